@@ -1,0 +1,123 @@
+/* libtnkit_hip -- C ABI of the MI355X (gfx950) two-site effective-Hamiltonian path.
+ *
+ * The reference (tnkit, Python) has no C ABI: its plug point is the duck-typed backend object
+ * returned by get_backend (tnkit/backend.py:192-240) whose einsum / vdot / norm calls the
+ * contraction helpers make. This library implements those contractions and the Krylov-vector
+ * work behind them. Each entry cites the tnkit function it replaces.
+ *
+ * Conventions
+ *   - Row-major C-contiguous arrays with tnkit's index order: MPS M (a, s, b); MPO W (wl, wr,
+ *     s_out, s_in); environments L, R (bra, mpo, ket)  (tnkit/_dmrg.py:7-11).
+ *   - Every pointer is a device pointer owned by the caller, except where marked "host".
+ *     Workspaces are caller-allocated; the library never allocates device memory.
+ *   - Calls are asynchronous on `stream` (a hipStream_t; NULL = the null stream) and never
+ *     synchronise the host.
+ *   - Every entry returns a TNH_* status; tnh_last_error() gives the message of the last
+ *     failure on the calling thread.
+ */
+#ifndef TNKIT_HIP_H
+#define TNKIT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* tnh_stream_t;
+
+/* dtype codes */
+enum { TNH_F64 = 0, TNH_F32 = 1, TNH_BF16 = 2, TNH_C128 = 3, TNH_C64 = 4 };
+/* status codes */
+enum { TNH_OK = 0, TNH_ERR_ARG = 1, TNH_ERR_WS = 2, TNH_ERR_HIP = 3, TNH_ERR_UNSUPPORTED = 4 };
+
+/* Library version (major*10000 + minor*100 + patch) and the last error on this thread. */
+int tnh_version(void);
+const char* tnh_last_error(void);
+
+/* C[m][n] (+)= sum_k A[k][m] * B[k][n], batched over `batch` with element strides.
+ * The GEMM every contraction below is built from; replaces the vendor GEMMs that tnkit's
+ * einsum reaches (tnkit/backend.py:317-322 numpy, torch.einsum on the torch backends).
+ * tile_hint: 0 = automatic, 1 = 128x128, 2 = 64x64, 3 = 32x32, 4 = 16x128. */
+int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long lda, const void* B, long ldb,
+                void* C, long ldc, int batch, long strideA, long strideB, long strideC,
+                int accumulate, int tile_hint, tnh_stream_t stream);
+
+/* fused_two_site (tnkit/_dmrg.py:172-180): the operator of one two-site problem, built once per
+ * local solve, in the layouts the matvec streams:
+ *   LT[q][p][wl] = L[p][wl][q]             (a, a, wl)
+ *   S[(s1 s2 wr)][(wl s1' s2')] = sum_m W1[wl][m][s1][s1'] W2[m][wr][s2][s2']
+ *   RT[wr][b][b'] = R[b'][wr][b]           (wr, b, b)
+ * dtype TNH_F64 or TNH_F32. */
+int tnh_fused_two_site(int dtype, int a, int wl, int wm, int wr, int d1, int d2, int b,
+                       const void* L, const void* W1, const void* W2, const void* R, void* LT,
+                       void* S, void* RT, tnh_stream_t stream);
+
+/* Workspace bytes of tnh_apply_two_site_fused. */
+size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr);
+
+/* apply_two_site_fused (tnkit/_dmrg.py:183-189): out = H_eff theta for theta (a, d1, d2, b).
+ * `dtype` is the arithmetic type of the operator (F64, or F32 on the F32 matrix cores);
+ * `io_dtype` the type of theta and out (F64 with an F32 operator gives the mixed-precision
+ * matvec with float64 Krylov vectors). tile1/tile3: tile hints of the two GEMMs (0 = auto). */
+int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl, int wr,
+                             const void* LT, const void* S, const void* RT, const void* theta,
+                             void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
+                             tnh_stream_t stream);
+
+/* Workspace bytes of the environment updates, for bonds (a, c), physical d, MPO bonds wl, wr. */
+size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr);
+
+/* update_left_env (tnkit/_dmrg.py:216-222): out (c, wr, c) = L (a, wl, a) grown past A (a, d, c)
+ * with W (wl, wr, d, d). Real dtypes (F64, F32). */
+int tnh_update_left_env(int dtype, int a, int d, int c, int wl, int wr, const void* L,
+                        const void* A, const void* W, void* out, void* ws, size_t ws_bytes,
+                        tnh_stream_t stream);
+
+/* update_right_env (tnkit/_dmrg.py:225-231): out (c, wl, c) = R (b, wr, b) grown past
+ * B (c, d, b) with W (wl, wr, d, d). Real dtypes (F64, F32). */
+int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const void* R,
+                         const void* B, const void* W, void* out, void* ws, size_t ws_bytes,
+                         tnh_stream_t stream);
+
+/* Workspace bytes of the Krylov-vector entries below. */
+size_t tnh_krylov_ws_bytes(long n, int nmax);
+
+/* One thick-restart Lanczos orthogonalisation step, float64 (tnkit/eigensolver.py:560-604):
+ *   w -= sum_{r<nsub} sub_coef[r] V[sub_base + r]      (three-term or arrowhead coupling;
+ *                                                       sub_coef is a HOST array)
+ *   scal[0] = alpha = <V[j], w>;  w -= alpha V[j]
+ *   two classical Gram-Schmidt passes against V[0..nlive)
+ *   scal[1] = beta = |w|
+ * V is (rows, ldv) row-major; nlive <= 64. scal is device memory (2 doubles). */
+int tnh_krylov_orth(long n, int nlive, const double* V, long ldv, double* w, int j, int nsub,
+                    int sub_base, const double* sub_coef, double* scal, void* ws, size_t ws_bytes,
+                    tnh_stream_t stream);
+
+/* out = beta * out + sum_{r<nrows} coef[r] V[r]  (coef HOST, nrows < 128): the Ritz vector
+ * reconstruction of tnkit/eigensolver.py:670-673 and the Krylov combination of
+ * tnkit/eigensolver.py:212. */
+int tnh_lincomb(long n, int nrows, const double* V, long ldv, const double* coef, double beta,
+                double* out, tnh_stream_t stream);
+
+/* out[0] = |x|  (tnkit NumpyBackend.norm, backend.py:295-296), float64, deterministic. */
+int tnh_norm(long n, const double* x, double* out, void* ws, size_t ws_bytes, tnh_stream_t stream);
+
+/* out[0] = <x, y>  (tnkit NumpyBackend.vdot for real data, backend.py:292-293). */
+int tnh_dot(long n, const double* x, const double* y, double* out, void* ws, size_t ws_bytes,
+            tnh_stream_t stream);
+
+/* y = x * (mul / max(div[0], 1e-200)); div may be NULL (then y = mul * x). The division guard is
+ * tnkit's guard_denom (tnkit/backend.py:699-708). */
+int tnh_scale_by(long n, const double* x, const double* div, double mul, double* y,
+                 tnh_stream_t stream);
+
+/* dst = (dst_dtype) src, for F64 <-> F32. */
+int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst,
+                tnh_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TNKIT_HIP_H */

@@ -1,0 +1,55 @@
+"""Closed forms and brute force that pin the oracle (TEST INFRASTRUCTURE ONLY).
+
+Only `tests/`, `__graft_entry__.smoke()` and `bench.py`'s `cpu_baseline` leg use this module.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def tfim_free_fermion_energy(N, g):
+    """Exact ground energy of the open chain H = -sum X_i X_{i+1} - sum g_i Z_i (g scalar or
+    length-N) by Jordan-Wigner: E0 = -1/2 sum_k Lambda_k with Lambda^2 = eig((A-B)(A+B)),
+    A = diag(2 g_i) - (nearest-neighbour 1), B_{i,i+1} = -1 = -B_{i+1,i}.
+    tnkit/free_fermion.py:19-31 (uniform g); the per-site form only changes diag(A)."""
+    g = np.broadcast_to(np.asarray(g, dtype=np.float64), (N,))
+    A = np.diag(2.0 * g)
+    B = np.zeros((N, N))
+    for i in range(N - 1):
+        A[i, i + 1] = A[i + 1, i] = -1.0
+        B[i, i + 1] = -1.0
+        B[i + 1, i] = 1.0
+    M = (A - B) @ (A + B)
+    w = np.linalg.eigvalsh(0.5 * (M + M.T))
+    return -0.5 * float(np.sum(np.sqrt(np.clip(w, 0.0, None))))
+
+
+def mpo_to_dense(mpo):
+    """The full 2^N x 2^N operator of an open MPO (W: wl, wr, s, s'), contracted left to right
+    with the boundary channels 0 (left) and 0 (right, the MPO's last tensor has wr = 1)."""
+    T = mpo[0][0]                                         # (wr, s, s')
+    T = T.reshape(T.shape[0], T.shape[1], T.shape[2])
+    for W in mpo[1:]:
+        # T: (w, S, S') ; W: (w, r, s, s') -> (r, S s, S' s')
+        T = np.einsum("wab,wrst->rasbt", T, W)
+        r, a, s, b, t = T.shape
+        T = T.reshape(r, a * s, b * t)
+    assert T.shape[0] == 1
+    return T[0]
+
+
+def ground_energy_dense(mpo):
+    """Lowest eigenvalue of `mpo_to_dense(mpo)` (exact diagonalisation, small N only)."""
+    H = mpo_to_dense(mpo)
+    H = 0.5 * (H + H.conj().T)
+    return float(np.linalg.eigvalsh(H)[0])
+
+
+def heff_dense(L, W1, W2, R):
+    """The two-site effective Hamiltonian as an explicit (a d1 d2 b)^2 matrix, by contracting
+    the network with identity inputs: H[(p s t d), (q i j b)] = sum L[p,w,q] W1[w,m,s,i]
+    W2[m,r,t,j] R[d,r,b].  A single einsum over the whole network (no pairwise order)."""
+    H = np.einsum("pwq,wmsi,mrtj,drb->pstdqijb", L, W1, W2, R)
+    a, d1, d2, b = L.shape[0], W1.shape[2], W2.shape[2], R.shape[0]
+    n = a * d1 * d2 * b
+    return H.reshape(n, L.shape[2] * W1.shape[3] * W2.shape[3] * R.shape[2])

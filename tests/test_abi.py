@@ -1,0 +1,47 @@
+"""The C-ABI library loads on the CPU host and exports every entry that include/tnkit_hip.h
+declares, with the Python binding covering each of them. No kernel is launched here."""
+import os
+import re
+
+from tnkit_amd import _lib
+
+from conftest import ROOT
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "tnkit_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(tnh_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entries():
+    names = _declared()
+    assert "tnh_apply_two_site_fused" in names and "tnh_krylov_orth" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_binding_covers_header():
+    assert set(_declared()) == set(_lib.SIGNATURES)
+
+
+def test_host_side_queries_need_no_device():
+    lib = _lib.load()
+    assert lib.tnh_version() >= 100
+    assert lib.tnh_two_site_ws_bytes(0, 1024, 2, 2, 1024, 5, 5) >= 2 * 4 * 1024 * 1024 * 5 * 8
+    assert lib.tnh_krylov_ws_bytes(4096, 41) > 0
+
+
+def test_bad_arguments_are_refused_before_any_launch():
+    lib = _lib.load()
+    rc = lib.tnh_apply_two_site_fused(0, 0, 0, 2, 2, 4, 3, 3, None, None, None, None, None, None,
+                                      0, 0, 0, None)
+    assert rc == _lib.ERR_ARG
+    assert b"positive" in lib.tnh_last_error()
+    rc = lib.tnh_gemm_tn(0, 8, 8, 8, None, 4, None, 8, None, 8, 1, 0, 0, 0, 0, 0, None)
+    assert rc == _lib.ERR_ARG

@@ -1,0 +1,178 @@
+"""`backend="hip"`: the MI355X backend object behind tnkit's backend interface.
+
+`get_backend(spec)` mirrors tnkit/backend.py:192-240 for the specs this package serves:
+'hip' (complex128), 'hip-real' (float64), 'auto' (real when the problem is real) and a backend
+object, which is returned unchanged. Arrays are torch tensors on the current HIP device; the
+hot contractions (`fused_two_site`, `apply_two_site_fused`, the environment updates) and the
+Krylov-vector work run in libtnkit_hip. There is no CPU path: constructing the backend without
+a HIP device or without the built library raises.
+"""
+from __future__ import annotations
+
+from typing import Any, Optional
+
+import numpy as np
+
+from . import _lib
+
+
+def _negligible_imag(a, tol=1e-12):
+    """(ok, max|imag|): whether the imaginary part is negligible relative to max(1, |real|),
+    the rule of tnkit/backend.py:243-276."""
+    a = np.asarray(a)
+    if not np.iscomplexobj(a):
+        return True, 0.0
+    im = np.abs(a.imag)
+    m = float(im.max(initial=0.0))
+    scale = np.maximum(1.0, np.abs(a.real))
+    return bool(np.all(im <= tol * scale)), m
+
+
+def is_real_problem(*groups, tol=1e-12) -> bool:
+    """True when every tensor of every group has a negligible imaginary part
+    (tnkit/backend.py:163-189)."""
+    for g in groups:
+        if g is None:
+            continue
+        for t in g:
+            arr = t.detach().cpu().numpy() if hasattr(t, "detach") else t
+            if not _negligible_imag(arr, tol)[0]:
+                return False
+    return True
+
+
+def get_backend(spec: Any = "hip", *, mpo=None, init_mps=None, real_tol: float = 1e-12):
+    """A backend from its name, or `spec` itself when it is already a backend object.
+
+    'hip-real' -> float64, 'hip' -> complex128, 'auto' -> 'hip-real' when `mpo` (and
+    `init_mps`) are real to `real_tol`, else 'hip'. Any other string is a ValueError
+    (tnkit/backend.py:240)."""
+    if not isinstance(spec, str):
+        return spec
+    if spec == "auto":
+        if mpo is None:
+            raise ValueError("get_backend(spec='auto') needs mpo=... to inspect")
+        spec = "hip-real" if is_real_problem(mpo, init_mps, tol=real_tol) else "hip"
+    if spec == "hip-real":
+        return HipBackend(real=True)
+    if spec == "hip":
+        return HipBackend(real=False)
+    raise ValueError(f"unknown backend {spec!r}; this package provides 'hip', 'hip-real' and 'auto'")
+
+
+class HipBackend:
+    """tnkit's backend duck-type (tnkit/backend.py:491-816) on the HIP device."""
+
+    is_torch = True
+
+    def __init__(self, real: bool = True, device: Optional[str] = None):
+        import torch
+        self.torch = torch
+        _lib.load()                                  # fails loudly if the library is missing
+        if not torch.cuda.is_available():
+            raise RuntimeError("backend 'hip' needs a ROCm GPU (torch.cuda.is_available() is "
+                               "False); there is no CPU fallback")
+        self.device = torch.device(device) if device else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.is_complex = not real
+        self.dtype = torch.complex128 if self.is_complex else torch.float64
+        self.rdtype = torch.float64
+        self.name = "hip" if self.is_complex else "hip-real"
+
+    # --- creation / movement ------------------------------------------------------------------
+    def array(self, x):
+        torch = self.torch
+        if isinstance(x, torch.Tensor):
+            t = x
+            if not self.is_complex and t.is_complex():
+                ok, m = _negligible_imag(t.detach().cpu().numpy())
+                if not ok:
+                    raise TypeError(f"real backend received an array with a significant "
+                                    f"imaginary part (max|imag|={m:.3e})")
+                t = t.real
+            return t.to(device=self.device, dtype=self.dtype).contiguous()
+        a = np.asarray(x)
+        if not self.is_complex and np.iscomplexobj(a):
+            ok, m = _negligible_imag(a)
+            if not ok:
+                raise TypeError(f"real backend received an array with a significant imaginary "
+                                f"part (max|imag|={m:.3e}). Use a complex backend.")
+            a = a.real
+        return torch.as_tensor(np.ascontiguousarray(a)).to(device=self.device, dtype=self.dtype)
+
+    def to_numpy(self, x):
+        if isinstance(x, self.torch.Tensor):
+            return x.detach().cpu().numpy()
+        return np.asarray(x)
+
+    def randn(self, rng, shape):
+        """Host numpy draw, then moved: the same state as the numpy backend for the same rng
+        (tnkit/backend.py:258-263)."""
+        r = rng.standard_normal(shape)
+        if self.is_complex:
+            r = r + 1j * rng.standard_normal(shape)
+        return self.array(r)
+
+    def zeros(self, shape):
+        return self.torch.zeros(shape, dtype=self.dtype, device=self.device)
+
+    def ones(self, shape):
+        return self.torch.ones(shape, dtype=self.dtype, device=self.device)
+
+    def zeros_like(self, a):
+        return self.torch.zeros_like(a)
+
+    def astype(self, x, which):
+        """'low' -> float32 (complex64), 'high' -> float64 (complex128). Unlike tnkit's torch
+        backend, a real problem stays real in low precision."""
+        torch = self.torch
+        if which == "low":
+            return x.to(torch.complex64 if x.is_complex() else torch.float32)
+        return x.to(torch.complex128 if x.is_complex() else torch.float64)
+
+    # --- elementwise / shape ------------------------------------------------------------------
+    def conj(self, a):
+        return a.conj() if a.is_complex() else a
+
+    def transpose(self, x, perm):
+        return x.permute(*perm)
+
+    def diag(self, v):
+        return self.torch.diag(v)
+
+    # --- reductions ---------------------------------------------------------------------------
+    def vdot(self, a, b):
+        torch = self.torch
+        if a.dtype == torch.float64 and b.dtype == torch.float64 and not a.is_complex():
+            from . import kernels
+            return kernels.dot(a.reshape(-1).contiguous(), b.reshape(-1).contiguous())
+        return torch.vdot(a.reshape(-1), b.reshape(-1))
+
+    def norm(self, a):
+        if a.dtype == self.torch.float64:
+            from . import kernels
+            return kernels.norm(a.reshape(-1).contiguous())
+        return self.torch.linalg.norm(a.reshape(-1))
+
+    def to_scalar(self, x):
+        return complex(x.item()) if hasattr(x, "item") else complex(x)
+
+    def norm_axis(self, x, axis=-1):
+        return self.torch.linalg.norm(x, dim=axis)
+
+    # --- contraction (plumbing outside the hot path) ------------------------------------------
+    def einsum(self, sub, *ops):
+        return self.torch.einsum(sub, *ops)
+
+    # --- factorisations -----------------------------------------------------------------------
+    def svd(self, A):
+        return self.torch.linalg.svd(A, full_matrices=False)
+
+    def qr(self, A):
+        return self.torch.linalg.qr(A, mode="reduced")
+
+    def eigh(self, A):
+        return self.torch.linalg.eigh(A)
+
+    def sync(self):
+        self.torch.cuda.synchronize(self.device)

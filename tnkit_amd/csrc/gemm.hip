@@ -1,0 +1,89 @@
+// Tile selection and launch of the K-major MFMA GEMM, plus the C-ABI entry tnh_gemm_tn.
+#include "gemm_tn.h"
+
+namespace tnh {
+
+namespace {
+
+template <typename TC>
+struct TileBK;
+template <>
+struct TileBK<double> {
+  static constexpr int big = 8;  // keeps the double-buffered 128x128 tiles under 40 KB of LDS
+  static constexpr int small = 16;
+};
+template <>
+struct TileBK<float> {
+  static constexpr int big = 16;
+  static constexpr int small = 16;
+};
+
+inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
+
+template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
+int launch_one(const GemmArgs& g, int batch, hipStream_t stream) {
+  const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
+  if (tiles == 0 || batch == 0) return TNH_OK;
+  TNH_REQUIRE(tiles < (1L << 31) && batch <= 65535, TNH_ERR_ARG, "gemm_tn: grid too large");
+  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(batch));
+  hipLaunchKernelGGL((gemm_tn_kernel<TC, TA, TB, TO, BM, BN, BK, WM, WN>), grid, dim3(WM * WN * 64),
+                     0, stream, g);
+  TNH_CHECK_LAUNCH("gemm_tn");
+  return TNH_OK;
+}
+
+}  // namespace
+
+// Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64), 2 = 64x64, 3 = 32x32, 4 = 16x128 (for
+// skinny M such as the thick-restart rotation). 0 picks the largest square tile that still gives
+// every CU two workgroups, and the skinny tile when M <= 16.
+template <typename TC, typename TA, typename TB, typename TO>
+int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint) {
+  TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn: negative size");
+  int cfg = tile_hint;
+  if (cfg == 0) {
+    const long b = batch > 0 ? batch : 1;
+    const long t128 = ceil_div(g.M, 128) * ceil_div(g.N, 128) * b;
+    const long t64 = ceil_div(g.M, 64) * ceil_div(g.N, 64) * b;
+    if (g.M <= 16)
+      cfg = 4;
+    else if (t128 >= 512)
+      cfg = 1;
+    else if (t64 >= 256)
+      cfg = 2;
+    else
+      cfg = 3;
+  }
+  constexpr int BKb = TileBK<TC>::big, BKs = TileBK<TC>::small;
+  switch (cfg) {
+    case 1: return launch_one<TC, TA, TB, TO, 128, 128, BKb, 2, 2>(g, batch, stream);
+    case 2: return launch_one<TC, TA, TB, TO, 64, 64, BKs, 2, 2>(g, batch, stream);
+    case 3: return launch_one<TC, TA, TB, TO, 32, 32, BKs, 2, 2>(g, batch, stream);
+    case 4: return launch_one<TC, TA, TB, TO, 16, 128, BKs, 1, 4>(g, batch, stream);
+    default: set_error("gemm_tn: unknown tile config %d", cfg); return TNH_ERR_ARG;
+  }
+}
+
+template int launch_gemm_tn<double, double, double, double>(const GemmArgs&, int, hipStream_t, int);
+template int launch_gemm_tn<float, float, float, float>(const GemmArgs&, int, hipStream_t, int);
+template int launch_gemm_tn<float, double, float, float>(const GemmArgs&, int, hipStream_t, int);
+template int launch_gemm_tn<float, float, float, double>(const GemmArgs&, int, hipStream_t, int);
+
+}  // namespace tnh
+
+using namespace tnh;
+
+extern "C" int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long lda, const void* B,
+                           long ldb, void* C, long ldc, int batch, long strideA, long strideB,
+                           long strideC, int accumulate, int tile_hint, tnh_stream_t stream) {
+  GemmArgs g{M, N, K, A, lda, strideA, B, ldb, strideB, C, ldc, strideC, accumulate};
+  TNH_REQUIRE(lda >= M && ldb >= N && ldc >= N, TNH_ERR_ARG,
+              "tnh_gemm_tn: leading dimensions (%ld, %ld, %ld) smaller than (M=%d, N=%d, N=%d)",
+              lda, ldb, ldc, M, N, N);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case TNH_F64: return launch_gemm_tn<double, double, double, double>(g, batch, s, tile_hint);
+    case TNH_F32: return launch_gemm_tn<float, float, float, float>(g, batch, s, tile_hint);
+    default: set_error("tnh_gemm_tn: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+}

@@ -1,0 +1,202 @@
+// K-major ("TN") GEMM on CDNA4 matrix cores:  C[m][n] = sum_k A[k][m] * B[k][n]
+//
+// Every large contraction of the two-site path is written so that both operands are stored
+// with the contracted index outermost (K rows of contiguous M or N entries). Tiles then move
+// from HBM to LDS as plain row copies, and the MFMA fragments are read from LDS as 16
+// consecutive entries per k row, which is bank-conflict free with the 16-element row pad.
+//
+// MFMA shapes (wave64):
+//   double : v_mfma_f64_16x16x4f64   lane l holds A[l%16][l/16], B[l/16][l%16]
+//   float  : v_mfma_f32_16x16x4f32   same operand layout
+//   result : lane l holds D[4*(l/16)+r][l%16], r = 0..3 (both)
+//
+// Storage types may differ from the compute type (A or B stored in double and rounded to
+// float on the way into LDS, C stored in double from a float accumulator), which is how the
+// mixed-precision matvec takes float64 Krylov vectors without a separate conversion pass.
+#pragma once
+#include "tnh_common.h"
+
+namespace tnh {
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A;
+  long lda;
+  long strideA;  // batch strides (elements)
+  const void* B;
+  long ldb;
+  long strideB;
+  void* C;
+  long ldc;
+  long strideC;
+  int accumulate;  // C += A^T B instead of C = A^T B
+};
+
+// Tile -> block mapping. Workgroups are dispatched round-robin over the 8 XCDs, each with its
+// own L2; consecutive logical tiles are remapped onto one XCD, and tiles are walked in groups
+// of `kGroup` tile rows so that neighbours share A and B panels in that XCD's L2.
+__device__ __forceinline__ void tile_of(int bid, int nbm, int nbn, int& tm, int& tn) {
+  constexpr int kXcd = 8;
+  constexpr int kGroup = 8;
+  const int total = nbm * nbn;
+  int pid = bid;
+  if (total % kXcd == 0) pid = (bid % kXcd) * (total / kXcd) + bid / kXcd;
+  const int per_group = kGroup * nbn;
+  const int g = pid / per_group;
+  const int first = g * kGroup;
+  const int gsize = min(nbm - first, kGroup);
+  const int in = pid - g * per_group;
+  tm = first + in % gsize;
+  tn = in / gsize;
+}
+
+template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(WM* WN * 64)
+    gemm_tn_kernel(GemmArgs g) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NT = WM * WN * 64;
+  constexpr int PAD = 16;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int FM = BM / WM / 16;
+  constexpr int FN = BN / WN / 16;
+  constexpr int EA = BK * BM / NT;  // elements per thread per A tile
+  constexpr int EB = BK * BN / NT;
+  static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k=4");
+  static_assert((BK * BM) % NT == 0 && (BK * BN) % NT == 0, "tile not divisible by threads");
+  static_assert(FM >= 1 && FN >= 1, "wave tile too small");
+
+  __shared__ TC As[2][BK * LDA_S];
+  __shared__ TC Bs[2][BK * LDB_S];
+
+  const int M = g.M, N = g.N, K = g.K;
+  const int nbm = (M + BM - 1) / BM;
+  const int nbn = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_of(blockIdx.x, nbm, nbn, tm, tn);
+  const int m0 = tm * BM;
+  const int n0 = tn * BN;
+  const long bz = blockIdx.y;
+  const TA* __restrict__ A = static_cast<const TA*>(g.A) + bz * g.strideA;
+  const TB* __restrict__ B = static_cast<const TB*>(g.B) + bz * g.strideB;
+  TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
+  const int row16 = lane & 15;
+  const int kq = lane >> 4;
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+  TC ra[EA];
+  TC rb[EB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int e = t + i * NT;
+      const int kr = e / BM, mc = e % BM;
+      const int k = k0 + kr, m = m0 + mc;
+      ra[i] = (k < K && m < M) ? static_cast<TC>(A[(long)k * g.lda + m]) : TC(0);
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int e = t + i * NT;
+      const int kr = e / BN, nc = e % BN;
+      const int k = k0 + kr, n = n0 + nc;
+      rb[i] = (k < K && n < N) ? static_cast<TC>(B[(long)k * g.ldb + n]) : TC(0);
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int e = t + i * NT;
+      As[buf][(e / BM) * LDA_S + e % BM] = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int e = t + i * NT;
+      Bs[buf][(e / BN) * LDB_S + e % BN] = rb[i];
+    }
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    const TC* as = As[cur];
+    const TC* bs = Bs[cur];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      TC af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * LDA_S + wm * (BM / WM) + i * 16 + row16];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + kq) * LDB_S + wn * (BN / WN) + j * 16 + row16];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * (BN / WN) + j * 16 + row16;
+      if (n >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * (BM / WM) + i * 16 + kq * 4 + r;
+        if (m < M) {
+          TO* dst = C + (long)m * g.ldc + n;
+          if (g.accumulate)
+            *dst = static_cast<TO>(static_cast<TC>(*dst) + acc[i][j][r]);
+          else
+            *dst = static_cast<TO>(acc[i][j][r]);
+        }
+      }
+    }
+  }
+}
+
+// Host-side launcher: picks a tile shape for the problem size and launches on `stream`.
+// Returns a TNH_* status.
+template <typename TC, typename TA, typename TB, typename TO>
+int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint = 0);
+
+}  // namespace tnh

@@ -1,0 +1,301 @@
+// Float64 Krylov-vector kernels: the Lanczos three-term / arrowhead subtraction, alpha, the two
+// classical Gram-Schmidt passes and beta of one thick-restart Lanczos iteration
+// (tnkit/eigensolver.py:560-604), plus linear combinations of basis rows (the Ritz vector,
+// tnkit/eigensolver.py:670-673) and the small helpers around them.
+//
+// Every reduction is two-stage and deterministic: each workgroup writes one partial per
+// coefficient, and a second kernel sums the partials in a fixed order. The basis V is read once
+// per Gram-Schmidt pass: a thread loads V[0..nv)[i] into registers, subtracts the previous
+// pass's projection from w[i] and accumulates the next pass's dot products from the same
+// registers, so one iteration reads V three times (the reference form reads it four times).
+#include "tnh_common.h"
+
+namespace tnh {
+
+constexpr int kHostCoef = 128;  // coefficients passed by value (Krylov windows up to 127 rows)
+
+struct HostCoef {
+  double c[kHostCoef];
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sums `vals[0..NV)` over the block and writes them to part[blockIdx.x*ld + k].
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&vals)[NV], int nv, double* part, int ld) {
+  __shared__ double red[4][NV];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    if (k < nv) {
+      const double s = wave_sum(vals[k]);
+      if (lane == 0) red[wave][k] = s;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nv; k += blockDim.x)
+    part[(long)blockIdx.x * ld + k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+}
+
+// w[i] -= sum_r hc[r] * V[sub_base + r][i];   partial <V[j], w>
+__global__ void __launch_bounds__(256)
+    orth_pre_kernel(long n, const double* __restrict__ V, long ldv, double* __restrict__ w,
+                    int nsub, int sub_base, HostCoef hc, int j, double* __restrict__ part) {
+  double acc[1] = {0.0};
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    double wi = w[i];
+    for (int r = 0; r < nsub; ++r) wi -= hc.c[r] * V[(long)(sub_base + r) * ldv + i];
+    if (nsub > 0) w[i] = wi;
+    acc[0] += V[(long)j * ldv + i] * wi;
+  }
+  block_partials<1>(acc, 1, part, 1);
+}
+
+// w[i] -= sum_{k<nv} dc[k] * V[k][i]; then partial dots <V[k], w> (mode 0), or partial |w|^2
+// (mode 1). V rows are held in registers between the subtraction and the dots.
+template <int NV>
+__global__ void __launch_bounds__(256)
+    orth_pass_kernel(long n, int nv, const double* __restrict__ V, long ldv, double* __restrict__ w,
+                     const double* __restrict__ dc, int mode, double* __restrict__ part) {
+  __shared__ double cs[NV];
+  for (int k = threadIdx.x; k < NV; k += blockDim.x) cs[k] = (k < nv && dc) ? dc[k] : 0.0;
+  __syncthreads();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    double v[NV];
+    double wi = w[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      v[k] = (k < nv) ? V[(long)k * ldv + i] : 0.0;
+      wi -= cs[k] * v[k];
+    }
+    if (nv > 0) w[i] = wi;
+    if (mode == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) acc[k] += v[k] * wi;
+    } else {
+      acc[0] += wi * wi;
+    }
+  }
+  block_partials<NV>(acc, mode == 0 ? nv : 1, part, mode == 0 ? nv : 1);
+}
+
+// out[k] = sum_b part[b*ld + k]  for k < ncol, in block order. mode 0: plain; 1: sqrt (norms);
+// 2: alpha -- scal[0] = sum and coef[k] = (k == jsel) ? sum : 0 for k < ncoef.
+__global__ void __launch_bounds__(256)
+    reduce_kernel(int nb, int ncol, const double* __restrict__ part, int ld, double* __restrict__ out,
+                  int mode, int jsel, int ncoef, double* __restrict__ coef) {
+  __shared__ double red[4];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[(long)b * ld + k];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = red[0] + red[1] + red[2] + red[3];
+    if (mode == 1) tot = sqrt(tot);
+    out[k] = tot;
+    if (mode == 2)
+      for (int r = 0; r < ncoef; ++r) coef[r] = (r == jsel) ? tot : 0.0;
+  }
+}
+
+// out[i] = beta * out[i] + sum_r hc[r] * V[r][i]
+__global__ void __launch_bounds__(256)
+    lincomb_kernel(long n, int nrows, const double* __restrict__ V, long ldv, HostCoef hc,
+                   double beta, double* __restrict__ out) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    double s = (beta != 0.0) ? beta * out[i] : 0.0;
+    for (int r = 0; r < nrows; ++r) s += hc.c[r] * V[(long)r * ldv + i];
+    out[i] = s;
+  }
+}
+
+// y[i] = x[i] * (mul / div[0])  -- division by a device scalar (beta), with tnkit's
+// guard_denom clamp: a divisor below 1e-200 is replaced by 1e-200 (tnkit/backend.py:699-708)
+__global__ void scale_by_kernel(long n, const double* __restrict__ x, const double* __restrict__ div,
+                                double mul, double* __restrict__ y) {
+  double d = div ? div[0] : 1.0;
+  if (div && d < 1e-200) d = 1e-200;
+  const double s = mul / d;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = x[i] * s;
+}
+
+template <typename TO, typename TI>
+__global__ void convert_kernel(long n, const TI* __restrict__ x, TO* __restrict__ y) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+    y[i] = static_cast<TO>(x[i]);
+}
+
+namespace {
+
+inline int nblocks(long n) {
+  long b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 1024) b = 1024;  // four workgroups per CU saturate HBM; more only grows the partials
+  return static_cast<int>(b);
+}
+
+template <int NV>
+int pass(long n, int nv, const double* V, long ldv, double* w, const double* dc, int mode,
+         double* part, int nb, hipStream_t st) {
+  hipLaunchKernelGGL(orth_pass_kernel<NV>, dim3(nb), dim3(256), 0, st, n, nv, V, ldv, w, dc, mode,
+                     part);
+  TNH_CHECK_LAUNCH("orth_pass");
+  return TNH_OK;
+}
+
+int pass_any(long n, int nv, const double* V, long ldv, double* w, const double* dc, int mode,
+             double* part, int nb, hipStream_t st) {
+  if (nv <= 4) return pass<4>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  if (nv <= 8) return pass<8>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  if (nv <= 16) return pass<16>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  if (nv <= 32) return pass<32>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  if (nv <= 48) return pass<48>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  if (nv <= 64) return pass<64>(n, nv, V, ldv, w, dc, mode, part, nb, st);
+  set_error("krylov: at most 64 live basis vectors are supported, got %d", nv);
+  return TNH_ERR_ARG;
+}
+
+int reduce(int nb, int ncol, const double* part, int ld, double* out, int mode, int jsel, int ncoef,
+           double* coef, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_kernel, dim3(ncol), dim3(256), 0, st, nb, ncol, part, ld, out, mode, jsel,
+                     ncoef, coef);
+  TNH_CHECK_LAUNCH("reduce");
+  return TNH_OK;
+}
+
+#define TNH_TRY(x)                 \
+  do {                             \
+    int rc_ = (x);                 \
+    if (rc_ != TNH_OK) return rc_; \
+  } while (0)
+
+}  // namespace
+}  // namespace tnh
+
+using namespace tnh;
+
+extern "C" {
+
+size_t tnh_krylov_ws_bytes(long n, int nmax) {
+  (void)n;
+  // partials: 1024 blocks x 64 columns, plus three coefficient vectors
+  return align_up(1024 * 64 * sizeof(double), 256) + 3 * align_up(64 * sizeof(double), 256) +
+         256 + (size_t)0 * nmax;
+}
+
+int tnh_krylov_orth(long n, int nlive, const double* V, long ldv, double* w, int j, int nsub,
+                    int sub_base, const double* sub_coef, double* scal, void* ws, size_t ws_bytes,
+                    tnh_stream_t stream) {
+  TNH_REQUIRE(n > 0 && ldv >= n, TNH_ERR_ARG, "tnh_krylov_orth: bad n=%ld ldv=%ld", n, ldv);
+  TNH_REQUIRE(nlive >= 1 && nlive <= 64 && j >= 0 && j < nlive, TNH_ERR_ARG,
+              "tnh_krylov_orth: need 1 <= nlive <= 64 and 0 <= j < nlive (nlive=%d, j=%d)", nlive, j);
+  TNH_REQUIRE(nsub >= 0 && nsub < kHostCoef && sub_base >= 0, TNH_ERR_ARG,
+              "tnh_krylov_orth: nsub=%d out of range", nsub);
+  HostCoef hc;
+  memset(&hc, 0, sizeof(hc));
+  for (int r = 0; r < nsub; ++r) hc.c[r] = sub_coef[r];
+  Arena ar(ws, ws_bytes);
+  double* part = ar.take<double>(1024 * 64);
+  double* c0 = ar.take<double>(64);
+  double* c1 = ar.take<double>(64);
+  double* c2 = ar.take<double>(64);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_krylov_orth: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = nblocks(n);
+  // w -= sub; alpha = <V[j], w>
+  hipLaunchKernelGGL(orth_pre_kernel, dim3(nb), dim3(256), 0, st, n, V, ldv, w, nsub, sub_base, hc,
+                     j, part);
+  TNH_CHECK_LAUNCH("orth_pre");
+  TNH_TRY(reduce(nb, 1, part, 1, scal + 0, 2, j, nlive, c0, st));
+  // w -= alpha V[j]; c1 = V w
+  TNH_TRY(pass_any(n, nlive, V, ldv, w, c0, 0, part, nb, st));
+  TNH_TRY(reduce(nb, nlive, part, nlive, c1, 0, 0, 0, nullptr, st));
+  // w -= c1 V; c2 = V w
+  TNH_TRY(pass_any(n, nlive, V, ldv, w, c1, 0, part, nb, st));
+  TNH_TRY(reduce(nb, nlive, part, nlive, c2, 0, 0, 0, nullptr, st));
+  // w -= c2 V; beta = |w|
+  TNH_TRY(pass_any(n, nlive, V, ldv, w, c2, 1, part, nb, st));
+  TNH_TRY(reduce(nb, 1, part, 1, scal + 1, 1, 0, 0, nullptr, st));
+  return TNH_OK;
+}
+
+int tnh_lincomb(long n, int nrows, const double* V, long ldv, const double* coef, double beta,
+                double* out, tnh_stream_t stream) {
+  TNH_REQUIRE(n >= 0 && nrows >= 0 && nrows < kHostCoef && (nrows == 0 || ldv >= n), TNH_ERR_ARG,
+              "tnh_lincomb: bad sizes n=%ld nrows=%d ldv=%ld", n, nrows, ldv);
+  if (n == 0) return TNH_OK;
+  HostCoef hc;
+  memset(&hc, 0, sizeof(hc));
+  for (int r = 0; r < nrows; ++r) hc.c[r] = coef[r];
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks(n)), dim3(256), 0, st, n, nrows, V, ldv, hc, beta,
+                     out);
+  TNH_CHECK_LAUNCH("lincomb");
+  return TNH_OK;
+}
+
+int tnh_norm(long n, const double* x, double* out, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(n > 0, TNH_ERR_ARG, "tnh_norm: n must be positive");
+  Arena ar(ws, ws_bytes);
+  double* part = ar.take<double>(1024 * 64);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_norm: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = nblocks(n);
+  // a pass with no subtraction over a zero-row basis leaves x unchanged and sums x^2
+  TNH_TRY(pass<4>(n, 0, x, n, const_cast<double*>(x), nullptr, 1, part, nb, st));
+  return reduce(nb, 1, part, 1, out, 1, 0, 0, nullptr, st);
+}
+
+int tnh_dot(long n, const double* x, const double* y, double* out, void* ws, size_t ws_bytes,
+            tnh_stream_t stream) {
+  TNH_REQUIRE(n > 0, TNH_ERR_ARG, "tnh_dot: n must be positive");
+  Arena ar(ws, ws_bytes);
+  double* part = ar.take<double>(1024 * 64);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_dot: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  HostCoef hc;
+  memset(&hc, 0, sizeof(hc));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = nblocks(n);
+  hipLaunchKernelGGL(orth_pre_kernel, dim3(nb), dim3(256), 0, st, n, x, n, const_cast<double*>(y), 0,
+                     0, hc, 0, part);
+  TNH_CHECK_LAUNCH("dot");
+  return reduce(nb, 1, part, 1, out, 0, 0, 0, nullptr, st);
+}
+
+int tnh_scale_by(long n, const double* x, const double* div, double mul, double* y,
+                 tnh_stream_t stream) {
+  if (n <= 0) return TNH_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(scale_by_kernel, dim3(nblocks(n)), dim3(256), 0, st, n, x, div, mul, y);
+  TNH_CHECK_LAUNCH("scale_by");
+  return TNH_OK;
+}
+
+int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst,
+                tnh_stream_t stream) {
+  if (n <= 0) return TNH_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dst_dtype == TNH_F32 && src_dtype == TNH_F64) {
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nblocks(n)), dim3(256), 0, st, n,
+                       static_cast<const double*>(src), static_cast<float*>(dst));
+  } else if (dst_dtype == TNH_F64 && src_dtype == TNH_F32) {
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblocks(n)), dim3(256), 0, st, n,
+                       static_cast<const float*>(src), static_cast<double*>(dst));
+  } else {
+    set_error("tnh_convert: unsupported pair (%d <- %d)", dst_dtype, src_dtype);
+    return TNH_ERR_UNSUPPORTED;
+  }
+  TNH_CHECK_LAUNCH("convert");
+  return TNH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,352 @@
+// The two-site effective Hamiltonian and the environment updates as K-major MFMA GEMMs joined
+// by small "mix" kernels that apply the MPO tensors.
+//
+// Two-site matvec (tnkit/_dmrg.py:172-189, apply_two_site_fused), bonds a -> b, d1, d2, wl, wr:
+//   GEMM1  X1[(i j b)][(p w)]  = sum_q theta[q][(i j b)] * LT[q][(p w)]      LT[q][p][w] = L[p][w][q]
+//   S-mix  X2[r][(b p)][(s t)] = sum_{w i j} S[(s t r)][(w i j)] * X1[(i j)][(b p)][w]
+//   GEMM3  out[(p s t)][d]     = sum_{(r b)} X2[(r b)][(p s t)] * RT[(r b)][d]  RT[r][b][d] = R[d][r][b]
+// Left environment (tnkit/_dmrg.py:216-222), L(a, wl, a), A(a, d, c), W(wl, wr, d, d):
+//   GEMM   T1[(w q)][(s e)]   = sum_p L[p][(w q)] * A[p][(s e)]
+//   W-mix  T2[(q y)][(e r)]   = sum_{w s} W[w][r][s][y] * T1[w][q][s][e]
+//   GEMM   L'[(e r)][f]       = sum_{(q y)} T2[(q y)][(e r)] * A[(q y)][f]
+// Right environment (tnkit/_dmrg.py:225-231), R(b, wr, b), B(c, d, b):
+//   GEMM   T1[(r y)][(u s)]   = sum_x R[x][(r y)] * Bx[x][(u s)]               Bx[x][u][s] = B[u][s][x]
+//   W-mix  T2[(p y)][(u w)]   = sum_{r s} W[w][r][s][p] * T1[r][y][u][s]
+//   GEMM   R'[(u w)][v]       = sum_{(p y)} T2[(p y)][(u w)] * Bp[(p y)][v]     Bp[p][y][v] = B[v][p][y]
+// Real dtypes (the statics path); conj() is the identity.
+#include "gemm_tn.h"
+
+namespace tnh {
+
+// ---------------------------------------------------------------------------------------------
+// small kernels
+// ---------------------------------------------------------------------------------------------
+
+// out[j0][j1][j2] = in[j0*s0 + j1*s1 + j2*s2]   (a 3-axis permutation given by input strides)
+template <typename T>
+__global__ void permute3_kernel(long n0, long n1, long n2, long s0, long s1, long s2,
+                                const T* __restrict__ in, T* __restrict__ out) {
+  const long total = n0 * n1 * n2;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total;
+       o += (long)gridDim.x * blockDim.x) {
+    const long j2 = o % n2;
+    const long r = o / n2;
+    const long j1 = r % n1;
+    const long j0 = r / n1;
+    out[o] = in[j0 * s0 + j1 * s1 + j2 * s2];
+  }
+}
+
+// S[(s t r)][(w i j)] = sum_m W1[w][m][s][i] * W2[m][r][t][j]
+template <typename T>
+__global__ void build_s_kernel(int wl, int wm, int wr, int d1, int d2, const T* __restrict__ W1,
+                               const T* __restrict__ W2, T* __restrict__ S) {
+  const int ncol = wl * d1 * d2;
+  const int total = d1 * d2 * wr * ncol;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
+    const int col = o % ncol, row = o / ncol;
+    const int j = col % d2, i = (col / d2) % d1, w = col / (d1 * d2);
+    const int r = row % wr, t = (row / wr) % d2, s = row / (wr * d2);
+    T acc = 0;
+    for (int m = 0; m < wm; ++m)
+      acc += W1[((w * wm + m) * d1 + s) * d1 + i] * W2[((m * wr + r) * d2 + t) * d2 + j];
+    S[o] = acc;
+  }
+}
+
+// Mx[(o1 o2)][(i1 i2)] = W[w][r][s][y] with the four MPO axes routed to (o1, o2, i1, i2) by
+// `which`: 0 (left env)  Mx[(y r)][(w s)];   1 (right env)  Mx[(p w)][(r s)] = W[w][r][s][p].
+template <typename T>
+__global__ void build_wmix_kernel(int which, int wl, int wr, int d, const T* __restrict__ W,
+                                  T* __restrict__ Mx) {
+  const int total = wl * wr * d * d;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
+    int w, r, s, y;
+    if (which == 0) {  // o = ((y*wr + r)*wl + w)*d + s
+      s = o % d;
+      w = (o / d) % wl;
+      r = (o / (d * wl)) % wr;
+      y = o / (d * wl * wr);
+    } else {  // o = ((p*wl + w)*wr + r)*d + s, p -> y
+      s = o % d;
+      r = (o / d) % wr;
+      w = (o / (d * wr)) % wl;
+      y = o / (d * wr * wl);
+    }
+    Mx[o] = W[((w * wr + r) * d + s) * d + y];
+  }
+}
+
+// Generic small-matrix mix over a 2-level group index (g1, g2):
+//   out[g1*os1 + g2*os2 + jo1*ojs1 + jo2*ojs2] = sum_ji Mx[jo][ji] * in[g1*is1 + g2*is2 + ji1*ijs1 + ji2*ijs2]
+// with jo = jo1*njo2 + jo2, ji = ji1*nji2 + ji2. One thread per (jo, g1, g2), g2 fastest so a wave
+// shares jo and the branch on a zero MPO entry is uniform.
+struct MixArgs {
+  long G1, G2;
+  int njo, njo2, nji, nji2;
+  long is1, is2, ijs1, ijs2;
+  long os1, os2, ojs1, ojs2;
+};
+
+template <typename TC, typename TI, typename TO>
+__global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restrict__ Mx,
+                                                  const TI* __restrict__ in, TO* __restrict__ out) {
+  const long total = a.G1 * a.G2 * a.njo;
+  for (long tid = blockIdx.x * (long)blockDim.x + threadIdx.x; tid < total;
+       tid += (long)gridDim.x * blockDim.x) {
+    const long g2 = tid % a.G2;
+    const long rr = tid / a.G2;
+    const long g1 = rr % a.G1;
+    const int jo = static_cast<int>(rr / a.G1);
+    const long base = g1 * a.is1 + g2 * a.is2;
+    const TC* mrow = Mx + (long)jo * a.nji;
+    TC acc = 0;
+    for (int ji = 0; ji < a.nji; ++ji) {
+      const TC m = mrow[ji];
+      if (m != TC(0)) {
+        const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
+        acc += m * static_cast<TC>(in[base + ji1 * a.ijs1 + ji2 * a.ijs2]);
+      }
+    }
+    const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
+    out[g1 * a.os1 + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(acc);
+  }
+}
+
+namespace {
+
+inline unsigned grid_for(long total, int threads = 256, long cap = 1L << 16) {
+  long g = (total + threads - 1) / threads;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+template <typename T>
+int permute3(long n0, long n1, long n2, long s0, long s1, long s2, const void* in, void* out,
+             hipStream_t st) {
+  const long total = n0 * n1 * n2;
+  if (total == 0) return TNH_OK;
+  hipLaunchKernelGGL(permute3_kernel<T>, dim3(grid_for(total)), dim3(256), 0, st, n0, n1, n2, s0,
+                     s1, s2, static_cast<const T*>(in), static_cast<T*>(out));
+  TNH_CHECK_LAUNCH("permute3");
+  return TNH_OK;
+}
+
+template <typename TC, typename TI, typename TO>
+int mix(const MixArgs& a, const void* Mx, const void* in, void* out, hipStream_t st) {
+  const long total = a.G1 * a.G2 * a.njo;
+  if (total == 0) return TNH_OK;
+  hipLaunchKernelGGL((mix_kernel<TC, TI, TO>), dim3(grid_for(total, 256, 1L << 20)), dim3(256), 0,
+                     st, a, static_cast<const TC*>(Mx), static_cast<const TI*>(in),
+                     static_cast<TO*>(out));
+  TNH_CHECK_LAUNCH("mix");
+  return TNH_OK;
+}
+
+size_t dsize(int dtype) {
+  switch (dtype) {
+    case TNH_F64: return 8;
+    case TNH_F32: return 4;
+    default: return 0;
+  }
+}
+
+#define TNH_TRY(x)              \
+  do {                          \
+    int rc_ = (x);              \
+    if (rc_ != TNH_OK) return rc_; \
+  } while (0)
+
+// ---- two-site ----------------------------------------------------------------------------------
+
+template <typename T>
+int fused_two_site(int a, int wl, int wm, int wr, int d1, int d2, int b, const void* L,
+                   const void* W1, const void* W2, const void* R, void* LT, void* S, void* RT,
+                   hipStream_t st) {
+  // LT[q][p][w] = L[p][w][q]: L strides (wl*a, a, 1) for (p, w, q)
+  TNH_TRY(permute3<T>(a, a, wl, 1, (long)wl * a, a, L, LT, st));
+  // RT[r][b][d] = R[d][r][b]: R strides (wr*b, b, 1) for (d, r, b)
+  TNH_TRY(permute3<T>(wr, b, b, b, 1, (long)wr * b, R, RT, st));
+  const int total = d1 * d2 * wr * wl * d1 * d2;
+  hipLaunchKernelGGL(build_s_kernel<T>, dim3(grid_for(total)), dim3(256), 0, st, wl, wm, wr, d1, d2,
+                     static_cast<const T*>(W1), static_cast<const T*>(W2), static_cast<T*>(S));
+  TNH_CHECK_LAUNCH("build_s");
+  return TNH_OK;
+}
+
+template <typename TC, typename TIO>
+int apply_two_site(int a, int d1, int d2, int b, int wl, int wr, const void* LT, const void* S,
+                   const void* RT, const void* theta, void* out, void* ws, size_t ws_bytes,
+                   hipStream_t st, int tile1, int tile3) {
+  const long P = a, Q = a, B = b, dd = (long)d1 * d2;
+  Arena ar(ws, ws_bytes);
+  TC* X1 = ar.take<TC>(dd * B * P * wl);
+  TC* X2 = ar.take<TC>((long)wr * B * P * dd);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
+              ar.used);
+  // GEMM1: M = d1 d2 b, N = p wl, K = q
+  {
+    GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
+               dd * B, 0, LT, P * wl, 0, X1, P * wl, 0, 0};
+    TNH_TRY((launch_gemm_tn<TC, TIO, TC, TC>(g, 1, st, tile1)));
+  }
+  // S-mix
+  {
+    MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
+              P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd};
+    TNH_TRY((mix<TC, TC, TC>(m, S, X1, X2, st)));
+  }
+  // GEMM3: M = p d1 d2, N = b', K = wr b
+  {
+    GemmArgs g{static_cast<int>(P * dd), static_cast<int>(B), static_cast<int>(wr * B), X2, P * dd,
+               0, RT, B, 0, out, B, 0, 0};
+    TNH_TRY((launch_gemm_tn<TC, TC, TC, TIO>(g, 1, st, tile3)));
+  }
+  return TNH_OK;
+}
+
+// ---- environments ------------------------------------------------------------------------------
+
+template <typename T>
+int left_env(int a, int d, int c, int wl, int wr, const void* L, const void* A, const void* W,
+             void* out, void* ws, size_t ws_bytes, hipStream_t st) {
+  Arena ar(ws, ws_bytes);
+  T* T1 = ar.take<T>((long)wl * a * d * c);
+  T* T2 = ar.take<T>((long)a * d * c * wr);
+  T* Mx = ar.take<T>((long)wl * wr * d * d);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "update_left_env: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for((long)wl * wr * d * d)), dim3(256), 0, st,
+                     0, wl, wr, d, static_cast<const T*>(W), Mx);
+  TNH_CHECK_LAUNCH("build_wmix");
+  {  // T1[(w q)][(s e)] = sum_p L[p][(w q)] A[p][(s e)]
+    GemmArgs g{wl * a, d * c, a, L, (long)wl * a, 0, A, (long)d * c, 0, T1, (long)d * c, 0, 0};
+    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  }
+  {  // T2[q][y][e][r] = sum_{w s} Mx[(y r)][(w s)] T1[w][q][s][e]; groups (q, e)
+    MixArgs m{a, c, d * wr, wr, wl * d, d, (long)d * c, 1, (long)a * d * c, c,
+              (long)d * c * wr, wr, (long)c * wr, 1};
+    TNH_TRY((mix<T, T, T>(m, Mx, T1, T2, st)));
+  }
+  {  // L'[(e r)][f] = sum_{(q y)} T2[(q y)][(e r)] A[(q y)][f]
+    GemmArgs g{c * wr, c, a * d, T2, (long)c * wr, 0, A, c, 0, out, c, 0, 0};
+    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  }
+  return TNH_OK;
+}
+
+template <typename T>
+int right_env(int c, int d, int b, int wl, int wr, const void* R, const void* B, const void* W,
+              void* out, void* ws, size_t ws_bytes, hipStream_t st) {
+  Arena ar(ws, ws_bytes);
+  T* Bx = ar.take<T>((long)b * c * d);
+  T* Bp = ar.take<T>((long)d * b * c);
+  T* T1 = ar.take<T>((long)wr * b * c * d);
+  T* T2 = ar.take<T>((long)d * b * c * wl);
+  T* Mx = ar.take<T>((long)wl * wr * d * d);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "update_right_env: workspace %zu < %zu bytes", ws_bytes,
+              ar.used);
+  // B(c, d, b) strides (d*b, b, 1) for (u, s, x)
+  TNH_TRY(permute3<T>(b, c, d, 1, (long)d * b, b, B, Bx, st));  // Bx[x][u][s]
+  TNH_TRY(permute3<T>(d, b, c, b, 1, (long)d * b, B, Bp, st));  // Bp[p][y][v]
+  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for((long)wl * wr * d * d)), dim3(256), 0, st,
+                     1, wl, wr, d, static_cast<const T*>(W), Mx);
+  TNH_CHECK_LAUNCH("build_wmix");
+  {  // T1[(r y)][(u s)] = sum_x R[x][(r y)] Bx[x][(u s)]
+    GemmArgs g{wr * b, c * d, b, R, (long)wr * b, 0, Bx, (long)c * d, 0, T1, (long)c * d, 0, 0};
+    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  }
+  {  // T2[p][y][u][w] = sum_{r s} Mx[(p w)][(r s)] T1[r][y][u][s]; groups (y, u)
+    MixArgs m{b, c, d * wl, wl, wr * d, d, (long)c * d, d, (long)b * c * d, 1,
+              (long)c * wl, wl, (long)b * c * wl, 1};
+    TNH_TRY((mix<T, T, T>(m, Mx, T1, T2, st)));
+  }
+  {  // R'[(u w)][v] = sum_{(p y)} T2[(p y)][(u w)] Bp[(p y)][v]
+    GemmArgs g{c * wl, c, d * b, T2, (long)c * wl, 0, Bp, c, 0, out, c, 0, 0};
+    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  }
+  return TNH_OK;
+}
+
+}  // namespace
+}  // namespace tnh
+
+using namespace tnh;
+
+extern "C" {
+
+int tnh_fused_two_site(int dtype, int a, int wl, int wm, int wr, int d1, int d2, int b,
+                       const void* L, const void* W1, const void* W2, const void* R, void* LT,
+                       void* S, void* RT, tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wm > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_fused_two_site: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case TNH_F64: return fused_two_site<double>(a, wl, wm, wr, d1, d2, b, L, W1, W2, R, LT, S, RT, st);
+    case TNH_F32: return fused_two_site<float>(a, wl, wm, wr, d1, d2, b, L, W1, W2, R, LT, S, RT, st);
+    default: set_error("tnh_fused_two_site: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+}
+
+size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr) {
+  const size_t e = dsize(dtype);
+  const size_t dd = (size_t)d1 * d2;
+  return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + 256;
+}
+
+int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl, int wr,
+                             const void* LT, const void* S, const void* RT, const void* theta,
+                             void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
+                             tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_apply_two_site_fused: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == TNH_F64 && io_dtype == TNH_F64)
+    return apply_two_site<double, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                          st, tile1, tile3);
+  if (dtype == TNH_F32 && io_dtype == TNH_F32)
+    return apply_two_site<float, float>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                        st, tile1, tile3);
+  if (dtype == TNH_F32 && io_dtype == TNH_F64)
+    return apply_two_site<float, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                         st, tile1, tile3);
+  set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr) {
+  // the larger of the left (bonds a -> c) and right (bonds a <- c) workspaces
+  const size_t e = dsize(dtype);
+  const size_t w = (size_t)wl > (size_t)wr ? wl : wr;
+  const size_t big = (size_t)a * d * c;
+  return 2 * align_up(big * e, 256) + 2 * align_up(big * w * e, 256) +
+         align_up((size_t)wl * wr * d * d * e, 256) + 256;
+}
+
+int tnh_update_left_env(int dtype, int a, int d, int c, int wl, int wr, const void* L,
+                        const void* A, const void* W, void* out, void* ws, size_t ws_bytes,
+                        tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
+              "tnh_update_left_env: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case TNH_F64: return left_env<double>(a, d, c, wl, wr, L, A, W, out, ws, ws_bytes, st);
+    case TNH_F32: return left_env<float>(a, d, c, wl, wr, L, A, W, out, ws, ws_bytes, st);
+    default: set_error("tnh_update_left_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+}
+
+int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const void* R,
+                         const void* B, const void* W, void* out, void* ws, size_t ws_bytes,
+                         tnh_stream_t stream) {
+  TNH_REQUIRE(b > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
+              "tnh_update_right_env: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case TNH_F64: return right_env<double>(c, d, b, wl, wr, R, B, W, out, ws, ws_bytes, st);
+    case TNH_F32: return right_env<float>(c, d, b, wl, wr, R, B, W, out, ws, ws_bytes, st);
+    default: set_error("tnh_update_right_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+"""Functional wrappers of libtnkit_hip over torch tensors on the current HIP device.
+
+torch supplies device memory and the stream; every contraction here is a libtnkit_hip kernel.
+Each function checks shapes, dtypes, devices and contiguity on the host before the launch, so a
+mismatch raises ValueError/TypeError instead of reaching a kernel.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Sequence, Tuple
+
+import torch
+
+from . import _lib
+from ._lib import check, dcode, ptr
+
+_REAL = (torch.float64, torch.float32)
+
+
+def _need(t: torch.Tensor, name: str, ndim: int, dtypes=_REAL) -> None:
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name}: must be on the HIP device, got {t.device}")
+    if t.dim() != ndim:
+        raise ValueError(f"{name}: expected {ndim} axes, got shape {tuple(t.shape)}")
+    if t.dtype not in dtypes:
+        raise TypeError(f"{name}: dtype {t.dtype} not supported here ({dtypes})")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+
+
+def _ws(nbytes: int, device) -> torch.Tensor:
+    return torch.empty(max(int(nbytes), 256), dtype=torch.uint8, device=device)
+
+
+def stream(device=None) -> int:
+    return _lib.stream_of(device)
+
+
+# ---------------------------------------------------------------------------------------------
+# GEMM
+# ---------------------------------------------------------------------------------------------
+
+
+def gemm_tn(A: torch.Tensor, B: torch.Tensor, out: Optional[torch.Tensor] = None,
+            accumulate: bool = False, tile: int = 0) -> torch.Tensor:
+    """out[m, n] (+)= sum_k A[k, m] B[k, n] for 2-D contiguous A (K, M), B (K, N)."""
+    _need(A, "A", 2)
+    _need(B, "B", 2)
+    if A.dtype != B.dtype or A.shape[0] != B.shape[0]:
+        raise ValueError(f"gemm_tn: A {tuple(A.shape)} {A.dtype} and B {tuple(B.shape)} {B.dtype}")
+    K, M = A.shape
+    N = B.shape[1]
+    if out is None:
+        out = torch.empty((M, N), dtype=A.dtype, device=A.device)
+        accumulate = False
+    _need(out, "out", 2)
+    if tuple(out.shape) != (M, N) or out.dtype != A.dtype:
+        raise ValueError(f"gemm_tn: out {tuple(out.shape)} {out.dtype}, expected {(M, N)}")
+    lib = _lib.load()
+    check(lib.tnh_gemm_tn(dcode(A.dtype), M, N, K, ptr(A), M, ptr(B), N, ptr(out), N, 1, 0, 0, 0,
+                          int(accumulate), int(tile), stream(A.device)), "gemm_tn")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# two-site operator
+# ---------------------------------------------------------------------------------------------
+
+
+def fused_two_site(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch.Tensor,
+                   dtype: Optional[torch.dtype] = None) -> Tuple[torch.Tensor, torch.Tensor,
+                                                                 torch.Tensor]:
+    """(LT, S, RT) of one two-site problem (tnkit/_dmrg.py:172-180) in the device layouts
+    LT (q, p, wl), S (d1 d2 wr, wl d1 d2), RT (wr, b, b'). `dtype` (float32) builds the operator
+    of the low-precision matvec from float64 inputs."""
+    for t, n in ((L, "L"), (W1, "W1"), (W2, "W2"), (R, "R")):
+        _need(t, n, 3 if n in ("L", "R") else 4)
+    a, wl, a2 = L.shape
+    b, wr, b2 = R.shape
+    wl1, wm, d1, d1b = W1.shape
+    wm2, wr2, d2, d2b = W2.shape
+    if not (a == a2 and b == b2 and wl1 == wl and wm2 == wm and wr2 == wr and d1 == d1b
+            and d2 == d2b):
+        raise ValueError(f"fused_two_site: inconsistent shapes L{tuple(L.shape)} "
+                         f"W1{tuple(W1.shape)} W2{tuple(W2.shape)} R{tuple(R.shape)}")
+    dt = dtype or L.dtype
+    if dt != L.dtype:
+        L, W1, W2, R = (x.to(dt) for x in (L, W1, W2, R))
+    W1, W2 = W1.to(dt).contiguous(), W2.to(dt).contiguous()
+    dev = L.device
+    LT = torch.empty((a, a, wl), dtype=dt, device=dev)
+    S = torch.empty((d1 * d2 * wr, wl * d1 * d2), dtype=dt, device=dev)
+    RT = torch.empty((wr, b, b), dtype=dt, device=dev)
+    check(_lib.load().tnh_fused_two_site(dcode(dt), a, wl, wm, wr, d1, d2, b, ptr(L), ptr(W1),
+                                         ptr(W2), ptr(R), ptr(LT), ptr(S), ptr(RT), stream(dev)),
+          "fused_two_site")
+    return LT, S, RT
+
+
+def apply_two_site_fused(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor,
+                         RT: torch.Tensor, out: Optional[torch.Tensor] = None,
+                         tiles: Tuple[int, int] = (0, 0)) -> torch.Tensor:
+    """H_eff theta (tnkit/_dmrg.py:183-189) for theta (a, d1, d2, b). The operator's dtype sets
+    the arithmetic; theta/out may be float64 around a float32 operator (mixed precision)."""
+    _need(theta, "theta", 4)
+    _need(LT, "LT", 3)
+    _need(S, "S", 2)
+    _need(RT, "RT", 3)
+    q, d1, d2, b = theta.shape
+    a, p, wl = LT.shape
+    wr, bb, bp = RT.shape
+    if not (a == q and p == q and bb == b and bp == b and S.shape == (d1 * d2 * wr, wl * d1 * d2)):
+        raise ValueError(f"apply_two_site_fused: theta{tuple(theta.shape)} LT{tuple(LT.shape)} "
+                         f"S{tuple(S.shape)} RT{tuple(RT.shape)} do not match")
+    if not (LT.dtype == S.dtype == RT.dtype):
+        raise TypeError("apply_two_site_fused: LT, S, RT must share a dtype")
+    if theta.dtype != LT.dtype and not (theta.dtype == torch.float64 and LT.dtype == torch.float32):
+        raise TypeError(f"apply_two_site_fused: theta {theta.dtype} with operator {LT.dtype}")
+    if out is None:
+        out = torch.empty_like(theta)
+    elif out.shape != theta.shape or out.dtype != theta.dtype or not out.is_contiguous():
+        raise ValueError("apply_two_site_fused: out must match theta")
+    lib = _lib.load()
+    dt, io = dcode(LT.dtype), dcode(theta.dtype)
+    nbytes = lib.tnh_two_site_ws_bytes(dt, a, d1, d2, b, wl, wr)
+    ws = _ws(nbytes, theta.device)
+    check(lib.tnh_apply_two_site_fused(dt, io, a, d1, d2, b, wl, wr, ptr(LT), ptr(S), ptr(RT),
+                                       ptr(theta), ptr(out), ptr(ws), ws.numel(), int(tiles[0]),
+                                       int(tiles[1]), stream(theta.device)),
+          "apply_two_site_fused")
+    return out
+
+
+def update_left_env(L: torch.Tensor, A: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """L' (c, wr, c) past a left-canonical A (a, d, c)  (tnkit/_dmrg.py:216-222)."""
+    _need(L, "L", 3)
+    _need(A, "A", 3)
+    _need(W, "W", 4)
+    a, wl, a2 = L.shape
+    a3, d, c = A.shape
+    wl2, wr, d2, d3 = W.shape
+    if not (a == a2 == a3 and wl == wl2 and d == d2 == d3) or not (L.dtype == A.dtype == W.dtype):
+        raise ValueError(f"update_left_env: L{tuple(L.shape)} A{tuple(A.shape)} W{tuple(W.shape)}")
+    lib = _lib.load()
+    dt = dcode(L.dtype)
+    out = torch.empty((c, wr, c), dtype=L.dtype, device=L.device)
+    ws = _ws(lib.tnh_env_ws_bytes(dt, a, d, c, wl, wr), L.device)
+    check(lib.tnh_update_left_env(dt, a, d, c, wl, wr, ptr(L), ptr(A), ptr(W), ptr(out), ptr(ws),
+                                  ws.numel(), stream(L.device)), "update_left_env")
+    return out
+
+
+def update_right_env(R: torch.Tensor, B: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """R' (c, wl, c) past a right-canonical B (c, d, b)  (tnkit/_dmrg.py:225-231)."""
+    _need(R, "R", 3)
+    _need(B, "B", 3)
+    _need(W, "W", 4)
+    b, wr, b2 = R.shape
+    c, d, b3 = B.shape
+    wl, wr2, d2, d3 = W.shape
+    if not (b == b2 == b3 and wr == wr2 and d == d2 == d3) or not (R.dtype == B.dtype == W.dtype):
+        raise ValueError(f"update_right_env: R{tuple(R.shape)} B{tuple(B.shape)} W{tuple(W.shape)}")
+    lib = _lib.load()
+    dt = dcode(R.dtype)
+    out = torch.empty((c, wl, c), dtype=R.dtype, device=R.device)
+    ws = _ws(lib.tnh_env_ws_bytes(dt, b, d, c, wl, wr), R.device)
+    check(lib.tnh_update_right_env(dt, c, d, b, wl, wr, ptr(R), ptr(B), ptr(W), ptr(out), ptr(ws),
+                                   ws.numel(), stream(R.device)), "update_right_env")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# Krylov vectors (float64)
+# ---------------------------------------------------------------------------------------------
+
+
+class KrylovWorkspace:
+    """Device scratch of the Krylov entries: partial sums and the (alpha, beta) pair."""
+
+    def __init__(self, n: int, device):
+        lib = _lib.load()
+        self.ws = _ws(lib.tnh_krylov_ws_bytes(n, 64), device)
+        self.scal = torch.zeros(2, dtype=torch.float64, device=device)
+
+
+def _coef(vals: Sequence[float]):
+    arr = (ctypes.c_double * max(1, len(vals)))(*[float(v) for v in vals])
+    return ctypes.cast(arr, ctypes.POINTER(ctypes.c_double)), arr
+
+
+def krylov_orth(V: torch.Tensor, nlive: int, w: torch.Tensor, j: int, sub_coef: Sequence[float],
+                sub_base: int, kw: KrylovWorkspace) -> None:
+    """In place on w: subtract sum_r sub_coef[r] V[sub_base + r], alpha = <V[j], w>, w -= alpha
+    V[j], CGS2 against V[:nlive], beta = |w|; (alpha, beta) land in kw.scal (device)."""
+    _need(V, "V", 2, (torch.float64,))
+    _need(w, "w", 1, (torch.float64,))
+    n = w.numel()
+    if V.shape[1] != n or nlive > V.shape[0] or sub_base + len(sub_coef) > V.shape[0]:
+        raise ValueError("krylov_orth: basis and vector do not match")
+    pc, keep = _coef(sub_coef)
+    lib = _lib.load()
+    check(lib.tnh_krylov_orth(n, nlive, ptr(V), V.stride(0), ptr(w), j, len(sub_coef), sub_base,
+                              pc, ptr(kw.scal), ptr(kw.ws), kw.ws.numel(), stream(w.device)),
+          "krylov_orth")
+    del keep
+
+
+def lincomb(V: torch.Tensor, coef: Sequence[float], out: torch.Tensor, beta: float = 0.0) -> torch.Tensor:
+    """out = beta * out + sum_r coef[r] V[r] (coefficients from the host)."""
+    _need(V, "V", 2, (torch.float64,))
+    _need(out, "out", 1, (torch.float64,))
+    if len(coef) > V.shape[0] or V.shape[1] != out.numel():
+        raise ValueError("lincomb: sizes do not match")
+    pc, keep = _coef(coef)
+    check(_lib.load().tnh_lincomb(out.numel(), len(coef), ptr(V), V.stride(0), pc, float(beta),
+                                  ptr(out), stream(out.device)), "lincomb")
+    del keep
+    return out
+
+
+def norm(x: torch.Tensor, kw: Optional[KrylovWorkspace] = None) -> torch.Tensor:
+    """|x| as a 0-d float64 device tensor (deterministic two-stage reduction)."""
+    _need(x, "x", 1, (torch.float64,))
+    kw = kw or KrylovWorkspace(x.numel(), x.device)
+    out = torch.empty((), dtype=torch.float64, device=x.device)
+    check(_lib.load().tnh_norm(x.numel(), ptr(x), ptr(out), ptr(kw.ws), kw.ws.numel(),
+                               stream(x.device)), "norm")
+    return out
+
+
+def dot(x: torch.Tensor, y: torch.Tensor, kw: Optional[KrylovWorkspace] = None) -> torch.Tensor:
+    """<x, y> as a 0-d float64 device tensor."""
+    _need(x, "x", 1, (torch.float64,))
+    _need(y, "y", 1, (torch.float64,))
+    if x.numel() != y.numel():
+        raise ValueError("dot: sizes differ")
+    kw = kw or KrylovWorkspace(x.numel(), x.device)
+    out = torch.empty((), dtype=torch.float64, device=x.device)
+    check(_lib.load().tnh_dot(x.numel(), ptr(x), ptr(y), ptr(out), ptr(kw.ws), kw.ws.numel(),
+                              stream(x.device)), "dot")
+    return out
+
+
+def scale_by(x: torch.Tensor, div: Optional[torch.Tensor], mul: float, out: torch.Tensor) -> torch.Tensor:
+    """out = x * mul / max(div, 1e-200), div a 1-element device tensor (or None)."""
+    _need(x, "x", 1, (torch.float64,))
+    _need(out, "out", 1, (torch.float64,))
+    check(_lib.load().tnh_scale_by(x.numel(), ptr(x), ptr(div) if div is not None else None,
+                                   float(mul), ptr(out), stream(x.device)), "scale_by")
+    return out
