@@ -8,7 +8,8 @@
 // MFMA shapes (wave64):
 //   double : v_mfma_f64_16x16x4f64   lane l holds A[l%16][l/16], B[l/16][l%16]
 //   float  : v_mfma_f32_16x16x4f32   same operand layout
-//   result : lane l holds D[4*(l/16)+r][l%16], r = 0..3 (both)
+//   result : float  -- lane l, item r holds D[4*(l/16) + r][l%16]
+//            double -- lane l, item r holds D[4*r + l/16][l%16]   (rows interleaved)
 //
 // Storage types may differ from the compute type (A or B stored in double and rounded to
 // float on the way into LDS, C stored in double from a float accumulator), which is how the
@@ -27,6 +28,8 @@ struct Mfma<double> {
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // row of result item r held by a lane in row-group kq = lane / 16
+  static __device__ __forceinline__ int row(int kq, int r) { return 4 * r + kq; }
 };
 
 template <>
@@ -35,6 +38,7 @@ struct Mfma<float> {
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
 };
 
 struct GemmArgs {
@@ -181,7 +185,7 @@ __global__ void __launch_bounds__(WM* WN * 64)
       if (n >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm * (BM / WM) + i * 16 + kq * 4 + r;
+        const int m = m0 + wm * (BM / WM) + i * 16 + MF::row(kq, r);
         if (m < M) {
           TO* dst = C + (long)m * g.ldc + n;
           if (g.accumulate)
