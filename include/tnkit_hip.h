@@ -65,6 +65,14 @@ int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int
                              void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
                              tnh_stream_t stream);
 
+/* tnh_apply_two_site_fused restricted to some of its kernels, for timing one at a time:
+ * stages bit 0 = GEMM1 (theta x L), bit 1 = the MPO mix, bit 2 = GEMM3 (x R); 7 = all. No
+ * counterpart in tnkit; a profiling hook over the same workspace layout. */
+int tnh_apply_two_site_stages(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl,
+                              int wr, const void* LT, const void* S, const void* RT,
+                              const void* theta, void* out, void* ws, size_t ws_bytes, int tile1,
+                              int tile3, int stages, tnh_stream_t stream);
+
 /* Workspace bytes of the environment updates, for bonds (a, c), physical d, MPO bonds wl, wr. */
 size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr);
 

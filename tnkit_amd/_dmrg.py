@@ -91,6 +91,24 @@ def right_canonicalize(M, be=None):
     return M
 
 
+def right_canonicalize_lq(M, be=None):
+    """Right-canonical form by an LQ factorisation per site (QR of the transpose) instead of an
+    SVD; the same state up to a gauge on each bond, as tnkit's block path does
+    (tnkit/_dmrg.py:295-306). Much cheaper than a device SVD at large bond dimension."""
+    be = be or get_backend("hip-real")
+    N = len(M)
+    for i in range(N - 1, 0, -1):
+        Dl, d, Dr = M[i].shape
+        Q, Rr = torch.linalg.qr(M[i].reshape(Dl, d * Dr).t())      # M^T = Q R  ->  M = R^T Q^T
+        M[i] = Q.t().reshape(-1, d, Dr).contiguous()
+        nxt = torch.matmul(M[i - 1].reshape(-1, Dl), Rr.t()).reshape(M[i - 1].shape[0], d, -1)
+        M[i - 1] = (nxt / torch.linalg.norm(nxt)).contiguous()
+    M[0] = (M[0] / torch.linalg.norm(M[0])).contiguous()
+    if isinstance(M, MPS):
+        M.center = 0
+    return M
+
+
 # --------------------------------------------------------------------------------------------
 # options and the front door
 # --------------------------------------------------------------------------------------------
@@ -183,69 +201,100 @@ def _check(o: DMRGOptions, mpo) -> None:
         raise ValueError("reorth='periodic' with stop='residual' is not supported")
 
 
-def _dmrg(mpo, chi: int, o: DMRGOptions):
-    _check(o, mpo)
-    be = get_backend(o.backend if o.backend != "auto" else "auto", mpo=mpo, init_mps=o.init_mps)
-    if not isinstance(be, HipBackend):
-        raise ValueError(f"backend {o.backend!r} is not a HIP backend")
-    if be.is_complex:
-        raise ValueError("the HIP DMRG sweep serves real Hamiltonians (backend 'hip-real'); "
-                         "this MPO has a significant imaginary part")
-    N = len(mpo)
-    d = int(mpo[0].shape[2])
-    mpo = [be.array(W) for W in mpo]
-    if o.init_mps is not None:
-        M = right_canonicalize([be.array(t) for t in o.init_mps], be)
-    else:
-        M = right_canonicalize(random_mps(N, chi, d, np.random.default_rng(o.seed), be), be)
-    mixed_counts = {"lo": 0, "hi": 0}
-    rng = np.random.default_rng(o.seed + 1)
+class DMRGEngine:
+    """The state of one dense two-site DMRG run on the HIP device: MPS, environments, MPO and
+    counters, with `solve_block` the local step of tnkit/_dmrg.py:677-727 and `sweep` the
+    right-then-left pass of tnkit/_dmrg.py:775-793."""
 
-    L: List[Optional[torch.Tensor]] = [None] * (N + 1)
-    R: List[Optional[torch.Tensor]] = [None] * (N + 1)
-    L[0] = be.ones((1, 1, 1))
-    R[N] = be.ones((1, 1, 1))
-    for i in range(N - 1, 0, -1):
-        R[i] = update_right_env(R[i + 1], M[i], mpo[i])
+    def __init__(self, mpo, chi: int, o: DMRGOptions, canonical: str = "svd"):
+        _check(o, mpo)
+        be = get_backend(o.backend, mpo=mpo, init_mps=o.init_mps)
+        if not isinstance(be, HipBackend):
+            raise ValueError(f"backend {o.backend!r} is not a HIP backend")
+        if be.is_complex:
+            raise ValueError("the HIP DMRG sweep serves real Hamiltonians (backend 'hip-real'); "
+                             "this MPO has a significant imaginary part")
+        self.be, self.o, self.chi = be, o, int(chi)
+        N = len(mpo)
+        self.N = N
+        d = int(mpo[0].shape[2])
+        self.mpo = [be.array(W) for W in mpo]
+        canon = right_canonicalize if canonical == "svd" else right_canonicalize_lq
+        if o.init_mps is not None:
+            self.M = canon([be.array(t) for t in o.init_mps], be)
+        else:
+            self.M = canon(random_mps(N, chi, d, np.random.default_rng(o.seed), be), be)
+        self.mixed_counts = {"lo": 0, "hi": 0}
+        self.rng = np.random.default_rng(o.seed + 1)
+        self.L: List[Optional[torch.Tensor]] = [None] * (N + 1)
+        self.R: List[Optional[torch.Tensor]] = [None] * (N + 1)
+        self.L[0] = be.ones((1, 1, 1))
+        self.R[N] = be.ones((1, 1, 1))
+        for i in range(N - 1, 0, -1):
+            self.R[i] = update_right_env(self.R[i + 1], self.M[i], self.mpo[i])
+        self.total_disc = 0.0
+        self.matvec_calls = 0
+        self.phase = {"eig": 0.0, "trunc": 0.0, "env": 0.0}
+        self.last_eig_info: Dict[str, Any] = {}
 
-    energies: List[float] = []
-    total_disc = 0.0
-    matvec_calls = 0
-    phase = {"eig": 0.0, "trunc": 0.0, "env": 0.0}
+    @classmethod
+    def from_state(cls, mpo, chi: int, o: DMRGOptions, M, Ls: Dict[int, torch.Tensor],
+                   Rs: Sequence[Optional[torch.Tensor]]) -> "DMRGEngine":
+        """An engine over a given mixed-canonical state and the environments it already has
+        (device tensors), for resuming or timing a part of a sweep."""
+        eng = cls.__new__(cls)
+        _check(o, mpo)
+        eng.be = get_backend(o.backend, mpo=mpo)
+        eng.o, eng.chi, eng.N = o, int(chi), len(mpo)
+        eng.mpo = [eng.be.array(W) for W in mpo]
+        eng.M = list(M)
+        eng.L = [None] * (eng.N + 1)
+        for k, v in Ls.items():
+            eng.L[k] = v
+        eng.R = list(Rs)
+        eng.mixed_counts = {"lo": 0, "hi": 0}
+        eng.rng = np.random.default_rng(o.seed + 1)
+        eng.total_disc, eng.matvec_calls = 0.0, 0
+        eng.phase = {"eig": 0.0, "trunc": 0.0, "env": 0.0}
+        eng.last_eig_info = {}
+        return eng
 
-    def eigsolve(theta0, op, op_lo, cur_tol):
-        nonlocal matvec_calls
+    def eigsolve(self, theta0, op, op_lo, cur_tol):
+        o, be = self.o, self.be
         shape = tuple(int(s) for s in theta0.shape)
         n = int(np.prod(shape))
         count = {"n": 0}
+        ws = kernels.two_site_ws(theta0, *op)
 
         def matvec(x):
             count["n"] += 1
-            return kernels.apply_two_site_fused(x.reshape(shape), *op).reshape(-1)
+            return kernels.apply_two_site_fused(x.reshape(shape), *op, ws=ws).reshape(-1)
 
         def matvec_lo(x):
             count["n"] += 1
-            return kernels.apply_two_site_fused(x.reshape(shape), *op_lo).reshape(-1)
+            return kernels.apply_two_site_fused(x.reshape(shape), *op_lo, ws=ws).reshape(-1)
 
-        v0 = theta0.reshape(-1) if o.warm_start else be.randn(rng, n)
-        mi = o.max_iter if o.max_iter is not None else min(n, max(200, 4 * chi))
+        v0 = theta0.reshape(-1) if o.warm_start else be.randn(self.rng, n)
+        mi = o.max_iter if o.max_iter is not None else min(n, max(200, 4 * self.chi))
         be.sync()
         t0 = time.perf_counter()
         if o.eig == "lanczos-mixed":
-            e, x, _ = lanczos_ground_mixed(matvec_lo, matvec, v0, be, tol=cur_tol, max_iter=mi,
-                                           m=o.restart_m, k=o.restart_k, tol_lo=o.mixed_tol_lo,
-                                           counts=mixed_counts)
+            e, x, inf = lanczos_ground_mixed(matvec_lo, matvec, v0, be, tol=cur_tol, max_iter=mi,
+                                             m=o.restart_m, k=o.restart_k, tol_lo=o.mixed_tol_lo,
+                                             counts=self.mixed_counts)
         else:
-            e, x, _ = lanczos_ground_restart(matvec, v0, be, tol=cur_tol, max_iter=mi,
-                                             m=o.restart_m, k=o.restart_k, reorth_alg=o.reorth_alg)
+            e, x, inf = lanczos_ground_restart(matvec, v0, be, tol=cur_tol, max_iter=mi,
+                                               m=o.restart_m, k=o.restart_k,
+                                               reorth_alg=o.reorth_alg)
         be.sync()
-        phase["eig"] += time.perf_counter() - t0
-        matvec_calls += count["n"]
+        self.phase["eig"] += time.perf_counter() - t0
+        self.matvec_calls += count["n"]
+        self.last_eig_info = dict(inf, matvecs=count["n"])
         return e, x.reshape(shape)
 
-    def solve_block(i, direction, cur_tol):
-        nonlocal total_disc
-        W1, W2 = mpo[i], mpo[i + 1]
+    def solve_block(self, i: int, direction: str, cur_tol: float) -> float:
+        o, be, M, L, R, chi = self.o, self.be, self.M, self.L, self.R, self.chi
+        W1, W2 = self.mpo[i], self.mpo[i + 1]
         a, s1 = M[i].shape[0], M[i].shape[1]
         s2, b = M[i + 1].shape[1], M[i + 1].shape[2]
         theta0 = torch.matmul(M[i].reshape(a * s1, -1), M[i + 1].reshape(M[i + 1].shape[0], -1))
@@ -253,7 +302,8 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
         op = fused_two_site(L[i], W1, W2, R[i + 2])
         op_lo = fused_two_site(L[i], W1, W2, R[i + 2], dtype=torch.float32) \
             if o.eig == "lanczos-mixed" else None
-        e, x = eigsolve(theta0, op, op_lo, cur_tol)
+        e, x = self.eigsolve(theta0, op, op_lo, cur_tol)
+        del op, op_lo
         t0 = time.perf_counter()
         mat = x.reshape(a * s1, s2 * b)
         if o.trunc == "svd":
@@ -271,17 +321,30 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
             M[i] = (U * S).reshape(a, s1, -1).contiguous()
             M[i + 1] = Vh.reshape(-1, s2, b).contiguous()
         be.sync()
-        phase["trunc"] += time.perf_counter() - t0
-        total_disc += disc
+        self.phase["trunc"] += time.perf_counter() - t0
+        self.total_disc += disc
         t0 = time.perf_counter()
         if direction == "right":
             L[i + 1] = update_left_env(L[i], M[i], W1)
         else:
             R[i + 1] = update_right_env(R[i + 2], M[i + 1], W2)
         be.sync()
-        phase["env"] += time.perf_counter() - t0
+        self.phase["env"] += time.perf_counter() - t0
         return e
 
+    def sweep(self, cur_tol: float) -> float:
+        E = None
+        for i in range(0, self.N - 1):
+            E = self.solve_block(i, "right", cur_tol)
+        for i in range(self.N - 2, -1, -1):
+            E = self.solve_block(i, "left", cur_tol)
+        return E
+
+
+def _dmrg(mpo, chi: int, o: DMRGOptions):
+    eng = DMRGEngine(mpo, chi, o)
+    be = eng.be
+    energies: List[float] = []
     sweep_s: List[float] = []
     prev_E = None
     E = None
@@ -290,10 +353,7 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
         t_sweep = time.perf_counter()
         cur_tol = (o.eig_tol_warmup if (o.eig_tol_warmup is not None and sweep < o.warmup_sweeps)
                    else o.eig_tol)
-        for i in range(0, N - 1):
-            E = solve_block(i, "right", cur_tol)
-        for i in range(N - 2, -1, -1):
-            E = solve_block(i, "left", cur_tol)
+        E = eng.sweep(cur_tol)
         energies.append(E)
         be.sync()
         sweep_s.append(time.perf_counter() - t_sweep)
@@ -303,7 +363,7 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
             break
         prev_E = E
     info = {"sweeps": sweep + 1, "energies": energies, "backend": be.name, "blocks": False,
-            "eig": o.eig, "discarded_weight": total_disc, "matvec_calls": matvec_calls,
-            "phase_time": phase, "mixed_counts": mixed_counts, "sites": 2, "expand": 0,
+            "eig": o.eig, "discarded_weight": eng.total_disc, "matvec_calls": eng.matvec_calls,
+            "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2, "expand": 0,
             "expand_alphas": [0.0] * (sweep + 1), "sweep_s": sweep_s}
-    return E, MPS(M, center=0), info
+    return E, MPS(eng.M, center=0), info

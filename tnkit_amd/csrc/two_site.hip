@@ -175,10 +175,12 @@ int fused_two_site(int a, int wl, int wm, int wr, int d1, int d2, int b, const v
   return TNH_OK;
 }
 
+// stages: bit 0 GEMM1, bit 1 S-mix, bit 2 GEMM3 (7 = the whole matvec). A partial run reads and
+// writes the workspace as the whole one does; it exists to time one stage at a time.
 template <typename TC, typename TIO>
 int apply_two_site(int a, int d1, int d2, int b, int wl, int wr, const void* LT, const void* S,
                    const void* RT, const void* theta, void* out, void* ws, size_t ws_bytes,
-                   hipStream_t st, int tile1, int tile3) {
+                   hipStream_t st, int tile1, int tile3, int stages = 7) {
   const long P = a, Q = a, B = b, dd = (long)d1 * d2;
   Arena ar(ws, ws_bytes);
   TC* X1 = ar.take<TC>(dd * B * P * wl);
@@ -186,19 +188,19 @@ int apply_two_site(int a, int d1, int d2, int b, int wl, int wr, const void* LT,
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
   // GEMM1: M = d1 d2 b, N = p wl, K = q
-  {
+  if (stages & 1) {
     GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
                dd * B, 0, LT, P * wl, 0, X1, P * wl, 0, 0};
     TNH_TRY((launch_gemm_tn<TC, TIO, TC, TC>(g, 1, st, tile1)));
   }
   // S-mix
-  {
+  if (stages & 2) {
     MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
               P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd};
     TNH_TRY((mix<TC, TC, TC>(m, S, X1, X2, st)));
   }
   // GEMM3: M = p d1 d2, N = b', K = wr b
-  {
+  if (stages & 4) {
     GemmArgs g{static_cast<int>(P * dd), static_cast<int>(B), static_cast<int>(wr * B), X2, P * dd,
                0, RT, B, 0, out, B, 0, 0};
     TNH_TRY((launch_gemm_tn<TC, TC, TC, TIO>(g, 1, st, tile3)));
@@ -294,24 +296,33 @@ size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, in
   return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + 256;
 }
 
+int tnh_apply_two_site_stages(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl,
+                              int wr, const void* LT, const void* S, const void* RT,
+                              const void* theta, void* out, void* ws, size_t ws_bytes, int tile1,
+                              int tile3, int stages, tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_apply_two_site_fused: sizes must be positive");
+  TNH_REQUIRE(stages >= 1 && stages <= 7, TNH_ERR_ARG, "tnh_apply_two_site_stages: stages %d", stages);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == TNH_F64 && io_dtype == TNH_F64)
+    return apply_two_site<double, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                          st, tile1, tile3, stages);
+  if (dtype == TNH_F32 && io_dtype == TNH_F32)
+    return apply_two_site<float, float>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                        st, tile1, tile3, stages);
+  if (dtype == TNH_F32 && io_dtype == TNH_F64)
+    return apply_two_site<float, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
+                                         st, tile1, tile3, stages);
+  set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
 int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl, int wr,
                              const void* LT, const void* S, const void* RT, const void* theta,
                              void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
                              tnh_stream_t stream) {
-  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
-              "tnh_apply_two_site_fused: sizes must be positive");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == TNH_F64 && io_dtype == TNH_F64)
-    return apply_two_site<double, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                          st, tile1, tile3);
-  if (dtype == TNH_F32 && io_dtype == TNH_F32)
-    return apply_two_site<float, float>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                        st, tile1, tile3);
-  if (dtype == TNH_F32 && io_dtype == TNH_F64)
-    return apply_two_site<float, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                         st, tile1, tile3);
-  set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
-  return TNH_ERR_UNSUPPORTED;
+  return tnh_apply_two_site_stages(dtype, io_dtype, a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws,
+                                   ws_bytes, tile1, tile3, 7, stream);
 }
 
 size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr) {

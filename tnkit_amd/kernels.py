@@ -101,7 +101,8 @@ def fused_two_site(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch
 
 def apply_two_site_fused(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor,
                          RT: torch.Tensor, out: Optional[torch.Tensor] = None,
-                         tiles: Tuple[int, int] = (0, 0)) -> torch.Tensor:
+                         tiles: Tuple[int, int] = (0, 0), ws: Optional[torch.Tensor] = None,
+                         stages: int = 7) -> torch.Tensor:
     """H_eff theta (tnkit/_dmrg.py:183-189) for theta (a, d1, d2, b). The operator's dtype sets
     the arithmetic; theta/out may be float64 around a float32 operator (mixed precision)."""
     _need(theta, "theta", 4)
@@ -125,12 +126,22 @@ def apply_two_site_fused(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor,
     lib = _lib.load()
     dt, io = dcode(LT.dtype), dcode(theta.dtype)
     nbytes = lib.tnh_two_site_ws_bytes(dt, a, d1, d2, b, wl, wr)
-    ws = _ws(nbytes, theta.device)
-    check(lib.tnh_apply_two_site_fused(dt, io, a, d1, d2, b, wl, wr, ptr(LT), ptr(S), ptr(RT),
-                                       ptr(theta), ptr(out), ptr(ws), ws.numel(), int(tiles[0]),
-                                       int(tiles[1]), stream(theta.device)),
+    if ws is None or ws.numel() < nbytes:
+        ws = _ws(nbytes, theta.device)
+    check(lib.tnh_apply_two_site_stages(dt, io, a, d1, d2, b, wl, wr, ptr(LT), ptr(S), ptr(RT),
+                                        ptr(theta), ptr(out), ptr(ws), ws.numel(),
+                                        int(tiles[0]), int(tiles[1]), int(stages),
+                                        stream(theta.device)),
           "apply_two_site_fused")
     return out
+
+
+def two_site_ws(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor, RT: torch.Tensor):
+    """A workspace tensor for repeated `apply_two_site_fused` calls on one operator."""
+    q, d1, d2, b = theta.shape
+    nbytes = _lib.load().tnh_two_site_ws_bytes(dcode(LT.dtype), q, d1, d2, b, LT.shape[2],
+                                                RT.shape[0])
+    return _ws(nbytes, theta.device)
 
 
 def update_left_env(L: torch.Tensor, A: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
