@@ -1,42 +1,61 @@
 // Tile selection and launch of the K-major MFMA GEMM, plus the C-ABI entry tnh_gemm_tn.
+#include <cstdint>
+
 #include "gemm_tn.h"
 
 namespace tnh {
 
 namespace {
 
-template <typename TC>
-struct TileBK;
-template <>
-struct TileBK<double> {
-  static constexpr int big = 8;  // keeps the double-buffered 128x128 tiles under 40 KB of LDS
-  static constexpr int small = 16;
-};
-template <>
-struct TileBK<float> {
-  static constexpr int big = 16;
-  static constexpr int small = 16;
-};
-
 inline long ceil_div(long a, long b) { return (a + b - 1) / b; }
 
+template <int V, typename T>
+bool vec_ok(const void* p, long ld, long cols) {
+  return V > 1 && ld % V == 0 && cols % V == 0 && (reinterpret_cast<uintptr_t>(p) % (V * sizeof(T))) == 0;
+}
+
+template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN,
+          int VA, int VB>
+int launch_v(const GemmArgs& g, int batch, hipStream_t stream) {
+  const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
+  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(batch));
+  hipLaunchKernelGGL((gemm_tn_kernel<TC, TA, TB, TO, BM, BN, BK, WM, WN, VA, VB>), grid,
+                     dim3(WM * WN * 64), 0, stream, g);
+  TNH_CHECK_LAUNCH("gemm_tn");
+  return TNH_OK;
+}
+
+// Picks 16-byte vector loads for an operand when its leading dimension, column count, base
+// pointer and batch stride allow them.
 template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
 int launch_one(const GemmArgs& g, int batch, hipStream_t stream) {
   const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
   if (tiles == 0 || batch == 0) return TNH_OK;
   TNH_REQUIRE(tiles < (1L << 31) && batch <= 65535, TNH_ERR_ARG, "gemm_tn: grid too large");
-  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(batch));
-  hipLaunchKernelGGL((gemm_tn_kernel<TC, TA, TB, TO, BM, BN, BK, WM, WN>), grid, dim3(WM * WN * 64),
-                     0, stream, g);
-  TNH_CHECK_LAUNCH("gemm_tn");
-  return TNH_OK;
+  constexpr int VA = 16 / sizeof(TA), VB = 16 / sizeof(TB);
+  constexpr bool can_a = (BK * BM) % (VA * WM * WN * 64) == 0;
+  constexpr bool can_b = (BK * BN) % (VB * WM * WN * 64) == 0;
+  const bool va = can_a && vec_ok<VA, TA>(g.A, g.lda, g.M) && (batch == 1 || g.strideA % VA == 0);
+  const bool vb = can_b && vec_ok<VB, TB>(g.B, g.ldb, g.N) && (batch == 1 || g.strideB % VB == 0);
+  if constexpr (can_a && can_b) {
+    if (va && vb) return launch_v<TC, TA, TB, TO, BM, BN, BK, WM, WN, VA, VB>(g, batch, stream);
+  }
+  if constexpr (can_a) {
+    if (va) return launch_v<TC, TA, TB, TO, BM, BN, BK, WM, WN, VA, 1>(g, batch, stream);
+  }
+  if constexpr (can_b) {
+    if (vb) return launch_v<TC, TA, TB, TO, BM, BN, BK, WM, WN, 1, VB>(g, batch, stream);
+  }
+  return launch_v<TC, TA, TB, TO, BM, BN, BK, WM, WN, 1, 1>(g, batch, stream);
 }
 
 }  // namespace
 
-// Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64), 2 = 64x64, 3 = 32x32, 4 = 16x128 (for
-// skinny M such as the thick-restart rotation). 0 picks the largest square tile that still gives
-// every CU two workgroups, and the skinny tile when M <= 16.
+// Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64, K step 16 doubles / 32 floats),
+// 2 = 64x64, 3 = 32x32, 4 = 16x128 (skinny M such as the thick-restart rotation),
+// 5 = 128x128 with the short K step (8 doubles / 16 floats), 6 = 256x128 (8 waves).
+// 0 picks the largest square tile that still gives every CU two workgroups, and the skinny tile
+// when M <= 16.
 template <typename TC, typename TA, typename TB, typename TO>
 int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint) {
   TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn: negative size");
@@ -54,12 +73,16 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
     else
       cfg = 3;
   }
-  constexpr int BKb = TileBK<TC>::big, BKs = TileBK<TC>::small;
+  constexpr bool dbl = sizeof(TC) == 8;
+  constexpr int BKb = dbl ? 16 : 32;
+  constexpr int BKs = dbl ? 8 : 16;
   switch (cfg) {
     case 1: return launch_one<TC, TA, TB, TO, 128, 128, BKb, 2, 2>(g, batch, stream);
-    case 2: return launch_one<TC, TA, TB, TO, 64, 64, BKs, 2, 2>(g, batch, stream);
-    case 3: return launch_one<TC, TA, TB, TO, 32, 32, BKs, 2, 2>(g, batch, stream);
-    case 4: return launch_one<TC, TA, TB, TO, 16, 128, BKs, 1, 4>(g, batch, stream);
+    case 2: return launch_one<TC, TA, TB, TO, 64, 64, 16, 2, 2>(g, batch, stream);
+    case 3: return launch_one<TC, TA, TB, TO, 32, 32, 16, 2, 2>(g, batch, stream);
+    case 4: return launch_one<TC, TA, TB, TO, 16, 128, 16, 1, 4>(g, batch, stream);
+    case 5: return launch_one<TC, TA, TB, TO, 128, 128, BKs, 2, 2>(g, batch, stream);
+    case 6: return launch_one<TC, TA, TB, TO, 256, 128, BKs, 4, 2>(g, batch, stream);
     default: set_error("gemm_tn: unknown tile config %d", cfg); return TNH_ERR_ARG;
   }
 }

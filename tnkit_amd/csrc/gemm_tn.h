@@ -73,7 +73,48 @@ __device__ __forceinline__ void tile_of(int bid, int nbm, int nbn, int& tm, int&
   tn = in / gsize;
 }
 
-template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
+// One thread's share of a K-major tile: BK rows of BMN contiguous entries, moved as vectors of V
+// elements (V = 1 or 16 bytes' worth when the leading dimension and base allow it).
+template <typename TC, typename TS, int BK, int BMN, int NT, int V>
+struct TileLoader {
+  static constexpr int NV = BK * BMN / V / NT;  // vectors per thread
+  static_assert((BK * BMN) % (V * NT) == 0, "tile not divisible by threads");
+  TC r[NV * V];
+  __device__ __forceinline__ void load(const TS* __restrict__ src, long ld, int k0, int K, int c0,
+                                       int C, int t) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + i * NT;
+      const int kr = e / (BMN / V), cc = (e % (BMN / V)) * V;
+      const int k = k0 + kr, c = c0 + cc;
+      if constexpr (V == 1) {
+        r[i] = (k < K && c < C) ? static_cast<TC>(src[(long)k * ld + c]) : TC(0);
+      } else {
+        typedef TS vec_t __attribute__((ext_vector_type(V)));
+        if (k < K && c < C) {  // C % V == 0 and c % V == 0, so the whole vector is in bounds
+          const vec_t x = *reinterpret_cast<const vec_t*>(src + (long)k * ld + c);
+#pragma unroll
+          for (int v = 0; v < V; ++v) r[i * V + v] = static_cast<TC>(x[v]);
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) r[i * V + v] = TC(0);
+        }
+      }
+    }
+  }
+  __device__ __forceinline__ void store(TC* __restrict__ dst, int lds, int t) const {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int e = t + i * NT;
+      const int kr = e / (BMN / V), cc = (e % (BMN / V)) * V;
+#pragma unroll
+      for (int v = 0; v < V; ++v) dst[kr * lds + cc + v] = r[i * V + v];
+    }
+  }
+};
+
+template <typename TC, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN,
+          int VA, int VB>
 __global__ void __launch_bounds__(WM* WN * 64)
     gemm_tn_kernel(GemmArgs g) {
   using MF = Mfma<TC>;
@@ -84,10 +125,7 @@ __global__ void __launch_bounds__(WM* WN * 64)
   constexpr int LDB_S = BN + PAD;
   constexpr int FM = BM / WM / 16;
   constexpr int FN = BN / WN / 16;
-  constexpr int EA = BK * BM / NT;  // elements per thread per A tile
-  constexpr int EB = BK * BN / NT;
   static_assert(BK % 4 == 0, "BK must be a multiple of the MFMA k=4");
-  static_assert((BK * BM) % NT == 0 && (BK * BN) % NT == 0, "tile not divisible by threads");
   static_assert(FM >= 1 && FN >= 1, "wave tile too small");
 
   __shared__ TC As[2][BK * LDA_S];
@@ -119,46 +157,23 @@ __global__ void __launch_bounds__(WM* WN * 64)
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
 
-  TC ra[EA];
-  TC rb[EB];
-  auto load = [&](int k0) {
-#pragma unroll
-    for (int i = 0; i < EA; ++i) {
-      const int e = t + i * NT;
-      const int kr = e / BM, mc = e % BM;
-      const int k = k0 + kr, m = m0 + mc;
-      ra[i] = (k < K && m < M) ? static_cast<TC>(A[(long)k * g.lda + m]) : TC(0);
-    }
-#pragma unroll
-    for (int i = 0; i < EB; ++i) {
-      const int e = t + i * NT;
-      const int kr = e / BN, nc = e % BN;
-      const int k = k0 + kr, n = n0 + nc;
-      rb[i] = (k < K && n < N) ? static_cast<TC>(B[(long)k * g.ldb + n]) : TC(0);
-    }
-  };
-  auto store = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < EA; ++i) {
-      const int e = t + i * NT;
-      As[buf][(e / BM) * LDA_S + e % BM] = ra[i];
-    }
-#pragma unroll
-    for (int i = 0; i < EB; ++i) {
-      const int e = t + i * NT;
-      Bs[buf][(e / BN) * LDB_S + e % BN] = rb[i];
-    }
-  };
+  TileLoader<TC, TA, BK, BM, NT, VA> la;
+  TileLoader<TC, TB, BK, BN, NT, VB> lb;
 
   const int nk = (K + BK - 1) / BK;
   if (nk > 0) {
-    load(0);
-    store(0);
+    la.load(A, g.lda, 0, K, m0, M, t);
+    lb.load(B, g.ldb, 0, K, n0, N, t);
+    la.store(As[0], LDA_S, t);
+    lb.store(Bs[0], LDB_S, t);
   }
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) load((kt + 1) * BK);
+    if (kt + 1 < nk) {
+      la.load(A, g.lda, (kt + 1) * BK, K, m0, M, t);
+      lb.load(B, g.ldb, (kt + 1) * BK, K, n0, N, t);
+    }
     const TC* as = As[cur];
     const TC* bs = Bs[cur];
 #pragma unroll
@@ -173,7 +188,10 @@ __global__ void __launch_bounds__(WM* WN * 64)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+    if (kt + 1 < nk) {
+      la.store(As[cur ^ 1], LDA_S, t);
+      lb.store(Bs[cur ^ 1], LDB_S, t);
+    }
     __syncthreads();
   }
 

@@ -79,8 +79,10 @@ __global__ void build_wmix_kernel(int which, int wl, int wr, int d, const T* __r
 
 // Generic small-matrix mix over a 2-level group index (g1, g2):
 //   out[g1*os1 + g2*os2 + jo1*ojs1 + jo2*ojs2] = sum_ji Mx[jo][ji] * in[g1*is1 + g2*is2 + ji1*ijs1 + ji2*ijs2]
-// with jo = jo1*njo2 + jo2, ji = ji1*nji2 + ji2. One thread per (jo, g1, g2), g2 fastest so a wave
-// shares jo and the branch on a zero MPO entry is uniform.
+// with jo = jo1*njo2 + jo2, ji = ji1*nji2 + ji2. One thread per group and chunk of CO outputs: the
+// thread loads its group's inputs once (CI at a time) into registers and writes its outputs;
+// g2 runs fastest, so a wave reads and writes contiguous runs. Mx sits in LDS, and its zero
+// entries (most of an MPO's) are skipped by a branch that is uniform across the workgroup.
 struct MixArgs {
   long G1, G2;
   int njo, njo2, nji, nji2;
@@ -88,28 +90,78 @@ struct MixArgs {
   long os1, os2, ojs1, ojs2;
 };
 
+// outputs / inputs per register chunk: 32 floats or 16 doubles stay in registers without spills
+template <typename T>
+struct MixChunk {
+  static constexpr int value = 32;
+};
+template <>
+struct MixChunk<double> {
+  static constexpr int value = 16;
+};
+constexpr int kMixLds = 4096;  // Mx entries held in LDS (larger operators read global memory)
+
 template <typename TC, typename TI, typename TO>
 __global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restrict__ Mx,
                                                   const TI* __restrict__ in, TO* __restrict__ out) {
-  const long total = a.G1 * a.G2 * a.njo;
+  constexpr int kMixChunk = MixChunk<TC>::value;
+  __shared__ TC ms[kMixLds];
+  const int nm = a.njo * a.nji;
+  const bool in_lds = nm <= kMixLds;
+  if (in_lds)
+    for (int k = threadIdx.x; k < nm; k += blockDim.x) ms[k] = Mx[k];
+  __syncthreads();
+  const TC* M = in_lds ? ms : Mx;
+  const long G = a.G1 * a.G2;
+  const int nco = (a.njo + kMixChunk - 1) / kMixChunk;
+  const long total = G * nco;
   for (long tid = blockIdx.x * (long)blockDim.x + threadIdx.x; tid < total;
        tid += (long)gridDim.x * blockDim.x) {
-    const long g2 = tid % a.G2;
-    const long rr = tid / a.G2;
-    const long g1 = rr % a.G1;
-    const int jo = static_cast<int>(rr / a.G1);
-    const long base = g1 * a.is1 + g2 * a.is2;
-    const TC* mrow = Mx + (long)jo * a.nji;
-    TC acc = 0;
-    for (int ji = 0; ji < a.nji; ++ji) {
-      const TC m = mrow[ji];
-      if (m != TC(0)) {
-        const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
-        acc += m * static_cast<TC>(in[base + ji1 * a.ijs1 + ji2 * a.ijs2]);
+    const long g = tid % G;
+    const int co = static_cast<int>(tid / G);
+    const long g2 = g % a.G2;
+    const long g1 = g / a.G2;
+    const long ibase = g1 * a.is1 + g2 * a.is2;
+    const int jo0 = co * kMixChunk;
+    TC acc[kMixChunk];
+#pragma unroll
+    for (int o = 0; o < kMixChunk; ++o) acc[o] = TC(0);
+    for (int ji0 = 0; ji0 < a.nji; ji0 += kMixChunk) {
+      TC x[kMixChunk];
+#pragma unroll
+      for (int i = 0; i < kMixChunk; ++i) {
+        const int ji = ji0 + i;
+        if (ji < a.nji) {
+          const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
+          x[i] = static_cast<TC>(in[ibase + ji1 * a.ijs1 + ji2 * a.ijs2]);
+        } else {
+          x[i] = TC(0);
+        }
+      }
+#pragma unroll
+      for (int o = 0; o < kMixChunk; ++o) {
+        const int jo = jo0 + o;
+        if (jo < a.njo) {
+          const TC* mrow = M + (long)jo * a.nji + ji0;
+#pragma unroll
+          for (int i = 0; i < kMixChunk; ++i) {
+            if (ji0 + i < a.nji) {
+              const TC m = mrow[i];
+              if (m != TC(0)) acc[o] += m * x[i];
+            }
+          }
+        }
       }
     }
-    const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
-    out[g1 * a.os1 + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(acc);
+    const long obase = g1 * a.os1 + g2 * a.os2;
+#pragma unroll
+    for (int o = 0; o < kMixChunk; ++o) {
+      const int jo = jo0 + o;
+      if (jo < a.njo) {
+        const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
+        out[obase + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(acc[o]);
+      }
+    }
   }
 }
 
@@ -135,7 +187,8 @@ int permute3(long n0, long n1, long n2, long s0, long s1, long s2, const void* i
 
 template <typename TC, typename TI, typename TO>
 int mix(const MixArgs& a, const void* Mx, const void* in, void* out, hipStream_t st) {
-  const long total = a.G1 * a.G2 * a.njo;
+  constexpr int kMixChunk = MixChunk<TC>::value;
+  const long total = a.G1 * a.G2 * ((a.njo + kMixChunk - 1) / kMixChunk);
   if (total == 0) return TNH_OK;
   hipLaunchKernelGGL((mix_kernel<TC, TI, TO>), dim3(grid_for(total, 256, 1L << 20)), dim3(256), 0,
                      st, a, static_cast<const TC*>(Mx), static_cast<const TI*>(in),
