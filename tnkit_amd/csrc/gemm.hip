@@ -54,8 +54,7 @@ int launch_one(const GemmArgs& g, int batch, hipStream_t stream) {
 // Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64, K step 16 doubles / 32 floats),
 // 2 = 64x64, 3 = 32x32, 4 = 16x128 (skinny M such as the thick-restart rotation),
 // 5 = 128x128 with the short K step (8 doubles / 16 floats), 6 = 256x128 (8 waves).
-// 0 picks the largest square tile that still gives every CU two workgroups, and the skinny tile
-// when M <= 16.
+// 0 picks by dtype and tile count (below), and the skinny tile when M <= 16.
 template <typename TC, typename TA, typename TB, typename TO>
 int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint) {
   TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn: negative size");
@@ -64,10 +63,13 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
     const long b = batch > 0 ? batch : 1;
     const long t128 = ceil_div(g.M, 128) * ceil_div(g.N, 128) * b;
     const long t64 = ceil_div(g.M, 64) * ceil_div(g.N, 64) * b;
+    // measured on MI355X at the chi = 512 / 1024 two-site shapes: 64x64 tiles lead for float64
+    // (two waves per SIMD hide the load latency better than one 128x128 wave), and 128x128 with
+    // the short K step leads for float32 once there are >= 4 tiles per CU
     if (g.M <= 16)
       cfg = 4;
-    else if (t128 >= 512)
-      cfg = 1;
+    else if (sizeof(TC) == 4 && t128 >= 1024)
+      cfg = 5;
     else if (t64 >= 256)
       cfg = 2;
     else
