@@ -137,6 +137,29 @@ def cpu_baseline(prob, budget_s=20.0):
                       f"({sec * 1e3:.0f} ms each; BLAS: {', '.join(blas) or 'numpy default'})"}
 
 
+def local_solves(th, op64, op32, ws, chi):
+    """The local ground state by float64 thick-restart Lanczos and by the mixed two-stage form."""
+    import torch
+    from tnkit_amd import kernels
+    from tnkit_amd.eigensolver import lanczos_ground_mixed, lanczos_ground_restart
+    shape = tuple(th.shape)
+    cnt = {"lo": 0, "hi": 0}
+    mvh = lambda x: kernels.apply_two_site_fused(x.reshape(shape), *op64, ws=ws).reshape(-1)  # noqa
+    mvl = lambda x: kernels.apply_two_site_fused(x.reshape(shape), *op32, ws=ws).reshape(-1)  # noqa
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    e64, _, i64 = lanczos_ground_restart(mvh, th.reshape(-1).clone(), tol=1e-9, max_iter=4 * chi)
+    torch.cuda.synchronize()
+    t_e64 = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    emx, _, imx = lanczos_ground_mixed(mvl, mvh, th.reshape(-1).clone(), tol=1e-9,
+                                       max_iter=4 * chi, counts=cnt)
+    torch.cuda.synchronize()
+    t_emx = time.perf_counter() - t1
+    return {"fp64_E": e64, "mixed_E": emx, "abs": abs(emx - e64), "fp64_matvecs": i64["iters"],
+            "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"], "fp64_s": t_e64, "mixed_s": t_emx}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,7 +169,20 @@ def main():
     ap.add_argument("--N", type=int, default=100)
     ap.add_argument("--sweep-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-local", action="store_true",
+                    help="skip the local ground-state solves (short profiled runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(args.copy_probe * 2 ** 20 // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": src.numel() * 8, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: the two-site path is measured on "
                                    "one MI355X and multi-GPU sharding is design-only",
@@ -160,7 +196,6 @@ def main():
     import torch
     from tnkit_amd import kernels
     from tnkit_amd import _dmrg as H
-    from tnkit_amd.eigensolver import lanczos_ground_mixed, lanczos_ground_restart
 
     t_setup = time.perf_counter()
     prob = build_local_problem(args.N, args.chi)
@@ -223,21 +258,9 @@ def main():
     roof["per_stage_ms"] = {names[k]: round(v, 4) for k, v in stage_ms.items()}
 
     # ---- local ground state: mixed vs float64 -------------------------------------------
-    shape = tuple(th.shape)
-    n = th.numel()
-    cnt = {"lo": 0, "hi": 0}
-    mvh = lambda x: kernels.apply_two_site_fused(x.reshape(shape), *op64, ws=ws).reshape(-1)  # noqa
-    mvl = lambda x: kernels.apply_two_site_fused(x.reshape(shape), *op32, ws=ws).reshape(-1)  # noqa
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    e64, x64, i64 = lanczos_ground_restart(mvh, th.reshape(-1).clone(), tol=1e-9, max_iter=4 * args.chi)
-    torch.cuda.synchronize()
-    t_e64 = time.perf_counter() - t1
-    t1 = time.perf_counter()
-    emx, xmx, imx = lanczos_ground_mixed(mvl, mvh, th.reshape(-1).clone(), tol=1e-9,
-                                         max_iter=4 * args.chi, counts=cnt)
-    torch.cuda.synchronize()
-    t_emx = time.perf_counter() - t1
+    local = {}
+    if not args.no_local:
+        local = local_solves(th, op64, op32, ws, args.chi)
     del op64, op32, ws, out, y32, y64
 
     # ---- bulk DMRG steps (eigensolve + Gram truncation + environment) ---------------------
@@ -282,9 +305,7 @@ def main():
         "fp64": {"ms": ms64, "tflops": fl / (ms64 * 1e-3) / 1e12},
         "speedup_mixed_vs_fp64": ms64 / ms32,
         "matvec_rel_err_f32": mv_rel,
-        "local_dE": {"fp64_E": e64, "mixed_E": emx, "abs": abs(emx - e64),
-                     "fp64_matvecs": i64["iters"], "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"],
-                     "fp64_s": t_e64, "mixed_s": t_emx},
+        "local_dE": local,
         "bulk_steps": steps,
         "reference_h100_mig_fp64_tflops": 5.67,
         "roofline": roof,

@@ -53,8 +53,16 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
+    # the C-ABI driver (host program linked against the library by rpath)
+    exe = os.path.join(HERE, "cabi_bench")
+    r = subprocess.run([hipcc(), "-O2", "-std=c++17", os.path.join(CSRC, "cabi_bench.cpp"),
+                        "-o", exe + ".tmp", "-L", HERE, "-ltnkit_hip", "-Wl,-rpath,$ORIGIN"],
+                       capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cabi_bench build failed:\n" + r.stdout + r.stderr)
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print(f"built {OUT} from {len(srcs)} sources")
+        print(f"built {OUT} from {len(srcs)} sources, and {exe}")
     return OUT
 
 

@@ -136,10 +136,20 @@ __global__ void convert_kernel(long n, const TI* __restrict__ x, TO* __restrict_
 
 namespace {
 
+// workgroups of the reducing kernels: each writes one partial per coefficient, so their number
+// is capped at the partial buffer's 1024 rows (four per CU)
 inline int nblocks(long n) {
   long b = (n + 255) / 256;
   if (b < 1) b = 1;
-  if (b > 1024) b = 1024;  // four workgroups per CU saturate HBM; more only grows the partials
+  if (b > 1024) b = 1024;
+  return static_cast<int>(b);
+}
+
+// workgroups of the elementwise kernels: enough waves in flight to cover HBM latency
+inline int nblocks_stream(long n) {
+  long b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 16384) b = 16384;
   return static_cast<int>(b);
 }
 
@@ -237,7 +247,7 @@ int tnh_lincomb(long n, int nrows, const double* V, long ldv, const double* coef
   memset(&hc, 0, sizeof(hc));
   for (int r = 0; r < nrows; ++r) hc.c[r] = coef[r];
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks(n)), dim3(256), 0, st, n, nrows, V, ldv, hc, beta,
+  hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks_stream(n)), dim3(256), 0, st, n, nrows, V, ldv, hc, beta,
                      out);
   TNH_CHECK_LAUNCH("lincomb");
   return TNH_OK;
@@ -275,7 +285,7 @@ int tnh_scale_by(long n, const double* x, const double* div, double mul, double*
                  tnh_stream_t stream) {
   if (n <= 0) return TNH_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(scale_by_kernel, dim3(nblocks(n)), dim3(256), 0, st, n, x, div, mul, y);
+  hipLaunchKernelGGL(scale_by_kernel, dim3(nblocks_stream(n)), dim3(256), 0, st, n, x, div, mul, y);
   TNH_CHECK_LAUNCH("scale_by");
   return TNH_OK;
 }
@@ -285,10 +295,10 @@ int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst
   if (n <= 0) return TNH_OK;
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dst_dtype == TNH_F32 && src_dtype == TNH_F64) {
-    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nblocks(n)), dim3(256), 0, st, n,
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nblocks_stream(n)), dim3(256), 0, st, n,
                        static_cast<const double*>(src), static_cast<float*>(dst));
   } else if (dst_dtype == TNH_F64 && src_dtype == TNH_F32) {
-    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblocks(n)), dim3(256), 0, st, n,
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblocks_stream(n)), dim3(256), 0, st, n,
                        static_cast<const float*>(src), static_cast<double*>(dst));
   } else {
     set_error("tnh_convert: unsupported pair (%d <- %d)", dst_dtype, src_dtype);

@@ -165,6 +165,91 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restric
   }
 }
 
+// LDS-staged form of mix_kernel for operators with nji, njo <= kStageMax. A workgroup takes kStageG
+// consecutive g2 at one g1: it copies the inputs of those groups into LDS in the order in which
+// they lie in memory (mode 0: g2 fastest; 1: ji1 fastest, then g2; 2: ji2 fastest, then g2), each
+// thread computes the outputs of one group from LDS, and the outputs leave through LDS in memory
+// order the same way. Every global access of a wave is then a contiguous run.
+constexpr int kStageG = 256;
+constexpr int kStageMax = 32;
+
+__device__ __forceinline__ void stage_decode(int mode, int L, int n, int n2, int& j, int& t) {
+  const int n1 = n / n2;
+  if (mode == 1) {  // L = (j2 * G + t) * n1 + j1
+    const int j1 = L % n1;
+    const int r = L / n1;
+    t = r % kStageG;
+    j = j1 * n2 + r / kStageG;
+  } else if (mode == 2) {  // L = (j1 * G + t) * n2 + j2
+    const int j2 = L % n2;
+    const int r = L / n2;
+    t = r % kStageG;
+    j = (r / kStageG) * n2 + j2;
+  } else {  // L = j * G + t
+    j = L / kStageG;
+    t = L % kStageG;
+  }
+}
+
+template <typename TC, typename TI, typename TO>
+__global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode, int out_mode,
+                                                         const TC* __restrict__ Mx,
+                                                         const TI* __restrict__ in,
+                                                         TO* __restrict__ out) {
+  // dynamic LDS: the operator (njo x nji), then max(nji, njo) rows of kStageG values
+  extern __shared__ __align__(16) unsigned char stage_lds[];
+  TC* ms = reinterpret_cast<TC*>(stage_lds);
+  const int nm = a.njo * a.nji;
+  TC* buf = ms + ((nm + 1) / 2) * 2;
+  for (int k = threadIdx.x; k < nm; k += blockDim.x) ms[k] = Mx[k];
+  const long g1 = blockIdx.y;
+  const long g20 = (long)blockIdx.x * kStageG;
+  const long ib = g1 * a.is1;
+  const int nin = a.nji * kStageG;
+  for (int L = threadIdx.x; L < nin; L += blockDim.x) {
+    int ji, t;
+    stage_decode(in_mode, L, a.nji, a.nji2, ji, t);
+    const long g2 = g20 + t;
+    TC v = TC(0);
+    if (g2 < a.G2) {
+      const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
+      v = static_cast<TC>(in[ib + g2 * a.is2 + ji1 * a.ijs1 + ji2 * a.ijs2]);
+    }
+    buf[ji * kStageG + t] = v;
+  }
+  __syncthreads();
+  TC acc[kStageMax];
+#pragma unroll
+  for (int o = 0; o < kStageMax; ++o) acc[o] = TC(0);
+  const int t = threadIdx.x;
+  for (int ji = 0; ji < a.nji; ++ji) {
+    const TC x = buf[ji * kStageG + t];
+#pragma unroll
+    for (int o = 0; o < kStageMax; ++o) {
+      if (o < a.njo) {
+        const TC m = ms[o * a.nji + ji];
+        if (m != TC(0)) acc[o] += m * x;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int o = 0; o < kStageMax; ++o)
+    if (o < a.njo) buf[o * kStageG + t] = acc[o];
+  __syncthreads();
+  const long ob = g1 * a.os1;
+  const int nout = a.njo * kStageG;
+  for (int L = threadIdx.x; L < nout; L += blockDim.x) {
+    int jo, tt;
+    stage_decode(out_mode, L, a.njo, a.njo2, jo, tt);
+    const long g2 = g20 + tt;
+    if (g2 < a.G2) {
+      const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
+      out[ob + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(buf[jo * kStageG + tt]);
+    }
+  }
+}
+
 namespace {
 
 inline unsigned grid_for(long total, int threads = 256, long cap = 1L << 16) {
@@ -185,8 +270,32 @@ int permute3(long n0, long n1, long n2, long s0, long s1, long s2, const void* i
   return TNH_OK;
 }
 
+// memory order of a 2-level index block: 0 = the group index g2 is the unit stride, 1 = the
+// first component is, with g2 stepping over all of it, 2 = the same for the second component;
+// -1 = none of these
+inline int stage_mode(long s_g2, long s1, long s2, int n, int n2) {
+  const int n1 = n / n2;
+  if (s_g2 == 1) return 0;
+  if (s1 == 1 && s_g2 == n1) return 1;
+  if (s2 == 1 && s_g2 == n2) return 2;
+  return -1;
+}
+
 template <typename TC, typename TI, typename TO>
 int mix(const MixArgs& a, const void* Mx, const void* in, void* out, hipStream_t st) {
+  const int im = stage_mode(a.is2, a.ijs1, a.ijs2, a.nji, a.nji2);
+  const int om = stage_mode(a.os2, a.ojs1, a.ojs2, a.njo, a.njo2);
+  const int rows = a.nji > a.njo ? a.nji : a.njo;
+  const size_t lds = (((size_t)a.njo * a.nji + 1) / 2 * 2 + (size_t)rows * kStageG) * sizeof(TC);
+  if (a.nji <= kStageMax && a.njo <= kStageMax && im >= 0 && om >= 0 && a.G1 <= 65535 &&
+      a.G1 * a.G2 > 0 && lds <= 65536) {
+    dim3 grid(static_cast<unsigned>((a.G2 + kStageG - 1) / kStageG), static_cast<unsigned>(a.G1));
+    hipLaunchKernelGGL((mix_staged_kernel<TC, TI, TO>), grid, dim3(kStageG), lds, st, a, im, om,
+                       static_cast<const TC*>(Mx), static_cast<const TI*>(in),
+                       static_cast<TO*>(out));
+    TNH_CHECK_LAUNCH("mix_staged");
+    return TNH_OK;
+  }
   constexpr int kMixChunk = MixChunk<TC>::value;
   const long total = a.G1 * a.G2 * ((a.njo + kMixChunk - 1) / kMixChunk);
   if (total == 0) return TNH_OK;
