@@ -160,6 +160,37 @@ def local_solves(th, op64, op32, ws, chi):
             "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"], "fp64_s": t_e64, "mixed_s": t_emx}
 
 
+def config2(N, stages, eig, trunc, seed=0, log=None):
+    """The precision study: XXZ chain N sites, dense two-site DMRG with the bond ramped through
+    `stages` = [(chi, sweeps), ...], each stage started from the previous state (init_mps), as
+    tnkit's benchmark runner chains its stages (bench/suite/runners/tnkit_runner.py:143-152).
+    Returns per-stage energies, sweep times, matvec counts and phase times."""
+    import torch
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd.models import xxz_mpo
+    mpo = xxz_mpo(N)
+    out, mps = [], None
+    for chi, n in stages:
+        kw = dict(n_sweeps=n, tol=0.0, rtol=0.0, seed=seed, backend="hip-real", eig=eig,
+                  trunc=trunc)
+        if mps is not None:
+            kw["init_mps"] = list(mps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        E, mps, info = H.dmrg(mpo, chi, **kw)
+        torch.cuda.synchronize()
+        rec = {"chi": chi, "sweeps": n, "E": E, "energies": info["energies"],
+               "sweep_s": info["sweep_s"], "wall_s": time.perf_counter() - t0,
+               "matvec_calls": info["matvec_calls"], "mixed_counts": info["mixed_counts"],
+               "phase_s": {k: round(v, 3) for k, v in info["phase_time"].items()},
+               "discarded_weight": info["discarded_weight"],
+               "max_bond": max(int(t.shape[2]) for t in mps)}
+        out.append(rec)
+        if log:
+            print(json.dumps({"eig": eig, **rec}), file=log, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,9 +202,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-local", action="store_true",
                     help="skip the local ground-state solves (short profiled runs)")
+    ap.add_argument("--config2", type=str, default="",
+                    help="run the chi-ramp precision study instead, e.g. '64:4,512:4,1024:4'")
+    ap.add_argument("--trunc", type=str, default="gram")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
+    if args.config2:
+        stages = [tuple(int(v) for v in st.split(":")) for st in args.config2.split(",")]
+        res = {"workload": f"XXZ N={args.N} two-site DMRG, stages {stages}, trunc={args.trunc}"}
+        for eig in ("lanczos-restart", "lanczos-mixed"):
+            res[eig] = config2(args.N, stages, eig, args.trunc, log=sys.stderr)
+        res["dE_per_stage"] = [abs(a["E"] - b["E"]) for a, b in
+                               zip(res["lanczos-restart"], res["lanczos-mixed"])]
+        print(json.dumps(res))
+        return
     if args.copy_probe:
         import torch
         src = torch.ones(args.copy_probe * 2 ** 20 // 8, dtype=torch.float64, device="cuda")

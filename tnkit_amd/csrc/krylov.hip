@@ -120,18 +120,35 @@ __global__ void __launch_bounds__(256)
 
 // y[i] = x[i] * (mul / div[0])  -- division by a device scalar (beta), with tnkit's
 // guard_denom clamp: a divisor below 1e-200 is replaced by 1e-200 (tnkit/backend.py:699-708)
+// Elementwise y[i] = f(x[i]): each thread moves 4 consecutive elements per step (two 16-byte
+// loads in flight for doubles) so that enough bytes are in flight to cover HBM latency.
+template <typename TO, typename TI, typename F>
+__device__ __forceinline__ void map4(long n, const TI* __restrict__ x, TO* __restrict__ y, F f) {
+  const long stride = (long)gridDim.x * 256 * 4;
+  for (long i0 = (blockIdx.x * 256L + threadIdx.x) * 4; i0 < n; i0 += stride) {
+    if (i0 + 3 < n) {
+      TI a0 = x[i0], a1 = x[i0 + 1], a2 = x[i0 + 2], a3 = x[i0 + 3];
+      y[i0] = f(a0);
+      y[i0 + 1] = f(a1);
+      y[i0 + 2] = f(a2);
+      y[i0 + 3] = f(a3);
+    } else {
+      for (long i = i0; i < n; ++i) y[i] = f(x[i]);
+    }
+  }
+}
+
 __global__ void scale_by_kernel(long n, const double* __restrict__ x, const double* __restrict__ div,
                                 double mul, double* __restrict__ y) {
   double d = div ? div[0] : 1.0;
   if (div && d < 1e-200) d = 1e-200;
   const double s = mul / d;
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) y[i] = x[i] * s;
+  map4(n, x, y, [s](double v) { return v * s; });
 }
 
 template <typename TO, typename TI>
 __global__ void convert_kernel(long n, const TI* __restrict__ x, TO* __restrict__ y) {
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256)
-    y[i] = static_cast<TO>(x[i]);
+  map4(n, x, y, [](TI v) { return static_cast<TO>(v); });
 }
 
 namespace {
@@ -145,9 +162,16 @@ inline int nblocks(long n) {
   return static_cast<int>(b);
 }
 
-// workgroups of the elementwise kernels: enough waves in flight to cover HBM latency
-inline int nblocks_stream(long n) {
+inline int nblocks_lin(long n) {
   long b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  if (b > 16384) b = 16384;
+  return static_cast<int>(b);
+}
+
+// workgroups of the elementwise kernels (4 elements per thread per step)
+inline int nblocks_stream(long n) {
+  long b = (n + 1023) / 1024;
   if (b < 1) b = 1;
   if (b > 16384) b = 16384;
   return static_cast<int>(b);
@@ -247,7 +271,7 @@ int tnh_lincomb(long n, int nrows, const double* V, long ldv, const double* coef
   memset(&hc, 0, sizeof(hc));
   for (int r = 0; r < nrows; ++r) hc.c[r] = coef[r];
   hipStream_t st = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks_stream(n)), dim3(256), 0, st, n, nrows, V, ldv, hc, beta,
+  hipLaunchKernelGGL(lincomb_kernel, dim3(nblocks_lin(n)), dim3(256), 0, st, n, nrows, V, ldv, hc, beta,
                      out);
   TNH_CHECK_LAUNCH("lincomb");
   return TNH_OK;

@@ -205,17 +205,29 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   const long g1 = blockIdx.y;
   const long g20 = (long)blockIdx.x * kStageG;
   const long ib = g1 * a.is1;
-  const int nin = a.nji * kStageG;
-  for (int L = threadIdx.x; L < nin; L += blockDim.x) {
-    int ji, t;
-    stage_decode(in_mode, L, a.nji, a.nji2, ji, t);
-    const long g2 = g20 + t;
-    TC v = TC(0);
-    if (g2 < a.G2) {
-      const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
-      v = static_cast<TC>(in[ib + g2 * a.is2 + ji1 * a.ijs1 + ji2 * a.ijs2]);
+  // all of a thread's loads are issued before any LDS store, so up to kStageMax are in flight
+  {
+    TC v[kStageMax];
+    int dst[kStageMax];
+#pragma unroll
+    for (int i = 0; i < kStageMax; ++i) {
+      v[i] = TC(0);
+      dst[i] = -1;
+      if (i < a.nji) {
+        const int L = threadIdx.x + i * kStageG;
+        int ji, t;
+        stage_decode(in_mode, L, a.nji, a.nji2, ji, t);
+        dst[i] = ji * kStageG + t;
+        const long g2 = g20 + t;
+        if (g2 < a.G2) {
+          const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
+          v[i] = static_cast<TC>(in[ib + g2 * a.is2 + ji1 * a.ijs1 + ji2 * a.ijs2]);
+        }
+      }
     }
-    buf[ji * kStageG + t] = v;
+#pragma unroll
+    for (int i = 0; i < kStageMax; ++i)
+      if (dst[i] >= 0) buf[dst[i]] = v[i];
   }
   __syncthreads();
   TC acc[kStageMax];
