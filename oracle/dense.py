@@ -119,6 +119,21 @@ def svd(A):
         return ssvd(A, full_matrices=False, lapack_driver="gesvd")
 
 
+def right_canonicalize_lq(M):
+    """Right-canonical form by an LQ factorisation per site (QR of the transpose), the form
+    tnkit uses for block tensors (tnkit/_dmrg.py:295-306); equal to `right_canonicalize` up to
+    an orthogonal gauge on every bond."""
+    M = list(M)
+    for i in range(len(M) - 1, 0, -1):
+        Dl, d, Dr = M[i].shape
+        Q, Rr = np.linalg.qr(M[i].reshape(Dl, d * Dr).T)
+        M[i] = Q.T.reshape(-1, d, Dr)
+        M[i - 1] = np.einsum("asb,bc->asc", M[i - 1], Rr.T, optimize=True)
+        M[i - 1] = M[i - 1] / np.linalg.norm(M[i - 1])
+    M[0] = M[0] / np.linalg.norm(M[0])
+    return M
+
+
 def right_canonicalize(M):
     """Right-canonical form by one SVD per site from the right, renormalising the running
     centre at every step; the centre ends at site 0.  tnkit/_dmrg.py:264-292"""
@@ -343,12 +358,14 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
          trunc: str = "svd", seed: int = 0, max_iter: Optional[int] = None,
          restart_m: int = 40, restart_k: int = 10, mixed_tol_lo: float = 1e-4,
          init_mps: Optional[List[np.ndarray]] = None, dtype=np.float64,
-         max_steps: Optional[int] = None) -> Tuple[float, List[np.ndarray], Dict]:
+         max_steps: Optional[int] = None, canonical: str = "svd"
+         ) -> Tuple[float, List[np.ndarray], Dict]:
     """Dense two-site DMRG, the sweep of tnkit/_dmrg.py:560-799 restricted to the dense
     path: warm-started thick-restart Lanczos (or the mixed two-stage form with a float32
     matvec), SVD or Gram truncation, S renormalised, environments updated after each cut.
 
     `max_steps` (oracle-only) stops after that many local steps, for timing samples.
+    `canonical` 'lq' starts from the LQ right-canonical form (the same state in another gauge).
     Returns (E, M, info) with tnkit's info keys 'sweeps', 'energies', 'discarded_weight',
     'matvec_calls', 'sweep_s', 'mixed_counts'."""
     N = len(mpo)
@@ -357,10 +374,11 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
     if real and any(np.abs(np.imag(W)).max(initial=0.0) > 1e-12 for W in mpo):
         raise TypeError("real dtype requested for an MPO with a significant imaginary part")
     mpo = [np.asarray(np.real(W) if real else W, dtype=dtype) for W in mpo]
+    canon = right_canonicalize if canonical == "svd" else right_canonicalize_lq
     if init_mps is not None:
-        M = right_canonicalize([np.asarray(x, dtype=dtype) for x in init_mps])
+        M = canon([np.asarray(x, dtype=dtype) for x in init_mps])
     else:
-        M = right_canonicalize(random_mps(N, chi, d, np.random.default_rng(seed), dtype))
+        M = canon(random_mps(N, chi, d, np.random.default_rng(seed), dtype))
     L = [None] * (N + 1)
     R = [None] * (N + 1)
     L[0] = np.ones((1, 1, 1), dtype=dtype)

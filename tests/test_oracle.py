@@ -283,3 +283,19 @@ def test_oracle_plants_are_caught():
     bad2 = D.apply_two_site_fused(th, *D.fused_two_site(L, W1b, W2, R))
     for bad in (bad1, bad2):
         assert np.abs(bad - ref).max() > 1e-3 * np.abs(ref).max()
+
+
+def test_lq_canonical_start_gives_the_same_trajectory():
+    """Two-site DMRG is covariant under the bond gauge: an LQ right-canonical start gives the
+    per-sweep energies of the SVD one to rounding (the HIP engine canonicalises by LQ)."""
+    run = GOLD["dmrg"][3]
+    E1, _, i1 = D.dmrg(_model(run["model"]), run["chi"], **run["kwargs"])
+    E2, _, i2 = D.dmrg(_model(run["model"]), run["chi"], canonical="lq", **run["kwargs"])
+    for a, b in zip(i1["energies"], i2["energies"]):
+        assert abs(a - b) < 1e-11 * max(1, abs(a))
+    assert abs(i1["matvec_calls"] - i2["matvec_calls"]) <= 0.02 * i1["matvec_calls"]
+    M = D.right_canonicalize_lq(D.random_mps(6, 4, 2, np.random.default_rng(0)))
+    for t in M[1:]:
+        a, d, b = t.shape
+        np.testing.assert_allclose(t.reshape(a, d * b) @ t.reshape(a, d * b).T, np.eye(a),
+                                   atol=1e-13)

@@ -204,9 +204,14 @@ def _check(o: DMRGOptions, mpo) -> None:
 class DMRGEngine:
     """The state of one dense two-site DMRG run on the HIP device: MPS, environments, MPO and
     counters, with `solve_block` the local step of tnkit/_dmrg.py:677-727 and `sweep` the
-    right-then-left pass of tnkit/_dmrg.py:775-793."""
+    right-then-left pass of tnkit/_dmrg.py:775-793.
 
-    def __init__(self, mpo, chi: int, o: DMRGOptions, canonical: str = "svd"):
+    The start is made right-canonical by LQ factorisations (`canonical='lq'`, default) rather
+    than tnkit's SVDs: the same state in another bond gauge, which two-site DMRG does not see
+    (its local problems are covariant under the gauge), at a fraction of the device cost of an
+    SVD at large bond dimension. `canonical='svd'` follows tnkit/_dmrg.py:264-292 exactly."""
+
+    def __init__(self, mpo, chi: int, o: DMRGOptions, canonical: str = "lq"):
         _check(o, mpo)
         be = get_backend(o.backend, mpo=mpo, init_mps=o.init_mps)
         if not isinstance(be, HipBackend):
