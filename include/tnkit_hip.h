@@ -88,6 +88,49 @@ int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const v
                          const void* B, const void* W, void* out, void* ws, size_t ws_bytes,
                          tnh_stream_t stream);
 
+/* ---- batched forms: `batch` independent problems of one shape, element strides s* between
+ * consecutive problems (the legs of tnkit's batched evolution, tnkit/batch_tdvp.py:211-243).
+ * Element types F64, F32, C128, C64; workspaces are `batch` times the single-problem size. */
+int tnh_fused_two_site_batched(int dtype, int batch, int a, int wl, int wm, int wr, int d1, int d2,
+                               int b, const void* L, long sL, const void* W1, long sW1,
+                               const void* W2, long sW2, const void* R, long sR, void* LT,
+                               long sLT, void* S, long sS, void* RT, long sRT, tnh_stream_t stream);
+
+/* apply_two_site_fused over a batch (tnkit/batch_tdvp.py:211-219 apply_two_site_b). io_dtype may
+ * be C128 with a C64 operator (complex64 matvec with complex128 Krylov vectors). */
+int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1, int d2, int b,
+                               int wl, int wr, const void* LT, long sLT, const void* S, long sS,
+                               const void* RT, long sRT, const void* theta, long sth, void* out,
+                               long sout, void* ws, size_t ws_bytes, int tile1, int tile3,
+                               int stages, tnh_stream_t stream);
+
+/* update_left_env / update_right_env over a batch (tnkit/batch_tdvp.py:231-243); complex inputs
+ * take conj(A) / conj(B) as tnkit does. */
+int tnh_update_left_env_batched(int dtype, int batch, int a, int d, int c, int wl, int wr,
+                                const void* L, long sL, const void* A, long sA, const void* W,
+                                long sW, void* out, long sout, void* ws, size_t ws_bytes,
+                                tnh_stream_t stream);
+int tnh_update_right_env_batched(int dtype, int batch, int c, int d, int b, int wl, int wr,
+                                 const void* R, long sR, const void* B, long sB, const void* W,
+                                 long sW, void* out, long sout, void* ws, size_t ws_bytes,
+                                 tnh_stream_t stream);
+
+/* ---- complex128 Krylov exponential, batched over legs (tnkit/eigensolver.py:95-217,
+ * tnkit/batch_tdvp.py:247-324). Per leg: basis V (m rows of ldv, leg stride sV), work vector w
+ * (leg stride sw), and ab[leg*sab + k] = alpha_k, ab[leg*sab + m + k] = beta_k (device). */
+size_t tnh_expm_ws_bytes(int batch);
+
+/* Step j of the fixed-m Lanczos exponential after w = H V[j]: three-term subtraction with
+ * beta_{j-1}, CGS2 against V[0..j] with alpha_j read from the first projection, beta_j = |w|,
+ * V[j+1] = w / max(beta_j, 1e-200). Nothing is read back to the host. m <= 32. */
+int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w, long sw,
+                    double* ab, long sab, void* ws, size_t ws_bytes, tnh_stream_t stream);
+
+/* out[leg] = sum_{k<nrows} c[leg][k] V[leg][k] with complex coefficients c (device, interleaved
+ * re/im, leg stride sc doubles): the final combination beta0 * expm(tau T)[:, 0] . V. */
+int tnh_lincomb_c(int batch, long n, int nrows, const void* V, long ldv, long sV, const double* c,
+                  long sc, void* out, long sout, tnh_stream_t stream);
+
 /* Workspace bytes of the Krylov-vector entries below. */
 size_t tnh_krylov_ws_bytes(long n, int nmax);
 

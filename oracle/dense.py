@@ -469,3 +469,86 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
             "matvec_calls": counts["matvec"], "sweep_s": sweep_s, "mixed_counts": mixed_counts,
             "steps": steps[0]}
     return E, M, info
+
+
+# --------------------------------------------------------------------------------------------
+# two-site TDVP
+# --------------------------------------------------------------------------------------------
+
+
+def tdvp2_evolve(mps, mpo, chi, dt, n_steps, krylov_m=20, measure=None, svd_tol=0.0):
+    """Two-site TDVP (tnkit/tdvp.py:555-800): each local step evolves theta by exp(-i H2 dt/2)
+    with the fixed-m Krylov exponential, cuts it with an SVD to at most `chi` values (and the
+    `svd_tol` relative budget), renormalises S, updates the environment and back-evolves the
+    one-site centre by exp(+i H1 dt/2) unless it is at the chain end; right then left sweeps.
+    Returns (M, records) with records [(t, measure(M))]."""
+    M = right_canonicalize_lq([np.asarray(t, dtype=np.complex128) for t in mps])
+    W = [np.asarray(w, dtype=np.complex128) for w in mpo]
+    N = len(M)
+    L = [None] * (N + 1)
+    R = [None] * (N + 1)
+    L[0] = np.ones((1, 1, 1), dtype=np.complex128)
+    R[N] = np.ones((1, 1, 1), dtype=np.complex128)
+    for i in range(N - 1, 0, -1):
+        R[i] = update_right_env(R[i + 1], M[i], W[i])
+    rec = [(0.0, measure(M))] if measure else []
+
+    def cut(mat):
+        U, S, Vh = svd(mat)
+        tot = float(np.sum(S ** 2)) or 1.0
+        keep = min(chi, S.shape[0])
+        if svd_tol > 0:
+            tail = np.cumsum((S ** 2)[::-1])[::-1] / tot
+            drop = np.append(tail, 0.0)
+            ok = np.nonzero(drop[1:keep + 1] <= svd_tol)[0]
+            keep = int(ok[0]) + 1 if len(ok) else keep
+        S = S[:keep] / np.linalg.norm(S[:keep])
+        return U[:, :keep], S, Vh[:keep]
+
+    def step(i, moving):
+        th = np.einsum("asc,ctb->astb", M[i], M[i + 1])
+        a, s1, s2, b = th.shape
+        op = fused_two_site(L[i], W[i], W[i + 1], R[i + 2])
+        th = lanczos_expm_fixed(lambda x: apply_two_site_fused(x.reshape(th.shape), *op)
+                                .reshape(-1), th.reshape(-1), -0.5j * dt, krylov_m)
+        U, S, Vh = cut(th.reshape(a * s1, s2 * b))
+        nk = S.shape[0]
+        if moving == "right":
+            M[i] = U.reshape(a, s1, nk)
+            L[i + 1] = update_left_env(L[i], M[i], W[i])
+            C = (S[:, None] * Vh).reshape(nk, s2, b)
+            if i < N - 2:
+                C = lanczos_expm_fixed(lambda x: apply_one_site(x.reshape(C.shape), L[i + 1],
+                                                                W[i + 1], R[i + 2]).reshape(-1),
+                                       C.reshape(-1), 0.5j * dt, krylov_m).reshape(C.shape)
+            M[i + 1] = C
+        else:
+            M[i + 1] = Vh.reshape(nk, s2, b)
+            R[i + 1] = update_right_env(R[i + 2], M[i + 1], W[i + 1])
+            C = (U * S).reshape(a, s1, nk)
+            if i > 0:
+                C = lanczos_expm_fixed(lambda x: apply_one_site(x.reshape(C.shape), L[i], W[i],
+                                                                R[i + 1]).reshape(-1),
+                                       C.reshape(-1), 0.5j * dt, krylov_m).reshape(C.shape)
+            M[i] = C
+
+    for n in range(n_steps):
+        for i in range(N - 1):
+            step(i, "right")
+        for i in range(N - 2, -1, -1):
+            step(i, "left")
+        if measure:
+            rec.append(((n + 1) * dt, measure(M)))
+    return M, rec
+
+
+def local_expectations(M, op):
+    """<op_i> of a dense MPS: right-canonicalise a copy, sweep the norm environment from the
+    left (tnkit/observables.py:112-147)."""
+    M = right_canonicalize_lq([np.asarray(t, dtype=np.complex128) for t in M])
+    Lenv = np.ones((1, 1), dtype=np.complex128)
+    out = []
+    for A in M:
+        out.append(np.einsum("ab,asc,st,btc->", Lenv, A, op, A.conj()).real)
+        Lenv = np.einsum("ab,asc,bsd->cd", Lenv, A, A.conj())
+    return np.array(out)

@@ -53,3 +53,29 @@ def heff_dense(L, W1, W2, R):
     a, d1, d2, b = L.shape[0], W1.shape[2], W2.shape[2], R.shape[0]
     n = a * d1 * d2 * b
     return H.reshape(n, L.shape[2] * W1.shape[3] * W2.shape[3] * R.shape[2])
+
+
+def mps_to_vector(M):
+    """The full state vector of an MPS with outer bonds 1."""
+    psi = np.asarray(M[0])
+    for t in M[1:]:
+        psi = np.tensordot(psi, np.asarray(t), axes=([psi.ndim - 1], [0]))
+    return psi.reshape(-1)
+
+
+def site_expectations(psi, op, N):
+    """<psi| op_i |psi> / <psi|psi> for every site of an N-site state vector of qubits."""
+    psi = psi.reshape((2,) * N)
+    nrm = np.vdot(psi, psi).real
+    out = []
+    for i in range(N):
+        phi = np.moveaxis(np.tensordot(op, psi, axes=([1], [i])), 0, i)
+        out.append(np.vdot(psi, phi).real / nrm)
+    return np.array(out)
+
+
+def evolve_dense(mpo, psi0, t):
+    """exp(-i H t) psi0 for the dense H of an MPO (small N only)."""
+    import scipy.linalg
+    H = mpo_to_dense(mpo)
+    return scipy.linalg.expm(-1j * t * H) @ psi0

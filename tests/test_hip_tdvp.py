@@ -1,0 +1,168 @@
+"""Complex and batched kernels, the device Krylov exponential, TDVP2 and the batched evolution
+against the oracle, scipy and exact evolution."""
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+from oracle import dense as D
+from oracle import exact as X
+from tnkit_amd import kernels
+from tnkit_amd import tdvp as T
+from tnkit_amd.models import tfim_mpo
+from tnkit_amd.observables import local_expectations
+
+pytestmark = pytest.mark.gpu
+
+Z = np.diag([1.0, -1.0])
+
+
+def _c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def _t(x, dev, dtype=torch.complex128):
+    return torch.as_tensor(np.ascontiguousarray(x)).to(dev, dtype)
+
+
+def _rel(got, ref):
+    got = got.detach().cpu().numpy() if isinstance(got, torch.Tensor) else got
+    return float(np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+@pytest.mark.parametrize("M,N,K", [(7, 9, 5), (64, 64, 64), (130, 70, 33), (10, 300, 41)])
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.complex64])
+def test_gemm_tn_complex(hip, M, N, K, dtype):
+    print(f"gemm_tn complex {M} {N} {K} {dtype}", flush=True)
+    rng = np.random.default_rng(M + N)
+    A, B = _c(rng, K, M), _c(rng, K, N)
+    got = kernels.gemm_tn(_t(A, hip, dtype), _t(B, hip, dtype))
+    tol = 1e-13 if dtype == torch.complex128 else 5e-6
+    assert _rel(got, A.T @ B) < tol
+
+
+@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", [(1, 1, 3, 3, 2, 2, 2), (6, 3, 3, 3, 2, 2, 7),
+                                                (40, 5, 5, 5, 2, 2, 33), (5, 3, 3, 1, 2, 1, 4)])
+def test_complex_matvec_and_envs(hip, a, wl, wm, wr, d1, d2, b):
+    print(f"complex two-site {(a, wl, wm, wr, d1, d2, b)}", flush=True)
+    rng = np.random.default_rng(a * b)
+    L, R = _c(rng, a, wl, a), _c(rng, b, wr, b)
+    W1, W2 = _c(rng, wl, wm, d1, d1), _c(rng, wm, wr, d2, d2)
+    th = _c(rng, a, d1, d2, b)
+    ref = D.apply_two_site_fused(th, *D.fused_two_site(L, W1, W2, R))
+    op = kernels.fused_two_site(_t(L, hip), _t(W1, hip), _t(W2, hip), _t(R, hip))
+    assert _rel(kernels.apply_two_site_fused(_t(th, hip), *op), ref) < 1e-12
+    op64 = kernels.fused_two_site(_t(L, hip), _t(W1, hip), _t(W2, hip), _t(R, hip),
+                                  dtype=torch.complex64)
+    got = kernels.apply_two_site_fused(_t(th, hip), *op64)
+    assert got.dtype == torch.complex128 and _rel(got, ref) < 5e-5
+    A = _c(rng, a, d1, 6)
+    W = _c(rng, wl, wr, d1, d1)
+    assert _rel(kernels.update_left_env(_t(L, hip), _t(A, hip), _t(W, hip)),
+                D.update_left_env(L, A, W)) < 1e-12
+    Bm = _c(rng, 6, d1, b)
+    W = _c(rng, 3, wr, d1, d1)
+    assert _rel(kernels.update_right_env(_t(R, hip), _t(Bm, hip), _t(W, hip)),
+                D.update_right_env(R, Bm, W)) < 1e-12
+
+
+def test_batched_kernels_match_per_leg(hip):
+    rng = np.random.default_rng(3)
+    k, a, w, d, b = 3, 9, 3, 2, 11
+    L, R = _c(rng, k, a, w, a), _c(rng, k, b, w, b)
+    W1, W2 = _c(rng, k, w, w, d, d), _c(rng, k, w, w, d, d)
+    th = _c(rng, k, a, d, d, b)
+    op = kernels.fused_two_site_b(_t(L, hip), _t(W1, hip), _t(W2, hip), _t(R, hip))
+    got = kernels.apply_two_site_b(_t(th, hip), *op).cpu().numpy()
+    A = _c(rng, k, a, d, 5)
+    gl = kernels.update_left_env_b(_t(L, hip), _t(A, hip), _t(W1, hip)).cpu().numpy()
+    Bm = _c(rng, k, 5, d, b)
+    gr = kernels.update_right_env_b(_t(R, hip), _t(Bm, hip), _t(W2, hip)).cpu().numpy()
+    for j in range(k):
+        ref = D.apply_two_site_fused(th[j], *D.fused_two_site(L[j], W1[j], W2[j], R[j]))
+        assert _rel(got[j], ref) < 1e-12
+        assert _rel(gl[j], D.update_left_env(L[j], A[j], W1[j])) < 1e-12
+        assert _rel(gr[j], D.update_right_env(R[j], Bm[j], W2[j])) < 1e-12
+
+
+@pytest.mark.parametrize("n,m", [(1, 5), (6, 10), (200, 20), (5000, 30)])
+def test_expm_batched_vs_scipy(hip, n, m):
+    print(f"expm n={n} m={m}", flush=True)
+    rng = np.random.default_rng(n)
+    k = 3
+    H = _c(rng, k, n, n)
+    H = 0.5 * (H + np.conj(np.swapaxes(H, 1, 2))) / np.sqrt(n)
+    v = _c(rng, k, n)
+    Hd = _t(H, hip)
+    tau = -0.05j
+    got = T.lanczos_expm_fixed_batched(lambda x: torch.bmm(Hd, x.reshape(k, n, 1)).reshape(k, n),
+                                       _t(v, hip), tau, m).cpu().numpy()
+    for j in range(k):
+        ref = scipy.linalg.expm(tau * H[j]) @ v[j]
+        assert np.abs(got[j] - ref).max() < 1e-11 * max(1, np.abs(ref).max())
+    ref0 = D.lanczos_expm_fixed(lambda x: H[0] @ x, v[0], tau, m)
+    assert np.abs(got[0] - ref0).max() < 1e-12 * np.abs(ref0).max()
+
+
+def _ground(N, g, chi, seed=0):
+    E, M, _ = D.dmrg(tfim_mpo(N, g), chi, n_sweeps=4, tol=0.0, rtol=0.0, seed=seed)
+    return M
+
+
+def test_tdvp2_exact_at_full_bond(hip):
+    """N = 8 at chi = 16: the HIP TDVP2 equals exact evolution to 1e-9 (quench g 1.5 -> 1)."""
+    N, dt, n = 8, 0.05, 6
+    M0 = _ground(N, 1.5, 16)
+    mpo = tfim_mpo(N, 1.0)
+    M, rec = T.tdvp2_evolve(M0, mpo, 16, dt, n, krylov_m=20,
+                            measure=lambda m: local_expectations(m, Z))
+    psi0 = X.mps_to_vector(M0)
+    assert len(rec) == n + 1
+    for t, z in rec:
+        ref = X.site_expectations(X.evolve_dense(mpo, psi0, t), Z, N)
+        assert np.abs(z - ref).max() < 1e-9, (t, np.abs(z - ref).max())
+
+
+def test_tdvp2_truncated_matches_oracle(hip):
+    """N = 10 with the cap chi = 6 < 2^5: HIP (Gram cut) and oracle (SVD cut) trajectories agree."""
+    N, dt, n = 10, 0.05, 5
+    M0 = _ground(N, 1.3, 6)
+    mpo = tfim_mpo(N, 1.0)
+    _, rh = T.tdvp2_evolve(M0, mpo, 6, dt, n, krylov_m=20,
+                           measure=lambda m: local_expectations(m, Z))
+    _, ro = D.tdvp2_evolve(M0, mpo, 6, dt, n, krylov_m=20,
+                           measure=lambda m: D.local_expectations(m, Z))
+    for (t1, z1), (t2, z2) in zip(rh, ro):
+        assert abs(t1 - t2) < 1e-12 and np.abs(z1 - z2).max() < 1e-7
+
+
+def test_evolve_batch_uneven_caps_match_single_runs(hip):
+    """Three quenches (g0 = 1.2, 1.6, 2.0 -> 1.0) with per-leg caps 4, 6, 8 in one batch equal
+    three single-trajectory runs (and the oracle), leg by leg."""
+    N, dt, n = 8, 0.05, 4
+    g0s, caps = [1.2, 1.6, 2.0], [4, 6, 8]
+    states = [_ground(N, g, 4, seed=j) for j, g in enumerate(g0s)]
+    mpo = tfim_mpo(N, 1.0)
+    stats = {}
+    legs, rec = T.evolve_batch(states, mpo, dt, n, chi_max=caps, krylov_m=20, keep_device=True,
+                               measure=lambda ss: [local_expectations(s, Z) for s in ss],
+                               stats=stats)
+    assert len(rec) == n + 1
+    for j in range(3):
+        _, r1 = T.tdvp2_evolve(states[j], mpo, caps[j], dt, n, krylov_m=20,
+                               measure=lambda m: local_expectations(m, Z))
+        _, ro = D.tdvp2_evolve(states[j], mpo, caps[j], dt, n, krylov_m=20,
+                               measure=lambda m: D.local_expectations(m, Z))
+        for (t, zs), (_, z1), (_, zo) in zip(rec, r1, ro):
+            assert np.abs(zs[j] - z1).max() < 1e-10
+            assert np.abs(zs[j] - zo).max() < 1e-7
+        assert max(int(t.shape[2]) for t in legs[j][:-1]) <= caps[j]
+    assert stats["krylov_m"] >= 20
+
+
+def test_batch_geometry_errors(hip):
+    with pytest.raises(T.BatchGeometryError):
+        T.evolve_batch([_ground(6, 1.2, 4), _ground(8, 1.2, 4)], tfim_mpo(6, 1.0), 0.05, 1,
+                       chi_max=4)
+    with pytest.raises(ValueError):
+        T.evolve_batch([_ground(6, 1.2, 4)], tfim_mpo(6, 1.0), 0.05, 1)

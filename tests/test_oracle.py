@@ -299,3 +299,21 @@ def test_lq_canonical_start_gives_the_same_trajectory():
         a, d, b = t.shape
         np.testing.assert_allclose(t.reshape(a, d * b) @ t.reshape(a, d * b).T, np.eye(a),
                                    atol=1e-13)
+
+
+def test_tdvp2_oracle_is_exact_at_full_bond_dimension():
+    """With bonds at 2^(N/2) the two-site projector is the identity, so TDVP2 reproduces the
+    exact evolution exp(-iHt)|psi0> (dense matrix exponential), N = 8, quench g 1.5 -> 1.0."""
+    N, dt, n = 8, 0.05, 6
+    E0, M0, _ = D.dmrg(tfim_mpo(N, 1.5), 16, n_sweeps=4, tol=0.0, rtol=0.0, seed=0)
+    mpo = tfim_mpo(N, 1.0)
+    Z = np.diag([1.0, -1.0])
+    M, rec = D.tdvp2_evolve(M0, mpo, 16, dt, n, krylov_m=20,
+                            measure=lambda m: D.local_expectations(m, Z))
+    psi0 = X.mps_to_vector(M0)
+    for t, z in rec:
+        ref = X.site_expectations(X.evolve_dense(mpo, psi0, t), Z, N)
+        assert np.abs(z - ref).max() < 1e-9, (t, np.abs(z - ref).max())
+    # and the local expectations themselves against the state vector
+    np.testing.assert_allclose(D.local_expectations(M0, Z), X.site_expectations(psi0, Z, N),
+                               atol=1e-12)

@@ -38,6 +38,16 @@ SIGNATURES = {
                                     c_size_t, _P]),
     "tnh_update_right_env": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P,
                                      c_size_t, _P]),
+    "tnh_fused_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 7 + [_P]),
+    "tnh_apply_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 5 + [_P, c_size_t, c_int,
+                                                                          c_int, c_int, _P]),
+    "tnh_update_left_env_batched": (c_int, [c_int] * 7 + [_P, c_long] * 4 + [_P, c_size_t, _P]),
+    "tnh_update_right_env_batched": (c_int, [c_int] * 7 + [_P, c_long] * 4 + [_P, c_size_t, _P]),
+    "tnh_expm_ws_bytes": (c_size_t, [c_int]),
+    "tnh_expm_step_c": (c_int, [c_int, c_long, c_int, c_int, _P, c_long, c_long, _P, c_long, _P,
+                                c_long, _P, c_size_t, _P]),
+    "tnh_lincomb_c": (c_int, [c_int, c_long, c_int, _P, c_long, c_long, _P, c_long, _P, c_long,
+                              _P]),
     "tnh_krylov_ws_bytes": (c_size_t, [c_long, c_int]),
     "tnh_krylov_orth": (c_int, [c_long, c_int, _P, c_long, _P, c_int, c_int, c_int, _PD, _P, _P,
                                 c_size_t, _P]),

@@ -89,6 +89,45 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
   }
 }
 
+namespace {
+template <typename TR, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
+int launch_c(const GemmArgs& g, int batch, int conj_b, hipStream_t stream) {
+  const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
+  if (tiles == 0 || batch == 0) return TNH_OK;
+  TNH_REQUIRE(tiles < (1L << 31) && batch <= 65535, TNH_ERR_ARG, "gemm_tn: grid too large");
+  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(batch));
+  hipLaunchKernelGGL((gemm_tn_cplx_kernel<TR, TA, TB, TO, BM, BN, BK, WM, WN>), grid,
+                     dim3(WM * WN * 64), 0, stream, g, conj_b);
+  TNH_CHECK_LAUNCH("gemm_tn_cplx");
+  return TNH_OK;
+}
+}  // namespace
+
+// Complex tiles: 1 = 64x64 (4 waves of 32x32), 2 = 32x32, 3 = 16x64 (skinny M). 0 = automatic.
+template <typename TR, typename TA, typename TB, typename TO>
+int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t stream,
+                        int tile_hint) {
+  TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn: negative size");
+  int cfg = tile_hint;
+  if (cfg == 0) {
+    const long b = batch > 0 ? batch : 1;
+    const long t64 = ceil_div(g.M, 64) * ceil_div(g.N, 64) * b;
+    cfg = g.M <= 16 ? 3 : (t64 >= 256 ? 1 : 2);
+  }
+  constexpr int BK = sizeof(TR) == 8 ? 8 : 16;
+  switch (cfg) {
+    case 1: return launch_c<TR, TA, TB, TO, 64, 64, BK, 2, 2>(g, batch, conj_b, stream);
+    case 2: return launch_c<TR, TA, TB, TO, 32, 32, BK, 2, 2>(g, batch, conj_b, stream);
+    case 3: return launch_c<TR, TA, TB, TO, 16, 64, 16, 1, 4>(g, batch, conj_b, stream);
+    default: set_error("gemm_tn: unknown complex tile config %d", cfg); return TNH_ERR_ARG;
+  }
+}
+
+template int launch_gemm_tn_cplx<double, c128, c128, c128>(const GemmArgs&, int, int, hipStream_t, int);
+template int launch_gemm_tn_cplx<float, c64, c64, c64>(const GemmArgs&, int, int, hipStream_t, int);
+template int launch_gemm_tn_cplx<float, c128, c64, c64>(const GemmArgs&, int, int, hipStream_t, int);
+template int launch_gemm_tn_cplx<float, c64, c64, c128>(const GemmArgs&, int, int, hipStream_t, int);
+
 template int launch_gemm_tn<double, double, double, double>(const GemmArgs&, int, hipStream_t, int);
 template int launch_gemm_tn<float, float, float, float>(const GemmArgs&, int, hipStream_t, int);
 template int launch_gemm_tn<float, double, float, float>(const GemmArgs&, int, hipStream_t, int);
@@ -109,6 +148,9 @@ extern "C" int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long l
   switch (dtype) {
     case TNH_F64: return launch_gemm_tn<double, double, double, double>(g, batch, s, tile_hint);
     case TNH_F32: return launch_gemm_tn<float, float, float, float>(g, batch, s, tile_hint);
+    case TNH_C128:
+      return launch_gemm_tn_cplx<double, c128, c128, c128>(g, batch, 0, s, tile_hint);
+    case TNH_C64: return launch_gemm_tn_cplx<float, c64, c64, c64>(g, batch, 0, s, tile_hint);
     default: set_error("tnh_gemm_tn: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
   }
 }

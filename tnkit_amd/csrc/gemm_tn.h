@@ -15,6 +15,7 @@
 // float on the way into LDS, C stored in double from a float accumulator), which is how the
 // mixed-precision matvec takes float64 Krylov vectors without a separate conversion pass.
 #pragma once
+#include "cplx.h"
 #include "tnh_common.h"
 
 namespace tnh {
@@ -216,9 +217,170 @@ __global__ void __launch_bounds__(WM* WN * 64)
   }
 }
 
+// Complex K-major GEMM: C[m][n] = sum_k A[k][m] * B'[k][n] with B' = B or conj(B), complex
+// elements stored interleaved. Real and imaginary parts sit in separate LDS planes and each
+// complex fragment product is four real MFMAs (re += ar br - ai bi, im += ar bi + ai br), so the
+// complex flop count is 4x the real one at the same matrix-core rate. TA/TB/TO are storage types
+// (cpx<double> or cpx<float>); TR is the compute type of the MFMA.
+template <typename TR, typename TA, typename TB, typename TO, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(WM* WN * 64)
+    gemm_tn_cplx_kernel(GemmArgs g, int conj_b) {
+  using MF = Mfma<TR>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NT = WM * WN * 64;
+  constexpr int PAD = 16;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int FM = BM / WM / 16;
+  constexpr int FN = BN / WN / 16;
+  constexpr int EA = BK * BM / NT;
+  constexpr int EB = BK * BN / NT;
+  static_assert(BK % 4 == 0 && (BK * BM) % NT == 0 && (BK * BN) % NT == 0, "tile shape");
+  __shared__ TR Ar[2][BK * LDA_S], Ai[2][BK * LDA_S];
+  __shared__ TR Br[2][BK * LDB_S], Bi[2][BK * LDB_S];
+
+  const int M = g.M, N = g.N, K = g.K;
+  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_of(blockIdx.x, nbm, nbn, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const long bz = blockIdx.y;
+  const TA* __restrict__ A = static_cast<const TA*>(g.A) + bz * g.strideA;
+  const TB* __restrict__ B = static_cast<const TB*>(g.B) + bz * g.strideB;
+  TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN, row16 = lane & 15, kq = lane >> 4;
+  const TR bsign = conj_b ? TR(-1) : TR(1);
+
+  acc_t cr[FM][FN], ci[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) cr[i][j] = ci[i][j] = acc_t{};
+
+  TR rar[EA], rai[EA], rbr[EB], rbi[EB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int e = t + i * NT, kr = e / BM, mc = e % BM, k = k0 + kr, m = m0 + mc;
+      if (k < K && m < M) {
+        const TA v = A[(long)k * g.lda + m];
+        rar[i] = static_cast<TR>(v.re);
+        rai[i] = static_cast<TR>(v.im);
+      } else {
+        rar[i] = rai[i] = TR(0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int e = t + i * NT, kr = e / BN, nc = e % BN, k = k0 + kr, n = n0 + nc;
+      if (k < K && n < N) {
+        const TB v = B[(long)k * g.ldb + n];
+        rbr[i] = static_cast<TR>(v.re);
+        rbi[i] = bsign * static_cast<TR>(v.im);
+      } else {
+        rbr[i] = rbi[i] = TR(0);
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < EA; ++i) {
+      const int e = t + i * NT, o = (e / BM) * LDA_S + e % BM;
+      Ar[buf][o] = rar[i];
+      Ai[buf][o] = rai[i];
+    }
+#pragma unroll
+    for (int i = 0; i < EB; ++i) {
+      const int e = t + i * NT, o = (e / BN) * LDB_S + e % BN;
+      Br[buf][o] = rbr[i];
+      Bi[buf][o] = rbi[i];
+    }
+  };
+  const int nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      TR ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int o = (kk + kq) * LDA_S + wm * (BM / WM) + i * 16 + row16;
+        ar[i] = Ar[cur][o];
+        ai[i] = Ai[cur][o];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int o = (kk + kq) * LDB_S + wn * (BN / WN) + j * 16 + row16;
+        br[j] = Br[cur][o];
+        bi[j] = Bi[cur][o];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          cr[i][j] = MF::run(ar[i], br[j], cr[i][j]);
+          cr[i][j] = MF::run(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = MF::run(ar[i], bi[j], ci[i][j]);
+          ci[i][j] = MF::run(ai[i], br[j], ci[i][j]);
+        }
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * (BN / WN) + j * 16 + row16;
+      if (n >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * (BM / WM) + i * 16 + MF::row(kq, r);
+        if (m < M) {
+          TO* dst = C + (long)m * g.ldc + n;
+          TO v(cr[i][j][r], ci[i][j][r]);
+          if (g.accumulate) v = TO(dst->re + v.re, dst->im + v.im);
+          *dst = v;
+        }
+      }
+    }
+}
+
+template <typename TR, typename TA, typename TB, typename TO>
+int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t stream,
+                        int tile_hint = 0);
+
 // Host-side launcher: picks a tile shape for the problem size and launches on `stream`.
 // Returns a TNH_* status.
 template <typename TC, typename TA, typename TB, typename TO>
 int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint = 0);
+
+// Routes a GEMM to the real or the complex kernel by the compute type.
+template <typename T>
+struct real_of {
+  using type = T;
+  static constexpr bool cplx = false;
+};
+template <typename T>
+struct real_of<cpx<T>> {
+  using type = T;
+  static constexpr bool cplx = true;
+};
+
+template <typename TC, typename TA, typename TB, typename TO>
+inline int gemm_any(const GemmArgs& g, int batch, int conj_b, hipStream_t st, int tile) {
+  if constexpr (real_of<TC>::cplx)
+    return launch_gemm_tn_cplx<typename real_of<TC>::type, TA, TB, TO>(g, batch, conj_b, st, tile);
+  else
+    return launch_gemm_tn<TC, TA, TB, TO>(g, batch, st, tile);
+}
 
 }  // namespace tnh

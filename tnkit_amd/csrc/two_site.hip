@@ -13,7 +13,10 @@
 //   GEMM   T1[(r y)][(u s)]   = sum_x R[x][(r y)] * Bx[x][(u s)]               Bx[x][u][s] = B[u][s][x]
 //   W-mix  T2[(p y)][(u w)]   = sum_{r s} W[w][r][s][p] * T1[r][y][u][s]
 //   GEMM   R'[(u w)][v]       = sum_{(p y)} T2[(p y)][(u w)] * Bp[(p y)][v]     Bp[p][y][v] = B[v][p][y]
-// Real dtypes (the statics path); conj() is the identity.
+// Real (float64, float32) and complex (complex128, complex64) elements; conj(A) and conj(B) are
+// taken inside the first GEMM of each environment update. Every entry has a batched form: k
+// independent problems of one shape with element strides between them (the legs of the batched
+// time evolution), one launch per kernel for all k.
 #include "gemm_tn.h"
 
 namespace tnh {
@@ -22,11 +25,14 @@ namespace tnh {
 // small kernels
 // ---------------------------------------------------------------------------------------------
 
-// out[j0][j1][j2] = in[j0*s0 + j1*s1 + j2*s2]   (a 3-axis permutation given by input strides)
+// out[j0][j1][j2] = in[j0*s0 + j1*s1 + j2*s2]   (a 3-axis permutation given by input strides);
+// batch element z = blockIdx.z at element offsets z*bin, z*bout
 template <typename T>
 __global__ void permute3_kernel(long n0, long n1, long n2, long s0, long s1, long s2,
-                                const T* __restrict__ in, T* __restrict__ out) {
+                                const T* __restrict__ in, long bin, T* __restrict__ out, long bout) {
   const long total = n0 * n1 * n2;
+  in += blockIdx.z * bin;
+  out += blockIdx.z * bout;
   for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total;
        o += (long)gridDim.x * blockDim.x) {
     const long j2 = o % n2;
@@ -37,17 +43,21 @@ __global__ void permute3_kernel(long n0, long n1, long n2, long s0, long s1, lon
   }
 }
 
-// S[(s t r)][(w i j)] = sum_m W1[w][m][s][i] * W2[m][r][t][j]
+// S[(s t r)][(w i j)] = sum_m W1[w][m][s][i] * W2[m][r][t][j]   (batch element blockIdx.z)
 template <typename T>
 __global__ void build_s_kernel(int wl, int wm, int wr, int d1, int d2, const T* __restrict__ W1,
-                               const T* __restrict__ W2, T* __restrict__ S) {
+                               long b1, const T* __restrict__ W2, long b2, T* __restrict__ S,
+                               long bs) {
+  W1 += blockIdx.z * b1;
+  W2 += blockIdx.z * b2;
+  S += blockIdx.z * bs;
   const int ncol = wl * d1 * d2;
   const int total = d1 * d2 * wr * ncol;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
     const int col = o % ncol, row = o / ncol;
     const int j = col % d2, i = (col / d2) % d1, w = col / (d1 * d2);
     const int r = row % wr, t = (row / wr) % d2, s = row / (wr * d2);
-    T acc = 0;
+    T acc = T(0);
     for (int m = 0; m < wm; ++m)
       acc += W1[((w * wm + m) * d1 + s) * d1 + i] * W2[((m * wr + r) * d2 + t) * d2 + j];
     S[o] = acc;
@@ -58,7 +68,9 @@ __global__ void build_s_kernel(int wl, int wm, int wr, int d1, int d2, const T* 
 // `which`: 0 (left env)  Mx[(y r)][(w s)];   1 (right env)  Mx[(p w)][(r s)] = W[w][r][s][p].
 template <typename T>
 __global__ void build_wmix_kernel(int which, int wl, int wr, int d, const T* __restrict__ W,
-                                  T* __restrict__ Mx) {
+                                  long bw, T* __restrict__ Mx, long bm) {
+  W += blockIdx.z * bw;
+  Mx += blockIdx.z * bm;
   const int total = wl * wr * d * d;
   for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
     int w, r, s, y;
@@ -88,6 +100,7 @@ struct MixArgs {
   int njo, njo2, nji, nji2;
   long is1, is2, ijs1, ijs2;
   long os1, os2, ojs1, ojs2;
+  long bm, bin, bout;  // batch strides of Mx, in and out (batch element blockIdx.z)
 };
 
 // outputs / inputs per register chunk: 32 floats or 16 doubles stay in registers without spills
@@ -99,13 +112,25 @@ template <>
 struct MixChunk<double> {
   static constexpr int value = 16;
 };
-constexpr int kMixLds = 4096;  // Mx entries held in LDS (larger operators read global memory)
+template <>
+struct MixChunk<c64> {
+  static constexpr int value = 16;
+};
+template <>
+struct MixChunk<c128> {
+  static constexpr int value = 8;
+};
+constexpr int kMixLds = 2048;  // Mx entries held in LDS (larger operators read global memory)
 
 template <typename TC, typename TI, typename TO>
 __global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restrict__ Mx,
                                                   const TI* __restrict__ in, TO* __restrict__ out) {
   constexpr int kMixChunk = MixChunk<TC>::value;
-  __shared__ TC ms[kMixLds];
+  Mx += blockIdx.z * a.bm;
+  in += blockIdx.z * a.bin;
+  out += blockIdx.z * a.bout;
+  __shared__ __align__(16) unsigned char ms_raw[kMixLds * sizeof(TC)];
+  TC* ms = reinterpret_cast<TC*>(ms_raw);
   const int nm = a.njo * a.nji;
   const bool in_lds = nm <= kMixLds;
   if (in_lds)
@@ -172,6 +197,10 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restric
 // order the same way. Every global access of a wave is then a contiguous run.
 constexpr int kStageG = 256;
 constexpr int kStageMax = 32;
+template <typename T>
+struct StageMax {  // register budget of the staged kernel: 32 real or 16 complex per thread
+  static constexpr int value = real_of<T>::cplx ? 16 : kStageMax;
+};
 
 __device__ __forceinline__ void stage_decode(int mode, int L, int n, int n2, int& j, int& t) {
   const int n1 = n / n2;
@@ -196,6 +225,10 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
                                                          const TC* __restrict__ Mx,
                                                          const TI* __restrict__ in,
                                                          TO* __restrict__ out) {
+  constexpr int kMax = StageMax<TC>::value;
+  Mx += blockIdx.z * a.bm;
+  in += blockIdx.z * a.bin;
+  out += blockIdx.z * a.bout;
   // dynamic LDS: the operator (njo x nji), then max(nji, njo) rows of kStageG values
   extern __shared__ __align__(16) unsigned char stage_lds[];
   TC* ms = reinterpret_cast<TC*>(stage_lds);
@@ -207,10 +240,10 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   const long ib = g1 * a.is1;
   // all of a thread's loads are issued before any LDS store, so up to kStageMax are in flight
   {
-    TC v[kStageMax];
-    int dst[kStageMax];
+    TC v[kMax];
+    int dst[kMax];
 #pragma unroll
-    for (int i = 0; i < kStageMax; ++i) {
+    for (int i = 0; i < kMax; ++i) {
       v[i] = TC(0);
       dst[i] = -1;
       if (i < a.nji) {
@@ -226,18 +259,18 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
       }
     }
 #pragma unroll
-    for (int i = 0; i < kStageMax; ++i)
+    for (int i = 0; i < kMax; ++i)
       if (dst[i] >= 0) buf[dst[i]] = v[i];
   }
   __syncthreads();
-  TC acc[kStageMax];
+  TC acc[kMax];
 #pragma unroll
-  for (int o = 0; o < kStageMax; ++o) acc[o] = TC(0);
+  for (int o = 0; o < kMax; ++o) acc[o] = TC(0);
   const int t = threadIdx.x;
   for (int ji = 0; ji < a.nji; ++ji) {
     const TC x = buf[ji * kStageG + t];
 #pragma unroll
-    for (int o = 0; o < kStageMax; ++o) {
+    for (int o = 0; o < kMax; ++o) {
       if (o < a.njo) {
         const TC m = ms[o * a.nji + ji];
         if (m != TC(0)) acc[o] += m * x;
@@ -246,7 +279,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   }
   __syncthreads();
 #pragma unroll
-  for (int o = 0; o < kStageMax; ++o)
+  for (int o = 0; o < kMax; ++o)
     if (o < a.njo) buf[o * kStageG + t] = acc[o];
   __syncthreads();
   const long ob = g1 * a.os1;
@@ -272,12 +305,12 @@ inline unsigned grid_for(long total, int threads = 256, long cap = 1L << 16) {
 }
 
 template <typename T>
-int permute3(long n0, long n1, long n2, long s0, long s1, long s2, const void* in, void* out,
-             hipStream_t st) {
+int permute3(int batch, long n0, long n1, long n2, long s0, long s1, long s2, const void* in,
+             long bin, void* out, long bout, hipStream_t st) {
   const long total = n0 * n1 * n2;
-  if (total == 0) return TNH_OK;
-  hipLaunchKernelGGL(permute3_kernel<T>, dim3(grid_for(total)), dim3(256), 0, st, n0, n1, n2, s0,
-                     s1, s2, static_cast<const T*>(in), static_cast<T*>(out));
+  if (total == 0 || batch == 0) return TNH_OK;
+  hipLaunchKernelGGL(permute3_kernel<T>, dim3(grid_for(total), 1, batch), dim3(256), 0, st, n0, n1,
+                     n2, s0, s1, s2, static_cast<const T*>(in), bin, static_cast<T*>(out), bout);
   TNH_CHECK_LAUNCH("permute3");
   return TNH_OK;
 }
@@ -294,14 +327,16 @@ inline int stage_mode(long s_g2, long s1, long s2, int n, int n2) {
 }
 
 template <typename TC, typename TI, typename TO>
-int mix(const MixArgs& a, const void* Mx, const void* in, void* out, hipStream_t st) {
+int mix(const MixArgs& a, int batch, const void* Mx, const void* in, void* out, hipStream_t st) {
+  if (a.G1 * a.G2 == 0 || batch == 0) return TNH_OK;
   const int im = stage_mode(a.is2, a.ijs1, a.ijs2, a.nji, a.nji2);
   const int om = stage_mode(a.os2, a.ojs1, a.ojs2, a.njo, a.njo2);
   const int rows = a.nji > a.njo ? a.nji : a.njo;
   const size_t lds = (((size_t)a.njo * a.nji + 1) / 2 * 2 + (size_t)rows * kStageG) * sizeof(TC);
-  if (a.nji <= kStageMax && a.njo <= kStageMax && im >= 0 && om >= 0 && a.G1 <= 65535 &&
-      a.G1 * a.G2 > 0 && lds <= 65536) {
-    dim3 grid(static_cast<unsigned>((a.G2 + kStageG - 1) / kStageG), static_cast<unsigned>(a.G1));
+  constexpr int kMax = StageMax<TC>::value;
+  if (a.nji <= kMax && a.njo <= kMax && im >= 0 && om >= 0 && a.G1 <= 65535 && lds <= 65536) {
+    dim3 grid(static_cast<unsigned>((a.G2 + kStageG - 1) / kStageG), static_cast<unsigned>(a.G1),
+              batch);
     hipLaunchKernelGGL((mix_staged_kernel<TC, TI, TO>), grid, dim3(kStageG), lds, st, a, im, om,
                        static_cast<const TC*>(Mx), static_cast<const TI*>(in),
                        static_cast<TO*>(out));
@@ -310,9 +345,8 @@ int mix(const MixArgs& a, const void* Mx, const void* in, void* out, hipStream_t
   }
   constexpr int kMixChunk = MixChunk<TC>::value;
   const long total = a.G1 * a.G2 * ((a.njo + kMixChunk - 1) / kMixChunk);
-  if (total == 0) return TNH_OK;
-  hipLaunchKernelGGL((mix_kernel<TC, TI, TO>), dim3(grid_for(total, 256, 1L << 20)), dim3(256), 0,
-                     st, a, static_cast<const TC*>(Mx), static_cast<const TI*>(in),
+  hipLaunchKernelGGL((mix_kernel<TC, TI, TO>), dim3(grid_for(total, 256, 1L << 16), 1, batch),
+                     dim3(256), 0, st, a, static_cast<const TC*>(Mx), static_cast<const TI*>(in),
                      static_cast<TO*>(out));
   TNH_CHECK_LAUNCH("mix");
   return TNH_OK;
@@ -322,29 +356,37 @@ size_t dsize(int dtype) {
   switch (dtype) {
     case TNH_F64: return 8;
     case TNH_F32: return 4;
+    case TNH_C128: return 16;
+    case TNH_C64: return 8;
     default: return 0;
   }
 }
 
-#define TNH_TRY(x)              \
-  do {                          \
-    int rc_ = (x);              \
+#define TNH_TRY(x)                 \
+  do {                             \
+    int rc_ = (x);                 \
     if (rc_ != TNH_OK) return rc_; \
   } while (0)
 
 // ---- two-site ----------------------------------------------------------------------------------
 
+struct Strides4 {  // element strides between batch elements
+  long a, b, c, d;
+};
+
 template <typename T>
-int fused_two_site(int a, int wl, int wm, int wr, int d1, int d2, int b, const void* L,
-                   const void* W1, const void* W2, const void* R, void* LT, void* S, void* RT,
+int fused_two_site(int batch, int a, int wl, int wm, int wr, int d1, int d2, int b, const void* L,
+                   long sL, const void* W1, long sW1, const void* W2, long sW2, const void* R,
+                   long sR, void* LT, long sLT, void* S, long sS, void* RT, long sRT,
                    hipStream_t st) {
   // LT[q][p][w] = L[p][w][q]: L strides (wl*a, a, 1) for (p, w, q)
-  TNH_TRY(permute3<T>(a, a, wl, 1, (long)wl * a, a, L, LT, st));
+  TNH_TRY(permute3<T>(batch, a, a, wl, 1, (long)wl * a, a, L, sL, LT, sLT, st));
   // RT[r][b][d] = R[d][r][b]: R strides (wr*b, b, 1) for (d, r, b)
-  TNH_TRY(permute3<T>(wr, b, b, b, 1, (long)wr * b, R, RT, st));
+  TNH_TRY(permute3<T>(batch, wr, b, b, b, 1, (long)wr * b, R, sR, RT, sRT, st));
   const int total = d1 * d2 * wr * wl * d1 * d2;
-  hipLaunchKernelGGL(build_s_kernel<T>, dim3(grid_for(total)), dim3(256), 0, st, wl, wm, wr, d1, d2,
-                     static_cast<const T*>(W1), static_cast<const T*>(W2), static_cast<T*>(S));
+  hipLaunchKernelGGL(build_s_kernel<T>, dim3(grid_for(total), 1, batch), dim3(256), 0, st, wl, wm,
+                     wr, d1, d2, static_cast<const T*>(W1), sW1, static_cast<const T*>(W2), sW2,
+                     static_cast<T*>(S), sS);
   TNH_CHECK_LAUNCH("build_s");
   return TNH_OK;
 }
@@ -352,32 +394,31 @@ int fused_two_site(int a, int wl, int wm, int wr, int d1, int d2, int b, const v
 // stages: bit 0 GEMM1, bit 1 S-mix, bit 2 GEMM3 (7 = the whole matvec). A partial run reads and
 // writes the workspace as the whole one does; it exists to time one stage at a time.
 template <typename TC, typename TIO>
-int apply_two_site(int a, int d1, int d2, int b, int wl, int wr, const void* LT, const void* S,
-                   const void* RT, const void* theta, void* out, void* ws, size_t ws_bytes,
-                   hipStream_t st, int tile1, int tile3, int stages = 7) {
+int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* LT,
+                   long sLT, const void* S, long sS, const void* RT, long sRT, const void* theta,
+                   long sth, void* out, long sout, void* ws, size_t ws_bytes, hipStream_t st,
+                   int tile1, int tile3, int stages) {
   const long P = a, Q = a, B = b, dd = (long)d1 * d2;
+  const long nx1 = dd * B * P * wl, nx2 = (long)wr * B * P * dd;
   Arena ar(ws, ws_bytes);
-  TC* X1 = ar.take<TC>(dd * B * P * wl);
-  TC* X2 = ar.take<TC>((long)wr * B * P * dd);
+  TC* X1 = ar.take<TC>(nx1 * batch);
+  TC* X2 = ar.take<TC>(nx2 * batch);
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
-  // GEMM1: M = d1 d2 b, N = p wl, K = q
-  if (stages & 1) {
+  if (stages & 1) {  // GEMM1: M = d1 d2 b, N = p wl, K = q
     GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
-               dd * B, 0, LT, P * wl, 0, X1, P * wl, 0, 0};
-    TNH_TRY((launch_gemm_tn<TC, TIO, TC, TC>(g, 1, st, tile1)));
+               dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+    TNH_TRY((gemm_any<TC, TIO, TC, TC>(g, batch, 0, st, tile1)));
   }
-  // S-mix
-  if (stages & 2) {
+  if (stages & 2) {  // S-mix
     MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
-              P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd};
-    TNH_TRY((mix<TC, TC, TC>(m, S, X1, X2, st)));
+              P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd, sS, nx1, nx2};
+    TNH_TRY((mix<TC, TC, TC>(m, batch, S, X1, X2, st)));
   }
-  // GEMM3: M = p d1 d2, N = b', K = wr b
-  if (stages & 4) {
+  if (stages & 4) {  // GEMM3: M = p d1 d2, N = b', K = wr b
     GemmArgs g{static_cast<int>(P * dd), static_cast<int>(B), static_cast<int>(wr * B), X2, P * dd,
-               0, RT, B, 0, out, B, 0, 0};
-    TNH_TRY((launch_gemm_tn<TC, TC, TC, TIO>(g, 1, st, tile3)));
+               nx2, RT, B, sRT, out, B, sout, 0};
+    TNH_TRY((gemm_any<TC, TC, TC, TIO>(g, batch, 0, st, tile3)));
   }
   return TNH_OK;
 }
@@ -385,61 +426,66 @@ int apply_two_site(int a, int d1, int d2, int b, int wl, int wr, const void* LT,
 // ---- environments ------------------------------------------------------------------------------
 
 template <typename T>
-int left_env(int a, int d, int c, int wl, int wr, const void* L, const void* A, const void* W,
-             void* out, void* ws, size_t ws_bytes, hipStream_t st) {
+int left_env(int batch, int a, int d, int c, int wl, int wr, const void* L, long sL,
+             const void* A, long sA, const void* W, long sW, void* out, long sout, void* ws,
+             size_t ws_bytes, hipStream_t st) {
+  const long n1 = (long)wl * a * d * c, n2 = (long)a * d * c * wr, nm = (long)wl * wr * d * d;
   Arena ar(ws, ws_bytes);
-  T* T1 = ar.take<T>((long)wl * a * d * c);
-  T* T2 = ar.take<T>((long)a * d * c * wr);
-  T* Mx = ar.take<T>((long)wl * wr * d * d);
+  T* T1 = ar.take<T>(n1 * batch);
+  T* T2 = ar.take<T>(n2 * batch);
+  T* Mx = ar.take<T>(nm * batch);
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "update_left_env: workspace %zu < %zu bytes", ws_bytes, ar.used);
-  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for((long)wl * wr * d * d)), dim3(256), 0, st,
-                     0, wl, wr, d, static_cast<const T*>(W), Mx);
+  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for(nm), 1, batch), dim3(256), 0, st, 0, wl,
+                     wr, d, static_cast<const T*>(W), sW, Mx, nm);
   TNH_CHECK_LAUNCH("build_wmix");
-  {  // T1[(w q)][(s e)] = sum_p L[p][(w q)] A[p][(s e)]
-    GemmArgs g{wl * a, d * c, a, L, (long)wl * a, 0, A, (long)d * c, 0, T1, (long)d * c, 0, 0};
-    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  {  // T1[(w q)][(s e)] = sum_p L[p][(w q)] conj(A)[p][(s e)]
+    GemmArgs g{wl * a, d * c, a, L, (long)wl * a, sL, A, (long)d * c, sA, T1, (long)d * c, n1, 0};
+    TNH_TRY((gemm_any<T, T, T, T>(g, batch, 1, st, 0)));
   }
   {  // T2[q][y][e][r] = sum_{w s} Mx[(y r)][(w s)] T1[w][q][s][e]; groups (q, e)
     MixArgs m{a, c, d * wr, wr, wl * d, d, (long)d * c, 1, (long)a * d * c, c,
-              (long)d * c * wr, wr, (long)c * wr, 1};
-    TNH_TRY((mix<T, T, T>(m, Mx, T1, T2, st)));
+              (long)d * c * wr, wr, (long)c * wr, 1, nm, n1, n2};
+    TNH_TRY((mix<T, T, T>(m, batch, Mx, T1, T2, st)));
   }
   {  // L'[(e r)][f] = sum_{(q y)} T2[(q y)][(e r)] A[(q y)][f]
-    GemmArgs g{c * wr, c, a * d, T2, (long)c * wr, 0, A, c, 0, out, c, 0, 0};
-    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+    GemmArgs g{c * wr, c, a * d, T2, (long)c * wr, n2, A, c, sA, out, c, sout, 0};
+    TNH_TRY((gemm_any<T, T, T, T>(g, batch, 0, st, 0)));
   }
   return TNH_OK;
 }
 
 template <typename T>
-int right_env(int c, int d, int b, int wl, int wr, const void* R, const void* B, const void* W,
-              void* out, void* ws, size_t ws_bytes, hipStream_t st) {
+int right_env(int batch, int c, int d, int b, int wl, int wr, const void* R, long sR,
+              const void* B, long sB, const void* W, long sW, void* out, long sout, void* ws,
+              size_t ws_bytes, hipStream_t st) {
+  const long nb = (long)b * c * d, n1 = (long)wr * b * c * d, n2 = (long)d * b * c * wl;
+  const long nm = (long)wl * wr * d * d;
   Arena ar(ws, ws_bytes);
-  T* Bx = ar.take<T>((long)b * c * d);
-  T* Bp = ar.take<T>((long)d * b * c);
-  T* T1 = ar.take<T>((long)wr * b * c * d);
-  T* T2 = ar.take<T>((long)d * b * c * wl);
-  T* Mx = ar.take<T>((long)wl * wr * d * d);
+  T* Bx = ar.take<T>(nb * batch);
+  T* Bp = ar.take<T>(nb * batch);
+  T* T1 = ar.take<T>(n1 * batch);
+  T* T2 = ar.take<T>(n2 * batch);
+  T* Mx = ar.take<T>(nm * batch);
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "update_right_env: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
   // B(c, d, b) strides (d*b, b, 1) for (u, s, x)
-  TNH_TRY(permute3<T>(b, c, d, 1, (long)d * b, b, B, Bx, st));  // Bx[x][u][s]
-  TNH_TRY(permute3<T>(d, b, c, b, 1, (long)d * b, B, Bp, st));  // Bp[p][y][v]
-  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for((long)wl * wr * d * d)), dim3(256), 0, st,
-                     1, wl, wr, d, static_cast<const T*>(W), Mx);
+  TNH_TRY(permute3<T>(batch, b, c, d, 1, (long)d * b, b, B, sB, Bx, nb, st));  // Bx[x][u][s]
+  TNH_TRY(permute3<T>(batch, d, b, c, b, 1, (long)d * b, B, sB, Bp, nb, st));  // Bp[p][y][v]
+  hipLaunchKernelGGL(build_wmix_kernel<T>, dim3(grid_for(nm), 1, batch), dim3(256), 0, st, 1, wl,
+                     wr, d, static_cast<const T*>(W), sW, Mx, nm);
   TNH_CHECK_LAUNCH("build_wmix");
-  {  // T1[(r y)][(u s)] = sum_x R[x][(r y)] Bx[x][(u s)]
-    GemmArgs g{wr * b, c * d, b, R, (long)wr * b, 0, Bx, (long)c * d, 0, T1, (long)c * d, 0, 0};
-    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+  {  // T1[(r y)][(u s)] = sum_x R[x][(r y)] conj(Bx)[x][(u s)]
+    GemmArgs g{wr * b, c * d, b, R, (long)wr * b, sR, Bx, (long)c * d, nb, T1, (long)c * d, n1, 0};
+    TNH_TRY((gemm_any<T, T, T, T>(g, batch, 1, st, 0)));
   }
   {  // T2[p][y][u][w] = sum_{r s} Mx[(p w)][(r s)] T1[r][y][u][s]; groups (y, u)
     MixArgs m{b, c, d * wl, wl, wr * d, d, (long)c * d, d, (long)b * c * d, 1,
-              (long)c * wl, wl, (long)b * c * wl, 1};
-    TNH_TRY((mix<T, T, T>(m, Mx, T1, T2, st)));
+              (long)c * wl, wl, (long)b * c * wl, 1, nm, n1, n2};
+    TNH_TRY((mix<T, T, T>(m, batch, Mx, T1, T2, st)));
   }
   {  // R'[(u w)][v] = sum_{(p y)} T2[(p y)][(u w)] Bp[(p y)][v]
-    GemmArgs g{c * wl, c, d * b, T2, (long)c * wl, 0, Bp, c, 0, out, c, 0, 0};
-    TNH_TRY((launch_gemm_tn<T, T, T, T>(g, 1, st, 0)));
+    GemmArgs g{c * wl, c, d * b, T2, (long)c * wl, n2, Bp, c, nb, out, c, sout, 0};
+    TNH_TRY((gemm_any<T, T, T, T>(g, batch, 0, st, 0)));
   }
   return TNH_OK;
 }
@@ -451,17 +497,32 @@ using namespace tnh;
 
 extern "C" {
 
+int tnh_fused_two_site_batched(int dtype, int batch, int a, int wl, int wm, int wr, int d1, int d2,
+                               int b, const void* L, long sL, const void* W1, long sW1,
+                               const void* W2, long sW2, const void* R, long sR, void* LT,
+                               long sLT, void* S, long sS, void* RT, long sRT,
+                               tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wm > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_fused_two_site: sizes must be positive");
+  TNH_REQUIRE(batch >= 1 && batch <= 65535, TNH_ERR_ARG, "tnh_fused_two_site: batch %d", batch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define TNH_FTS(T) \
+  fused_two_site<T>(batch, a, wl, wm, wr, d1, d2, b, L, sL, W1, sW1, W2, sW2, R, sR, LT, sLT, S, sS, RT, sRT, st)
+  switch (dtype) {
+    case TNH_F64: return TNH_FTS(double);
+    case TNH_F32: return TNH_FTS(float);
+    case TNH_C128: return TNH_FTS(c128);
+    case TNH_C64: return TNH_FTS(c64);
+    default: set_error("tnh_fused_two_site: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+#undef TNH_FTS
+}
+
 int tnh_fused_two_site(int dtype, int a, int wl, int wm, int wr, int d1, int d2, int b,
                        const void* L, const void* W1, const void* W2, const void* R, void* LT,
                        void* S, void* RT, tnh_stream_t stream) {
-  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wm > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
-              "tnh_fused_two_site: sizes must be positive");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  switch (dtype) {
-    case TNH_F64: return fused_two_site<double>(a, wl, wm, wr, d1, d2, b, L, W1, W2, R, LT, S, RT, st);
-    case TNH_F32: return fused_two_site<float>(a, wl, wm, wr, d1, d2, b, L, W1, W2, R, LT, S, RT, st);
-    default: set_error("tnh_fused_two_site: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
-  }
+  return tnh_fused_two_site_batched(dtype, 1, a, wl, wm, wr, d1, d2, b, L, 0, W1, 0, W2, 0, R, 0,
+                                    LT, 0, S, 0, RT, 0, stream);
 }
 
 size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr) {
@@ -470,33 +531,44 @@ size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, in
   return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + 256;
 }
 
+int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1, int d2, int b,
+                               int wl, int wr, const void* LT, long sLT, const void* S, long sS,
+                               const void* RT, long sRT, const void* theta, long sth, void* out,
+                               long sout, void* ws, size_t ws_bytes, int tile1, int tile3,
+                               int stages, tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_apply_two_site_fused: sizes must be positive");
+  TNH_REQUIRE(stages >= 1 && stages <= 7, TNH_ERR_ARG, "tnh_apply_two_site: stages %d", stages);
+  TNH_REQUIRE(batch >= 1 && batch <= 65535, TNH_ERR_ARG, "tnh_apply_two_site: batch %d", batch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define TNH_A2S(TC, TIO)                                                                       \
+  apply_two_site<TC, TIO>(batch, a, d1, d2, b, wl, wr, LT, sLT, S, sS, RT, sRT, theta, sth, out, \
+                          sout, ws, ws_bytes, st, tile1, tile3, stages)
+  if (dtype == TNH_F64 && io_dtype == TNH_F64) return TNH_A2S(double, double);
+  if (dtype == TNH_F32 && io_dtype == TNH_F32) return TNH_A2S(float, float);
+  if (dtype == TNH_F32 && io_dtype == TNH_F64) return TNH_A2S(float, double);
+  if (dtype == TNH_C128 && io_dtype == TNH_C128) return TNH_A2S(c128, c128);
+  if (dtype == TNH_C64 && io_dtype == TNH_C64) return TNH_A2S(c64, c64);
+  if (dtype == TNH_C64 && io_dtype == TNH_C128) return TNH_A2S(c64, c128);
+#undef TNH_A2S
+  set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
 int tnh_apply_two_site_stages(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl,
                               int wr, const void* LT, const void* S, const void* RT,
                               const void* theta, void* out, void* ws, size_t ws_bytes, int tile1,
                               int tile3, int stages, tnh_stream_t stream) {
-  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
-              "tnh_apply_two_site_fused: sizes must be positive");
-  TNH_REQUIRE(stages >= 1 && stages <= 7, TNH_ERR_ARG, "tnh_apply_two_site_stages: stages %d", stages);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dtype == TNH_F64 && io_dtype == TNH_F64)
-    return apply_two_site<double, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                          st, tile1, tile3, stages);
-  if (dtype == TNH_F32 && io_dtype == TNH_F32)
-    return apply_two_site<float, float>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                        st, tile1, tile3, stages);
-  if (dtype == TNH_F32 && io_dtype == TNH_F64)
-    return apply_two_site<float, double>(a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws, ws_bytes,
-                                         st, tile1, tile3, stages);
-  set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
-  return TNH_ERR_UNSUPPORTED;
+  return tnh_apply_two_site_batched(dtype, io_dtype, 1, a, d1, d2, b, wl, wr, LT, 0, S, 0, RT, 0,
+                                    theta, 0, out, 0, ws, ws_bytes, tile1, tile3, stages, stream);
 }
 
 int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl, int wr,
                              const void* LT, const void* S, const void* RT, const void* theta,
                              void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
                              tnh_stream_t stream) {
-  return tnh_apply_two_site_stages(dtype, io_dtype, a, d1, d2, b, wl, wr, LT, S, RT, theta, out, ws,
-                                   ws_bytes, tile1, tile3, 7, stream);
+  return tnh_apply_two_site_batched(dtype, io_dtype, 1, a, d1, d2, b, wl, wr, LT, 0, S, 0, RT, 0,
+                                    theta, 0, out, 0, ws, ws_bytes, tile1, tile3, 7, stream);
 }
 
 size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr) {
@@ -508,30 +580,56 @@ size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr) {
          align_up((size_t)wl * wr * d * d * e, 256) + 256;
 }
 
+int tnh_update_left_env_batched(int dtype, int batch, int a, int d, int c, int wl, int wr,
+                                const void* L, long sL, const void* A, long sA, const void* W,
+                                long sW, void* out, long sout, void* ws, size_t ws_bytes,
+                                tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
+              "tnh_update_left_env: sizes must be positive");
+  TNH_REQUIRE(batch >= 1 && batch <= 65535, TNH_ERR_ARG, "tnh_update_left_env: batch %d", batch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define TNH_LE(T) left_env<T>(batch, a, d, c, wl, wr, L, sL, A, sA, W, sW, out, sout, ws, ws_bytes, st)
+  switch (dtype) {
+    case TNH_F64: return TNH_LE(double);
+    case TNH_F32: return TNH_LE(float);
+    case TNH_C128: return TNH_LE(c128);
+    case TNH_C64: return TNH_LE(c64);
+    default: set_error("tnh_update_left_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+  }
+#undef TNH_LE
+}
+
 int tnh_update_left_env(int dtype, int a, int d, int c, int wl, int wr, const void* L,
                         const void* A, const void* W, void* out, void* ws, size_t ws_bytes,
                         tnh_stream_t stream) {
-  TNH_REQUIRE(a > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
-              "tnh_update_left_env: sizes must be positive");
+  return tnh_update_left_env_batched(dtype, 1, a, d, c, wl, wr, L, 0, A, 0, W, 0, out, 0, ws,
+                                     ws_bytes, stream);
+}
+
+int tnh_update_right_env_batched(int dtype, int batch, int c, int d, int b, int wl, int wr,
+                                 const void* R, long sR, const void* B, long sB, const void* W,
+                                 long sW, void* out, long sout, void* ws, size_t ws_bytes,
+                                 tnh_stream_t stream) {
+  TNH_REQUIRE(b > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
+              "tnh_update_right_env: sizes must be positive");
+  TNH_REQUIRE(batch >= 1 && batch <= 65535, TNH_ERR_ARG, "tnh_update_right_env: batch %d", batch);
   hipStream_t st = static_cast<hipStream_t>(stream);
+#define TNH_RE(T) right_env<T>(batch, c, d, b, wl, wr, R, sR, B, sB, W, sW, out, sout, ws, ws_bytes, st)
   switch (dtype) {
-    case TNH_F64: return left_env<double>(a, d, c, wl, wr, L, A, W, out, ws, ws_bytes, st);
-    case TNH_F32: return left_env<float>(a, d, c, wl, wr, L, A, W, out, ws, ws_bytes, st);
-    default: set_error("tnh_update_left_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
+    case TNH_F64: return TNH_RE(double);
+    case TNH_F32: return TNH_RE(float);
+    case TNH_C128: return TNH_RE(c128);
+    case TNH_C64: return TNH_RE(c64);
+    default: set_error("tnh_update_right_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
   }
+#undef TNH_RE
 }
 
 int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const void* R,
                          const void* B, const void* W, void* out, void* ws, size_t ws_bytes,
                          tnh_stream_t stream) {
-  TNH_REQUIRE(b > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
-              "tnh_update_right_env: sizes must be positive");
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  switch (dtype) {
-    case TNH_F64: return right_env<double>(c, d, b, wl, wr, R, B, W, out, ws, ws_bytes, st);
-    case TNH_F32: return right_env<float>(c, d, b, wl, wr, R, B, W, out, ws, ws_bytes, st);
-    default: set_error("tnh_update_right_env: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
-  }
+  return tnh_update_right_env_batched(dtype, 1, c, d, b, wl, wr, R, 0, B, 0, W, 0, out, 0, ws,
+                                      ws_bytes, stream);
 }
 
 }  // extern "C"

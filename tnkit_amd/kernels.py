@@ -15,9 +15,11 @@ from . import _lib
 from ._lib import check, dcode, ptr
 
 _REAL = (torch.float64, torch.float32)
+_ANY = (torch.float64, torch.float32, torch.complex128, torch.complex64)
+_LOW = {torch.float64: torch.float32, torch.complex128: torch.complex64}
 
 
-def _need(t: torch.Tensor, name: str, ndim: int, dtypes=_REAL) -> None:
+def _need(t: torch.Tensor, name: str, ndim: int, dtypes=_ANY) -> None:
     if not isinstance(t, torch.Tensor):
         raise TypeError(f"{name}: expected a torch tensor, got {type(t).__name__}")
     if t.device.type != "cuda":
@@ -87,7 +89,7 @@ def fused_two_site(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch
                          f"W1{tuple(W1.shape)} W2{tuple(W2.shape)} R{tuple(R.shape)}")
     dt = dtype or L.dtype
     if dt != L.dtype:
-        L, W1, W2, R = (x.to(dt) for x in (L, W1, W2, R))
+        L, R = L.to(dt), R.to(dt)
     W1, W2 = W1.to(dt).contiguous(), W2.to(dt).contiguous()
     dev = L.device
     LT = torch.empty((a, a, wl), dtype=dt, device=dev)
@@ -117,7 +119,7 @@ def apply_two_site_fused(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor,
                          f"S{tuple(S.shape)} RT{tuple(RT.shape)} do not match")
     if not (LT.dtype == S.dtype == RT.dtype):
         raise TypeError("apply_two_site_fused: LT, S, RT must share a dtype")
-    if theta.dtype != LT.dtype and not (theta.dtype == torch.float64 and LT.dtype == torch.float32):
+    if theta.dtype != LT.dtype and _LOW.get(theta.dtype) != LT.dtype:
         raise TypeError(f"apply_two_site_fused: theta {theta.dtype} with operator {LT.dtype}")
     if out is None:
         out = torch.empty_like(theta)
@@ -260,4 +262,136 @@ def scale_by(x: torch.Tensor, div: Optional[torch.Tensor], mul: float, out: torc
     _need(out, "out", 1, (torch.float64,))
     check(_lib.load().tnh_scale_by(x.numel(), ptr(x), ptr(div) if div is not None else None,
                                    float(mul), ptr(out), stream(x.device)), "scale_by")
+    return out
+
+
+# ---------------------------------------------------------------------------------------------
+# batched forms: a leading leg axis (tnkit/batch_tdvp.py:211-243)
+# ---------------------------------------------------------------------------------------------
+
+
+def _bstride(t: torch.Tensor) -> int:
+    return int(t.stride(0)) if t.shape[0] > 1 else int(t[0].numel())
+
+
+def _need_b(t: torch.Tensor, name: str, ndim: int) -> None:
+    """A batched operand: every leg contiguous, legs at a fixed stride (a row-strided view of a
+    Krylov basis qualifies)."""
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dim() != ndim:
+        raise ValueError(f"{name}: expected a {ndim}-axis tensor on the HIP device")
+    if t.dtype not in _ANY:
+        raise TypeError(f"{name}: dtype {t.dtype} not supported")
+    if not t[0].is_contiguous() or (t.shape[0] > 1 and t.stride(0) < t[0].numel()):
+        raise ValueError(f"{name}: each leg must be contiguous and legs must not overlap")
+
+
+def fused_two_site_b(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch.Tensor,
+                     dtype: Optional[torch.dtype] = None):
+    """`fused_two_site` for k legs stacked on axis 0: L (k, a, wl, a), W1 (k, wl, wm, d1, d1),
+    W2 (k, wm, wr, d2, d2), R (k, b, wr, b)."""
+    for t, n, nd in ((L, "L", 4), (W1, "W1", 5), (W2, "W2", 5), (R, "R", 4)):
+        _need(t, n, nd)
+    k, a, wl, _ = L.shape
+    _, b, wr, _ = R.shape
+    _, _, wm, d1, _ = W1.shape
+    d2 = W2.shape[3]
+    if not (W1.shape[0] == W2.shape[0] == R.shape[0] == k):
+        raise ValueError("fused_two_site_b: the leg axes differ")
+    dt = dtype or L.dtype
+    if dt != L.dtype:
+        L, R = L.to(dt), R.to(dt)
+    W1, W2 = W1.to(dt).contiguous(), W2.to(dt).contiguous()
+    dev = L.device
+    LT = torch.empty((k, a, a, wl), dtype=dt, device=dev)
+    S = torch.empty((k, d1 * d2 * wr, wl * d1 * d2), dtype=dt, device=dev)
+    RT = torch.empty((k, wr, b, b), dtype=dt, device=dev)
+    check(_lib.load().tnh_fused_two_site_batched(
+        dcode(dt), k, a, wl, wm, wr, d1, d2, b, ptr(L), _bstride(L), ptr(W1), _bstride(W1),
+        ptr(W2), _bstride(W2), ptr(R), _bstride(R), ptr(LT), _bstride(LT), ptr(S), _bstride(S),
+        ptr(RT), _bstride(RT), stream(dev)), "fused_two_site_b")
+    return LT, S, RT
+
+
+def apply_two_site_b(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor, RT: torch.Tensor,
+                     out: Optional[torch.Tensor] = None, ws: Optional[torch.Tensor] = None):
+    """H_eff theta for k legs: theta (k, a, d1, d2, b) with the operator of `fused_two_site_b`."""
+    _need_b(theta, "theta", 5)
+    k, q, d1, d2, b = theta.shape
+    if LT.shape[:2] != (k, q) or RT.shape[0] != k or RT.shape[2] != b:
+        raise ValueError("apply_two_site_b: operator and theta do not match")
+    if theta.dtype != LT.dtype and _LOW.get(theta.dtype) != LT.dtype:
+        raise TypeError(f"apply_two_site_b: theta {theta.dtype} with operator {LT.dtype}")
+    wl, wr = LT.shape[3], RT.shape[1]
+    lib = _lib.load()
+    dt, io = dcode(LT.dtype), dcode(theta.dtype)
+    nbytes = k * lib.tnh_two_site_ws_bytes(dt, q, d1, d2, b, wl, wr)
+    if ws is None or ws.numel() < nbytes:
+        ws = _ws(nbytes, theta.device)
+    if out is None:
+        out = torch.empty(theta.shape, dtype=theta.dtype, device=theta.device)
+    check(lib.tnh_apply_two_site_batched(
+        dt, io, k, q, d1, d2, b, wl, wr, ptr(LT), _bstride(LT), ptr(S), _bstride(S), ptr(RT),
+        _bstride(RT), ptr(theta), _bstride(theta), ptr(out), _bstride(out), ptr(ws), ws.numel(), 0, 0,
+        7, stream(theta.device)), "apply_two_site_b")
+    return out
+
+
+def update_left_env_b(L: torch.Tensor, A: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """update_left_env for k legs: L (k, a, wl, a), A (k, a, d, c), W (k, wl, wr, d, d)."""
+    _need(L, "L", 4)
+    _need(A, "A", 4)
+    _need(W, "W", 5)
+    k, a, wl, _ = L.shape
+    _, _, d, c = A.shape
+    wr = W.shape[2]
+    lib = _lib.load()
+    dt = dcode(L.dtype)
+    out = torch.empty((k, c, wr, c), dtype=L.dtype, device=L.device)
+    ws = _ws(k * lib.tnh_env_ws_bytes(dt, a, d, c, wl, wr), L.device)
+    check(lib.tnh_update_left_env_batched(dt, k, a, d, c, wl, wr, ptr(L), _bstride(L), ptr(A),
+                                          _bstride(A), ptr(W), _bstride(W), ptr(out),
+                                          _bstride(out), ptr(ws), ws.numel(), stream(L.device)),
+          "update_left_env_b")
+    return out
+
+
+def update_right_env_b(R: torch.Tensor, B: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """update_right_env for k legs: R (k, b, wr, b), B (k, c, d, b), W (k, wl, wr, d, d)."""
+    _need(R, "R", 4)
+    _need(B, "B", 4)
+    _need(W, "W", 5)
+    k, b, wr, _ = R.shape
+    _, c, d, _ = B.shape
+    wl = W.shape[1]
+    lib = _lib.load()
+    dt = dcode(R.dtype)
+    out = torch.empty((k, c, wl, c), dtype=R.dtype, device=R.device)
+    ws = _ws(k * lib.tnh_env_ws_bytes(dt, b, d, c, wl, wr), R.device)
+    check(lib.tnh_update_right_env_batched(dt, k, c, d, b, wl, wr, ptr(R), _bstride(R), ptr(B),
+                                           _bstride(B), ptr(W), _bstride(W), ptr(out),
+                                           _bstride(out), ptr(ws), ws.numel(), stream(R.device)),
+          "update_right_env_b")
+    return out
+
+
+def expm_step_c(V: torch.Tensor, w: torch.Tensor, ab: torch.Tensor, j: int, ws: torch.Tensor):
+    """Step j of the batched complex Krylov exponential (see tnh_expm_step_c): V (k, m, n),
+    w (k, n) complex128, ab (k, 2m) float64 on the device."""
+    k, m, n = V.shape
+    check(_lib.load().tnh_expm_step_c(k, n, m, j, ptr(V), V.stride(1), V.stride(0), ptr(w),
+                                      w.stride(0), ptr(ab), ab.stride(0), ptr(ws), ws.numel(),
+                                      stream(V.device)), "expm_step_c")
+
+
+def expm_ws(k: int, device) -> torch.Tensor:
+    return _ws(_lib.load().tnh_expm_ws_bytes(k), device)
+
+
+def lincomb_c(V: torch.Tensor, c: torch.Tensor, out: torch.Tensor, nrows: int) -> torch.Tensor:
+    """out[leg] = sum_{r<nrows} c[leg, r] V[leg, r] for complex128 V (k, m, n), c (k, m)."""
+    k, m, n = V.shape
+    cr = torch.view_as_real(c.contiguous())
+    check(_lib.load().tnh_lincomb_c(k, n, nrows, ptr(V), V.stride(1), V.stride(0), ptr(cr),
+                                    cr.stride(0), ptr(out), out.stride(0), stream(V.device)),
+          "lincomb_c")
     return out

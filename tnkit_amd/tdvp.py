@@ -1,0 +1,363 @@
+"""Two-site TDVP on the HIP device, single (`tdvp2_evolve`) and batched over k trajectories
+(`evolve_batch`), after tnkit/tdvp.py:555-800 and tnkit/batch_tdvp.py:572-1002.
+
+One implementation serves both: a trajectory is a batch of one. The k legs are stacked on a
+leading axis and zero-padded to the widest bond of each position ('pad' in tnkit's terms); every
+contraction is one batched libtnkit_hip launch for all legs (`kernels.*_b`), the Krylov
+exponential keeps its scalars on the device and reads them back once per solve, and the
+truncation is one batched Gram eigendecomposition with per-leg ranks and masks, so padding
+stays exact zeros. Per-leg bond caps (`chi_max` as a list) extend tnkit's single cap.
+
+The local step (tnkit/tdvp.py:555-650): theta = M[i] M[i+1]; theta <- exp(-i H2 dt/2) theta;
+cut to the leg's rank; renormalise S; update the environment; back-evolve the one-site centre
+with exp(+i H1 dt/2) unless it sits at the chain end. Sweep right then left (2nd-order Strang).
+"""
+from __future__ import annotations
+
+import dataclasses
+from typing import Any, Callable, Dict, List, Optional, Sequence, Tuple, Union
+
+import numpy as np
+import torch
+
+from . import kernels
+from .mps import MPS
+
+KRYLOV_TOL_DEFAULT = 1e-10
+
+
+class BatchGeometryError(ValueError):
+    """The legs are not one program on different data (tnkit/batch_tdvp.py:93-101)."""
+
+
+class BatchKrylovError(RuntimeError):
+    """A leg's Krylov residual exceeds the tolerance at the largest allowed m
+    (tnkit/batch_tdvp.py:104-116)."""
+
+
+@dataclasses.dataclass
+class BatchTDVPOptions:
+    """Settings of `evolve_batch` (tnkit/batch_tdvp.py:134-201). Served: evolver '2tdvp',
+    ragged 'pad', expm_where 'eigh'; `chi_max` may be one cap or one per leg."""
+
+    backend: Any = "hip"
+    evolver: str = "2tdvp"
+    krylov_m: Union[int, str] = 20
+    krylov_tol: float = KRYLOV_TOL_DEFAULT
+    krylov_m_max: Optional[int] = None
+    expm_where: str = "eigh"
+    ragged: str = "pad"
+    chi_max: Optional[Union[int, Sequence[int]]] = None
+    svd_tol: float = 0.0
+    renorm: bool = True
+    canonicalize: bool = True
+    keep_device: bool = False
+    measure: Optional[Callable[[Any], Any]] = None
+    meas_every: int = 1
+    stats: Optional[Dict[str, Any]] = None
+    low_precision: bool = False        # complex64 matvec with complex128 Krylov vectors
+
+
+# --------------------------------------------------------------------------------------------
+# the batched Krylov exponential
+# --------------------------------------------------------------------------------------------
+
+
+def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
+                               resid: Optional[np.ndarray] = None) -> torch.Tensor:
+    """exp(tau H_leg) v_leg for k legs at once, from a fixed m Krylov steps
+    (tnkit/batch_tdvp.py:247-324, tnkit/eigensolver.py:95-217): CGS2 with alpha from the first
+    projection, clamped division instead of a breakdown branch, exp of the k tridiagonals by
+    their eigendecompositions. `v` is (k, n) complex128. `resid` (length k) receives each leg's
+    Saad a-posteriori error estimate |beta_m tau [e^{tau T}]_{m-1,0}| beta0."""
+    k, n = v.shape
+    m = max(1, min(int(m), n))
+    dev = v.device
+    V = torch.zeros((k, m, n), dtype=torch.complex128, device=dev)
+    ab = torch.zeros((k, 2 * m), dtype=torch.float64, device=dev)
+    beta0 = torch.linalg.vector_norm(v, dim=1)
+    V[:, 0] = v / torch.clamp(beta0, min=1e-200)[:, None]
+    ws = kernels.expm_ws(k, dev)
+    for j in range(m):
+        w = matvec_b(V[:, j]).reshape(k, n).to(torch.complex128).contiguous()
+        kernels.expm_step_c(V, w, ab, j, ws)
+    abh = ab.cpu().numpy()
+    b0 = beta0.cpu().numpy()
+    al, be = abh[:, :m], abh[:, m:]
+    T = np.zeros((k, m, m))
+    idx = np.arange(m)
+    T[:, idx, idx] = al
+    if m > 1:
+        T[:, idx[:-1], idx[1:]] = be[:, :m - 1]
+        T[:, idx[1:], idx[:-1]] = be[:, :m - 1]
+    lam, Z = np.linalg.eigh(T)
+    E = np.einsum("kij,kj,klj->kil", Z, np.exp(tau * lam), Z)   # (k, m, m)
+    c = b0[:, None] * E[:, :, 0]
+    if resid is not None:
+        resid[:] = np.maximum(resid, np.abs(be[:, m - 1] * tau * E[:, m - 1, 0]) * b0)
+    out = torch.empty((k, n), dtype=torch.complex128, device=dev)
+    kernels.lincomb_c(V, torch.as_tensor(c).to(dev), out, m)
+    return out
+
+
+# --------------------------------------------------------------------------------------------
+# batch state
+# --------------------------------------------------------------------------------------------
+
+
+def _pad3(t: torch.Tensor, a: int, b: int) -> torch.Tensor:
+    out = torch.zeros((a, t.shape[1], b), dtype=t.dtype, device=t.device)
+    out[:t.shape[0], :, :t.shape[2]] = t
+    return out
+
+
+class _Batch:
+    """k legs on a leading axis, padded to the widest bond at each position, with the true
+    per-leg ranks (tnkit/batch_tdvp.py:403-496)."""
+
+    def __init__(self, states, mpos, dev, caps):
+        self.k = len(states)
+        self.N = len(states[0])
+        self.d = int(states[0][0].shape[1])
+        self.caps = np.asarray(caps, dtype=np.int64)
+        ranks = np.array([[1] + [int(t.shape[2]) for t in s[:-1]] + [1] for s in states])
+        self.rank = [ranks[:, i] for i in range(self.N + 1)]
+        self.dims = [int(ranks[:, i].max()) for i in range(self.N + 1)]
+        self.M = []
+        for i in range(self.N):
+            self.M.append(torch.stack([_pad3(s[i], self.dims[i], self.dims[i + 1])
+                                       for s in states]).contiguous())
+        self.W = [torch.stack([w[i] for w in mpos]).contiguous() for i in range(self.N)]
+        self.L: List[Optional[torch.Tensor]] = [None] * (self.N + 1)
+        self.R: List[Optional[torch.Tensor]] = [None] * (self.N + 1)
+        one = torch.ones((self.k, 1, 1, 1), dtype=torch.complex128, device=dev)
+        self.L[0], self.R[self.N] = one, one.clone()
+
+    def mask(self, bond: int, nk: int) -> torch.Tensor:
+        r = torch.as_tensor(self.rank[bond], device=self.M[0].device)
+        return (torch.arange(nk, device=r.device)[None, :] < r[:, None])
+
+
+def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
+    """Right-canonical form of one complex MPS by LQ per site, centre at 0, normalised."""
+    M = [t.clone() for t in M]
+    for i in range(len(M) - 1, 0, -1):
+        Dl, d, Dr = M[i].shape
+        Q, Rr = torch.linalg.qr(M[i].reshape(Dl, d * Dr).mT)
+        M[i] = Q.mT.reshape(-1, d, Dr).contiguous()
+        M[i - 1] = torch.tensordot(M[i - 1], Rr.mT, dims=([2], [0])).contiguous()
+        M[i - 1] = M[i - 1] / torch.linalg.norm(M[i - 1])
+    M[0] = M[0] / torch.linalg.norm(M[0])
+    return M
+
+
+def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
+              iso: str):
+    """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
+    the fewest values whose relative dropped weight stays under svd_tol, capped by the leg's
+    cap and structural ceiling (tnkit/batch_tdvp.py:572-599). Returns U (k, m, nk), S (k, nk),
+    Vh (k, nk, n), ranks, relative discarded weight per leg. The `iso` factor is an exact
+    isometry on each leg's kept columns (tnkit/linalg.py:162-204)."""
+    k, m, n = mat.shape
+    if iso == "right":
+        U, S, Vh, ranks, disc = _gram_cut(mat.mH.contiguous(), ranks_cap, ceiling, svd_tol, "left")
+        return Vh.mH, S, U.mH, ranks, disc
+    G = torch.matmul(mat, mat.mH.contiguous())          # (k, m, m), A A^H
+    G = 0.5 * (G + G.mH)
+    w, Z = torch.linalg.eigh(G)                         # ascending
+    w = torch.flip(w, dims=[1]).clamp(min=0.0)
+    Z = torch.flip(Z, dims=[2])
+    wh = w.cpu().numpy()
+    tot = wh.sum(axis=1)
+    tot = np.where(tot > 0, tot, 1.0)
+    tail = np.cumsum(wh[:, ::-1], axis=1)[:, ::-1]
+    ranks = np.empty(k, dtype=np.int64)
+    disc = np.empty(k)
+    for leg in range(k):
+        lim = max(1, min(int(ranks_cap[leg]), int(ceiling[leg]), m))
+        drop = np.append(tail[leg] / tot[leg], 0.0)
+        ok = np.nonzero(drop[1:lim + 1] <= svd_tol)[0]
+        ranks[leg] = int(ok[0]) + 1 if len(ok) else lim
+        disc[leg] = float(drop[ranks[leg]])
+    nk = int(ranks.max())
+    U = Z[:, :, :nk]
+    mk = torch.arange(nk, device=mat.device)[None, :] < torch.as_tensor(ranks, device=mat.device)[:, None]
+    U = U * mk[:, None, :]
+    C = torch.matmul(U.mH.contiguous(), mat)             # (k, nk, n) carried factor
+    S = torch.linalg.vector_norm(C, dim=2)
+    Vh = C / torch.where(S > 0, S, torch.ones_like(S))[:, :, None]
+    return U, S.to(torch.float64), Vh, ranks, disc
+
+
+# --------------------------------------------------------------------------------------------
+# the sweep
+# --------------------------------------------------------------------------------------------
+
+
+def _matvec_two(Bt: _Batch, i: int, shape, low: bool):
+    op = kernels.fused_two_site_b(Bt.L[i], Bt.W[i], Bt.W[i + 1], Bt.R[i + 2],
+                                  dtype=torch.complex64 if low else None)
+    ws = [None]
+
+    def mv(x):
+        y = kernels.apply_two_site_b(x.reshape(shape), *op, ws=ws[0])
+        return y.reshape(shape[0], -1)
+    return mv
+
+
+def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool):
+    """One-site H_eff (tnkit/_dmrg.py:192-199) as the two-site kernel with d2 = 1 and the
+    identity of the right MPO bond as the second MPO tensor: S[(s r)][(w s')] = W[w][r][s][s']."""
+    k, a, d, b = shape
+    wr = int(W.shape[2])
+    Wid = torch.eye(wr, dtype=W.dtype, device=W.device).reshape(1, wr, wr, 1, 1)
+    Wid = Wid.expand(k, -1, -1, -1, -1).contiguous()
+    op = kernels.fused_two_site_b(L, W, Wid, R, dtype=torch.complex64 if low else None)
+
+    def mv(x):
+        y = kernels.apply_two_site_b(x.reshape(k, a, d, 1, b), *op)
+        return y.reshape(k, -1)
+    return mv
+
+
+def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, moving: str,
+          resid: np.ndarray, low: bool) -> np.ndarray:
+    k, d = Bt.k, Bt.d
+    A, Bm = Bt.M[i], Bt.M[i + 1]
+    a, b = A.shape[1], Bm.shape[3]
+    theta = torch.matmul(A.reshape(k, a * d, -1), Bm.reshape(k, Bm.shape[1], -1))
+    shape = (k, a, d, d, b)
+    theta = lanczos_expm_fixed_batched(_matvec_two(Bt, i, shape, low), theta.reshape(k, -1),
+                                       -0.5j * dt, m, resid)
+    mat = theta.reshape(k, a * d, d * b)
+    ceiling = np.minimum(Bt.rank[i] * d, Bt.rank[i + 2] * d)
+    U, S, Vh, ranks, disc = _gram_cut(mat, Bt.caps, ceiling, svd_tol,
+                                      "left" if moving == "right" else "right")
+    nk = U.shape[2]
+    Bt.rank[i + 1] = ranks
+    Bt.dims[i + 1] = nk
+    if renorm:
+        S = S / torch.linalg.vector_norm(S, dim=1, keepdim=True)
+    Sc = S.to(torch.complex128)
+    if moving == "right":
+        Bt.M[i] = U.reshape(k, a, d, nk).contiguous()
+        Bt.L[i + 1] = kernels.update_left_env_b(Bt.L[i], Bt.M[i], Bt.W[i])
+        C = (Sc[:, :, None] * Vh).reshape(k, nk, d, b).contiguous()
+        if i < Bt.N - 2:
+            C = lanczos_expm_fixed_batched(
+                _matvec_one(Bt, Bt.L[i + 1], Bt.W[i + 1], Bt.R[i + 2], (k, nk, d, b), low),
+                C.reshape(k, -1), 0.5j * dt, m, resid).reshape(k, nk, d, b)
+        Bt.M[i + 1] = C.contiguous()
+    else:
+        Bt.M[i + 1] = Vh.reshape(k, nk, d, b).contiguous()
+        Bt.R[i + 1] = kernels.update_right_env_b(Bt.R[i + 2], Bt.M[i + 1], Bt.W[i + 1])
+        C = (U * Sc[:, None, :]).reshape(k, a, d, nk).contiguous()
+        if i > 0:
+            C = lanczos_expm_fixed_batched(
+                _matvec_one(Bt, Bt.L[i], Bt.W[i], Bt.R[i + 1], (k, a, d, nk), low),
+                C.reshape(k, -1), 0.5j * dt, m, resid).reshape(k, a, d, nk)
+        Bt.M[i] = C.contiguous()
+    return disc
+
+
+def _trim(Bt: _Batch, leg: int) -> MPS:
+    out = []
+    for i in range(Bt.N):
+        a, b = int(Bt.rank[i][leg]), int(Bt.rank[i + 1][leg])
+        out.append(Bt.M[i][leg, :a, :, :b].contiguous())
+    return MPS(out, center=0)
+
+
+def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps: int,
+                 options: Optional[Union[BatchTDVPOptions, Dict[str, Any]]] = None,
+                 **kwargs: Any) -> Tuple[List[MPS], List[Any]]:
+    """Evolve k MPS of one geometry as one batched program (tnkit/batch_tdvp.py:721-857).
+
+    `hams` is one MPO for every leg or a list of k MPOs with the same bond structure. Returns
+    (states, records): each leg trimmed to its own bonds, and [(t, measure(states_t)), ...]."""
+    if options is None:
+        o = BatchTDVPOptions(**kwargs)
+    elif isinstance(options, dict):
+        o = BatchTDVPOptions(**{**options, **kwargs})
+    else:
+        o = dataclasses.replace(options, **kwargs) if kwargs else options
+    if o.evolver != "2tdvp":
+        raise ValueError(f"evolver {o.evolver!r}: the HIP batch serves '2tdvp'")
+    if o.ragged != "pad":
+        raise ValueError(f"ragged {o.ragged!r}: the HIP batch serves 'pad'")
+    if o.chi_max is None:
+        raise ValueError("'2tdvp' needs chi_max")
+    if not isinstance(o.krylov_m, (int, np.integer)):
+        raise ValueError("krylov_m must be an int on the HIP backend")
+    k = len(states)
+    if k == 0:
+        return [], []
+    N = len(states[0])
+    if any(len(s) != N for s in states):
+        raise BatchGeometryError("legs differ in their number of sites")
+    mpos = hams if (isinstance(hams, (list, tuple)) and len(hams) == k
+                    and isinstance(hams[0], (list, tuple))) else [hams] * k
+    if len(mpos) != k or any(len(h) != N for h in mpos):
+        raise BatchGeometryError("one MPO per leg, each with one tensor per site")
+    for i in range(N):
+        if len({tuple(h[i].shape) for h in mpos}) != 1:
+            raise BatchGeometryError(f"MPO bond structure differs between legs at site {i}")
+    caps = (np.full(k, int(o.chi_max)) if np.isscalar(o.chi_max)
+            else np.asarray(list(o.chi_max), dtype=np.int64))
+    if len(caps) != k:
+        raise ValueError("chi_max: one cap, or one per leg")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    c = lambda x: torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x) \
+        .to(dev, torch.complex128)  # noqa: E731
+    st = [[c(t) for t in s] for s in states]
+    if o.canonicalize:
+        st = [_right_canonical(s) for s in st]
+    Bt = _Batch(st, [[c(w) for w in h] for h in mpos], dev, caps)
+    for i in range(N - 1, 0, -1):
+        Bt.R[i] = kernels.update_right_env_b(Bt.R[i + 1], Bt.M[i], Bt.W[i])
+    m = int(o.krylov_m)
+    m_max = int(o.krylov_m_max) if o.krylov_m_max is not None else 2 * m
+    records = []
+
+    def snapshot():
+        legs = [_trim(Bt, j) for j in range(k)]
+        return legs if o.keep_device else [MPS([t.cpu().numpy() for t in s], 0) for s in legs]
+
+    if o.measure is not None:
+        records.append((0.0, o.measure(snapshot())))
+    disc_tot = np.zeros(k)
+    resid_max = np.zeros(k)
+    for step in range(n_steps):
+        resid = np.zeros(k)
+        for i in range(0, N - 1):
+            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "right", resid, o.low_precision)
+        for i in range(N - 2, -1, -1):
+            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "left", resid, o.low_precision)
+        resid_max = np.maximum(resid_max, resid)
+        bad = np.nonzero(resid > o.krylov_tol)[0]
+        if len(bad):
+            if m >= m_max:
+                raise BatchKrylovError(f"leg {int(bad[0])}: Krylov residual {resid[bad[0]]:.2e} "
+                                       f"above krylov_tol={o.krylov_tol:g} at krylov_m={m} "
+                                       f"(krylov_m_max={m_max})")
+            m = min(m_max, m + 2)
+        if o.measure is not None and (step + 1) % o.meas_every == 0:
+            records.append(((step + 1) * dt, o.measure(snapshot())))
+    if o.stats is not None:
+        o.stats.update({"krylov_m": m, "resid_max": resid_max.tolist(),
+                        "disc_per_leg": disc_tot.tolist(), "bonds": [list(map(int, r))
+                                                                     for r in Bt.rank]})
+    return [_trim(Bt, j) for j in range(k)], records
+
+
+def tdvp2_evolve(mps: Sequence[Any], mpo: Sequence[Any], chi: int, dt: float, n_steps: int,
+                 krylov_m: int = 20, measure: Optional[Callable] = None, meas_every: int = 1,
+                 svd_tol: float = 0.0, renorm: bool = True, canonicalize: bool = True,
+                 low_precision: bool = False) -> Tuple[MPS, List[Tuple[float, Any]]]:
+    """Two-site TDVP of one state (tnkit/tdvp.py:653-800): a batch of one."""
+    meas = None if measure is None else (lambda legs: measure(legs[0]))
+    out, rec = evolve_batch([list(mps)], [list(mpo)], dt, n_steps, chi_max=int(chi),
+                            krylov_m=krylov_m, measure=meas, meas_every=meas_every,
+                            svd_tol=svd_tol, renorm=renorm, canonicalize=canonicalize,
+                            low_precision=low_precision, keep_device=True)
+    return out[0], rec
