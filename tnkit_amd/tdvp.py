@@ -153,7 +153,7 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
 
 def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
               iso: str):
-    """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
+    """Batched truncation of mat (k, m, n) by the Gram matrix A A^H: per-leg rank =
     the fewest values whose relative dropped weight stays under svd_tol, capped by the leg's
     cap and structural ceiling (tnkit/batch_tdvp.py:572-599). Returns U (k, m, nk), S (k, nk),
     Vh (k, nk, n), ranks, relative discarded weight per leg. The `iso` factor is an exact
@@ -176,8 +176,13 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     for leg in range(k):
         lim = max(1, min(int(ranks_cap[leg]), int(ceiling[leg]), m))
         drop = np.append(tail[leg] / tot[leg], 0.0)
-        ok = np.nonzero(drop[1:lim + 1] <= svd_tol)[0]
-        ranks[leg] = int(ok[0]) + 1 if len(ok) else lim
+        if svd_tol > 0.0:
+            ok = np.nonzero(drop[1:lim + 1] <= svd_tol)[0]
+            ranks[leg] = int(ok[0]) + 1 if len(ok) else lim
+        else:
+            # no budget: keep the cap. Gram eigenvalues of Schmidt values below ~1e-8 round to
+            # zero, and a zero-weight test on them would drop directions an SVD keeps.
+            ranks[leg] = lim
         disc[leg] = float(drop[ranks[leg]])
     nk = int(ranks.max())
     U = Z[:, :, :nk]
