@@ -153,7 +153,7 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
 
 def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
               iso: str):
-    """Batched truncation of mat (k, m, n) by the Gram matrix A A^H: per-leg rank =
+    """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
     the fewest values whose relative dropped weight stays under svd_tol, capped by the leg's
     cap and structural ceiling (tnkit/batch_tdvp.py:572-599). Returns U (k, m, nk), S (k, nk),
     Vh (k, nk, n), ranks, relative discarded weight per leg. The `iso` factor is an exact
@@ -162,11 +162,14 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     if iso == "right":
         U, S, Vh, ranks, disc = _gram_cut(mat.mH.contiguous(), ranks_cap, ceiling, svd_tol, "left")
         return Vh.mH, S, U.mH, ranks, disc
-    G = torch.matmul(mat, mat.mH.contiguous())          # (k, m, m), A A^H
+    tall = m > n                                        # the smaller Gram side, as tnkit does
+    G = (torch.matmul(mat.mH.contiguous(), mat) if tall
+         else torch.matmul(mat, mat.mH.contiguous()))   # (k, r, r)
     G = 0.5 * (G + G.mH)
     w, Z = torch.linalg.eigh(G)                         # ascending
     w = torch.flip(w, dims=[1]).clamp(min=0.0)
     Z = torch.flip(Z, dims=[2])
+    r = min(m, n)
     wh = w.cpu().numpy()
     tot = wh.sum(axis=1)
     tot = np.where(tot > 0, tot, 1.0)
@@ -174,7 +177,7 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     ranks = np.empty(k, dtype=np.int64)
     disc = np.empty(k)
     for leg in range(k):
-        lim = max(1, min(int(ranks_cap[leg]), int(ceiling[leg]), m))
+        lim = max(1, min(int(ranks_cap[leg]), int(ceiling[leg]), r))
         drop = np.append(tail[leg] / tot[leg], 0.0)
         if svd_tol > 0.0:
             ok = np.nonzero(drop[1:lim + 1] <= svd_tol)[0]
@@ -185,10 +188,15 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
             ranks[leg] = lim
         disc[leg] = float(drop[ranks[leg]])
     nk = int(ranks.max())
-    U = Z[:, :, :nk]
     mk = torch.arange(nk, device=mat.device)[None, :] < torch.as_tensor(ranks, device=mat.device)[:, None]
-    U = U * mk[:, None, :]
-    C = torch.matmul(U.mH.contiguous(), mat)             # (k, nk, n) carried factor
+    Zk = Z[:, :, :nk] * mk[:, None, :]
+    if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
+        Q, Rr = torch.linalg.qr(torch.matmul(mat, Zk))
+        U = Q * mk[:, None, :]
+        C = torch.matmul(Rr, Zk.mH.contiguous())        # (k, nk, n) carried factor
+    else:
+        U = Zk
+        C = torch.matmul(U.mH.contiguous(), mat)
     S = torch.linalg.vector_norm(C, dim=2)
     Vh = C / torch.where(S > 0, S, torch.ones_like(S))[:, :, None]
     return U, S.to(torch.float64), Vh, ranks, disc
