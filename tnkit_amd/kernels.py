@@ -30,6 +30,9 @@ def _need(t: torch.Tensor, name: str, ndim: int, dtypes=_ANY) -> None:
         raise TypeError(f"{name}: dtype {t.dtype} not supported here ({dtypes})")
     if not t.is_contiguous():
         raise ValueError(f"{name}: must be contiguous")
+    if t.is_conj() or t.is_neg():
+        raise ValueError(f"{name}: a lazily conjugated/negated view; call .resolve_conj() first "
+                         "(the kernels read memory as stored)")
 
 
 def _ws(nbytes: int, device) -> torch.Tensor:
@@ -283,6 +286,8 @@ def _need_b(t: torch.Tensor, name: str, ndim: int) -> None:
         raise TypeError(f"{name}: dtype {t.dtype} not supported")
     if not t[0].is_contiguous() or (t.shape[0] > 1 and t.stride(0) < t[0].numel()):
         raise ValueError(f"{name}: each leg must be contiguous and legs must not overlap")
+    if t.is_conj() or t.is_neg():
+        raise ValueError(f"{name}: a lazily conjugated/negated view; call .resolve_conj() first")
 
 
 def fused_two_site_b(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch.Tensor,

@@ -160,11 +160,13 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     isometry on each leg's kept columns (tnkit/linalg.py:162-204)."""
     k, m, n = mat.shape
     if iso == "right":
-        U, S, Vh, ranks, disc = _gram_cut(mat.mH.contiguous(), ranks_cap, ceiling, svd_tol, "left")
-        return Vh.mH, S, U.mH, ranks, disc
+        U, S, Vh, ranks, disc = _gram_cut(mat.mH.resolve_conj().contiguous(), ranks_cap, ceiling,
+                                          svd_tol, "left")
+        return (Vh.mH.resolve_conj().contiguous(), S, U.mH.resolve_conj().contiguous(), ranks,
+                disc)
     tall = m > n                                        # the smaller Gram side, as tnkit does
-    G = (torch.matmul(mat.mH.contiguous(), mat) if tall
-         else torch.matmul(mat, mat.mH.contiguous()))   # (k, r, r)
+    G = (torch.matmul(mat.mH.resolve_conj().contiguous(), mat) if tall
+         else torch.matmul(mat, mat.mH.resolve_conj().contiguous()))   # (k, r, r)
     G = 0.5 * (G + G.mH)
     w, Z = torch.linalg.eigh(G)                         # ascending
     w = torch.flip(w, dims=[1]).clamp(min=0.0)
@@ -193,10 +195,10 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
         Q, Rr = torch.linalg.qr(torch.matmul(mat, Zk))
         U = Q * mk[:, None, :]
-        C = torch.matmul(Rr, Zk.mH.contiguous())        # (k, nk, n) carried factor
+        C = torch.matmul(Rr, Zk.mH.resolve_conj().contiguous())        # (k, nk, n) carried factor
     else:
         U = Zk
-        C = torch.matmul(U.mH.contiguous(), mat)
+        C = torch.matmul(U.mH.resolve_conj().contiguous(), mat)
     S = torch.linalg.vector_norm(C, dim=2)
     Vh = C / torch.where(S > 0, S, torch.ones_like(S))[:, :, None]
     return U, S.to(torch.float64), Vh, ranks, disc
