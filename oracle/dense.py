@@ -552,3 +552,48 @@ def local_expectations(M, op):
         out.append(np.einsum("ab,asc,st,btc->", Lenv, A, op, A.conj()).real)
         Lenv = np.einsum("ab,asc,bsd->cd", Lenv, A, A.conj())
     return np.array(out)
+
+
+# --------------------------------------------------------------------------------------------
+# energy and its derivative with respect to the MPO (Hellmann-Feynman checks)
+# --------------------------------------------------------------------------------------------
+
+
+def mpo_energy(M, mpo):
+    """<psi|H|psi> / <psi|psi> by the left environment chain (norm by the same chain with the
+    identity MPO)."""
+    L = np.ones((1, 1, 1))
+    Ln = np.ones((1, 1))
+    for A, W in zip(M, mpo):
+        L = update_left_env(L, A, W)
+        Ln = np.einsum("pq,psd,qse->de", Ln, A.conj(), A, optimize=True)
+    return float(np.real(L.reshape(-1)[0] / Ln.reshape(-1)[0]))
+
+
+def mpo_gradient(L, A, W, R):
+    """d<psi|H|psi>/dW at one site for fixed environments:
+    dW[w,r,s,y] = sum L[p,w,q] conj(A[p,s,d]) A[q,y,e] R[d,r,e] (the site's term of the
+    environment chain, differentiated in W)."""
+    return np.einsum("pwq,psd,qye,dre->wrsy", L, A.conj(), A, R, optimize=True)
+
+
+def field_gradient(M, mpo_fields_fn, g):
+    """dE/dg_i for the per-site-field TFIM, E = <psi|H(g)|psi> / <psi|psi> at a right-canonical
+    state with its centre at site 0 (so <psi|psi> = |M[0]|^2): the MPO gradient at site i
+    contracted with dW_i/dg_i = -Z in the (0, last) channel."""
+    mpo = mpo_fields_fn(g)
+    N = len(mpo)
+    R = [None] * (N + 1)
+    R[N] = np.ones((1, 1, 1))
+    for i in range(N - 1, 0, -1):
+        R[i] = update_right_env(R[i + 1], M[i], mpo[i])
+    Z = np.diag([1.0, -1.0])
+    out = np.zeros(N)
+    L = np.ones((1, 1, 1))
+    norm2 = float(np.real(np.vdot(M[0], M[0])))
+    for i in range(N):
+        dW = mpo_gradient(L, M[i], mpo[i], R[i + 1])
+        wl, wr = mpo[i].shape[:2]
+        out[i] = float(np.real(np.sum(dW[0, wr - 1] * (-Z)))) / norm2
+        L = update_left_env(L, M[i], mpo[i])
+    return out
