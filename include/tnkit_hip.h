@@ -167,6 +167,38 @@ int tnh_scale_by(long n, const double* x, const double* div, double mul, double*
 int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst,
                 tnh_stream_t stream);
 
+/* ---- U(1) block-sparse contractions -------------------------------------------------------
+ * A block tensor is one flat buffer of blocks; the host builds its layouts and contraction plans
+ * (tnkit/blocks.py:220-253, :952-1100). */
+
+/* Output elements handled per workgroup by tnh_gather_runs (the chunk of its chunk table). */
+int tnh_gather_chunk(void);
+
+/* Replaces tnkit's gather _Index.take (tnkit/blocks.py:363-495, used by tensordot :1168-1169 and
+ * BlockTensor.canonical :755-760): dst[j] = (dst_dtype) src[sbeg[r] + (j - dbeg[r]) * sstep[r]]
+ * for the run r covering j, and dst[j] = 0 where no run covers j. `runs` (device) holds four
+ * arrays of nruns longs: sbeg, sstep, dbeg, len, sorted by dbeg with disjoint runs. `chunks`
+ * (device) holds 2 * nchunks ints, nchunks = ceil(n / tnh_gather_chunk()): the first run that
+ * may overlap chunk c, then the end of the runs that do. Dtypes F64/F32 in any combination. */
+int tnh_gather_runs(int src_dtype, int dst_dtype, long n, const void* src, void* dst, long nruns,
+                    const long* runs, const int* chunks, tnh_stream_t stream);
+
+/* Tile shape (bm x bn) of grouped-GEMM tile config `cfg` (1 = 128x128, 2 = 64x64, 3 = 32x32,
+ * 4 = 16x128, 5 = 64x128) and, when max_grid is not NULL, the persistent grid for it on the
+ * current device (resident workgroups on all CUs, a multiple of 8). */
+int tnh_grouped_tile(int dtype, int cfg, int* bm, int* bn, int* max_grid);
+
+/* All charge groups of one block contraction in one persistent launch; replaces the per-charge
+ * matmul loop of tnkit's tensordot (tnkit/blocks.py:1171-1190). Group g, descriptor
+ * desc[8g..8g+8) = (qa, lda, qb, ldb, po, R, K, C) (device longs):
+ *   C[po + r*C + c] = sum_{k<K} A[qa + k*lda + r] * B[qb + k*ldb + c],  r < R, c < C  (K = 0: 0)
+ * A and B are 16-byte aligned K-major stacks whose rows are padded with zeros to lda, ldb (both
+ * multiples of 4, as are qa and qb). tiles (device, ntiles x 3 ints) = (group, tile row, tile
+ * column) in config `tile_cfg`'s tile shape; workgroup b of `grid` (a multiple of 8) runs
+ * positions b, b + grid, ...; a group < 0 marks an empty position. dtype F64 or F32. */
+int tnh_grouped_gemm(int dtype, const long* desc, long ntiles, const int* tiles, const void* A,
+                     const void* B, void* C, int tile_cfg, int grid, tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,10 @@ SIGNATURES = {
     "tnh_dot": (c_int, [c_long, _P, _P, _P, _P, c_size_t, _P]),
     "tnh_scale_by": (c_int, [c_long, _P, _P, c_double, _P, _P]),
     "tnh_convert": (c_int, [c_int, c_int, c_long, _P, _P, _P]),
+    "tnh_gather_chunk": (c_int, []),
+    "tnh_gather_runs": (c_int, [c_int, c_int, c_long, _P, _P, c_long, _P, _P, _P]),
+    "tnh_grouped_tile": (c_int, [c_int, c_int, _P, _P, _P]),
+    "tnh_grouped_gemm": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, c_int, c_int, _P]),
 }
 
 

@@ -1,0 +1,301 @@
+// Block-sparse (U(1)) contractions: the run gather that stacks an operand into per-charge GEMM
+// matrices, and the persistent grouped GEMM that multiplies all charge groups of one
+// contraction in a single launch.
+//
+// Layouts (decided on the host, see tnkit_amd/blocks.py):
+//   gather : dst[j] = src[sbeg[r] + (j - dbeg[r]) * sstep[r]] for the run r covering j (runs
+//            sorted by dbeg, disjoint); positions no run covers are written as zero, so padded
+//            matrices come out with zero padding.
+//   grouped GEMM, group g with descriptor (qa, lda, qb, ldb, po, R, K, C):
+//            C[po + r*C + c] = sum_k A[qa + k*lda + r] * B[qb + k*ldb + c]     (K = 0: zeros)
+//            A and B are the K-major stacks written by the gather (lda, ldb, qa, qb multiples of 4
+//            elements, padding zero), so every tile row is read with 16-byte vector loads.
+//   tiles  : the host lists (group, tile row, tile column) per position; workgroup b of a grid of
+//            G (a multiple of 8) takes positions b, b + G, b + 2G, ... . Position p runs on XCD
+//            p % 8, which the host uses to keep a panel of tiles that share an A row block on one
+//            XCD's L2; entries with group < 0 are padding.
+#include "gemm_tn.h"
+
+namespace tnh {
+
+namespace {
+
+constexpr int kGatherThreads = 256;
+constexpr int kGatherChunk = 2048;  // output elements per workgroup
+
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(kGatherThreads)
+    gather_runs_kernel(const TS* __restrict__ src, TD* __restrict__ dst, long n,
+                       const long* __restrict__ sbeg, const long* __restrict__ sstep,
+                       const long* __restrict__ dbeg, const long* __restrict__ len,
+                       const int* __restrict__ clo, const int* __restrict__ chi) {
+  const long base = static_cast<long>(blockIdx.x) * kGatherChunk;
+  const int lo = clo[blockIdx.x];
+  const int hi = chi[blockIdx.x];  // runs [lo, hi) may overlap this chunk
+  int r = lo;
+  for (int i = 0; i < kGatherChunk / kGatherThreads; ++i) {
+    const long j = base + threadIdx.x + static_cast<long>(i) * kGatherThreads;
+    if (j >= n) break;
+    // the last run in [r, hi) starting at or before j (j grows, so r only moves forward)
+    int a = r, b = hi;
+    while (b - a > 1) {
+      const int mid = (a + b) >> 1;
+      if (dbeg[mid] <= j)
+        a = mid;
+      else
+        b = mid;
+    }
+    r = a;
+    TD v = TD(0);
+    if (r < hi) {
+      const long d0 = dbeg[r];
+      if (d0 <= j && j < d0 + len[r]) v = static_cast<TD>(src[sbeg[r] + (j - d0) * sstep[r]]);
+    }
+    dst[j] = v;
+  }
+}
+
+template <typename TS, typename TD>
+int launch_gather(long n, const void* src, void* dst, long nruns, const long* runs,
+                  const int* chunks, hipStream_t stream) {
+  if (n == 0) return TNH_OK;
+  const long nchunks = (n + kGatherChunk - 1) / kGatherChunk;
+  TNH_REQUIRE(nchunks < (1L << 31), TNH_ERR_ARG, "gather_runs: too many elements");
+  hipLaunchKernelGGL((gather_runs_kernel<TS, TD>), dim3(static_cast<unsigned>(nchunks)),
+                     dim3(kGatherThreads), 0, stream, static_cast<const TS*>(src),
+                     static_cast<TD*>(dst), n, runs, runs + nruns, runs + 2 * nruns,
+                     runs + 3 * nruns, chunks, chunks + nchunks);
+  TNH_CHECK_LAUNCH("gather_runs");
+  return TNH_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
+// grouped GEMM
+// ---------------------------------------------------------------------------------------------
+
+struct GroupArgs {
+  const long* desc;  // ngroups x 8: qa, lda, qb, ldb, po, R, K, C
+  const int* tiles;  // ntiles x 3: group, tile row, tile column
+  long ntiles;
+  const void* A;
+  const void* B;
+  void* C;
+};
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(WM* WN * 64) grouped_gemm_kernel(GroupArgs ga) {
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NT = WM * WN * 64;
+  constexpr int PAD = 16;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int FM = BM / WM / 16;
+  constexpr int FN = BN / WN / 16;
+  constexpr int V = 16 / sizeof(T);
+  // 16-byte loads where the tile splits evenly into them (the skinny 16-row A tile does not)
+  constexpr int VA = (BK * BM) % (V * NT) == 0 ? V : 1;
+  constexpr int VB = (BK * BN) % (V * NT) == 0 ? V : 1;
+  static_assert(BK % 4 == 0 && FM >= 1 && FN >= 1, "tile shape");
+
+  __shared__ T As[2][BK * LDA_S];
+  __shared__ T Bs[2][BK * LDB_S];
+
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int wm = wave / WN;
+  const int wn = wave % WN;
+  const int row16 = lane & 15;
+  const int kq = lane >> 4;
+
+  for (long pos = blockIdx.x; pos < ga.ntiles; pos += gridDim.x) {
+    const int g = ga.tiles[3 * pos];
+    if (g < 0) continue;
+    const int m0 = ga.tiles[3 * pos + 1] * BM;
+    const int n0 = ga.tiles[3 * pos + 2] * BN;
+    const long* d = ga.desc + 8 * static_cast<long>(g);
+    const T* __restrict__ A = static_cast<const T*>(ga.A) + d[0];
+    const long lda = d[1];
+    const T* __restrict__ B = static_cast<const T*>(ga.B) + d[2];
+    const long ldb = d[3];
+    T* __restrict__ C = static_cast<T*>(ga.C) + d[4];
+    const int M = static_cast<int>(d[5]);
+    const int K = static_cast<int>(d[6]);
+    const int N = static_cast<int>(d[7]);
+
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+
+    // vector loads may run past M (or N) into the zero padding of the row, never past lda/ldb
+    TileLoader<T, T, BK, BM, NT, VA> la;
+    TileLoader<T, T, BK, BN, NT, VB> lb;
+    const int nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+      la.load(A, lda, 0, K, m0, M, t);
+      lb.load(B, ldb, 0, K, n0, N, t);
+      la.store(As[0], LDA_S, t);
+      lb.store(Bs[0], LDB_S, t);
+    }
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      if (kt + 1 < nk) {
+        la.load(A, lda, (kt + 1) * BK, K, m0, M, t);
+        lb.load(B, ldb, (kt + 1) * BK, K, n0, N, t);
+      }
+      const T* as = As[cur];
+      const T* bs = Bs[cur];
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+          af[i] = as[(kk + kq) * LDA_S + wm * (BM / WM) + i * 16 + row16];
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          bf[j] = bs[(kk + kq) * LDB_S + wn * (BN / WN) + j * 16 + row16];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+      }
+      if (kt + 1 < nk) {
+        la.store(As[cur ^ 1], LDA_S, t);
+        lb.store(Bs[cur ^ 1], LDB_S, t);
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = n0 + wn * (BN / WN) + j * 16 + row16;
+        if (n >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * (BM / WM) + i * 16 + MF::row(kq, r);
+          if (m < M) C[static_cast<long>(m) * N + n] = acc[i][j][r];
+        }
+      }
+    }
+  }
+}
+
+struct TileShape {
+  int bm, bn, bk, wm, wn;
+};
+
+template <typename T>
+TileShape tile_shape(int cfg) {
+  constexpr bool dbl = sizeof(T) == 8;
+  switch (cfg) {
+    case 1: return {128, 128, dbl ? 16 : 32, 2, 2};
+    case 2: return {64, 64, 16, 2, 2};
+    case 3: return {32, 32, 16, 2, 2};
+    case 4: return {16, 128, 16, 1, 4};
+    case 5: return {64, 128, 16, 2, 2};
+    default: return {0, 0, 0, 0, 0};
+  }
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+int launch_grouped_cfg(const GroupArgs& ga, int grid, hipStream_t stream, int* max_grid) {
+  auto kern = grouped_gemm_kernel<T, BM, BN, BK, WM, WN>;
+  if (max_grid) {  // the persistent grid: resident workgroups on every CU, a multiple of 8
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WM * WN * 64, 0) !=
+            hipSuccess) {
+      set_error("grouped_gemm: device query failed");
+      return TNH_ERR_HIP;
+    }
+    *max_grid = (cus * (per_cu > 0 ? per_cu : 1) + 7) / 8 * 8;
+    return TNH_OK;
+  }
+  if (ga.ntiles == 0) return TNH_OK;
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(WM * WN * 64), 0, stream, ga);
+  TNH_CHECK_LAUNCH("grouped_gemm");
+  return TNH_OK;
+}
+
+template <typename T>
+int launch_grouped(int cfg, const GroupArgs& ga, int grid, hipStream_t stream, int* max_grid) {
+  constexpr bool dbl = sizeof(T) == 8;
+  constexpr int BKb = dbl ? 16 : 32;
+  switch (cfg) {
+    case 1: return launch_grouped_cfg<T, 128, 128, BKb, 2, 2>(ga, grid, stream, max_grid);
+    case 2: return launch_grouped_cfg<T, 64, 64, 16, 2, 2>(ga, grid, stream, max_grid);
+    case 3: return launch_grouped_cfg<T, 32, 32, 16, 2, 2>(ga, grid, stream, max_grid);
+    case 4: return launch_grouped_cfg<T, 16, 128, 16, 1, 4>(ga, grid, stream, max_grid);
+    case 5: return launch_grouped_cfg<T, 64, 128, 16, 2, 2>(ga, grid, stream, max_grid);
+    default: set_error("grouped_gemm: unknown tile config %d", cfg); return TNH_ERR_ARG;
+  }
+}
+
+}  // namespace
+}  // namespace tnh
+
+using namespace tnh;
+
+extern "C" int tnh_gather_chunk(void) { return kGatherChunk; }
+
+extern "C" int tnh_gather_runs(int src_dtype, int dst_dtype, long n, const void* src, void* dst,
+                               long nruns, const long* runs, const int* chunks,
+                               tnh_stream_t stream) {
+  TNH_REQUIRE(n >= 0 && nruns >= 0, TNH_ERR_ARG, "tnh_gather_runs: negative size");
+  TNH_REQUIRE(n == 0 || (dst && chunks && (nruns == 0 || (src && runs))), TNH_ERR_ARG,
+              "tnh_gather_runs: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (src_dtype == TNH_F64 && dst_dtype == TNH_F64)
+    return launch_gather<double, double>(n, src, dst, nruns, runs, chunks, s);
+  if (src_dtype == TNH_F32 && dst_dtype == TNH_F32)
+    return launch_gather<float, float>(n, src, dst, nruns, runs, chunks, s);
+  if (src_dtype == TNH_F64 && dst_dtype == TNH_F32)
+    return launch_gather<double, float>(n, src, dst, nruns, runs, chunks, s);
+  if (src_dtype == TNH_F32 && dst_dtype == TNH_F64)
+    return launch_gather<float, double>(n, src, dst, nruns, runs, chunks, s);
+  set_error("tnh_gather_runs: unsupported dtypes %d -> %d", src_dtype, dst_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+extern "C" int tnh_grouped_tile(int dtype, int cfg, int* bm, int* bn, int* max_grid) {
+  TileShape ts{};
+  int rc = TNH_OK;
+  if (dtype == TNH_F64) {
+    ts = tile_shape<double>(cfg);
+    if (ts.bm && max_grid) rc = launch_grouped<double>(cfg, GroupArgs{}, 0, nullptr, max_grid);
+  } else if (dtype == TNH_F32) {
+    ts = tile_shape<float>(cfg);
+    if (ts.bm && max_grid) rc = launch_grouped<float>(cfg, GroupArgs{}, 0, nullptr, max_grid);
+  } else {
+    set_error("tnh_grouped_tile: unsupported dtype %d", dtype);
+    return TNH_ERR_UNSUPPORTED;
+  }
+  TNH_REQUIRE(ts.bm > 0, TNH_ERR_ARG, "tnh_grouped_tile: unknown tile config %d", cfg);
+  if (bm) *bm = ts.bm;
+  if (bn) *bn = ts.bn;
+  return rc;
+}
+
+extern "C" int tnh_grouped_gemm(int dtype, const long* desc, long ntiles, const int* tiles,
+                                const void* A, const void* B, void* C, int tile_cfg, int grid,
+                                tnh_stream_t stream) {
+  TNH_REQUIRE(ntiles >= 0 && grid >= 0, TNH_ERR_ARG, "tnh_grouped_gemm: negative size");
+  if (ntiles == 0) return TNH_OK;
+  TNH_REQUIRE(grid > 0 && grid % 8 == 0, TNH_ERR_ARG,
+              "tnh_grouped_gemm: grid %d must be a positive multiple of 8", grid);
+  TNH_REQUIRE(desc && tiles && A && B && C, TNH_ERR_ARG, "tnh_grouped_gemm: null pointer");
+  TNH_REQUIRE(reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0,
+              TNH_ERR_ARG, "tnh_grouped_gemm: A and B must be 16-byte aligned");
+  GroupArgs ga{desc, tiles, ntiles, A, B, C};
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (dtype == TNH_F64) return launch_grouped<double>(tile_cfg, ga, grid, s, nullptr);
+  if (dtype == TNH_F32) return launch_grouped<float>(tile_cfg, ga, grid, s, nullptr);
+  set_error("tnh_grouped_gemm: unsupported dtype %d", dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
