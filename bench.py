@@ -160,6 +160,145 @@ def local_solves(th, op64, op32, ws, chi):
             "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"], "fp64_s": t_e64, "mixed_s": t_emx}
 
 
+def gaussian_sectors(chi, sigma, half_width):
+    """Sector dimensions of one even-charge bond: charges 2k, |k| <= half_width, dims proportional
+    to exp(-k^2 / (2 sigma^2)), at least 1, summing to chi (the largest ~ chi / (sigma sqrt(2 pi)))."""
+    import numpy as np
+    k = np.arange(-half_width, half_width + 1)
+    w = np.exp(-k ** 2 / (2.0 * sigma ** 2))
+    dims = np.maximum(1, np.floor(chi * w / w.sum())).astype(np.int64)
+    while dims.sum() < chi:
+        dims[np.argmax(chi * w / w.sum() - dims)] += 1
+    while dims.sum() > chi:
+        dims[np.argmax(dims)] -= 1
+    return {int(2 * kk): int(d) for kk, d in zip(k, dims)}
+
+
+def build_block_problem(chi=2048, sigma=2.7, half_width=15, seed=0):
+    """The bulk two-site problem of the U(1) XXZ chain (2 S^z_total = 0) at bond chi split over
+    2 half_width + 1 sectors of Gaussian sizes. The environments are those of a random block MPS
+    whose bonds grow from each edge (bond n: the charges n sites can reach, dims the Gaussian
+    profile capped by the number of basis states) over 2 half_width + 10 sites, built with the
+    block environment kernels and rescaled by their largest entry after each site; so H_eff is
+    symmetric, as a chain projection is."""
+    import math
+    import numpy as np
+    import torch
+    from tnkit_amd import blocks as B
+    from tnkit_amd import block_ops as BO
+    from tnkit_amd import block_mps as BM
+    from tnkit_amd.models import xxz_mpo
+
+    dev = torch.device("cuda", 0)
+    target = gaussian_sectors(chi, sigma, half_width)
+    peak = max(target.values())
+    qmax = 2 * half_width
+    nh = qmax + 10
+
+    def leg(n):
+        if n == nh:
+            cs = sorted(target)
+            return B.Leg(tuple(cs), tuple(target[c] for c in cs), 1)
+        cs = [q for q in range(-min(n, qmax + 1), min(n, qmax + 1) + 1) if (q - n) % 2 == 0]
+        dims = [min(max(1, int(round(peak * math.exp(-(q / 2.0) ** 2 / (2 * sigma ** 2))))),
+                    math.comb(n, (n + q) // 2)) for q in cs]
+        return B.Leg(tuple(cs), tuple(dims), 1)
+
+    mpo, _ = BM.to_block_mpo(xxz_mpo(2 * nh + 2), [[1, -1]] * (2 * nh + 2), dev)
+    Wf, Wb, Wl = mpo[0], mpo[1], mpo[-1]
+    p = Wb.legs[2]
+    rng = np.random.default_rng(seed)
+    legs = [leg(n) for n in range(nh + 1)]
+    L = BO.BlockTensor(B.canonical((legs[0], Wf.legs[0].flip(), legs[0].flip()), 0),
+                       torch.ones(1, dtype=torch.float64, device=dev))
+    for n in range(nh):
+        A = BO.random((legs[n], p, legs[n + 1].flip()), 0, rng, dev)
+        L = BM.update_left_env(L, A, Wf if n == 0 else Wb)
+        L = L.with_data(L.data / L.data.abs().max())
+    R = BO.BlockTensor(B.canonical((legs[0].flip(), Wl.legs[1].flip(), legs[0]), 0),
+                       torch.ones(1, dtype=torch.float64, device=dev))
+    for n in range(nh):
+        Bt = BO.random((legs[n + 1], p, legs[n].flip()), 0, rng, dev)
+        R = BM.update_right_env(R, Bt, Wl if n == 0 else Wb)
+        R = R.with_data(R.data / R.data.abs().max())
+    L, R = L.canonical(), R.canonical()
+    th = BO.random((legs[nh], p, p, legs[nh].flip()), 0, rng, dev)
+    th = th.with_data(th.data / th.data.norm())
+    return dict(L=L, R=R, W=Wb, theta=th, leg=legs[nh], sectors=target)
+
+
+def block_bench(steps, warmup, chi=2048, local=True):
+    """Config of the U(1) XXZ chain at chi = 2048 over 31 Gaussian sectors: the block H_eff
+    matvec in float64 and with the float32 operator (float64 vectors), per-kernel events, hipGraph
+    replay, and the local ground state by float64 vs mixed Lanczos."""
+    import torch
+    from tnkit_amd import block_ops as BO
+    from tnkit_amd.eigensolver import lanczos_ground_mixed, lanczos_ground_restart
+    t0 = time.perf_counter()
+    pr = build_block_problem(chi)
+    L, R, W, th = pr["L"], pr["R"], pr["W"], pr["theta"]
+    op64 = BO.BlockTwoSite(L, W, W, R)
+    op32 = BO.BlockTwoSite(L, W, W, R, dtype=torch.float32)
+    op64.tune()
+    op32.tune()
+    setup = time.perf_counter() - t0
+    x = th.data
+    out = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    y64 = op64.apply(x)
+    y32 = op32.apply(x)
+    rel32 = float((y32 - y64).abs().max() / y64.abs().max())
+    res = {"workload": f"U(1) XXZ (2Sz=0) bulk two-site problem, chi={chi} per bond over "
+                       f"{len(pr['sectors'])} sectors (Gaussian, largest {max(pr['sectors'].values())}, "
+                       f"smallest {min(pr['sectors'].values())}), theta {op64.size} elements",
+           "gflop_per_matvec": op64.flops / 1e9, "macs_per_contraction": op64.macs,
+           "groups_per_contraction": [len(st[0].desc) for st in op64.stages],
+           "tile_cfg_f64": [st[4] for st in op64.stages],
+           "tile_cfg_f32": [st[4] for st in op32.stages], "setup_s": setup,
+           "matvec_rel_err_f32": rel32}
+    for name, op in (("fp64", op64), ("mixed", op32)):
+        for _ in range(warmup):
+            op.apply(x, out)
+        ms = event_time(lambda: op.apply(x, out), steps, stream)
+        run = op.graph()
+        for _ in range(warmup):
+            run(x)
+        msg = event_time(lambda: run(x), steps, stream)
+        stages = []
+        for i, (p, A, Bb, o, cfg) in enumerate(op.stages):
+            src = x if i == 0 else op.stages[i - 1][3]
+            tg = event_time(lambda: BO.gather(p.dev_b, src, op.dtype, Bb), steps, stream)
+            tm = event_time(lambda: BO.grouped_gemm(p, A, Bb, o, cfg), steps, stream)
+            esz = 4 if op.dtype == torch.float32 else 8
+            mov = (src.numel() * src.element_size() + Bb.numel() * esz) / (tg * 1e-3) / 1e9
+            stages.append({"gather_ms": round(tg, 4), "gather_GBps": round(mov, 1),
+                           "gemm_ms": round(tm, 4),
+                           "gemm_tflops": round(2 * p.macs / (tm * 1e-3) / 1e12, 2),
+                           "cfg": cfg})
+        src = op.stages[-1][3]
+        tf = event_time(lambda: BO.gather(op.final, src, op.io_dtype, out), steps, stream)
+        res[name] = {"ms": ms, "tflops": op.flops / (ms * 1e-3) / 1e12, "graph_ms": msg,
+                     "graph_tflops": op.flops / (msg * 1e-3) / 1e12, "stages": stages,
+                     "final_gather_ms": round(tf, 4)}
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+    if local:
+        cnt = {"lo": 0, "hi": 0}
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        e64, _, i64 = lanczos_ground_restart(op64.apply, x.clone(), tol=1e-9, max_iter=4 * chi)
+        torch.cuda.synchronize()
+        t_e64 = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        emx, _, _ = lanczos_ground_mixed(op32.apply, op64.apply, x.clone(), tol=1e-9,
+                                         max_iter=4 * chi, counts=cnt)
+        torch.cuda.synchronize()
+        res["local_dE"] = {"fp64_E": e64, "mixed_E": emx, "abs": abs(emx - e64),
+                           "fp64_matvecs": i64["iters"], "mixed_lo": cnt["lo"],
+                           "mixed_hi": cnt["hi"], "fp64_s": t_e64,
+                           "mixed_s": time.perf_counter() - t1}
+    return res
+
+
 def config2(N, stages, eig, trunc, seed=0, log=None):
     """The precision study: XXZ chain N sites, dense two-site DMRG with the bond ramped through
     `stages` = [(chi, sweeps), ...], each stage started from the previous state (init_mps), as
@@ -205,6 +344,9 @@ def main():
     ap.add_argument("--config2", type=str, default="",
                     help="run the chi-ramp precision study instead, e.g. '64:4,512:4,1024:4'")
     ap.add_argument("--trunc", type=str, default="gram")
+    ap.add_argument("--block", action="store_true",
+                    help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
+    ap.add_argument("--no-block", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -231,6 +373,10 @@ def main():
                                    "one MI355X and multi-GPU sharding is design-only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    if args.block:
+        print(json.dumps(block_bench(args.steps, args.warmup, local=not args.no_local)))
+        return
 
     cores = len(os.sched_getaffinity(0))
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
@@ -333,6 +479,8 @@ def main():
                            "mixed_counts": eng.mixed_counts}
             del eng
     setup_s = time.perf_counter() - t_setup
+    block = {} if args.no_block else block_bench(max(5, args.steps // 2), args.warmup,
+                                                 local=not args.no_local)
 
     res = {
         "metric": METRIC, "value": fl / (ms32 * 1e-3) / 1e12, "unit": "TFLOP/s", "n_gpus": 1,
@@ -352,6 +500,7 @@ def main():
         "bulk_steps": steps,
         "reference_h100_mig_fp64_tflops": 5.67,
         "roofline": roof,
+        "block_u1": block,
         "setup_s": setup_s,
     }
     if not args.no_cpu:

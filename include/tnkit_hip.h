@@ -176,12 +176,20 @@ int tnh_gather_chunk(void);
 
 /* Replaces tnkit's gather _Index.take (tnkit/blocks.py:363-495, used by tensordot :1168-1169 and
  * BlockTensor.canonical :755-760): dst[j] = (dst_dtype) src[sbeg[r] + (j - dbeg[r]) * sstep[r]]
- * for the run r covering j, and dst[j] = 0 where no run covers j. `runs` (device) holds four
- * arrays of nruns longs: sbeg, sstep, dbeg, len, sorted by dbeg with disjoint runs. `chunks`
- * (device) holds 2 * nchunks ints, nchunks = ceil(n / tnh_gather_chunk()): the first run that
- * may overlap chunk c, then the end of the runs that do. Dtypes F64/F32 in any combination. */
+ * for the run r covering j; where no run covers j, dst[j] = 0 if `fill`, else untouched. `runs`
+ * (device) holds four arrays of nruns longs: sbeg, sstep, dbeg, len, sorted by dbeg with
+ * disjoint runs. `chunks` (device) holds 2 * nchunks ints, nchunks = ceil(n /
+ * tnh_gather_chunk()): the first run that may overlap chunk c, then the end of the runs that do.
+ * Dtypes F64/F32 in any combination. */
 int tnh_gather_runs(int src_dtype, int dst_dtype, long n, const void* src, void* dst, long nruns,
-                    const long* runs, const int* chunks, tnh_stream_t stream);
+                    const long* runs, const int* chunks, int fill, tnh_stream_t stream);
+
+/* The transposing part of the same gathers (blocks whose source-contiguous and
+ * destination-contiguous axes differ): tile t of `tiles` (device, ntiles x 6 longs: src0, scol,
+ * dst0, drow, nrows <= 64, ncols <= 64) moves dst[dst0 + i*drow + j] = src[src0 + i + j*scol]
+ * through LDS, contiguous on both sides. */
+int tnh_gather_tiles(int src_dtype, int dst_dtype, long ntiles, const void* src, void* dst,
+                     const long* tiles, tnh_stream_t stream);
 
 /* Tile shape (bm x bn) of grouped-GEMM tile config `cfg` (1 = 128x128, 2 = 64x64, 3 = 32x32,
  * 4 = 16x128, 5 = 64x128) and, when max_grid is not NULL, the persistent grid for it on the
@@ -198,6 +206,12 @@ int tnh_grouped_tile(int dtype, int cfg, int* bm, int* bn, int* max_grid);
  * positions b, b + grid, ...; a group < 0 marks an empty position. dtype F64 or F32. */
 int tnh_grouped_gemm(int dtype, const long* desc, long ntiles, const int* tiles, const void* A,
                      const void* B, void* C, int tile_cfg, int grid, tnh_stream_t stream);
+
+/* The same grouped product when every group has R <= 8 and K <= 8 (the MPO contractions of the
+ * two-site matvec, tnkit/_dmrg.py:166-167): memory bound, one thread per output column.
+ * chunks (device, nchunks x 2 ints) = (group, first column) of each 1024-column chunk. */
+int tnh_grouped_small(int dtype, const long* desc, long nchunks, const int* chunks, const void* A,
+                      const void* B, void* C, tnh_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -188,3 +188,24 @@ def test_tile_table_covers_every_tile_once(bm, bn):
             for i in range(-(-R[g] // bm)) for j in range(-(-C[g] // bn))}
     got = [tuple(int(v) for v in r) for r in live]
     assert len(got) == len(set(got)) and set(got) == want
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_moves_match_explicit_positions(seed):
+    """Runs plus transposing tiles cover the same positions as the explicit enumeration, with
+    boxes large enough (and transposed enough) for the tile path."""
+    rng = np.random.default_rng(100 + seed)
+    m = 3
+    shp = rng.integers(1, 90, size=(5, m))
+    shp[:, 1] = rng.integers(1, 3, size=5)
+    ss = np.stack([B._row_major_rows(s[None])[0][rng.permutation(m)] for s in shp])
+    src0 = np.cumsum(shp.prod(1)) - shp.prod(1)
+    sizes = shp.prod(1) + rng.integers(0, 3, size=5)
+    dst0 = np.cumsum(sizes) - sizes
+    ds = B._row_major_rows(shp)
+    runs, tiles = B.box_moves(src0, shp, ss, dst0, ds)
+    n = int(sizes.sum())
+    g = B.Gather(runs, n, int(shp.prod(1).sum()), tiles)
+    assert np.array_equal(g.expand(), _explicit(src0, shp, ss, dst0, ds, n))
+    if seed == 0:
+        assert len(tiles) > 0

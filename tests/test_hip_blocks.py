@@ -147,3 +147,34 @@ def test_block_two_site_random_vs_oracle_and_graph(hip):
             torch.cuda.synchronize()
             assert torch.equal(g, eager)
         _ = (x * 2.0).sum().item()                       # unrelated eager work between replays
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_small_groups_kernel(hip, case):
+    """The MPO contraction (W1 with L theta) through the memory-bound small-group kernel equals
+    the grouped MFMA GEMM and per-group torch matmuls."""
+    print(f"small groups {case}", flush=True)
+    ops = operands(case, hip)
+    x1 = BO.tensordot(ops["L"], ops["theta"], ((2,), (0,)))
+    W = ops["W"]
+    p = B.plan(W.struct, x1.struct, (0, 3), (1, 2))
+    assert BO._grouped(p, hip).small_ok()
+    for dt, tol in ((torch.float64, 1e-14), (torch.float32, 1e-6)):
+        A = BO.gather(p.dev_a, W.data, dt)
+        Bm = BO.gather(p.dev_b, x1.data, dt)
+        o1 = torch.full((p.out.size,), float("nan"), dtype=dt, device=hip)
+        o2 = torch.full((p.out.size,), float("nan"), dtype=dt, device=hip)
+        BO.grouped_gemm(p, A, Bm, o1, cfg=BO.SMALL)
+        BO.grouped_gemm(p, A, Bm, o2, cfg=4)
+        assert torch.isfinite(o1).all()
+        ia = torch.from_numpy(p.ref_a.expand()).to(hip)
+        ib = torch.from_numpy(p.ref_b.expand()).to(hip)
+        Ar, Br = W.data.double()[ia], x1.data.double()[ib]
+        ref = torch.zeros(p.out.size, dtype=torch.float64, device=hip)
+        for pa, pb, po, R, K, C in p.gemms.tolist():
+            if K:
+                ref[po:po + R * C] = (Ar[pa:pa + R * K].reshape(R, K)
+                                      @ Br[pb:pb + K * C].reshape(K, C)).reshape(-1)
+        scale = float(ref.abs().max())
+        assert float((o1.double() - ref).abs().max()) <= tol * scale
+        assert float((o2.double() - ref).abs().max()) <= tol * scale

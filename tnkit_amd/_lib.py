@@ -57,9 +57,11 @@ SIGNATURES = {
     "tnh_scale_by": (c_int, [c_long, _P, _P, c_double, _P, _P]),
     "tnh_convert": (c_int, [c_int, c_int, c_long, _P, _P, _P]),
     "tnh_gather_chunk": (c_int, []),
-    "tnh_gather_runs": (c_int, [c_int, c_int, c_long, _P, _P, c_long, _P, _P, _P]),
+    "tnh_gather_runs": (c_int, [c_int, c_int, c_long, _P, _P, c_long, _P, _P, c_int, _P]),
+    "tnh_gather_tiles": (c_int, [c_int, c_int, c_long, _P, _P, _P, _P]),
     "tnh_grouped_tile": (c_int, [c_int, c_int, _P, _P, _P]),
     "tnh_grouped_gemm": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, c_int, c_int, _P]),
+    "tnh_grouped_small": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, _P]),
 }
 
 

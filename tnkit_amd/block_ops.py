@@ -30,6 +30,8 @@ from ._lib import check, dcode, ptr
 _REAL = (torch.float64, torch.float32)
 TWO_SITE_SUBS = ("pwq,qijb->pwijb", "wmsi,pwijb->pmsjb", "mrtj,pmsjb->pstrb", "drb,pstrb->pstd")
 TILE_CONFIGS = (1, 2, 3, 4, 5)
+SMALL = 9              # the memory-bound kernel for groups with R, K <= 8 (not a tile shape)
+_SMALL_COLS = 1024
 
 
 def _stream(device) -> int:
@@ -42,7 +44,7 @@ def _stream(device) -> int:
 
 
 class _DevRuns:
-    __slots__ = ("runs", "chunks", "nruns", "n")
+    __slots__ = ("runs", "chunks", "nruns", "n", "tiles", "ntiles")
 
 
 def _device_runs(g: B.Gather, device, src_size: int) -> _DevRuns:
@@ -71,6 +73,17 @@ def _device_runs(g: B.Gather, device, src_size: int) -> _DevRuns:
     host = np.concatenate([src, sstep, dst, length]).astype(np.int64)
     d.runs = torch.from_numpy(host if host.size else np.zeros(4, np.int64)).to(device)
     d.chunks = torch.from_numpy(np.concatenate([lo, hi]).astype(np.int32)).to(device)
+    t = g.tiles
+    d.ntiles = len(t)
+    if d.ntiles:
+        s0, sc, d0, dr, nr, nc = t.T
+        if ((nr < 1) | (nc < 1) | (nr > B.TILE) | (nc > B.TILE)).any() or s0.min() < 0 \
+                or (s0 + (nr - 1) + (nc - 1) * sc).max() >= src_size or d0.min() < 0 \
+                or (d0 + (nr - 1) * dr + nc - 1).max() >= g.n:
+            raise ValueError("gather tiles outside their buffers")
+        d.tiles = torch.from_numpy(np.ascontiguousarray(t, np.int64).reshape(-1)).to(device)
+    else:
+        d.tiles = None
     g._dev[key] = d
     return d
 
@@ -88,9 +101,14 @@ def gather(g: B.Gather, src: torch.Tensor, dtype: Optional[torch.dtype] = None,
     if out.numel() != g.n or out.dtype != dtype or not out.is_contiguous():
         raise ValueError("gather: output buffer does not match")
     d = _device_runs(g, src.device, src.numel())
-    check(_lib.load().tnh_gather_runs(dcode(src.dtype), dcode(dtype), g.n, ptr(src), ptr(out),
-                                      d.nruns, ptr(d.runs), ptr(d.chunks), _stream(src.device)),
-          "gather_runs")
+    lib, st = _lib.load(), _stream(src.device)
+    # with tiles the run pass leaves uncovered positions alone (tiles and padding); padding is
+    # never read into a stored result by the grouped GEMM
+    check(lib.tnh_gather_runs(dcode(src.dtype), dcode(dtype), g.n, ptr(src), ptr(out), d.nruns,
+                              ptr(d.runs), ptr(d.chunks), int(d.ntiles == 0), st), "gather_runs")
+    if d.ntiles:
+        check(lib.tnh_gather_tiles(dcode(src.dtype), dcode(dtype), d.ntiles, ptr(src), ptr(out),
+                                   ptr(d.tiles), st), "gather_tiles")
     return out
 
 
@@ -176,10 +194,22 @@ class _Grouped:
         self.size_a, self.size_b, self.size_out = p.size_a, p.size_b, p.out.size
         self.macs = p.macs
 
+    def small_ok(self) -> bool:
+        d = self.desc_np
+        return len(d) > 0 and int(d[:, 5].max()) <= 8 and int(d[:, 6].max()) <= 8
+
     def table(self, dtype, cfg, device):
         key = (dtype, cfg)
         t = self.tables.get(key)
-        if t is None:
+        if t is None and cfg == SMALL:
+            C = self.desc_np[:, 7]
+            nch = -(-C // _SMALL_COLS)
+            g = np.repeat(np.arange(len(C)), nch)
+            c0 = (np.arange(g.size) - np.repeat(np.cumsum(nch) - nch, nch)) * _SMALL_COLS
+            tab = np.stack([g, c0], 1).astype(np.int32)
+            t = (torch.from_numpy(tab.reshape(-1)).to(device), len(tab), 0)
+            self.tables[key] = t
+        elif t is None:
             bm, bn, gmax = tile_info(dtype, cfg, device)
             tab = tile_table(self.desc_np, bm, bn)
             grid = int(min(gmax, len(tab)))
@@ -209,7 +239,14 @@ def grouped_gemm(p: B.Plan, A: torch.Tensor, Bm: torch.Tensor, out: torch.Tensor
     gs = _grouped(p, dev)
     if cfg is None:
         cfg = autotune(p, A, Bm, out)
+    if cfg == SMALL and not gs.small_ok():
+        raise ValueError("grouped_gemm: the small kernel needs R, K <= 8 in every group")
     tiles, ntiles, grid = gs.table(A.dtype, cfg, dev)
+    if cfg == SMALL:
+        check(_lib.load().tnh_grouped_small(dcode(A.dtype), ptr(gs.desc), ntiles, ptr(tiles),
+                                            ptr(A), ptr(Bm), ptr(out), _stream(dev)),
+              "grouped_small")
+        return out
     check(_lib.load().tnh_grouped_gemm(dcode(A.dtype), ptr(gs.desc), ntiles, ptr(tiles), ptr(A),
                                        ptr(Bm), ptr(out), cfg, grid, _stream(dev)),
           "grouped_gemm")
@@ -233,6 +270,8 @@ def heuristic_cfg(p: B.Plan, dtype) -> int:
     chip, else 32x32."""
     d = p.desc
     R, C = d[:, 5], d[:, 7]
+    if R.max(initial=0) <= 8 and d[:, 6].max(initial=0) <= 8:
+        return SMALL
     if R.max(initial=0) <= 16:
         return 4
     t64 = int(((-(-R // 64)) * (-(-C // 64))).sum())
@@ -256,6 +295,8 @@ def autotune(p: B.Plan, A, Bm, out, reps: int = 3) -> int:
         return cfg
     R = p.desc[:, 5]
     cands = [c for c in TILE_CONFIGS if not (c == 4 and R.max(initial=0) > 64)]
+    if _grouped(p, A.device).small_ok():
+        cands.append(SMALL)
     if torch.cuda.is_current_stream_capturing():
         cfg = heuristic_cfg(p, A.dtype)
         return cfg

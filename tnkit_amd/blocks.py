@@ -191,9 +191,11 @@ def dual(s: Struct) -> Struct:
 
 def box_runs(src0, shp, ss, dst0, ds):
     """Runs of a set of boxes. Box b covers multi-indices i < shp[b], read at src0[b] + i.ss[b]
-    and written at dst0[b] + i.ds[b], with ds[b, -1] == 1. Each box is cut into runs along its
-    trailing axes, merged while the destination stays contiguous and the source an arithmetic
-    progression. Returns (src start, src step, dst start, length), sorted by destination."""
+    and written at dst0[b] + i.ds[b]. Each box is cut into runs along its innermost axis of
+    extent > 1, merged outwards (axes of extent 1 are transparent) while the destination stays
+    contiguous and the source an arithmetic progression; a box whose innermost axis is not
+    destination-contiguous is cut into single elements. Returns (src start, src step, dst start,
+    length), sorted by destination."""
     src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
     shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
     B, m = shp.shape
@@ -203,19 +205,23 @@ def box_runs(src0, shp, ss, dst0, ds):
     if m == 0:
         one = np.ones(B, np.int64)
         return src0, one, dst0, one
-    if not (ds[:, -1] == 1).all():
-        raise ValueError("box_runs: the destination must be contiguous along the last axis")
-    tail = shp[:, -1].copy()
-    start = np.full(B, m - 1)
-    chain = np.ones(B, bool)
-    step = ss[:, -1]
-    for j in range(m - 2, -1, -1):
-        chain &= (ds[:, j] == tail) & (ss[:, j] == tail * step)
-        if not chain.any():
-            break
-        tail = np.where(chain, tail * shp[:, j], tail)
-        start = np.where(chain, j, start)
-    outer = np.where(np.arange(m)[None, :] >= start[:, None], 1, shp)
+    rows = np.arange(B)
+    big = shp > 1
+    inner = np.where(big.any(1), m - 1 - np.argmax(big[:, ::-1], 1), m - 1)
+    ok = ds[rows, inner] == 1
+    tail = np.where(ok, shp[rows, inner], 1)
+    step = np.where(ok, ss[rows, inner], 1)
+    in_run = np.zeros((B, m), bool)
+    in_run[rows, inner] = ok
+    chain = ok.copy()
+    for j in range(m - 1, -1, -1):
+        act = chain & (j < inner)
+        unit = shp[:, j] == 1
+        grow = act & ~unit & (ds[:, j] == tail) & (ss[:, j] == tail * step)
+        in_run[:, j] |= act & (unit | grow)
+        chain &= ~act | unit | grow
+        tail = np.where(grow, tail * shp[:, j], tail)
+    outer = np.where(in_run, 1, shp)
     nrun = outer.prod(1)
     box = np.repeat(np.arange(B), nrun)
     local = np.arange(box.size) - np.repeat(np.cumsum(nrun) - nrun, nrun)
@@ -243,20 +249,88 @@ def expand_runs(runs, n: int, fill: int = -1) -> np.ndarray:
     return out
 
 
-class Gather:
-    """One operand's stacking into GEMM matrices: destination-sorted runs over a buffer of `n`
-    elements (positions outside every run are zero), and whether it is the identity."""
-    __slots__ = ("runs", "n", "identity", "_dev")
+TILE = 64          # edge of the transposing tiles
+TILE_MIN = 8       # boxes with both contiguous extents at least this go through tiles
 
-    def __init__(self, runs, n: int, src_size: int):
+
+def box_moves(src0, shp, ss, dst0, ds):
+    """Split boxes between the two gather kernels. A box whose destination-contiguous axis a_d
+    (innermost of extent > 1, ds = 1) is strided in the source, and which has another axis a_s
+    contiguous in the source (ss = 1), both of extent >= TILE_MIN, is a batch of 2-D transposes:
+    it is cut into tiles (src0, scol, dst0, drow, nrows, ncols) of at most TILE x TILE moving
+    dst[dst0 + i drow + j] = src[src0 + i + j scol]. Every other box becomes runs (`box_runs`).
+    Returns (runs, tiles) with tiles a (T, 6) int64 array."""
+    src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
+    shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
+    nb, m = shp.shape
+    empty = np.zeros((0, 6), np.int64)
+    if nb == 0 or m < 2:
+        return box_runs(src0, shp, ss, dst0, ds), empty
+    rows = np.arange(nb)
+    big = shp > 1
+    a_d = np.where(big.any(1), m - 1 - np.argmax(big[:, ::-1], 1), m - 1)
+    cand = (ss == 1) & (shp >= TILE_MIN)
+    cand[rows, a_d] = False
+    a_s = np.argmax(cand, 1)
+    use = (cand.any(1) & (ds[rows, a_d] == 1) & (ss[rows, a_d] != 1)
+           & (shp[rows, a_d] >= TILE_MIN))
+    runs = box_runs(src0[~use], shp[~use], ss[~use], dst0[~use], ds[~use])
+    if not use.any():
+        return runs, empty
+    b = np.flatnonzero(use)
+    ad, as_ = a_d[b], a_s[b]
+    outer = shp[b].copy()
+    outer[np.arange(len(b)), ad] = 1
+    outer[np.arange(len(b)), as_] = 1
+    nout = outer.prod(1)
+    box = np.repeat(np.arange(len(b)), nout)
+    local = np.arange(box.size) - np.repeat(np.cumsum(nout) - nout, nout)
+    sb, db = src0[b][box].copy(), dst0[b][box].copy()
+    for k in range(m - 1, -1, -1):
+        d = outer[box, k]
+        i = local % d
+        local //= d
+        sb += i * ss[b][box, k]
+        db += i * ds[b][box, k]
+    ns = shp[b][box, as_[box]]
+    nd = shp[b][box, ad[box]]
+    scol = ss[b][box, ad[box]]
+    drow = ds[b][box, as_[box]]
+    ti, tj = -(-ns // TILE), -(-nd // TILE)
+    nt = ti * tj
+    slab = np.repeat(np.arange(box.size), nt)
+    loc = np.arange(slab.size) - np.repeat(np.cumsum(nt) - nt, nt)
+    i0 = (loc // tj[slab]) * TILE
+    j0 = (loc % tj[slab]) * TILE
+    tiles = np.stack([sb[slab] + i0 + j0 * scol[slab], scol[slab],
+                      db[slab] + i0 * drow[slab] + j0, drow[slab],
+                      np.minimum(TILE, ns[slab] - i0), np.minimum(TILE, nd[slab] - j0)], 1)
+    return runs, tiles.astype(np.int64)
+
+
+class Gather:
+    """One operand's stacking into GEMM matrices over a destination of `n` elements: runs
+    (destination-sorted, see `box_runs`) plus transposing tiles (see `box_moves`); positions
+    covered by neither are padding. `identity` says the stacking is the buffer itself."""
+    __slots__ = ("runs", "tiles", "n", "identity", "_dev")
+
+    def __init__(self, runs, n: int, src_size: int, tiles=None):
         self.runs, self.n = runs, int(n)
+        self.tiles = np.zeros((0, 6), np.int64) if tiles is None else tiles
         src, sstep, dst, length = runs
-        self.identity = bool(self.n == src_size and int(length.sum()) == self.n
-                             and (src == dst).all() and ((sstep == 1) | (length == 1)).all())
+        self.identity = bool(len(self.tiles) == 0 and self.n == src_size
+                             and int(length.sum()) == self.n and (src == dst).all()
+                             and ((sstep == 1) | (length == 1)).all())
         self._dev = {}
 
     def expand(self) -> np.ndarray:
-        return expand_runs(self.runs, self.n)
+        """idx[dst] = src over the destination, -1 at padding."""
+        out = expand_runs(self.runs, self.n)
+        for s0, sc, d0, dr, nr, nc in self.tiles.tolist():
+            i = np.arange(nr)[:, None]
+            j = np.arange(nc)[None, :]
+            out[(d0 + i * dr + j).ravel()] = (s0 + i + j * sc).ravel()
+        return out
 
 
 # ---------------------------------------------------------------------------------------------
@@ -420,8 +494,11 @@ def _stack(sp: SymPlan, s: Struct, axes, side: int, ld_pad: bool):
     ds = np.concatenate([_row_major_rows(bs[:, :nrow]) * ld[g][:, None],
                          _row_major_rows(bs[:, nrow:])], 1)
     total = int(_pad(gsz).sum()) if ld_pad else int(gsz.sum())
-    runs = box_runs(s.off[blk], bs, bst, dst0, ds)
-    return runs, total, base, ld, (oshp, o, n), K
+    if ld_pad:
+        moves = box_moves(s.off[blk], bs, bst, dst0, ds)
+    else:
+        moves = (box_runs(s.off[blk], bs, bst, dst0, ds), None)
+    return moves, total, base, ld, (oshp, o, n), K
 
 
 _PLANS: Dict = {}
@@ -442,10 +519,11 @@ def plan(sa: Struct, sb: Struct, ax_a: Tuple[int, ...], ax_b: Tuple[int, ...]) -
     p = Plan()
     ra, na_, _, _, (r_shp, ro, R), K = _stack(sp, sa, ax_a, 0, False)
     rb, nb_, _, _, (c_shp, co, C), _ = _stack(sp, sb, ax_b, 1, False)
-    p.ref_a, p.ref_b = Gather(ra, na_, sa.size), Gather(rb, nb_, sb.size)
+    p.ref_a, p.ref_b = Gather(ra[0], na_, sa.size), Gather(rb[0], nb_, sb.size)
     da, p.size_a, qa, lda, _, _ = _stack(sp, sa, ax_a, 0, True)
     db, p.size_b, qb, ldb, _, _ = _stack(sp, sb, ax_b, 1, True)
-    p.dev_a, p.dev_b = Gather(da, p.size_a, sa.size), Gather(db, p.size_b, sb.size)
+    p.dev_a = Gather(da[0], p.size_a, sa.size, da[1])
+    p.dev_b = Gather(db[0], p.size_b, sb.size, db[1])
     pa, pb, po = _excl(R * K), _excl(K * C), _excl(R * C)
     p.gemms = np.stack([pa, pb, po, R, K, C], 1).astype(np.int64).reshape(len(R), 6)
     p.desc = np.stack([qa, lda, qb, ldb, po, R, K, C], 1).astype(np.int64).reshape(len(R), 8)
@@ -472,7 +550,8 @@ def to_canonical(s: Struct) -> Gather:
     if len(k) == 0:
         z = np.zeros(0, np.int64)
         return Gather((z, z, z, z), c.size, s.size)
-    return Gather(box_runs(s.off[k], c.shp, s.st[k], c.off, c.st), c.size, s.size)
+    runs, tiles = box_moves(s.off[k], c.shp, s.st[k], c.off, c.st)
+    return Gather(runs, c.size, s.size, tiles)
 
 
 def clear_cache() -> None:

@@ -28,7 +28,7 @@ __global__ void __launch_bounds__(kGatherThreads)
     gather_runs_kernel(const TS* __restrict__ src, TD* __restrict__ dst, long n,
                        const long* __restrict__ sbeg, const long* __restrict__ sstep,
                        const long* __restrict__ dbeg, const long* __restrict__ len,
-                       const int* __restrict__ clo, const int* __restrict__ chi) {
+                       const int* __restrict__ clo, const int* __restrict__ chi, int fill) {
   const long base = static_cast<long>(blockIdx.x) * kGatherChunk;
   const int lo = clo[blockIdx.x];
   const int hi = chi[blockIdx.x];  // runs [lo, hi) may overlap this chunk
@@ -46,27 +46,108 @@ __global__ void __launch_bounds__(kGatherThreads)
         b = mid;
     }
     r = a;
+    bool hit = false;
     TD v = TD(0);
     if (r < hi) {
       const long d0 = dbeg[r];
-      if (d0 <= j && j < d0 + len[r]) v = static_cast<TD>(src[sbeg[r] + (j - d0) * sstep[r]]);
+      if (d0 <= j && j < d0 + len[r]) {
+        v = static_cast<TD>(src[sbeg[r] + (j - d0) * sstep[r]]);
+        hit = true;
+      }
     }
-    dst[j] = v;
+    if (hit || fill) dst[j] = v;
   }
 }
 
 template <typename TS, typename TD>
 int launch_gather(long n, const void* src, void* dst, long nruns, const long* runs,
-                  const int* chunks, hipStream_t stream) {
+                  const int* chunks, int fill, hipStream_t stream) {
   if (n == 0) return TNH_OK;
   const long nchunks = (n + kGatherChunk - 1) / kGatherChunk;
   TNH_REQUIRE(nchunks < (1L << 31), TNH_ERR_ARG, "gather_runs: too many elements");
   hipLaunchKernelGGL((gather_runs_kernel<TS, TD>), dim3(static_cast<unsigned>(nchunks)),
                      dim3(kGatherThreads), 0, stream, static_cast<const TS*>(src),
                      static_cast<TD*>(dst), n, runs, runs + nruns, runs + 2 * nruns,
-                     runs + 3 * nruns, chunks, chunks + nchunks);
+                     runs + 3 * nruns, chunks, chunks + nchunks, fill);
   TNH_CHECK_LAUNCH("gather_runs");
   return TNH_OK;
+}
+
+// Transposing gather: tile t = (src0, scol, dst0, drow, nrows, ncols) moves
+//   dst[dst0 + i*drow + j] = src[src0 + i + j*scol],   i < nrows <= 64, j < ncols <= 64,
+// through a 64 x 65 LDS tile, so that both the reads (along i) and the writes (along j) are
+// contiguous across a wavefront.
+constexpr int kTile = 64;
+
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256)
+    gather_tiles_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
+                        const long* __restrict__ tiles) {
+  __shared__ TD buf[kTile][kTile + 1];
+  const long* tl = tiles + 6 * static_cast<long>(blockIdx.x);
+  const long src0 = tl[0], scol = tl[1], dst0 = tl[2], drow = tl[3];
+  const int nrows = static_cast<int>(tl[4]), ncols = static_cast<int>(tl[5]);
+  const int t = threadIdx.x;
+  const int i = t & 63;
+  for (int j = t >> 6; j < ncols; j += 4)
+    if (i < nrows) buf[j][i] = static_cast<TD>(src[src0 + i + static_cast<long>(j) * scol]);
+  __syncthreads();
+  const int jj = t & 63;
+  for (int ii = t >> 6; ii < nrows; ii += 4)
+    if (jj < ncols) dst[dst0 + static_cast<long>(ii) * drow + jj] = buf[jj][ii];
+}
+
+template <typename TS, typename TD>
+int launch_tiles(long ntiles, const void* src, void* dst, const long* tiles, hipStream_t stream) {
+  if (ntiles == 0) return TNH_OK;
+  TNH_REQUIRE(ntiles < (1L << 31), TNH_ERR_ARG, "gather_tiles: too many tiles");
+  hipLaunchKernelGGL((gather_tiles_kernel<TS, TD>), dim3(static_cast<unsigned>(ntiles)), dim3(256),
+                     0, stream, static_cast<const TS*>(src), static_cast<TD*>(dst), tiles);
+  TNH_CHECK_LAUNCH("gather_tiles");
+  return TNH_OK;
+}
+
+// Small grouped products (every group R <= 8 and K <= 8, the MPO contractions of the two-site
+// matvec): one thread per output column, the group's A (K x R) in registers, B rows read and
+// C rows written contiguously across the wavefront; memory bound, no matrix cores.
+// chunks (device, nchunks x 2 ints): (group, first column) of each workgroup's 1024 columns.
+constexpr int kSmall = 8;
+constexpr int kSmallCols = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    grouped_small_kernel(const long* __restrict__ desc, const int* __restrict__ chunks,
+                         const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C) {
+  const int g = chunks[2 * blockIdx.x];
+  const int c0 = chunks[2 * blockIdx.x + 1];
+  const long* d = desc + 8 * static_cast<long>(g);
+  const T* Ag = A + d[0];
+  const long lda = d[1];
+  const T* Bg = B + d[2];
+  const long ldb = d[3];
+  T* Cg = C + d[4];
+  const int R = static_cast<int>(d[5]), K = static_cast<int>(d[6]), N = static_cast<int>(d[7]);
+  T a[kSmall][kSmall];
+#pragma unroll
+  for (int k = 0; k < kSmall; ++k)
+#pragma unroll
+    for (int r = 0; r < kSmall; ++r) a[k][r] = (k < K && r < R) ? Ag[k * lda + r] : T(0);
+  for (int q = 0; q < kSmallCols / 256; ++q) {
+    const int c = c0 + q * 256 + threadIdx.x;
+    if (c >= N || c >= c0 + kSmallCols) break;
+    T b[kSmall];
+#pragma unroll
+    for (int k = 0; k < kSmall; ++k) b[k] = k < K ? Bg[k * ldb + c] : T(0);
+#pragma unroll
+    for (int r = 0; r < kSmall; ++r) {
+      if (r < R) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < kSmall; ++k) acc += a[k][r] * b[k];
+        Cg[static_cast<long>(r) * N + c] = acc;
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -245,22 +326,64 @@ using namespace tnh;
 extern "C" int tnh_gather_chunk(void) { return kGatherChunk; }
 
 extern "C" int tnh_gather_runs(int src_dtype, int dst_dtype, long n, const void* src, void* dst,
-                               long nruns, const long* runs, const int* chunks,
+                               long nruns, const long* runs, const int* chunks, int fill,
                                tnh_stream_t stream) {
   TNH_REQUIRE(n >= 0 && nruns >= 0, TNH_ERR_ARG, "tnh_gather_runs: negative size");
   TNH_REQUIRE(n == 0 || (dst && chunks && (nruns == 0 || (src && runs))), TNH_ERR_ARG,
               "tnh_gather_runs: null pointer");
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (src_dtype == TNH_F64 && dst_dtype == TNH_F64)
-    return launch_gather<double, double>(n, src, dst, nruns, runs, chunks, s);
+    return launch_gather<double, double>(n, src, dst, nruns, runs, chunks, fill, s);
   if (src_dtype == TNH_F32 && dst_dtype == TNH_F32)
-    return launch_gather<float, float>(n, src, dst, nruns, runs, chunks, s);
+    return launch_gather<float, float>(n, src, dst, nruns, runs, chunks, fill, s);
   if (src_dtype == TNH_F64 && dst_dtype == TNH_F32)
-    return launch_gather<double, float>(n, src, dst, nruns, runs, chunks, s);
+    return launch_gather<double, float>(n, src, dst, nruns, runs, chunks, fill, s);
   if (src_dtype == TNH_F32 && dst_dtype == TNH_F64)
-    return launch_gather<float, double>(n, src, dst, nruns, runs, chunks, s);
+    return launch_gather<float, double>(n, src, dst, nruns, runs, chunks, fill, s);
   set_error("tnh_gather_runs: unsupported dtypes %d -> %d", src_dtype, dst_dtype);
   return TNH_ERR_UNSUPPORTED;
+}
+
+extern "C" int tnh_gather_tiles(int src_dtype, int dst_dtype, long ntiles, const void* src,
+                                void* dst, const long* tiles, tnh_stream_t stream) {
+  TNH_REQUIRE(ntiles >= 0, TNH_ERR_ARG, "tnh_gather_tiles: negative size");
+  if (ntiles == 0) return TNH_OK;
+  TNH_REQUIRE(src && dst && tiles, TNH_ERR_ARG, "tnh_gather_tiles: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (src_dtype == TNH_F64 && dst_dtype == TNH_F64)
+    return launch_tiles<double, double>(ntiles, src, dst, tiles, s);
+  if (src_dtype == TNH_F32 && dst_dtype == TNH_F32)
+    return launch_tiles<float, float>(ntiles, src, dst, tiles, s);
+  if (src_dtype == TNH_F64 && dst_dtype == TNH_F32)
+    return launch_tiles<double, float>(ntiles, src, dst, tiles, s);
+  if (src_dtype == TNH_F32 && dst_dtype == TNH_F64)
+    return launch_tiles<float, double>(ntiles, src, dst, tiles, s);
+  set_error("tnh_gather_tiles: unsupported dtypes %d -> %d", src_dtype, dst_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+extern "C" int tnh_grouped_small(int dtype, const long* desc, long nchunks, const int* chunks,
+                                 const void* A, const void* B, void* C, tnh_stream_t stream) {
+  TNH_REQUIRE(nchunks >= 0 && nchunks < (1L << 31), TNH_ERR_ARG,
+              "tnh_grouped_small: bad chunk count");
+  if (nchunks == 0) return TNH_OK;
+  TNH_REQUIRE(desc && chunks && A && B && C, TNH_ERR_ARG, "tnh_grouped_small: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(static_cast<unsigned>(nchunks));
+  if (dtype == TNH_F64) {
+    hipLaunchKernelGGL(grouped_small_kernel<double>, grid, dim3(256), 0, s, desc, chunks,
+                       static_cast<const double*>(A), static_cast<const double*>(B),
+                       static_cast<double*>(C));
+  } else if (dtype == TNH_F32) {
+    hipLaunchKernelGGL(grouped_small_kernel<float>, grid, dim3(256), 0, s, desc, chunks,
+                       static_cast<const float*>(A), static_cast<const float*>(B),
+                       static_cast<float*>(C));
+  } else {
+    set_error("tnh_grouped_small: unsupported dtype %d", dtype);
+    return TNH_ERR_UNSUPPORTED;
+  }
+  TNH_CHECK_LAUNCH("grouped_small");
+  return TNH_OK;
 }
 
 extern "C" int tnh_grouped_tile(int dtype, int cfg, int* bm, int* bn, int* max_grid) {
