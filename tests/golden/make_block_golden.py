@@ -15,6 +15,10 @@ of theta (a, s1, s2, b), of the environments and of the bulk XXZ MPO (from tnkit
   * the output layout (blocks, offsets, strides, shapes, size) before and after the einsum's
     leg permutation,
 and finally the gather of the result into the canonical layout and the result's data.
+
+tests/golden/block_dmrg.json holds tnkit's block DMRG runs (subspace expansion off) on small U(1)
+XXZ chains: per-sweep energies, matvec counts, discarded weight, the final bond legs, and the
+bond legs and norms of its random start state.
 """
 from __future__ import annotations
 
@@ -111,6 +115,29 @@ def case(name, a_leg, b_leg, seed, out):
     dense = D.apply_two_site(theta.to_dense(), L.to_dense(), W.to_dense(), W.to_dense(),
                              R.to_dense())
     out[k + "final/dense"] = np.asarray(dense)
+    # truncation maps: fuse theta to (a s1, s2 b), truncated SVD, split both factors back
+    from tnkit.legs import fuse as lfuse
+    mat, rec = lfuse(theta, ((0, 1), (2, 3)), be)
+    out[k + "fuse/ix"] = index_array(B._fuse_index(theta.struct, rec)[1])
+    struct_arrays(k + "fuse/", mat.struct, out)
+    for g, fl in enumerate(rec.fused):
+        leg_arrays(k + f"fuse/leg{g}/", fl, out)
+    chi = max(2, theta.struct.size // 40)
+    U, S, Vh, disc = B.truncated_svd(mat, chi)
+    out[k + "tsvd/S"] = np.asarray(S)
+    out[k + "tsvd/disc"] = np.array(disc)
+    out[k + "tsvd/chi"] = np.array(chi)
+    leg_arrays(k + "tsvd/bond/", Vh.legs[0], out)
+    struct_arrays(k + "tsvd/U/", U.struct, out)
+    struct_arrays(k + "tsvd/Vh/", Vh.struct, out)
+    su, iu = B._split_index(U.struct, rec, (0,))
+    sv, iv = B._split_index(Vh.struct, rec, (1,))
+    out[k + "split_u/ix"] = index_array(iu)
+    struct_arrays(k + "split_u/", su, out)
+    out[k + "split_v/ix"] = index_array(iv)
+    struct_arrays(k + "split_v/", sv, out)
+    out[k + "scale_u/ix"] = index_array(B._leg_positions(U.struct, 1))
+    out[k + "scale_v/ix"] = index_array(B._leg_positions(Vh.struct, 0))
     print(name, "theta", theta.struct.size, "blocks", len(theta.struct.blocks),
           "gemms", [len(out[k + f"c{c + 1}/gemms"]) for c in range(4)])
 
@@ -140,5 +167,33 @@ def main():
     print("wrote blocks.npz with", len(out), "arrays")
 
 
+def block_dmrg_runs():
+    """tnkit's block DMRG (expand=0) on small U(1) XXZ chains: per-sweep energies, matvec counts,
+    discarded weight and the final bond legs; plus the legs of random_block_mps."""
+    import json
+    from tnkit.api import dmrg
+    from tnkit.block_backend import random_block_mps
+    be = get_backend("numpy-real")
+    runs = []
+    for N, chi, charge, seed, sweeps in ((10, 16, 0, 1, 3), (12, 24, 0, 2, 3), (10, 12, 2, 3, 3)):
+        mpo, _ = to_block_mpo(xxz_mpo(N), [[1, -1]] * N, be, sym=U1())
+        E, mps, info = dmrg(mpo, chi, charge=charge, expand=0, n_sweeps=sweeps, tol=0.0,
+                            rtol=0.0, seed=seed, backend="numpy-real")
+        start = random_block_mps(mpo, chi, be, charge, np.random.default_rng(seed))
+        runs.append({"N": N, "chi": chi, "charge": charge, "seed": seed, "n_sweeps": sweeps,
+                     "energy": float(E).hex(),
+                     "energies": [float(e).hex() for e in info["energies"]],
+                     "matvec_calls": int(info["matvec_calls"]),
+                     "discarded_weight": float(info["discarded_weight"]).hex(),
+                     "bonds": [[list(t.legs[2].charges), list(t.legs[2].dims)] for t in mps],
+                     "start_bonds": [[list(t.legs[2].charges), list(t.legs[2].dims)]
+                                     for t in start],
+                     "start_norms": [float(np.linalg.norm(t.data)).hex() for t in start]})
+        print("block dmrg", N, chi, charge, E, info["matvec_calls"])
+    with open(os.path.join(HERE, "block_dmrg.json"), "w") as f:
+        json.dump(runs, f, indent=1)
+
+
 if __name__ == "__main__":
     main()
+    block_dmrg_runs()

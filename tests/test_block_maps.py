@@ -209,3 +209,65 @@ def test_moves_match_explicit_positions(seed):
     assert np.array_equal(g.expand(), _explicit(src0, shp, ss, dst0, ds, n))
     if seed == 0:
         assert len(tiles) > 0
+
+
+def test_block_mpo_matches_tnkit():
+    """to_block_mpo of the XXZ chain: channel charges, legs and the bulk tensor's buffer equal
+    tnkit's (bit for bit)."""
+    import torch
+    from tnkit_amd import block_mps as BM
+    from tnkit_amd.models import xxz_mpo
+    mpo, orders = BM.to_block_mpo(xxz_mpo(6), [[1, -1]] * 6, torch.device("cpu"))
+    W = mpo[2]
+    assert W.legs == (leg("small", "wl"), leg("small", "wr"), leg("small", "p"),
+                      leg("small", "p").flip())
+    assert np.array_equal(W.data.numpy(), GOLD["small/W/data"])
+    chans = BM.mpo_charges(xxz_mpo(6), [[1, -1]] * 6)
+    assert [c.tolist() for c in chans] == [[0]] + [[0, 2, -2, 0, 0]] * 5 + [[0]]
+
+
+def test_random_block_mps_matches_tnkit():
+    """The bond sectors, dimensions and random entries of tnkit's random block MPS."""
+    import json
+    import torch
+    from tnkit_amd import block_mps as BM
+    from tnkit_amd.models import xxz_mpo
+    runs = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "block_dmrg.json")))
+    for r in runs:
+        mpo, _ = BM.to_block_mpo(xxz_mpo(r["N"]), [[1, -1]] * r["N"], torch.device("cpu"))
+        M = BM.random_block_mps(mpo, r["chi"], r["charge"], np.random.default_rng(r["seed"]),
+                                torch.device("cpu"))
+        assert [[list(t.legs[2].charges), list(t.legs[2].dims)] for t in M] == r["start_bonds"]
+        for t, h in zip(M, r["start_norms"]):
+            assert abs(float(np.linalg.norm(t.data.numpy())) - float.fromhex(h)) \
+                <= 1e-14 * float.fromhex(h)
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fuse_split_maps_match_tnkit(case):
+    """theta fused to (a s1, s2 b), and the factors of tnkit's truncated SVD split back: fused
+    legs, layouts and gather indices equal tnkit's; so do the leg positions of the S scaling."""
+    legs = operand_legs(case)
+    th = B.canonical(legs["theta"], 0)
+    rec = B.fuse_record(th.legs, 0, ((0, 1), (2, 3)))
+    for g in range(2):
+        k = f"{case}/fuse/leg{g}/"
+        assert rec.fused[g] == B.Leg(tuple(GOLD[k + "charges"].tolist()),
+                                     tuple(GOLD[k + "dims"].tolist()), int(GOLD[k + "sign"]))
+    out, g = B.fuse_map(th, rec)
+    same_struct(out, case, "fuse")
+    assert np.array_equal(g.expand(), GOLD[f"{case}/fuse/ix"])
+    bond = B.Leg(tuple(GOLD[f"{case}/tsvd/bond/charges"].tolist()),
+                 tuple(GOLD[f"{case}/tsvd/bond/dims"].tolist()), 1)
+    U = B.canonical((rec.fused[0], bond.flip()), 0)
+    Vh = B.canonical((bond, rec.fused[1]), 0)
+    same_struct(U, case, "tsvd/U")
+    same_struct(Vh, case, "tsvd/Vh")
+    su, gu = B.split_map(U, rec, (0,))
+    sv, gv = B.split_map(Vh, rec, (1,))
+    same_struct(su, case, "split_u")
+    same_struct(sv, case, "split_v")
+    assert np.array_equal(gu.expand(), GOLD[f"{case}/split_u/ix"])
+    assert np.array_equal(gv.expand(), GOLD[f"{case}/split_v/ix"])
+    assert np.array_equal(B.leg_positions(U, 1).expand(), GOLD[f"{case}/scale_u/ix"])
+    assert np.array_equal(B.leg_positions(Vh, 0).expand(), GOLD[f"{case}/scale_v/ix"])

@@ -347,6 +347,9 @@ class DMRGEngine:
 
 
 def _dmrg(mpo, chi: int, o: DMRGOptions):
+    if getattr(mpo[0], "is_block", False):          # U(1) block tensors: the block sweep
+        from .block_dmrg import block_dmrg
+        return block_dmrg(mpo, chi, o)
     eng = DMRGEngine(mpo, chi, o)
     be = eng.be
     energies: List[float] = []

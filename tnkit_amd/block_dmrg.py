@@ -1,0 +1,177 @@
+"""Two-site DMRG on U(1) block MPOs and MPSs, on the HIP device.
+
+The sweep of tnkit/_dmrg.py:560-799 taking its `block` branches, with subspace expansion off
+(expand=0): a random block MPS (`block_mps.random_block_mps`) made right-canonical by an LQ per
+block (tnkit/_dmrg.py:295-306), then per bond theta = M[i] M[i+1], the lowest eigenpair of the
+block H_eff (`block_ops.BlockTwoSite`, all sector GEMMs of each contraction in one grouped
+launch) by thick-restart or mixed-precision Lanczos on the flat canonical buffer
+(tnkit/_dmrg.py:638-644), the truncated SVD across sectors, and the block environment update.
+"""
+from __future__ import annotations
+
+import time
+from typing import Any, Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import block_linalg as BL
+from . import block_mps as BM
+from .block_ops import BlockTensor, BlockTwoSite, einsum
+from .eigensolver import lanczos_ground_mixed, lanczos_ground_restart
+from .mps import MPS
+
+
+def right_canonicalize(M: List[BlockTensor]) -> List[BlockTensor]:
+    """Every tensor right-canonical by an LQ per block, centre at site 0, the running centre
+    renormalised at each step (tnkit/_dmrg.py:295-306)."""
+    for i in range(len(M) - 1, 0, -1):
+        f, rec = BL.fuse(M[i], ((0,), (1, 2)))
+        Lm, Q = BL.qr(f, right=True)
+        M[i], Lm = BL.split(Q, rec, legs=(1,)), BL.split(Lm, rec, legs=(0,))
+        nxt = einsum("asb,bc->asc", M[i - 1], Lm)
+        M[i - 1] = nxt.with_data(nxt.data / BL.norm(nxt))
+    M[0] = M[0].with_data(M[0].data / BL.norm(M[0]))
+    return M
+
+
+def _check(o, mpo) -> None:
+    if o.sites != 2:
+        raise ValueError(f"sites must be 2 on the HIP backend, got {o.sites!r}")
+    if o.eig not in ("lanczos-restart", "lanczos-mixed"):
+        raise ValueError(f"unknown eig solver {o.eig!r} for the HIP block sweep; expected one of "
+                         "['lanczos-mixed', 'lanczos-restart']")
+    if o.trunc not in ("svd", "gram"):
+        raise ValueError(f"unknown trunc {o.trunc!r}; expected 'svd' or 'gram'")
+    if o.charge is None:
+        raise ValueError("block MPOs need charge= (the total charge of the state)")
+    if o.expand not in (None, 0):
+        raise ValueError("subspace expansion is not served by the HIP block sweep (expand=0)")
+    if o.init_mps is not None and not getattr(o.init_mps[0], "is_block", False):
+        raise ValueError("init_mps for a block MPO must hold block tensors")
+    if any(W.dtype != torch.float64 for W in mpo):
+        raise TypeError("the HIP block sweep takes float64 block MPOs")
+
+
+class BlockDMRGEngine:
+    """State of one block two-site DMRG run: MPS, environments, MPO, counters."""
+
+    def __init__(self, mpo: Sequence[BlockTensor], chi: int, o):
+        _check(o, mpo)
+        self.o, self.chi, self.N = o, int(chi), len(mpo)
+        self.mpo = list(mpo)
+        if o.init_mps is not None:
+            self.M = right_canonicalize([t for t in o.init_mps])
+        else:
+            self.M = right_canonicalize(BM.random_block_mps(self.mpo, chi, o.charge,
+                                                            np.random.default_rng(o.seed)))
+        self.rng = np.random.default_rng(o.seed + 1)
+        self.mixed_counts = {"lo": 0, "hi": 0}
+        N = self.N
+        self.L: List[Optional[BlockTensor]] = [None] * (N + 1)
+        self.R: List[Optional[BlockTensor]] = [None] * (N + 1)
+        self.L[0], self.R[N] = BM.edge_envs(self.M, self.mpo)
+        for i in range(N - 1, 0, -1):
+            self.R[i] = BM.update_right_env(self.R[i + 1], self.M[i], self.mpo[i])
+        self.total_disc = 0.0
+        self.matvec_calls = 0
+        self.phase = {"eig": 0.0, "trunc": 0.0, "env": 0.0}
+
+    def eigsolve(self, theta0: BlockTensor, op: BlockTwoSite, op_lo, cur_tol: float):
+        o = self.o
+        x0 = theta0.canonical()
+        count = {"n": 0}
+
+        def matvec(x):
+            count["n"] += 1
+            return op.apply(x)
+
+        def matvec_lo(x):
+            count["n"] += 1
+            return op_lo.apply(x)
+
+        n = x0.struct.size
+        v0 = x0.data if o.warm_start else torch.from_numpy(
+            self.rng.standard_normal(n)).to(x0.data.device)
+        mi = o.max_iter if o.max_iter is not None else min(n, max(200, 4 * self.chi))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if o.eig == "lanczos-mixed":
+            e, x, _ = lanczos_ground_mixed(matvec_lo, matvec, v0.clone(), tol=cur_tol,
+                                           max_iter=mi, m=o.restart_m, k=o.restart_k,
+                                           tol_lo=o.mixed_tol_lo, counts=self.mixed_counts)
+        else:
+            e, x, _ = lanczos_ground_restart(matvec, v0.clone(), tol=cur_tol, max_iter=mi,
+                                             m=o.restart_m, k=o.restart_k)
+        torch.cuda.synchronize()
+        self.phase["eig"] += time.perf_counter() - t0
+        self.matvec_calls += count["n"]
+        return e, BlockTensor(x0.struct, x.reshape(-1).contiguous())
+
+    def solve_block(self, i: int, direction: str, cur_tol: float) -> float:
+        o, M, L, R = self.o, self.M, self.L, self.R
+        W1, W2 = self.mpo[i], self.mpo[i + 1]
+        theta0 = einsum("asc,ctb->astb", M[i], M[i + 1])
+        op = BlockTwoSite(L[i], W1, W2, R[i + 2], theta_legs=theta0.legs,
+                          charge=theta0.charge)
+        op_lo = (BlockTwoSite(L[i], W1, W2, R[i + 2], theta_legs=theta0.legs,
+                              charge=theta0.charge, dtype=torch.float32)
+                 if o.eig == "lanczos-mixed" else None)
+        e, x = self.eigsolve(theta0, op, op_lo, cur_tol)
+        del op, op_lo
+        t0 = time.perf_counter()
+        mat, rec = BL.fuse(x, ((0, 1), (2, 3)))
+        U, S, Vh, disc = BL.truncated_svd(mat, self.chi, o.trunc)
+        S = S / torch.linalg.norm(S)
+        if direction == "right":
+            M[i] = BL.split(U, rec, legs=(0,))
+            M[i + 1] = BL.split(BL.scale(Vh, S, 0), rec, legs=(1,))
+        else:
+            M[i] = BL.split(BL.scale(U, S, 1), rec, legs=(0,))
+            M[i + 1] = BL.split(Vh, rec, legs=(1,))
+        torch.cuda.synchronize()
+        self.phase["trunc"] += time.perf_counter() - t0
+        self.total_disc += disc
+        t0 = time.perf_counter()
+        if direction == "right":
+            L[i + 1] = BM.update_left_env(L[i], M[i], W1)
+        else:
+            R[i + 1] = BM.update_right_env(R[i + 2], M[i + 1], W2)
+        torch.cuda.synchronize()
+        self.phase["env"] += time.perf_counter() - t0
+        return e
+
+    def sweep(self, cur_tol: float) -> float:
+        E = None
+        for i in range(0, self.N - 1):
+            E = self.solve_block(i, "right", cur_tol)
+        for i in range(self.N - 2, -1, -1):
+            E = self.solve_block(i, "left", cur_tol)
+        return E
+
+
+def block_dmrg(mpo: Sequence[BlockTensor], chi: int, o) -> tuple:
+    """The sweep loop of tnkit/_dmrg.py:767-799 on block tensors; returns (E, MPS, info)."""
+    eng = BlockDMRGEngine(mpo, chi, o)
+    energies: List[float] = []
+    sweep_s: List[float] = []
+    prev_E, E, sweep = None, None, -1
+    for sweep in range(o.n_sweeps):
+        t0 = time.perf_counter()
+        cur_tol = (o.eig_tol_warmup if (o.eig_tol_warmup is not None and sweep < o.warmup_sweeps)
+                   else o.eig_tol)
+        E = eng.sweep(cur_tol)
+        energies.append(E)
+        torch.cuda.synchronize()
+        sweep_s.append(time.perf_counter() - t0)
+        if o.verbose:
+            print(f"  sweep {sweep + 1:2d}: E = {E:.12f}  (chi={chi}, blocks, hip)")
+        if prev_E is not None and abs(E - prev_E) < max(o.tol, o.rtol * abs(E)):
+            break
+        prev_E = E
+    info: Dict[str, Any] = {
+        "sweeps": sweep + 1, "energies": energies, "backend": "hip-real", "blocks": True,
+        "eig": o.eig, "discarded_weight": eng.total_disc, "matvec_calls": eng.matvec_calls,
+        "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2, "expand": 0,
+        "expand_alphas": [0.0] * (sweep + 1), "sweep_s": sweep_s}
+    return E, MPS(eng.M, center=0), info

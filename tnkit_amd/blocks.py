@@ -572,3 +572,144 @@ def parse_einsum(sub: str):
     free = [c for c in ta if c not in shared] + [c for c in tb if c not in shared]
     return (tuple(ta.index(c) for c in shared), tuple(tb.index(c) for c in shared),
             tuple(free.index(c) for c in out))
+
+
+# ---------------------------------------------------------------------------------------------
+# fusing and splitting legs (the matrix views of truncation and canonicalisation)
+# ---------------------------------------------------------------------------------------------
+
+
+@dataclass(frozen=True)
+class FuseRecord:
+    """How groups of legs were fused (tnkit/blocks.py:1197-1224). Fused leg g is fused[g]
+    (sign +1); its sector k stacks the sub-blocks sectors[g][k], each (sector tuple over
+    groups[g], row offset, size), in lexicographic order."""
+    legs: Tuple[Leg, ...]
+    charge: int
+    groups: Tuple[Tuple[int, ...], ...]
+    fused: Tuple[Leg, ...]
+    sectors: Tuple
+
+    @property
+    def perm(self) -> Tuple[int, ...]:
+        return tuple(a for g in self.groups for a in g)
+
+
+@lru_cache(maxsize=4096)
+def fuse_record(legs: Tuple[Leg, ...], charge: int, groups) -> FuseRecord:
+    labels = tuple((leg.charges, leg.sign) for leg in legs)
+    fused, sectors = [], []
+    for g in groups:
+        by_q = _label_groups(labels, g)
+        qs = sorted(by_q)
+        secs, dims = [], []
+        for q in qs:
+            items, off = [], 0
+            for t in by_q[q]:
+                n = 1
+                for a, s in zip(g, t):
+                    n *= legs[a].dims[s]
+                items.append((t, off, n))
+                off += n
+            secs.append(tuple(items))
+            dims.append(off)
+        sectors.append(tuple(secs))
+        fused.append(Leg(tuple(qs), tuple(dims), 1))
+    return FuseRecord(tuple(legs), int(charge), tuple(tuple(g) for g in groups), tuple(fused),
+                      tuple(sectors))
+
+
+def _row_major(shape) -> List[int]:
+    st, n = [], 1
+    for d in reversed(shape):
+        st.append(n)
+        n *= d
+    return list(reversed(st))
+
+
+@lru_cache(maxsize=4096)
+def fuse_map(s: Struct, rec: FuseRecord) -> Tuple[Struct, Gather]:
+    """The fused layout (canonical over rec.fused) and the gather into it from layout s
+    (tnkit/blocks.py:1227-1241)."""
+    out = canonical(rec.fused, rec.charge)
+    perm, n = list(rec.perm), len(rec.legs)
+    kb_l, d0_l, ds_l = [], [], []
+    for k, fb in enumerate(out.blocks):
+        ost = out.st[k].tolist()
+        for subs in itertools.product(*(rec.sectors[g][fb[g]] for g in range(len(rec.groups)))):
+            kb = s.index[_compose(n, ((rec.groups[g], t) for g, (t, _, _) in enumerate(subs)))]
+            dims = s.shp[kb].tolist()
+            ds = [r * ost[g] for g, grp in enumerate(rec.groups)
+                  for r in _row_major([dims[a] for a in grp])]
+            kb_l.append(kb)
+            d0_l.append(int(out.off[k]) + sum(off * ost[g] for g, (_, off, _) in enumerate(subs)))
+            ds_l.append(ds)
+    if not kb_l:
+        z = np.zeros(0, np.int64)
+        return out, Gather((z, z, z, z), out.size, s.size)
+    kb = np.array(kb_l, np.int64)
+    runs, tiles = box_moves(s.off[kb], s.shp[kb][:, perm], s.st[kb][:, perm],
+                            np.array(d0_l, np.int64), np.array(ds_l, np.int64).reshape(len(kb), n))
+    return out, Gather(runs, out.size, s.size, tiles)
+
+
+@lru_cache(maxsize=4096)
+def split_map(s: Struct, rec: FuseRecord, todo: Tuple[int, ...]) -> Tuple[Struct, Gather]:
+    """Split the fused legs `todo` of a tensor laid out as s, keeping its other legs; a full split
+    also undoes the fuse's permutation (tnkit/blocks.py:1244-1286). Returns the canonical output
+    layout and the gather into it."""
+    where = {g: {t: (k, off, nn) for k, sec in enumerate(rec.sectors[g]) for t, off, nn in sec}
+             for g in todo}
+    legs, origin = [], []
+    for g in range(len(s.legs)):
+        if g in where:
+            legs += [rec.legs[a] for a in rec.groups[g]]
+            origin += [(g, p) for p in range(len(rec.groups[g]))]
+        else:
+            legs.append(s.legs[g])
+            origin.append((g, None))
+    full = len(todo) == len(rec.groups)
+    perm = tuple(int(i) for i in np.argsort(rec.perm)) if full else tuple(range(len(legs)))
+    out = canonical(tuple(legs[i] for i in perm), s.charge)
+    rows = []
+    for b in out.blocks:
+        bg = [0] * len(legs)
+        for j, i in enumerate(perm):
+            bg[i] = b[j]
+        fb, sub = [0] * len(s.legs), {}
+        for i, (g, p) in enumerate(origin):
+            if p is None:
+                fb[g] = bg[i]
+            else:
+                sub.setdefault(g, []).append(bg[i])
+        start = {}
+        for g, t in sub.items():
+            fb[g], start[g], _ = where[g][tuple(t)]
+        kb = s.index[tuple(fb)]
+        sst = s.st[kb].tolist()
+        inner = {g: _row_major([legs[i].dims[bg[i]] for i, (h, p) in enumerate(origin)
+                                if h == g and p is not None]) for g in sub}
+        st = [sst[g] if p is None else sst[g] * inner[g][p] for g, p in origin]
+        rows.append((int(s.off[kb]) + sum(start[g] * sst[g] for g in sub), [st[i] for i in perm]))
+    if not rows:
+        z = np.zeros(0, np.int64)
+        return out, Gather((z, z, z, z), out.size, s.size)
+    src0 = np.array([r[0] for r in rows], np.int64)
+    ss = np.array([r[1] for r in rows], np.int64).reshape(len(rows), len(legs))
+    runs, tiles = box_moves(src0, out.shp, ss, out.off, out.st)
+    return out, Gather(runs, out.size, s.size, tiles)
+
+
+@lru_cache(maxsize=4096)
+def leg_positions(s: Struct, axis: int) -> Gather:
+    """For every element of layout s, its sector-ordered index along leg `axis`, as a gather
+    from a vector over that leg (tnkit/blocks.py:670-679); used to scale a leg by a vector."""
+    if not len(s.off):
+        z = np.zeros(0, np.int64)
+        return Gather((z, z, z, z), s.size, s.legs[axis].size)
+    start = np.concatenate([[0], np.cumsum(s.legs[axis].dims)[:-1]]).astype(np.int64)
+    sec = np.array(s.blocks, np.int64).reshape(len(s.off), -1)[:, axis]
+    step = np.zeros_like(s.st)
+    step[:, axis] = 1
+    # destination positions: the buffer itself; its row-major order for canonical layouts
+    return Gather(box_runs(start[sec], s.shp, step, s.off, s.st), s.size, s.legs[axis].size)
