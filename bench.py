@@ -299,6 +299,40 @@ def block_bench(steps, warmup, chi=2048, local=True):
     return res
 
 
+def sweep_cost(N, chis, forms, log=None):
+    """tnkit's per-sweep cost study (BENCHMARKS.md, "Cost per DMRG sweep"): the Heisenberg chain
+    (XXZ, Jxy = Jz = 1) of N sites, exactly 3 sweeps from a random state at full chi, seconds per
+    sweep as the mean of sweeps 2-3; dense and U(1) block forms (2 S^z_total = 0)."""
+    import torch
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd import block_mps as BM
+    from tnkit_amd.models import xxz_mpo
+    dense_mpo = xxz_mpo(N)
+    dev = torch.device("cuda", 0)
+    block_mpo, _ = BM.to_block_mpo(dense_mpo, [[1, -1]] * N, dev)
+    out = []
+    for form in forms:
+        for chi in chis:
+            kw = dict(n_sweeps=3, tol=0.0, rtol=0.0, seed=0, expand=0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if form.startswith("u1"):
+                E, _, info = H.dmrg(block_mpo, chi, charge=0, eig=form.split(":")[1],
+                                    trunc="gram", **kw)
+            else:
+                E, _, info = H.dmrg(dense_mpo, chi, backend="hip-real", trunc="gram",
+                                    eig=form.split(":")[1], **kw)
+            rec = {"form": form, "chi": chi, "E": E, "sweep_s": info["sweep_s"],
+                   "s_per_sweep_2_3": sum(info["sweep_s"][1:3]) / 2,
+                   "wall_s": time.perf_counter() - t0, "matvec_calls": info["matvec_calls"],
+                   "phase_s": {k: round(v, 2) for k, v in info["phase_time"].items()},
+                   "rel_err_vs_ref": (E + 28.17542485974319) / 28.17542485974319}
+            out.append(rec)
+            if log:
+                print(json.dumps(rec), file=log, flush=True)
+    return out
+
+
 def config2(N, stages, eig, trunc, seed=0, log=None):
     """The precision study: XXZ chain N sites, dense two-site DMRG with the bond ramped through
     `stages` = [(chi, sweeps), ...], each stage started from the previous state (init_mps), as
@@ -344,6 +378,9 @@ def main():
     ap.add_argument("--config2", type=str, default="",
                     help="run the chi-ramp precision study instead, e.g. '64:4,512:4,1024:4'")
     ap.add_argument("--trunc", type=str, default="gram")
+    ap.add_argument("--sweep-cost", type=str, default="",
+                    help="per-sweep cost study, e.g. '256,512,1024' (Heisenberg chain, N=--N)")
+    ap.add_argument("--forms", type=str, default="u1:lanczos-restart,dense:lanczos-restart")
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
     ap.add_argument("--no-block", action="store_true")
@@ -376,6 +413,13 @@ def main():
 
     if args.block:
         print(json.dumps(block_bench(args.steps, args.warmup, local=not args.no_local)))
+        return
+    if args.sweep_cost:
+        chis = [int(c) for c in args.sweep_cost.split(",")]
+        res = {"workload": f"Heisenberg chain N={args.N}, 3 sweeps from random at full chi, "
+                           "s/sweep = mean of sweeps 2-3",
+               "runs": sweep_cost(args.N, chis, args.forms.split(","), log=sys.stderr)}
+        print(json.dumps(res))
         return
 
     cores = len(os.sched_getaffinity(0))

@@ -90,3 +90,45 @@ def test_block_dmrg_exact_in_sector(hip, q):
     Em, _, info = dmrg(mpo, 32, charge=q, expand=0, n_sweeps=4, tol=0.0, rtol=0.0, seed=5,
                        eig="lanczos-mixed")
     assert abs(Em - e0) < 1e-9 and info["mixed_counts"]["lo"] > 0
+
+
+@pytest.mark.parametrize("case", ["medium", "odd"])
+@pytest.mark.parametrize("iso", ["left", "right"])
+def test_truncated_gram_matches_svd(hip, case, iso):
+    """The batched Gram truncation keeps the same sectors and singular values as the SVD one,
+    gives an exact isometry on its `iso` side, and the same rank-chi product."""
+    from tnkit_amd.block_ops import einsum
+    a, b, p = _leg(f"{case}/leg_a/"), _leg(f"{case}/leg_b/"), _leg(f"{case}/leg_p/")
+    th = BlockTensor(B.canonical((a, p, p, b.flip()), 0),
+                     torch.from_numpy(GOLD[f"{case}/theta/data"]).to(hip))
+    mat, rec = BL.fuse(th, ((0, 1), (2, 3)))
+    chi = int(GOLD[f"{case}/tsvd/chi"])
+    U1, S1, V1, d1 = BL.truncated_svd(mat, chi)
+    U2, S2, V2, d2 = BL.truncated_gram(mat, chi, iso)
+    assert V1.legs[0] == V2.legs[0]
+    assert float((S1 - S2).abs().max()) <= 1e-10 * float(S1.max())
+    assert abs(d1 - d2) <= 1e-10 * float((S1 ** 2).sum() + d1)
+    iso_t = U2 if iso == "left" else V2
+    gram = einsum("ab,ac->bc", iso_t, iso_t.conj()) if iso == "left" else \
+        einsum("ba,ca->bc", iso_t, iso_t.conj())
+    for r, c, G in BL._matrix_blocks(gram.canonical()):
+        assert float((G - torch.eye(G.shape[0], dtype=G.dtype, device=hip)).abs().max()) < 1e-12
+    prod = lambda U, S, V: einsum("asc,ctb->astb", BL.split(U, rec, legs=(0,)),
+                                  BL.split(BL.scale(V, S, 0), rec, legs=(1,))).canonical().data
+    assert float((prod(U1, S1, V1) - prod(U2, S2, V2)).abs().max()) \
+        <= 1e-9 * float(th.data.abs().max())
+
+
+@pytest.mark.parametrize("q", [0, 2])
+def test_block_dmrg_gram_truncation_exact_in_sector(hip, q):
+    N = 10
+    mpo, _ = BM.to_block_mpo(xxz_mpo(N), [[1, -1]] * N, hip)
+    E, _, _ = dmrg(mpo, 32, charge=q, expand=0, n_sweeps=4, tol=0.0, rtol=0.0, seed=6,
+                   trunc="gram")
+    assert abs(E - _sector_ground(N, q)) < 1e-9
+    run = RUNS[1]
+    mpo, _ = BM.to_block_mpo(xxz_mpo(run["N"]), [[1, -1]] * run["N"], hip)
+    E, mps, info = dmrg(mpo, run["chi"], charge=run["charge"], expand=0,
+                        n_sweeps=run["n_sweeps"], tol=0.0, rtol=0.0, seed=run["seed"],
+                        trunc="gram")
+    assert abs(E - float.fromhex(run["energy"])) < 1e-8

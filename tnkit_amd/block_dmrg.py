@@ -121,7 +121,11 @@ class BlockDMRGEngine:
         del op, op_lo
         t0 = time.perf_counter()
         mat, rec = BL.fuse(x, ((0, 1), (2, 3)))
-        U, S, Vh, disc = BL.truncated_svd(mat, self.chi, o.trunc)
+        if o.trunc == "gram":
+            U, S, Vh, disc = BL.truncated_gram(mat, self.chi,
+                                               "left" if direction == "right" else "right")
+        else:
+            U, S, Vh, disc = BL.truncated_svd(mat, self.chi)
         S = S / torch.linalg.norm(S)
         if direction == "right":
             M[i] = BL.split(U, rec, legs=(0,))

@@ -145,3 +145,82 @@ def qr(m: BlockTensor, right: bool = False) -> Tuple[BlockTensor, BlockTensor]:
     Bq = torch.cat([b.reshape(-1) for _, _, _, b in new])
     return (BlockTensor(B.canonical((lr, bond.flip()), 0), A.contiguous()),
             BlockTensor(B.canonical((bond, lc), m.charge), Bq.contiguous()))
+
+
+def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
+    """`truncated_svd` through the Gram matrix of each block on the `iso` side: 'left' gives
+    U = the eigenvectors of M M^T (an exact left isometry) and S Vh = U^T M; 'right' gives
+    Vh = the eigenvectors of M^T M (an exact right isometry) and U S = M Vh^T. The per-block
+    Gram matrices (K-major MFMA GEMMs) are padded to a power-of-two size with -1 on the padding
+    diagonal, which Jacobi and divide-and-conquer solvers keep apart from the nonnegative Gram
+    spectrum, and diagonalised in a few batched calls; one host read of all eigenvalues makes
+    the global cut (chi largest, as `truncated_svd`). S is taken as the norms of the carried
+    factor, as tnkit's dense Gram truncation does (tnkit/linalg.py:138-204). Same return
+    values and new-leg rule as `truncated_svd`."""
+    if iso not in ("left", "right"):
+        raise ValueError(f"iso must be 'left' or 'right', got {iso!r}")
+    if m.ndim != 2 or m.dtype != torch.float64:
+        raise ValueError("truncated_gram needs a two-leg float64 block tensor")
+    m = m.canonical()
+    lr, lc = m.legs
+    blocks = list(_matrix_blocks(m))
+    if not blocks:
+        raise ValueError("the matrix has no allowed blocks")
+    dev = m.data.device
+    mats, grams = [], []
+    for r, c, M in blocks:
+        Mt = M.t().contiguous()
+        mats.append((M, Mt))
+        grams.append(kernels.gemm_tn(Mt, Mt) if iso == "left" else kernels.gemm_tn(M, M))
+    sizes = [g.shape[0] for g in grams]
+    # batched eigh per power-of-two size class
+    cls = {}
+    for j, n in enumerate(sizes):
+        cls.setdefault(max(32, 1 << (int(n) - 1).bit_length()), []).append(j)
+    w_all = [None] * len(grams)
+    v_all = [None] * len(grams)
+    for nmax, idx in cls.items():
+        Gp = torch.zeros((len(idx), nmax, nmax), dtype=torch.float64, device=dev)
+        diag = torch.full((len(idx), nmax), -1.0, dtype=torch.float64, device=dev)
+        for t, j in enumerate(idx):
+            n = sizes[j]
+            Gp[t, :n, :n] = grams[j]
+            diag[t, :n] = 0.0
+        Gp = Gp + torch.diag_embed(diag)
+        w, V = torch.linalg.eigh(Gp)
+        for t, j in enumerate(idx):
+            n = sizes[j]
+            # the n real eigenpairs are the largest n (the padding sits at -1), descending
+            w_all[j] = torch.flip(w[t, nmax - n:], (0,))
+            v_all[j] = torch.flip(V[t, :n, nmax - n:], (1,))
+    w_host = torch.cat(w_all).cpu().numpy()
+    s_host = np.sqrt(np.clip(w_host, 0.0, None))
+    owner = np.concatenate([np.full(n, j) for j, n in enumerate(sizes)])
+    order = np.lexsort((np.arange(s_host.size), -s_host))
+    k = min(int(chi), s_host.size)
+    kept = np.bincount(owner[order[:k]], minlength=len(blocks))
+    parts = []
+    for j, ((r, c, _), (M, Mt), n) in enumerate(zip(blocks, mats, kept)):
+        n = int(n)
+        if n == 0:
+            continue
+        Q = v_all[j][:, :n].contiguous()                   # exact isometry, (rows or cols) x n
+        if iso == "left":
+            carried = kernels.gemm_tn(Q, M)                # S Vh = U^T M   (n x C)
+            S = torch.linalg.norm(carried, dim=1)
+            Vh = carried / torch.where(S > 0, S, torch.ones_like(S))[:, None]
+            parts.append((r, Q, S, Vh))
+        else:
+            carried = kernels.gemm_tn(Mt, Q)               # U S = M Vh^T   (R x n)
+            S = torch.linalg.norm(carried, dim=0)
+            U = carried / torch.where(S > 0, S, torch.ones_like(S))[None, :]
+            parts.append((r, U, S, Q.t()))
+    disc = float((s_host[order[k:]] ** 2).sum())
+    label = (lambda r: lr.charges[r]) if lr.sign > 0 else (lambda r: -lr.charges[r])
+    new = sorted(((label(r), r, U, S, Vh) for r, U, S, Vh in parts), key=lambda x: (x[0], x[1]))
+    bond = B.Leg(tuple(q for q, *_ in new), tuple(int(S.numel()) for _, _, _, S, _ in new), 1)
+    U = torch.cat([u.reshape(-1) for _, _, u, _, _ in sorted(new, key=lambda x: x[1])])
+    S = torch.cat([s for _, _, _, s, _ in new])
+    Vh = torch.cat([v.reshape(-1) for _, _, _, _, v in new])
+    return (BlockTensor(B.canonical((lr, bond.flip()), 0), U.contiguous()), S,
+            BlockTensor(B.canonical((bond, lc), m.charge), Vh.contiguous()), disc)
