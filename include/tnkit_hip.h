@@ -184,10 +184,10 @@ int tnh_gather_chunk(void);
 int tnh_gather_runs(int src_dtype, int dst_dtype, long n, const void* src, void* dst, long nruns,
                     const long* runs, const int* chunks, int fill, tnh_stream_t stream);
 
-/* The transposing part of the same gathers (blocks whose source-contiguous and
- * destination-contiguous axes differ): tile t of `tiles` (device, ntiles x 6 longs: src0, scol,
- * dst0, drow, nrows <= 64, ncols <= 64) moves dst[dst0 + i*drow + j] = src[src0 + i + j*scol]
- * through LDS, contiguous on both sides. */
+/* The 2-D part of the same gathers (transposed blocks, and blocks of many short runs): tile t
+ * of `tiles` (device, ntiles x 7 longs: src0, srow, scol, dst0, drow, nrows <= 64,
+ * ncols <= 64) moves dst[dst0 + i*drow + j] = (dst_dtype) src[src0 + i*srow + j*scol];
+ * transposes (scol != 1) go through LDS. */
 int tnh_gather_tiles(int src_dtype, int dst_dtype, long ntiles, const void* src, void* dst,
                      const long* tiles, tnh_stream_t stream);
 

@@ -151,7 +151,8 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     """`truncated_svd` through the Gram matrix of each block on the `iso` side: 'left' gives
     U = the eigenvectors of M M^T (an exact left isometry) and S Vh = U^T M; 'right' gives
     Vh = the eigenvectors of M^T M (an exact right isometry) and U S = M Vh^T. The per-block
-    Gram matrices (K-major MFMA GEMMs) are padded to a power-of-two size with -1 on the padding
+    Gram matrices (K-major MFMA GEMMs) are padded to a power-of-two size (at least 128: small
+    batched eigensolves are slower than a few larger ones) with -1 on the padding
     diagonal, which Jacobi and divide-and-conquer solvers keep apart from the nonnegative Gram
     spectrum, and diagonalised in a few batched calls; one host read of all eigenvalues makes
     the global cut (chi largest, as `truncated_svd`). S is taken as the norms of the carried
@@ -176,7 +177,7 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     # batched eigh per power-of-two size class
     cls = {}
     for j, n in enumerate(sizes):
-        cls.setdefault(max(32, 1 << (int(n) - 1).bit_length()), []).append(j)
+        cls.setdefault(max(128, 1 << (int(n) - 1).bit_length()), []).append(j)
     w_all = [None] * len(grams)
     v_all = [None] * len(grams)
     for nmax, idx in cls.items():

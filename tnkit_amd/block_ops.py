@@ -76,9 +76,10 @@ def _device_runs(g: B.Gather, device, src_size: int) -> _DevRuns:
     t = g.tiles
     d.ntiles = len(t)
     if d.ntiles:
-        s0, sc, d0, dr, nr, nc = t.T
+        s0, sr, sc, d0, dr, nr, nc = t.T
         if ((nr < 1) | (nc < 1) | (nr > B.TILE) | (nc > B.TILE)).any() or s0.min() < 0 \
-                or (s0 + (nr - 1) + (nc - 1) * sc).max() >= src_size or d0.min() < 0 \
+                or (sr < 0).any() or (sc < 0).any() or (dr < 0).any() \
+                or (s0 + (nr - 1) * sr + (nc - 1) * sc).max() >= src_size or d0.min() < 0 \
                 or (d0 + (nr - 1) * dr + nc - 1).max() >= g.n:
             raise ValueError("gather tiles outside their buffers")
         d.tiles = torch.from_numpy(np.ascontiguousarray(t, np.int64).reshape(-1)).to(device)

@@ -189,22 +189,12 @@ def dual(s: Struct) -> Struct:
 # ---------------------------------------------------------------------------------------------
 
 
-def box_runs(src0, shp, ss, dst0, ds):
-    """Runs of a set of boxes. Box b covers multi-indices i < shp[b], read at src0[b] + i.ss[b]
-    and written at dst0[b] + i.ds[b]. Each box is cut into runs along its innermost axis of
-    extent > 1, merged outwards (axes of extent 1 are transparent) while the destination stays
-    contiguous and the source an arithmetic progression; a box whose innermost axis is not
-    destination-contiguous is cut into single elements. Returns (src start, src step, dst start,
-    length), sorted by destination."""
-    src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
-    shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
+def _run_shape(shp, ss, ds):
+    """Per box: which axes form its runs, the run length and the run's source step. A run starts
+    at the innermost axis of extent > 1 and merges outwards (axes of extent 1 are transparent)
+    while the destination stays contiguous and the source an arithmetic progression; a box whose
+    innermost axis is not destination-contiguous has runs of one element."""
     B, m = shp.shape
-    if B == 0:
-        z = np.zeros(0, np.int64)
-        return z, z, z, z
-    if m == 0:
-        one = np.ones(B, np.int64)
-        return src0, one, dst0, one
     rows = np.arange(B)
     big = shp > 1
     inner = np.where(big.any(1), m - 1 - np.argmax(big[:, ::-1], 1), m - 1)
@@ -221,17 +211,42 @@ def box_runs(src0, shp, ss, dst0, ds):
         in_run[:, j] |= act & (unit | grow)
         chain &= ~act | unit | grow
         tail = np.where(grow, tail * shp[:, j], tail)
-    outer = np.where(in_run, 1, shp)
+    return in_run, tail, step, inner
+
+
+def _enumerate(src0, ss, dst0, ds, outer):
+    """(box, src start, dst start) of every multi-index over the `outer` extents of each box."""
+    B, m = outer.shape
     nrun = outer.prod(1)
     box = np.repeat(np.arange(B), nrun)
     local = np.arange(box.size) - np.repeat(np.cumsum(nrun) - nrun, nrun)
     src, dst = src0[box].copy(), dst0[box].copy()
     for k in range(m - 1, -1, -1):
         d = outer[box, k]
+        if not (d > 1).any():
+            continue
         i = local % d
         local //= d
         src += i * ss[box, k]
         dst += i * ds[box, k]
+    return box, src, dst
+
+
+def box_runs(src0, shp, ss, dst0, ds):
+    """Runs of a set of boxes. Box b covers multi-indices i < shp[b], read at src0[b] + i.ss[b]
+    and written at dst0[b] + i.ds[b]; each box is cut into runs as `_run_shape` says. Returns
+    (src start, src step, dst start, length), sorted by destination."""
+    src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
+    shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
+    B, m = shp.shape
+    if B == 0:
+        z = np.zeros(0, np.int64)
+        return z, z, z, z
+    if m == 0:
+        one = np.ones(B, np.int64)
+        return src0, one, dst0, one
+    in_run, tail, step, _ = _run_shape(shp, ss, ds)
+    box, src, dst = _enumerate(src0, ss, dst0, ds, np.where(in_run, 1, shp))
     o = np.argsort(dst, kind="stable")
     return src[o], step[box][o], dst[o], tail[box][o]
 
@@ -249,63 +264,75 @@ def expand_runs(runs, n: int, fill: int = -1) -> np.ndarray:
     return out
 
 
-TILE = 64          # edge of the transposing tiles
-TILE_MIN = 8       # boxes with both contiguous extents at least this go through tiles
+TILE = 64          # edge of the 2-D tiles
+TILE_MIN = 8       # smallest extent worth a tile
+SHORT_RUN = 32     # runs shorter than this are grouped into row tiles
 
 
-def box_moves(src0, shp, ss, dst0, ds):
-    """Split boxes between the two gather kernels. A box whose destination-contiguous axis a_d
-    (innermost of extent > 1, ds = 1) is strided in the source, and which has another axis a_s
-    contiguous in the source (ss = 1), both of extent >= TILE_MIN, is a batch of 2-D transposes:
-    it is cut into tiles (src0, scol, dst0, drow, nrows, ncols) of at most TILE x TILE moving
-    dst[dst0 + i drow + j] = src[src0 + i + j scol]. Every other box becomes runs (`box_runs`).
-    Returns (runs, tiles) with tiles a (T, 6) int64 array."""
-    src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
-    shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
-    nb, m = shp.shape
-    empty = np.zeros((0, 6), np.int64)
-    if nb == 0 or m < 2:
-        return box_runs(src0, shp, ss, dst0, ds), empty
-    rows = np.arange(nb)
-    big = shp > 1
-    a_d = np.where(big.any(1), m - 1 - np.argmax(big[:, ::-1], 1), m - 1)
-    cand = (ss == 1) & (shp >= TILE_MIN)
-    cand[rows, a_d] = False
-    a_s = np.argmax(cand, 1)
-    use = (cand.any(1) & (ds[rows, a_d] == 1) & (ss[rows, a_d] != 1)
-           & (shp[rows, a_d] >= TILE_MIN))
-    runs = box_runs(src0[~use], shp[~use], ss[~use], dst0[~use], ds[~use])
-    if not use.any():
-        return runs, empty
-    b = np.flatnonzero(use)
-    ad, as_ = a_d[b], a_s[b]
-    outer = shp[b].copy()
-    outer[np.arange(len(b)), ad] = 1
-    outer[np.arange(len(b)), as_] = 1
-    nout = outer.prod(1)
-    box = np.repeat(np.arange(len(b)), nout)
-    local = np.arange(box.size) - np.repeat(np.cumsum(nout) - nout, nout)
-    sb, db = src0[b][box].copy(), dst0[b][box].copy()
-    for k in range(m - 1, -1, -1):
-        d = outer[box, k]
-        i = local % d
-        local //= d
-        sb += i * ss[b][box, k]
-        db += i * ds[b][box, k]
-    ns = shp[b][box, as_[box]]
-    nd = shp[b][box, ad[box]]
-    scol = ss[b][box, ad[box]]
-    drow = ds[b][box, as_[box]]
-    ti, tj = -(-ns // TILE), -(-nd // TILE)
+def _tiles(sb, db, srow, drow, scol, nrows, ncols):
+    """Cut 2-D slabs (i < nrows, j < ncols; src = sb + i srow + j scol, dst = db + i drow + j)
+    into TILE x TILE tiles (src0, srow, scol, dst0, drow, nrows, ncols)."""
+    ti, tj = -(-nrows // TILE), -(-ncols // TILE)
     nt = ti * tj
-    slab = np.repeat(np.arange(box.size), nt)
+    slab = np.repeat(np.arange(sb.size), nt)
     loc = np.arange(slab.size) - np.repeat(np.cumsum(nt) - nt, nt)
     i0 = (loc // tj[slab]) * TILE
     j0 = (loc % tj[slab]) * TILE
-    tiles = np.stack([sb[slab] + i0 + j0 * scol[slab], scol[slab],
-                      db[slab] + i0 * drow[slab] + j0, drow[slab],
-                      np.minimum(TILE, ns[slab] - i0), np.minimum(TILE, nd[slab] - j0)], 1)
-    return runs, tiles.astype(np.int64)
+    return np.stack([sb[slab] + i0 * srow[slab] + j0 * scol[slab], srow[slab], scol[slab],
+                     db[slab] + i0 * drow[slab] + j0, drow[slab],
+                     np.minimum(TILE, nrows[slab] - i0), np.minimum(TILE, ncols[slab] - j0)],
+                    1).astype(np.int64)
+
+
+def box_moves(src0, shp, ss, dst0, ds):
+    """Split boxes between the gather kernels. Tiles (src0, srow, scol, dst0, drow, nrows, ncols)
+    move dst[dst0 + i drow + j] = src[src0 + i srow + j scol] for i < nrows, j < ncols (at most
+    TILE each); everything else becomes runs (`box_runs`). Two kinds of box are tiled:
+      * transposes: the destination-contiguous axis a_d is strided in the source and another
+        axis a_s is source-contiguous, both of extent >= TILE_MIN (i over a_s, j over a_d);
+      * short runs: the run is shorter than SHORT_RUN and an axis outside it has extent
+        >= TILE_MIN (i over that axis, j along the run), so many short runs move as one tile.
+    Returns (runs, tiles) with tiles a (T, 7) int64 array."""
+    src0, dst0 = np.asarray(src0, np.int64), np.asarray(dst0, np.int64)
+    shp, ss, ds = (np.asarray(x, np.int64) for x in (shp, ss, ds))
+    nb, m = shp.shape
+    empty = np.zeros((0, 7), np.int64)
+    if nb == 0 or m < 2:
+        return box_runs(src0, shp, ss, dst0, ds), empty
+    rows = np.arange(nb)
+    in_run, tail, step, a_d = _run_shape(shp, ss, ds)
+    ok_d = ds[rows, a_d] == 1
+    cand = (ss == 1) & (shp >= TILE_MIN)
+    cand[rows, a_d] = False
+    a_s = np.argmax(cand, 1)
+    trans = (cand.any(1) & ok_d & (ss[rows, a_d] != 1) & (shp[rows, a_d] >= TILE_MIN))
+    out_ext = np.where(in_run, 0, shp)
+    a_o = np.argmax(out_ext, 1)
+    short = (~trans & ok_d & (tail < SHORT_RUN) & (out_ext[rows, a_o] >= TILE_MIN))
+    rest = ~(trans | short)
+    runs = box_runs(src0[rest], shp[rest], ss[rest], dst0[rest], ds[rest])
+    parts = []
+    if trans.any():
+        b = np.flatnonzero(trans)
+        k = np.arange(len(b))
+        outer = shp[b].copy()
+        outer[k, a_d[b]] = 1
+        outer[k, a_s[b]] = 1
+        box, sb, db = _enumerate(src0[b], ss[b], dst0[b], ds[b], outer)
+        bb = b[box]
+        parts.append(_tiles(sb, db, np.ones_like(sb), ds[bb, a_s[bb]], ss[bb, a_d[bb]],
+                            shp[bb, a_s[bb]], shp[bb, a_d[bb]]))
+    if short.any():
+        b = np.flatnonzero(short)
+        k = np.arange(len(b))
+        outer = np.where(in_run[b], 1, shp[b])
+        outer[k, a_o[b]] = 1
+        box, sb, db = _enumerate(src0[b], ss[b], dst0[b], ds[b], outer)
+        bb = b[box]
+        parts.append(_tiles(sb, db, ss[bb, a_o[bb]], ds[bb, a_o[bb]], step[bb],
+                            shp[bb, a_o[bb]], tail[bb]))
+    tiles = np.concatenate(parts) if parts else empty
+    return runs, tiles
 
 
 class Gather:
@@ -316,7 +343,7 @@ class Gather:
 
     def __init__(self, runs, n: int, src_size: int, tiles=None):
         self.runs, self.n = runs, int(n)
-        self.tiles = np.zeros((0, 6), np.int64) if tiles is None else tiles
+        self.tiles = np.zeros((0, 7), np.int64) if tiles is None else tiles
         src, sstep, dst, length = runs
         self.identity = bool(len(self.tiles) == 0 and self.n == src_size
                              and int(length.sum()) == self.n and (src == dst).all()
@@ -326,10 +353,10 @@ class Gather:
     def expand(self) -> np.ndarray:
         """idx[dst] = src over the destination, -1 at padding."""
         out = expand_runs(self.runs, self.n)
-        for s0, sc, d0, dr, nr, nc in self.tiles.tolist():
+        for s0, sr, sc, d0, dr, nr, nc in self.tiles.tolist():
             i = np.arange(nr)[:, None]
             j = np.arange(nc)[None, :]
-            out[(d0 + i * dr + j).ravel()] = (s0 + i + j * sc).ravel()
+            out[(d0 + i * dr + j).ravel()] = (s0 + i * sr + j * sc).ravel()
         return out
 
 
@@ -453,8 +480,27 @@ class Plan:
              offset qb; padding zero.
     desc   : (G, 8) int64 rows (qa, lda, qb, ldb, po, R, K, C) for the grouped GEMM.
     """
-    __slots__ = ("gemms", "macs", "nmm", "out", "ref_a", "ref_b", "dev_a", "dev_b", "desc",
-                 "size_a", "size_b", "_cache")
+    __slots__ = ("gemms", "macs", "nmm", "out", "dev_a", "dev_b", "desc", "size_a", "size_b",
+                 "_cache", "_ref")
+
+    @property
+    def ref_a(self) -> "Gather":
+        return self._reference(0)
+
+    @property
+    def ref_b(self) -> "Gather":
+        return self._reference(1)
+
+    def _reference(self, side: int) -> "Gather":
+        """tnkit's own stacking of operand `side` (built on demand: the device path does not
+        use it)."""
+        key = ("ref", side)
+        g = self._cache.get(key)
+        if g is None:
+            sp, s, axes = self._ref[side]
+            moves, n, *_ = _stack(sp, s, axes, side, False)
+            g = self._cache[key] = Gather(moves[0], n, s.size)
+        return g
 
 
 def _stack(sp: SymPlan, s: Struct, axes, side: int, ld_pad: bool):
@@ -517,11 +563,8 @@ def plan(sa: Struct, sb: Struct, ax_a: Tuple[int, ...], ax_b: Tuple[int, ...]) -
                              "match and the signs be opposite")
     sp = _sym_plan(sa.labels, sb.labels, tuple(ax_a), tuple(ax_b))
     p = Plan()
-    ra, na_, _, _, (r_shp, ro, R), K = _stack(sp, sa, ax_a, 0, False)
-    rb, nb_, _, _, (c_shp, co, C), _ = _stack(sp, sb, ax_b, 1, False)
-    p.ref_a, p.ref_b = Gather(ra[0], na_, sa.size), Gather(rb[0], nb_, sb.size)
-    da, p.size_a, qa, lda, _, _ = _stack(sp, sa, ax_a, 0, True)
-    db, p.size_b, qb, ldb, _, _ = _stack(sp, sb, ax_b, 1, True)
+    da, p.size_a, qa, lda, (r_shp, ro, R), K = _stack(sp, sa, ax_a, 0, True)
+    db, p.size_b, qb, ldb, (c_shp, co, C), _ = _stack(sp, sb, ax_b, 1, True)
     p.dev_a = Gather(da[0], p.size_a, sa.size, da[1])
     p.dev_b = Gather(db[0], p.size_b, sb.size, db[1])
     pa, pb, po = _excl(R * K), _excl(K * C), _excl(R * C)
@@ -538,6 +581,7 @@ def plan(sa: Struct, sb: Struct, ax_a: Tuple[int, ...], ax_b: Tuple[int, ...]) -
                     sp.charge, sp.out_blocks, po[go] + ro[orr] * C[go] + co[oc], st,
                     np.concatenate([r_shp[orr], c_shp[oc]], 1), int((R * C).sum()))
     p._cache = {}
+    p._ref = ((sp, sa, tuple(ax_a)), (sp, sb, tuple(ax_b)))
     _PLANS[key] = p
     return p
 

@@ -73,10 +73,12 @@ int launch_gather(long n, const void* src, void* dst, long nruns, const long* ru
   return TNH_OK;
 }
 
-// Transposing gather: tile t = (src0, scol, dst0, drow, nrows, ncols) moves
-//   dst[dst0 + i*drow + j] = src[src0 + i + j*scol],   i < nrows <= 64, j < ncols <= 64,
-// through a 64 x 65 LDS tile, so that both the reads (along i) and the writes (along j) are
-// contiguous across a wavefront.
+// Tiled gather: tile t = (src0, srow, scol, dst0, drow, nrows, ncols) moves
+//   dst[dst0 + i*drow + j] = src[src0 + i*srow + j*scol],   i < nrows <= 64, j < ncols <= 64.
+// With scol == 1 (rows contiguous on both sides: many short runs moved together) the tile is
+// copied directly, its elements spread over all lanes; otherwise it is a transpose staged through
+// a 64 x 65 LDS tile, read along i (contiguous in the source when srow == 1) and written along j
+// (contiguous in the destination).
 constexpr int kTile = 64;
 
 template <typename TS, typename TD>
@@ -84,13 +86,24 @@ __global__ void __launch_bounds__(256)
     gather_tiles_kernel(const TS* __restrict__ src, TD* __restrict__ dst,
                         const long* __restrict__ tiles) {
   __shared__ TD buf[kTile][kTile + 1];
-  const long* tl = tiles + 6 * static_cast<long>(blockIdx.x);
-  const long src0 = tl[0], scol = tl[1], dst0 = tl[2], drow = tl[3];
-  const int nrows = static_cast<int>(tl[4]), ncols = static_cast<int>(tl[5]);
+  const long* tl = tiles + 7 * static_cast<long>(blockIdx.x);
+  const long src0 = tl[0], srow = tl[1], scol = tl[2], dst0 = tl[3], drow = tl[4];
+  const int nrows = static_cast<int>(tl[5]), ncols = static_cast<int>(tl[6]);
   const int t = threadIdx.x;
+  if (scol == 1) {
+    const int n = nrows * ncols;
+    for (int e = t; e < n; e += 256) {
+      const int i = e / ncols, j = e - i * ncols;
+      dst[dst0 + static_cast<long>(i) * drow + j] =
+          static_cast<TD>(src[src0 + static_cast<long>(i) * srow + j]);
+    }
+    return;
+  }
   const int i = t & 63;
   for (int j = t >> 6; j < ncols; j += 4)
-    if (i < nrows) buf[j][i] = static_cast<TD>(src[src0 + i + static_cast<long>(j) * scol]);
+    if (i < nrows)
+      buf[j][i] = static_cast<TD>(src[src0 + static_cast<long>(i) * srow +
+                                      static_cast<long>(j) * scol]);
   __syncthreads();
   const int jj = t & 63;
   for (int ii = t >> 6; ii < nrows; ii += 4)
