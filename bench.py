@@ -333,6 +333,74 @@ def sweep_cost(N, chis, forms, log=None):
     return out
 
 
+def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, log=None):
+    """Batched time evolution: `legs` TFIM quenches (N sites) from DMRG ground states at chi = 16
+    for fields g0 ~ U[1.2, 2.0] (default_rng(1)) to g = 1, two-site TDVP with per-leg bond caps
+    spread over 64..256 and a relative discarded-weight budget svd_tol per cut (so each leg's
+    bond follows its entanglement up to its cap), complex128 and complex64-operator (complex128
+    Krylov) forms, <Z_j>(t) against the exact free-fermion curves and against single-leg
+    tdvp2_evolve for two legs."""
+    import numpy as np
+    import torch
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd import free_fermion as F
+    from tnkit_amd import tdvp as T
+    from tnkit_amd.models import tfim_mpo
+    from tnkit_amd.observables import local_expectations
+    Z = np.diag([1.0, -1.0])
+    g0 = np.random.default_rng(1).uniform(1.2, 2.0, legs)
+    caps = [int(c) for c in np.linspace(64, 256, legs).round()]
+    t0 = time.perf_counter()
+    states = []
+    for j, g in enumerate(g0):
+        _, mps, _ = H.dmrg(tfim_mpo(N, float(g)), 16, n_sweeps=4, tol=1e-12, rtol=0.0, seed=j,
+                           backend="hip-real")
+        states.append([t.to(torch.complex128) for t in mps])
+        if log:
+            print(f"ground state {j} g0={g:.3f}", file=log, flush=True)
+    t_gs = time.perf_counter() - t0
+    times = [k * meas_every * dt for k in range(n_steps // meas_every + 1)]
+    ff = [F.quench_z(N, float(g), 1.0, times) for g in g0]
+    mpo1 = tfim_mpo(N, 1.0)
+    meas = lambda ss: [local_expectations(s, Z) for s in ss]  # noqa: E731
+    out = {"workload": f"{legs} TFIM quenches N={N}, g0~U[1.2,2.0] (seed 1) -> g=1, TDVP2 "
+                       f"dt={dt} x {n_steps} steps, per-leg caps 64..256, svd_tol={svd_tol}",
+           "legs": legs, "N": N, "caps": caps, "ground_state_s": t_gs}
+    seq = {}
+    for name, low in (("c128", False), ("c64_krylov_c128", True)):
+        stats = {}
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prog = (lambda k, t: print(f"{name} step {k} t={t:.2f} "
+                                   f"{time.perf_counter() - t1:.1f}s", file=log, flush=True)) \
+            if log else None
+        res, rec = T.evolve_batch(states, mpo1, dt, n_steps, chi_max=caps, krylov_m=20,
+                                  keep_device=True, measure=meas, meas_every=meas_every,
+                                  stats=stats, low_precision=low, progress=prog,
+                                  svd_tol=svd_tol)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        err = max(float(np.abs(np.asarray(zs[j]) - ff[j][k]).max())
+                  for k, (_, zs) in enumerate(rec) for j in range(legs))
+        out[name] = {"wall_s": wall, "s_per_step": wall / n_steps,
+                     "leg_steps_per_s": legs * n_steps / wall,
+                     "z_max_err_vs_free_fermion": err, "krylov_m": stats.get("krylov_m"),
+                     "max_bond": max(max(b) for b in stats["bonds"]),
+                     "z_final_leg0": [float(v) for v in rec[-1][1][0][:4]]}
+        if not low:
+            for j in (0, legs - 1):
+                _, r1 = T.tdvp2_evolve(states[j], mpo1, caps[j], dt, n_steps, krylov_m=20,
+                                       measure=lambda m: local_expectations(m, Z),
+                                       meas_every=meas_every, svd_tol=svd_tol)
+                seq[j] = r1
+            out[name]["z_max_diff_vs_sequential"] = max(
+                float(np.abs(np.asarray(rec[k][1][j]) - np.asarray(seq[j][k][1])).max())
+                for j in seq for k in range(len(rec)))
+        if log:
+            print(json.dumps({name: out[name]}), file=log, flush=True)
+    return out
+
+
 def config2(N, stages, eig, trunc, seed=0, log=None):
     """The precision study: XXZ chain N sites, dense two-site DMRG with the bond ramped through
     `stages` = [(chi, sweeps), ...], each stage started from the previous state (init_mps), as
@@ -381,6 +449,10 @@ def main():
     ap.add_argument("--sweep-cost", type=str, default="",
                     help="per-sweep cost study, e.g. '256,512,1024' (Heisenberg chain, N=--N)")
     ap.add_argument("--forms", type=str, default="u1:lanczos-restart,dense:lanczos-restart")
+    ap.add_argument("--config4", action="store_true",
+                    help="batched TDVP study: 32 TFIM quenches N=64, per-leg caps 64..256")
+    ap.add_argument("--legs", type=int, default=32)
+    ap.add_argument("--tsteps", type=int, default=40)
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
     ap.add_argument("--no-block", action="store_true")
@@ -413,6 +485,10 @@ def main():
 
     if args.block:
         print(json.dumps(block_bench(args.steps, args.warmup, local=not args.no_local)))
+        return
+    if args.config4:
+        print(json.dumps(config4(args.legs, 64 if args.N == 100 else args.N, args.tsteps,
+                                 log=sys.stderr)))
         return
     if args.sweep_cost:
         chis = [int(c) for c in args.sweep_cost.split(",")]

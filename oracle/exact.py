@@ -79,3 +79,43 @@ def evolve_dense(mpo, psi0, t):
     import scipy.linalg
     H = mpo_to_dense(mpo)
     return scipy.linalg.expm(-1j * t * H) @ psi0
+
+
+def tfim_majorana_matrix(g):
+    """A (2N x 2N, real antisymmetric) of H = -sum X_j X_{j+1} - sum g_j Z_j = (i/4) g^T A g in
+    the Majoranas g_{2j} = (prod_{k<j} Z_k) X_j, g_{2j+1} = (prod_{k<j} Z_k) Y_j, for which
+    Z_j = -i g_{2j} g_{2j+1} and X_j X_{j+1} = -i g_{2j+1} g_{2j+2}: the field puts 2 g_j at
+    (2j, 2j+1), the bond 2 at (2j+1, 2j+2). The Jordan-Wigner construction behind
+    tnkit/free_fermion_dynamics.py:31-51, with per-site fields."""
+    g = np.asarray(g, dtype=np.float64)
+    N = g.size
+    A = np.zeros((2 * N, 2 * N))
+    for j in range(N):
+        A[2 * j, 2 * j + 1] = 2.0 * g[j]
+        if j + 1 < N:
+            A[2 * j + 1, 2 * j + 2] = 2.0
+    return A - A.T
+
+
+def tfim_ground_covariance(g):
+    """Majorana covariance Gamma_kl = i <g_k g_l> (k != l) of the ground state of
+    `tfim_majorana_matrix(g)`: Gamma = i sign(iA)."""
+    A = tfim_majorana_matrix(g)
+    w, V = np.linalg.eigh(1j * A)
+    return np.real(1j * (V * np.sign(w)) @ V.conj().T)
+
+
+def tfim_quench_z(g0, g1, times):
+    """<Z_j>(t) after the quench from the ground state of fields g0 to fields g1: Heisenberg
+    evolution g(t) = exp(A1 t) g(0), Gamma(t) = R Gamma(0) R^T, <Z_j> = -Gamma_{2j, 2j+1}
+    (tnkit/free_fermion_dynamics.py:54-115). Returns an array (len(times), N)."""
+    import scipy.linalg
+    G0 = tfim_ground_covariance(g0)
+    A1 = tfim_majorana_matrix(g1)
+    N = A1.shape[0] // 2
+    out = []
+    for t in times:
+        R = scipy.linalg.expm(A1 * t)
+        G = R @ G0 @ R.T
+        out.append(-np.array([G[2 * j, 2 * j + 1] for j in range(N)]))
+    return np.array(out)

@@ -166,3 +166,27 @@ def test_batch_geometry_errors(hip):
                        chi_max=4)
     with pytest.raises(ValueError):
         T.evolve_batch([_ground(6, 1.2, 4)], tfim_mpo(6, 1.0), 0.05, 1)
+
+
+def test_evolve_batch_wide_caps_small_chain(hip):
+    """Four quenches on N = 16 with caps 64..256, far above the bonds the chain can hold: padded
+    legs of low Schmidt rank give Gram matrices with large null spaces (the case where the
+    batched Hermitian eigensolver needs its SVD fallback); every leg stays finite and matches the
+    exact free-fermion <Z_j>(t)."""
+    from tnkit_amd import free_fermion as F
+    from tnkit_amd._dmrg import dmrg as hdmrg
+    N, legs = 16, 4
+    g0 = np.random.default_rng(1).uniform(1.2, 2.0, legs)
+    caps = [int(c) for c in np.linspace(64, 256, legs).round()]
+    states = []
+    for j, g in enumerate(g0):
+        _, mps, _ = hdmrg(tfim_mpo(N, float(g)), 16, n_sweeps=4, tol=1e-12, rtol=0.0, seed=j,
+                          backend="hip-real")
+        states.append([t.to(torch.complex128) for t in mps])
+    _, rec = T.evolve_batch(states, tfim_mpo(N, 1.0), 0.05, 4, chi_max=caps, krylov_m=20,
+                            keep_device=True, meas_every=2,
+                            measure=lambda ss: [local_expectations(s, Z) for s in ss])
+    for t, zs in rec:
+        for j in range(legs):
+            ref = F.quench_z(N, float(g0[j]), 1.0, [t])[0]
+            assert np.isfinite(zs[j]).all() and np.abs(zs[j] - ref).max() < 1e-6, (t, j)

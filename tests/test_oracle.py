@@ -339,3 +339,44 @@ def test_mpo_gradient_vs_finite_differences_and_hellmann_feynman():
         assert abs(grad[i] - ff) < 1e-7
     z = D.local_expectations(M, np.diag([1.0, -1.0]))
     np.testing.assert_allclose(grad, -z, atol=1e-7)
+
+
+def test_free_fermion_quench_matches_exact_evolution():
+    """The Majorana covariance dynamics (TFIM quench g0 -> g1, per-site fields) against dense
+    exact evolution of the exact ground state, N = 7."""
+    from oracle import exact as X
+    from tnkit_amd.models import tfim_mpo_fields
+    N = 7
+    rng = np.random.default_rng(4)
+    g0 = 1.3 + 0.2 * rng.standard_normal(N)
+    g1 = np.full(N, 0.9)
+    times = [0.0, 0.3, 1.1]
+    ff = X.tfim_quench_z(g0, g1, times)
+    H0 = X.mpo_to_dense(tfim_mpo_fields(g0))
+    w, V = np.linalg.eigh(0.5 * (H0 + H0.conj().T))
+    psi0 = V[:, 0]
+    mpo1 = tfim_mpo_fields(g1)
+    Z = np.diag([1.0, -1.0])
+    for t, z in zip(times, ff):
+        ref = X.site_expectations(X.evolve_dense(mpo1, psi0, t), Z, N)
+        assert np.abs(z - ref).max() < 1e-10
+    # the ground state energy from the covariance: E = (i/4) sum A_kl <g_k g_l> = -tr(A Gamma)/4
+    A = X.tfim_majorana_matrix(g0)
+    G = X.tfim_ground_covariance(g0)
+    assert abs(-0.25 * np.trace(A @ G) - w[0]) < 1e-10
+    assert abs(X.tfim_free_fermion_energy(N, g0) - w[0]) < 1e-10
+
+
+def test_product_free_fermion_matches_oracle():
+    """tnkit_amd.free_fermion (host reference curves used by the benchmarks) against the oracle's
+    covariance dynamics and free-fermion energy, N = 40 with per-site fields."""
+    from oracle import exact as X
+    from tnkit_amd import free_fermion as F
+    rng = np.random.default_rng(9)
+    N = 40
+    g0 = rng.uniform(1.2, 2.0, N)
+    times = [0.0, 0.5, 2.0]
+    assert np.abs(F.quench_z(N, g0, 1.0, times) - X.tfim_quench_z(g0, np.ones(N), times)).max() \
+        < 1e-10
+    assert abs(F.tfim_ground_energy(N, g0) - X.tfim_free_fermion_energy(N, g0)) < 1e-10
+    assert abs(F.tfim_ground_energy(32, 1.0) - X.tfim_free_fermion_energy(32, 1.0)) < 1e-12

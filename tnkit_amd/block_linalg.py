@@ -16,6 +16,7 @@ import torch
 from . import blocks as B
 from . import kernels
 from .block_ops import BlockTensor, gather
+from .tdvp import _eigh_robust
 
 
 def fuse(x: BlockTensor, groups) -> Tuple[BlockTensor, B.FuseRecord]:
@@ -152,9 +153,9 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     U = the eigenvectors of M M^T (an exact left isometry) and S Vh = U^T M; 'right' gives
     Vh = the eigenvectors of M^T M (an exact right isometry) and U S = M Vh^T. The per-block
     Gram matrices (K-major MFMA GEMMs) are padded to a power-of-two size (at least 128: small
-    batched eigensolves are slower than a few larger ones) with -1 on the padding
-    diagonal, which Jacobi and divide-and-conquer solvers keep apart from the nonnegative Gram
-    spectrum, and diagonalised in a few batched calls; one host read of all eigenvalues makes
+    batched eigensolves are slower than a few larger ones) with distinct values in [-2, -1) on
+    the padding diagonal, which keeps the padding apart from the nonnegative Gram spectrum and
+    free of degeneracy, and diagonalised in a few batched calls; one host read of all eigenvalues makes
     the global cut (chi largest, as `truncated_svd`). S is taken as the norms of the carried
     factor, as tnkit's dense Gram truncation does (tnkit/linalg.py:138-204). Same return
     values and new-leg rule as `truncated_svd`."""
@@ -182,16 +183,16 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     v_all = [None] * len(grams)
     for nmax, idx in cls.items():
         Gp = torch.zeros((len(idx), nmax, nmax), dtype=torch.float64, device=dev)
-        diag = torch.full((len(idx), nmax), -1.0, dtype=torch.float64, device=dev)
+        diag = (-(1.0 + torch.arange(nmax, dtype=torch.float64, device=dev) / nmax))[None, :] \
+            .repeat(len(idx), 1)
         for t, j in enumerate(idx):
             n = sizes[j]
             Gp[t, :n, :n] = grams[j]
             diag[t, :n] = 0.0
-        Gp = Gp + torch.diag_embed(diag)
-        w, V = torch.linalg.eigh(Gp)
+        w, V = _eigh_robust(Gp, diag)
         for t, j in enumerate(idx):
             n = sizes[j]
-            # the n real eigenpairs are the largest n (the padding sits at -1), descending
+            # the n real eigenpairs are the largest n (the padding sits below -1), descending
             w_all[j] = torch.flip(w[t, nmax - n:], (0,))
             v_all[j] = torch.flip(V[t, :n, nmax - n:], (1,))
     w_host = torch.cat(w_all).cpu().numpy()

@@ -56,6 +56,7 @@ class BatchTDVPOptions:
     meas_every: int = 1
     stats: Optional[Dict[str, Any]] = None
     low_precision: bool = False        # complex64 matvec with complex128 Krylov vectors
+    progress: Optional[Callable[[int, float], None]] = None   # called after each time step
 
 
 # --------------------------------------------------------------------------------------------
@@ -151,6 +152,26 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
     return M
 
 
+def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None):
+    """Batched eigh (ascending) of the Hermitian positive semidefinite G + diag(add) on the
+    device. rocSOLVER's batched Hermitian eigensolver returns non-finite eigenvectors for some
+    complex Gram matrices of low rank in wide padded bonds; those legs are decomposed instead by
+    an SVD of the unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it returns
+    ascending like eigh. Raises if a leg stays non-finite."""
+    Gs = G if add is None else G + torch.diag_embed(add.to(G.dtype))
+    w, Z = torch.linalg.eigh(Gs)
+    bad = (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
+    if not bool(bad.any()):
+        return w, Z
+    idx = torch.nonzero(bad).flatten()
+    U, S, _ = torch.linalg.svd(G[idx])
+    w[idx] = torch.flip(S, dims=[1]).to(w.dtype)
+    Z[idx] = torch.flip(U, dims=[2])
+    if not (bool(torch.isfinite(Z).all()) and bool(torch.isfinite(w).all())):
+        raise FloatingPointError("Hermitian eigendecomposition returned non-finite values")
+    return w, Z
+
+
 def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
               iso: str):
     """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
@@ -168,7 +189,13 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     G = (torch.matmul(mat.mH.resolve_conj().contiguous(), mat) if tall
          else torch.matmul(mat, mat.mH.resolve_conj().contiguous()))   # (k, r, r)
     G = 0.5 * (G + G.mH)
-    w, Z = torch.linalg.eigh(G)                         # ascending
+    # padded legs leave exact zero rows; a large exactly degenerate zero block makes the batched
+    # eigensolver return NaN, so those rows get distinct negative diagonal entries, which keeps
+    # them below the nonnegative Gram spectrum and out of every kept set
+    r_ = G.shape[1]
+    dg = torch.diagonal(G, dim1=1, dim2=2).real
+    shift = -(1.0 + torch.arange(r_, device=G.device, dtype=torch.float64) / r_)
+    w, Z = _eigh_robust(G, torch.where(dg == 0, shift[None, :], torch.zeros_like(dg)))
     w = torch.flip(w, dims=[1]).clamp(min=0.0)
     Z = torch.flip(Z, dims=[2])
     r = min(m, n)
@@ -193,7 +220,17 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     mk = torch.arange(nk, device=mat.device)[None, :] < torch.as_tensor(ranks, device=mat.device)[:, None]
     Zk = Z[:, :, :nk] * mk[:, None, :]
     if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
-        Q, Rr = torch.linalg.qr(torch.matmul(mat, Zk))
+        # a leg keeping fewer than nk columns has exact zero columns at the end of A Z, which the
+        # batched Householder QR turns into NaN; they are filled with fixed Gaussian columns. The
+        # kept columns come first, so their Q and R do not see the fill, and the fill's rows of
+        # R meet the zero rows of Z^H below.
+        AZ = torch.matmul(mat, Zk)
+        if bool((~mk).any()):
+            gen = torch.Generator(device=mat.device)
+            gen.manual_seed(12345)
+            fill = torch.randn(AZ.shape, generator=gen, device=mat.device, dtype=torch.float64)
+            AZ = torch.where(mk[:, None, :], AZ, fill.to(AZ.dtype))
+        Q, Rr = torch.linalg.qr(AZ)
         U = Q * mk[:, None, :]
         C = torch.matmul(Rr, Zk.mH.resolve_conj().contiguous())        # (k, nk, n) carried factor
     else:
@@ -358,6 +395,8 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
             m = min(m_max, m + 2)
         if o.measure is not None and (step + 1) % o.meas_every == 0:
             records.append(((step + 1) * dt, o.measure(snapshot())))
+        if o.progress is not None:
+            o.progress(step + 1, (step + 1) * dt)
     if o.stats is not None:
         o.stats.update({"krylov_m": m, "resid_max": resid_max.tolist(),
                         "disc_per_leg": disc_tot.tolist(), "bonds": [list(map(int, r))
