@@ -401,6 +401,60 @@ def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, lo
     return out
 
 
+def config5(N=64, chi=256, fd_sites=(0, 21, 42, 63), h=1e-5, log=None):
+    """Differentiation: the TFIM with per-site fields g_i = 1 + 0.1 N(0,1) (default_rng(2)),
+    two-site DMRG at chi (float64), then dE/dg_i by the hand-written backward kernels
+    (MPOExpectation), compared with -<Z_i> (Hellmann-Feynman), with central differences of the
+    exact free-fermion ground energy, and with central differences of warm-started DMRG energies
+    at a few sites."""
+    import numpy as np
+    import torch
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd import free_fermion as F
+    from tnkit_amd.autodiff import field_gradient
+    from tnkit_amd.models import tfim_mpo_fields
+    from tnkit_amd.observables import local_expectations
+    g = 1.0 + 0.1 * np.random.default_rng(2).standard_normal(N)
+    opts = dict(n_sweeps=6, tol=1e-12, rtol=0.0, seed=0, backend="hip-real", trunc="gram")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    E, mps, info = H.dmrg(tfim_mpo_fields(g), chi, **opts)
+    torch.cuda.synchronize()
+    t_dmrg = time.perf_counter() - t0
+    if log:
+        print(f"dmrg E={E:.14f} sweeps={info['sweeps']} {t_dmrg:.1f}s", file=log, flush=True)
+    t0 = time.perf_counter()
+    grad = field_gradient(list(mps), torch.as_tensor(g)).cpu().numpy()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    z = local_expectations(list(mps), np.diag([1.0, -1.0]))
+    E_exact = F.tfim_ground_energy(N, g)
+    fd_exact = np.empty(N)
+    for i in range(N):
+        e = np.zeros(N)
+        e[i] = h
+        fd_exact[i] = (F.tfim_ground_energy(N, g + e) - F.tfim_ground_energy(N, g - e)) / (2 * h)
+    fd_dmrg = {}
+    for i in fd_sites:
+        e = np.zeros(N)
+        e[i] = h
+        Ep, _, _ = H.dmrg(tfim_mpo_fields(g + e), chi, **dict(opts, n_sweeps=3,
+                                                                init_mps=list(mps)))
+        Em, _, _ = H.dmrg(tfim_mpo_fields(g - e), chi, **dict(opts, n_sweeps=3,
+                                                                init_mps=list(mps)))
+        fd_dmrg[i] = (Ep - Em) / (2 * h)
+        if log:
+            print(f"fd site {i}: {fd_dmrg[i]:.10f} vs grad {grad[i]:.10f}", file=log, flush=True)
+    return {"workload": f"TFIM N={N}, g_i = 1 + 0.1 N(0,1) (seed 2), two-site DMRG chi={chi}",
+            "E": E, "E_exact_free_fermion": E_exact, "dE_abs": abs(E - E_exact),
+            "dmrg_s": t_dmrg, "gradient_s": t_grad,
+            "max_abs_grad_plus_z": float(np.abs(grad + z).max()),
+            "max_abs_grad_minus_exact_fd": float(np.abs(grad - fd_exact).max()),
+            "max_abs_grad_minus_dmrg_fd": float(max(abs(grad[i] - v) for i, v in
+                                                    fd_dmrg.items())),
+            "fd_h": h, "fd_sites": list(fd_sites)}
+
+
 def config2(N, stages, eig, trunc, seed=0, log=None):
     """The precision study: XXZ chain N sites, dense two-site DMRG with the bond ramped through
     `stages` = [(chi, sweeps), ...], each stage started from the previous state (init_mps), as
@@ -452,6 +506,8 @@ def main():
     ap.add_argument("--config4", action="store_true",
                     help="batched TDVP study: 32 TFIM quenches N=64, per-leg caps 64..256")
     ap.add_argument("--legs", type=int, default=32)
+    ap.add_argument("--config5", action="store_true",
+                    help="differentiation study: TFIM N=64 per-site fields, chi=256")
     ap.add_argument("--tsteps", type=int, default=40)
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
@@ -485,6 +541,9 @@ def main():
 
     if args.block:
         print(json.dumps(block_bench(args.steps, args.warmup, local=not args.no_local)))
+        return
+    if args.config5:
+        print(json.dumps(config5(64 if args.N == 100 else args.N, log=sys.stderr)))
         return
     if args.config4:
         print(json.dumps(config4(args.legs, 64 if args.N == 100 else args.N, args.tsteps,
