@@ -196,6 +196,9 @@ __global__ void __launch_bounds__(256) mix_kernel(MixArgs a, const TC* __restric
 // thread computes the outputs of one group from LDS, and the outputs leave through LDS in memory
 // order the same way. Every global access of a wave is then a contiguous run.
 constexpr int kStageG = 256;
+// LDS row stride of the staging buffer: one more than kStageG, so that lanes holding the same
+// group t in different rows (the strided load and store orders) fall in different banks
+constexpr int kStageLd = kStageG + 1;
 constexpr int kStageMax = 32;
 template <typename T>
 struct StageMax {  // register budget of the staged kernel: 32 real or 16 complex per thread
@@ -250,7 +253,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
         const int L = threadIdx.x + i * kStageG;
         int ji, t;
         stage_decode(in_mode, L, a.nji, a.nji2, ji, t);
-        dst[i] = ji * kStageG + t;
+        dst[i] = ji * kStageLd + t;
         const long g2 = g20 + t;
         if (g2 < a.G2) {
           const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
@@ -268,7 +271,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   for (int o = 0; o < kMax; ++o) acc[o] = TC(0);
   const int t = threadIdx.x;
   for (int ji = 0; ji < a.nji; ++ji) {
-    const TC x = buf[ji * kStageG + t];
+    const TC x = buf[ji * kStageLd + t];
 #pragma unroll
     for (int o = 0; o < kMax; ++o) {
       if (o < a.njo) {
@@ -280,7 +283,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   __syncthreads();
 #pragma unroll
   for (int o = 0; o < kMax; ++o)
-    if (o < a.njo) buf[o * kStageG + t] = acc[o];
+    if (o < a.njo) buf[o * kStageLd + t] = acc[o];
   __syncthreads();
   const long ob = g1 * a.os1;
   const int nout = a.njo * kStageG;
@@ -290,7 +293,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
     const long g2 = g20 + tt;
     if (g2 < a.G2) {
       const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
-      out[ob + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(buf[jo * kStageG + tt]);
+      out[ob + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(buf[jo * kStageLd + tt]);
     }
   }
 }
@@ -332,7 +335,7 @@ int mix(const MixArgs& a, int batch, const void* Mx, const void* in, void* out, 
   const int im = stage_mode(a.is2, a.ijs1, a.ijs2, a.nji, a.nji2);
   const int om = stage_mode(a.os2, a.ojs1, a.ojs2, a.njo, a.njo2);
   const int rows = a.nji > a.njo ? a.nji : a.njo;
-  const size_t lds = (((size_t)a.njo * a.nji + 1) / 2 * 2 + (size_t)rows * kStageG) * sizeof(TC);
+  const size_t lds = (((size_t)a.njo * a.nji + 1) / 2 * 2 + (size_t)rows * kStageLd) * sizeof(TC);
   constexpr int kMax = StageMax<TC>::value;
   if (a.nji <= kMax && a.njo <= kMax && im >= 0 && om >= 0 && a.G1 <= 65535 && lds <= 65536) {
     dim3 grid(static_cast<unsigned>((a.G2 + kStageG - 1) / kStageG), static_cast<unsigned>(a.G1),
