@@ -190,3 +190,23 @@ def test_evolve_batch_wide_caps_small_chain(hip):
         for j in range(legs):
             ref = F.quench_z(N, float(g0[j]), 1.0, [t])[0]
             assert np.isfinite(zs[j]).all() and np.abs(zs[j] - ref).max() < 1e-6, (t, j)
+
+
+def test_graphed_krylov_equals_eager(hip, monkeypatch):
+    """The batched evolution with the Krylov loops captured in hipGraphs (each signature eager
+    once, then captured and replayed, with eager work between replays) equals the eager run."""
+    N, legs = 10, 3
+    states = [_ground(N, g, 6, seed=j) for j, g in enumerate((1.2, 1.5, 1.9))]
+    mpo = tfim_mpo(N, 1.0)
+    kw = dict(chi_max=[6, 8, 10], krylov_m=12, keep_device=True)
+    monkeypatch.setenv("TNKIT_AMD_GRAPHS", "0")
+    ref, _ = T.evolve_batch(states, mpo, 0.05, 4, **kw)
+    monkeypatch.setenv("TNKIT_AMD_GRAPHS", "1")
+    T.clear_graphs()
+    got, _ = T.evolve_batch(states, mpo, 0.05, 4, **kw)
+    assert len(T._GRAPHS) > 0
+    for a, b in zip(ref, got):
+        for x, y in zip(a, b):
+            assert x.shape == y.shape
+            assert float((x - y).abs().max()) <= 1e-12 * max(1.0, float(x.abs().max()))
+    T.clear_graphs()

@@ -15,6 +15,8 @@ with exp(+i H1 dt/2) unless it sits at the chain end. Sweep right then left (2nd
 from __future__ import annotations
 
 import dataclasses
+import os
+from collections import OrderedDict
 from typing import Any, Callable, Dict, List, Optional, Sequence, Tuple, Union
 
 import numpy as np
@@ -64,24 +66,90 @@ class BatchTDVPOptions:
 # --------------------------------------------------------------------------------------------
 
 
+_GRAPHS: "OrderedDict" = OrderedDict()       # key -> (graph, static op, V, ab)
+_GRAPH_SEEN: set = set()
+_GRAPH_MAX_BYTES = 256 << 20                 # largest Krylov basis captured (bytes)
+_GRAPH_CACHE_BYTES = 2048 << 20              # all captured bases together
+_graph_bytes = [0]
+
+
+def graphs_enabled() -> bool:
+    return os.environ.get("TNKIT_AMD_GRAPHS", "1") != "0"
+
+
+def clear_graphs() -> None:
+    _GRAPHS.clear()
+    _GRAPH_SEEN.clear()
+    _graph_bytes[0] = 0
+
+
+def _krylov_graph(key, op, make_mv, k: int, m: int, n: int, dev):
+    """The captured Krylov loop for `key`, or None to run eagerly. As tnkit's site-step graphs
+    (tnkit/cudagraph.py:87-342, block_graph.py:137-151) a shape signature runs eagerly the first
+    time and is captured the second: the m matvecs and orthogonalisation steps become one
+    hipGraph over static copies of the operator, the basis V and the coefficients ab. Bases
+    above _GRAPH_MAX_BYTES are not captured (their kernels are long enough on their own); the
+    cache keeps the most recent signatures within _GRAPH_CACHE_BYTES."""
+    g = _GRAPHS.get(key)
+    if g is not None:
+        _GRAPHS.move_to_end(key)
+        return g
+    nbytes = k * m * n * 16
+    if nbytes > _GRAPH_MAX_BYTES or torch.cuda.is_current_stream_capturing():
+        return None
+    if key not in _GRAPH_SEEN:
+        _GRAPH_SEEN.add(key)
+        return None
+    while _GRAPHS and _graph_bytes[0] + nbytes > _GRAPH_CACHE_BYTES:
+        _, old = _GRAPHS.popitem(last=False)
+        _graph_bytes[0] -= old[4]
+    sop = tuple(t.clone() for t in op)
+    V = torch.zeros((k, m, n), dtype=torch.complex128, device=dev)
+    ab = torch.zeros((k, 2 * m), dtype=torch.float64, device=dev)
+    ws = kernels.expm_ws(k, dev)
+    mv = make_mv(sop)
+    torch.cuda.synchronize(dev)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for j in range(m):
+            w = mv(V[:, j]).reshape(k, n)
+            kernels.expm_step_c(V, w, ab, j, ws)
+    entry = (graph, sop, V, ab, nbytes, ws)
+    _GRAPHS[key] = entry
+    _graph_bytes[0] += nbytes
+    return entry
+
+
 def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
-                               resid: Optional[np.ndarray] = None) -> torch.Tensor:
+                               resid: Optional[np.ndarray] = None, graph=None) -> torch.Tensor:
     """exp(tau H_leg) v_leg for k legs at once, from a fixed m Krylov steps
     (tnkit/batch_tdvp.py:247-324, tnkit/eigensolver.py:95-217): CGS2 with alpha from the first
     projection, clamped division instead of a breakdown branch, exp of the k tridiagonals by
     their eigendecompositions. `v` is (k, n) complex128. `resid` (length k) receives each leg's
-    Saad a-posteriori error estimate |beta_m tau [e^{tau T}]_{m-1,0}| beta0."""
+    Saad a-posteriori error estimate |beta_m tau [e^{tau T}]_{m-1,0}| beta0. `graph` =
+    (key, operator tensors, make_mv) lets the Krylov loop run as a captured hipGraph."""
     k, n = v.shape
     m = max(1, min(int(m), n))
     dev = v.device
-    V = torch.zeros((k, m, n), dtype=torch.complex128, device=dev)
-    ab = torch.zeros((k, 2 * m), dtype=torch.float64, device=dev)
     beta0 = torch.linalg.vector_norm(v, dim=1)
-    V[:, 0] = v / torch.clamp(beta0, min=1e-200)[:, None]
-    ws = kernels.expm_ws(k, dev)
-    for j in range(m):
-        w = matvec_b(V[:, j]).reshape(k, n).to(torch.complex128).contiguous()
-        kernels.expm_step_c(V, w, ab, j, ws)
+    entry = None
+    if graph is not None and graphs_enabled():
+        key, op, make_mv = graph
+        entry = _krylov_graph((key, m), op, make_mv, k, m, n, dev)
+    if entry is not None:
+        g, sop, V, ab, _, _ = entry
+        for st, t in zip(sop, op):
+            st.copy_(t)
+        V[:, 0].copy_(v / torch.clamp(beta0, min=1e-200)[:, None])
+        g.replay()
+    else:
+        V = torch.zeros((k, m, n), dtype=torch.complex128, device=dev)
+        ab = torch.zeros((k, 2 * m), dtype=torch.float64, device=dev)
+        V[:, 0] = v / torch.clamp(beta0, min=1e-200)[:, None]
+        ws = kernels.expm_ws(k, dev)
+        for j in range(m):
+            w = matvec_b(V[:, j]).reshape(k, n).to(torch.complex128).contiguous()
+            kernels.expm_step_c(V, w, ab, j, ws)
     abh = ab.cpu().numpy()
     b0 = beta0.cpu().numpy()
     al, be = abh[:, :m], abh[:, m:]
@@ -247,29 +315,34 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
 
 
 def _matvec_two(Bt: _Batch, i: int, shape, low: bool):
+    """(matvec, graph context) of the batched two-site H_eff at bond i."""
     op = kernels.fused_two_site_b(Bt.L[i], Bt.W[i], Bt.W[i + 1], Bt.R[i + 2],
                                   dtype=torch.complex64 if low else None)
-    ws = [None]
 
-    def mv(x):
-        y = kernels.apply_two_site_b(x.reshape(shape), *op, ws=ws[0])
-        return y.reshape(shape[0], -1)
-    return mv
+    def make(o):
+        def mv(x):
+            return kernels.apply_two_site_b(x.reshape(shape), *o).reshape(shape[0], -1)
+        return mv
+    key = ("two", tuple(shape), tuple(tuple(t.shape) for t in op), low)
+    return make(op), (key, op, make)
 
 
 def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool):
     """One-site H_eff (tnkit/_dmrg.py:192-199) as the two-site kernel with d2 = 1 and the
-    identity of the right MPO bond as the second MPO tensor: S[(s r)][(w s')] = W[w][r][s][s']."""
+    identity of the right MPO bond as the second MPO tensor: S[(s r)][(w s')] = W[w][r][s][s'].
+    Returns (matvec, graph context)."""
     k, a, d, b = shape
     wr = int(W.shape[2])
     Wid = torch.eye(wr, dtype=W.dtype, device=W.device).reshape(1, wr, wr, 1, 1)
     Wid = Wid.expand(k, -1, -1, -1, -1).contiguous()
     op = kernels.fused_two_site_b(L, W, Wid, R, dtype=torch.complex64 if low else None)
 
-    def mv(x):
-        y = kernels.apply_two_site_b(x.reshape(k, a, d, 1, b), *op)
-        return y.reshape(k, -1)
-    return mv
+    def make(o):
+        def mv(x):
+            return kernels.apply_two_site_b(x.reshape(k, a, d, 1, b), *o).reshape(k, -1)
+        return mv
+    key = ("one", tuple(shape), tuple(tuple(t.shape) for t in op), low)
+    return make(op), (key, op, make)
 
 
 def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, moving: str,
@@ -279,8 +352,8 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
     a, b = A.shape[1], Bm.shape[3]
     theta = torch.matmul(A.reshape(k, a * d, -1), Bm.reshape(k, Bm.shape[1], -1))
     shape = (k, a, d, d, b)
-    theta = lanczos_expm_fixed_batched(_matvec_two(Bt, i, shape, low), theta.reshape(k, -1),
-                                       -0.5j * dt, m, resid)
+    mv, gctx = _matvec_two(Bt, i, shape, low)
+    theta = lanczos_expm_fixed_batched(mv, theta.reshape(k, -1), -0.5j * dt, m, resid, gctx)
     mat = theta.reshape(k, a * d, d * b)
     ceiling = np.minimum(Bt.rank[i] * d, Bt.rank[i + 2] * d)
     U, S, Vh, ranks, disc = _gram_cut(mat, Bt.caps, ceiling, svd_tol,
@@ -296,18 +369,18 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
         Bt.L[i + 1] = kernels.update_left_env_b(Bt.L[i], Bt.M[i], Bt.W[i])
         C = (Sc[:, :, None] * Vh).reshape(k, nk, d, b).contiguous()
         if i < Bt.N - 2:
-            C = lanczos_expm_fixed_batched(
-                _matvec_one(Bt, Bt.L[i + 1], Bt.W[i + 1], Bt.R[i + 2], (k, nk, d, b), low),
-                C.reshape(k, -1), 0.5j * dt, m, resid).reshape(k, nk, d, b)
+            mv, gctx = _matvec_one(Bt, Bt.L[i + 1], Bt.W[i + 1], Bt.R[i + 2], (k, nk, d, b), low)
+            C = lanczos_expm_fixed_batched(mv, C.reshape(k, -1), 0.5j * dt, m, resid,
+                                           gctx).reshape(k, nk, d, b)
         Bt.M[i + 1] = C.contiguous()
     else:
         Bt.M[i + 1] = Vh.reshape(k, nk, d, b).contiguous()
         Bt.R[i + 1] = kernels.update_right_env_b(Bt.R[i + 2], Bt.M[i + 1], Bt.W[i + 1])
         C = (U * Sc[:, None, :]).reshape(k, a, d, nk).contiguous()
         if i > 0:
-            C = lanczos_expm_fixed_batched(
-                _matvec_one(Bt, Bt.L[i], Bt.W[i], Bt.R[i + 1], (k, a, d, nk), low),
-                C.reshape(k, -1), 0.5j * dt, m, resid).reshape(k, a, d, nk)
+            mv, gctx = _matvec_one(Bt, Bt.L[i], Bt.W[i], Bt.R[i + 1], (k, a, d, nk), low)
+            C = lanczos_expm_fixed_batched(mv, C.reshape(k, -1), 0.5j * dt, m, resid,
+                                           gctx).reshape(k, a, d, nk)
         Bt.M[i] = C.contiguous()
     return disc
 
