@@ -137,8 +137,10 @@ def cpu_baseline(prob, budget_s=20.0):
                       f"({sec * 1e3:.0f} ms each; BLAS: {', '.join(blas) or 'numpy default'})"}
 
 
-def local_solves(th, op64, op32, ws, chi):
-    """The local ground state by float64 thick-restart Lanczos and by the mixed two-stage form."""
+def local_solves(th, op64, op32, ws, chi, op16=None):
+    """The local ground state by float64 thick-restart Lanczos and by the mixed two-stage form
+    (float32 operator, then float64); with `op16`, also the two-stage form on the BF16 operator,
+    whose stage 1 stops at a 3e-2 residual (the bf16 operator's own error is a few 1e-3)."""
     import torch
     from tnkit_amd import kernels
     from tnkit_amd.eigensolver import lanczos_ground_mixed, lanczos_ground_restart
@@ -156,8 +158,19 @@ def local_solves(th, op64, op32, ws, chi):
                                        max_iter=4 * chi, counts=cnt)
     torch.cuda.synchronize()
     t_emx = time.perf_counter() - t1
-    return {"fp64_E": e64, "mixed_E": emx, "abs": abs(emx - e64), "fp64_matvecs": i64["iters"],
-            "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"], "fp64_s": t_e64, "mixed_s": t_emx}
+    res = {"fp64_E": e64, "mixed_E": emx, "abs": abs(emx - e64), "fp64_matvecs": i64["iters"],
+           "mixed_lo": cnt["lo"], "mixed_hi": cnt["hi"], "fp64_s": t_e64, "mixed_s": t_emx}
+    if op16 is not None:
+        c16 = {"lo": 0, "hi": 0}
+        mv16 = lambda x: kernels.apply_two_site_fused(x.reshape(shape), *op16, ws=ws).reshape(-1)  # noqa
+        t1 = time.perf_counter()
+        e16, _, i16 = lanczos_ground_mixed(mv16, mvh, th.reshape(-1).clone(), tol=1e-9,
+                                           max_iter=4 * chi, tol_lo=3e-2, counts=c16)
+        torch.cuda.synchronize()
+        res.update({"bf16_E": e16, "bf16_abs": abs(e16 - e64), "bf16_lo": c16["lo"],
+                    "bf16_hi": c16["hi"], "bf16_stage1_E": i16["e_lo"],
+                    "bf16_s": time.perf_counter() - t1})
+    return res
 
 
 def gaussian_sectors(chi, sigma, half_width):
@@ -575,6 +588,7 @@ def main():
     stream = torch.cuda.current_stream()
     op64 = kernels.fused_two_site(L, W, W, R)
     op32 = kernels.fused_two_site(L, W, W, R, dtype=torch.float32)
+    op16 = kernels.fused_two_site(L, W, W, R, dtype=torch.bfloat16)
     ws = kernels.two_site_ws(th, *op64)
     out = torch.empty_like(th)
 
@@ -584,10 +598,16 @@ def main():
     def mv64():
         kernels.apply_two_site_fused(th, *op64, out=out, ws=ws)
 
+    def mv16():
+        kernels.apply_two_site_fused(th, *op16, out=out, ws=ws)
+
     # accuracy of one mixed matvec against float64
     y64 = kernels.apply_two_site_fused(th, *op64)
     y32 = kernels.apply_two_site_fused(th, *op32)
     mv_rel = float((y32 - y64).abs().max() / y64.abs().max())
+    y16 = kernels.apply_two_site_fused(th, *op16)
+    mv_rel16 = float((y16 - y64).abs().max() / y64.abs().max())
+    del y16
 
     # ---- timed region: K mixed matvecs --------------------------------------------------
     for _ in range(args.warmup):
@@ -601,6 +621,14 @@ def main():
     for _ in range(args.warmup):
         mv64()
     ms64 = event_time(mv64, args.steps, stream)
+    for _ in range(args.warmup):
+        mv16()
+    ms16 = event_time(mv16, args.steps, stream)
+    st16 = {}
+    for st in (1, 2, 4):
+        fn = lambda st=st: kernels.apply_two_site_fused(th, *op16, out=out, ws=ws, stages=st)  # noqa
+        fn()
+        st16[st] = event_time(fn, max(5, args.steps // 2), stream)
 
     # ---- per-kernel events for the roofline ----------------------------------------------
     sflops = stage_flops(a, b, d1, d2, wl, wr)
@@ -628,8 +656,8 @@ def main():
     # ---- local ground state: mixed vs float64 -------------------------------------------
     local = {}
     if not args.no_local:
-        local = local_solves(th, op64, op32, ws, args.chi)
-    del op64, op32, ws, out, y32, y64
+        local = local_solves(th, op64, op32, ws, args.chi, op16=op16)
+    del op64, op32, op16, ws, out, y32, y64
 
     # ---- bulk DMRG steps (eigensolve + Gram truncation + environment) ---------------------
     steps = {}
@@ -675,6 +703,13 @@ def main():
         "fp64": {"ms": ms64, "tflops": fl / (ms64 * 1e-3) / 1e12},
         "speedup_mixed_vs_fp64": ms64 / ms32,
         "matvec_rel_err_f32": mv_rel,
+        "bf16": {"ms": ms16, "tflops": fl / (ms16 * 1e-3) / 1e12, "matvec_rel_err": mv_rel16,
+                 "speedup_vs_mixed": ms32 / ms16,
+                 "per_stage_ms": {"GEMM1 (gemm_tn_bf16_kernel<double,bf16,float>)": st16[1],
+                                  "MPO mix (float, bf16 out)": st16[2],
+                                  "GEMM3 (gemm_tn_bf16_kernel<bf16,bf16,double>)": st16[4]},
+                 "note": "explored, not the headline: bf16 operator (LT, RT) and bf16 theta / "
+                         "mixed intermediate on the bf16 matrix cores, float accumulation"},
         "local_dE": local,
         "bulk_steps": steps,
         "reference_h100_mig_fp64_tflops": 5.67,

@@ -38,7 +38,8 @@ const char* tnh_last_error(void);
 /* C[m][n] (+)= sum_k A[k][m] * B[k][n], batched over `batch` with element strides.
  * The GEMM every contraction below is built from; replaces the vendor GEMMs that tnkit's
  * einsum reaches (tnkit/backend.py:317-322 numpy, torch.einsum on the torch backends).
- * tile_hint: 0 = automatic, 1 = 128x128, 2 = 64x64, 3 = 32x32, 4 = 16x128. */
+ * tile_hint: 0 = automatic, 1 = 128x128, 2 = 64x64, 3 = 32x32, 4 = 16x128. TNH_BF16: A and B
+ * in bf16 on v_mfma_f32_16x16x32_bf16, C in float32 (tiles 1 = 128x128, 2 = 64x64). */
 int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long lda, const void* B, long ldb,
                 void* C, long ldc, int batch, long strideA, long strideB, long strideC,
                 int accumulate, int tile_hint, tnh_stream_t stream);
@@ -48,7 +49,8 @@ int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long lda, const v
  *   LT[q][p][wl] = L[p][wl][q]             (a, a, wl)
  *   S[(s1 s2 wr)][(wl s1' s2')] = sum_m W1[wl][m][s1][s1'] W2[m][wr][s2][s2']
  *   RT[wr][b][b'] = R[b'][wr][b]           (wr, b, b)
- * dtype TNH_F64 or TNH_F32. */
+ * dtype TNH_F64 or TNH_F32; TNH_BF16 builds the bf16 operator of the BF16 matvec: L, R, LT and
+ * RT in bf16, W1, W2 and S in float32. */
 int tnh_fused_two_site(int dtype, int a, int wl, int wm, int wr, int d1, int d2, int b,
                        const void* L, const void* W1, const void* W2, const void* R, void* LT,
                        void* S, void* RT, tnh_stream_t stream);
@@ -59,7 +61,9 @@ size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, in
 /* apply_two_site_fused (tnkit/_dmrg.py:183-189): out = H_eff theta for theta (a, d1, d2, b).
  * `dtype` is the arithmetic type of the operator (F64, or F32 on the F32 matrix cores);
  * `io_dtype` the type of theta and out (F64 with an F32 operator gives the mixed-precision
- * matvec with float64 Krylov vectors). tile1/tile3: tile hints of the two GEMMs (0 = auto). */
+ * matvec with float64 Krylov vectors; TNH_BF16 with TNH_F64 runs both GEMMs on the bf16 matrix
+ * cores with float accumulation and the MPO mix in float). tile1/tile3: tile hints of the two
+ * GEMMs (0 = auto). */
 int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int b, int wl, int wr,
                              const void* LT, const void* S, const void* RT, const void* theta,
                              void* out, void* ws, size_t ws_bytes, int tile1, int tile3,

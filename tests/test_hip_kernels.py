@@ -41,6 +41,28 @@ def test_gemm_tn(hip, M, N, K, tile, dtype):
         assert _rel(acc, 2 * ref) < tol
 
 
+@pytest.mark.parametrize("M,N,K,tile", [(1, 1, 1, 0), (17, 33, 9, 0), (128, 128, 128, 1),
+                                        (130, 200, 77, 1), (64, 96, 40, 2), (257, 129, 1030, 0),
+                                        (300, 520, 64, 0), (0, 5, 5, 0), (5, 5, 0, 0)])
+def test_gemm_tn_bf16(hip, M, N, K, tile):
+    """bf16 operands on the bf16 matrix cores: with bf16-representable inputs every product is
+    exact in float32, so the result must equal the float64 product of the rounded inputs to
+    float32 accumulation error; a wrong lane/k layout gives O(1) errors."""
+    print(f"gemm_tn bf16 M={M} N={N} K={K} tile={tile}", flush=True)
+    rng = np.random.default_rng(5 * M + N + K)
+    A = torch.as_tensor(rng.standard_normal((K, M))).to(torch.bfloat16)
+    B = torch.as_tensor(rng.standard_normal((K, N))).to(torch.bfloat16)
+    ref = A.double().numpy().T @ B.double().numpy()
+    got = kernels.gemm_tn(A.to(hip), B.to(hip), tile=tile)
+    assert got.dtype == torch.float32 and tuple(got.shape) == (M, N)
+    tol = 2e-6 * np.sqrt(max(K, 1))
+    if M * N:
+        assert _rel(got, ref) < tol
+    acc = kernels.gemm_tn(A.to(hip), B.to(hip), out=got.clone(), accumulate=True, tile=tile)
+    if M * N:
+        assert _rel(acc, 2 * ref) < tol
+
+
 SHAPES = [  # (a, wl, wm, wr, d1, d2, b) -- tnkit/tests/test_two_site_fused.py plus bulk sizes
     (1, 1, 5, 5, 2, 2, 2), (4, 5, 5, 1, 2, 2, 1), (7, 20, 20, 20, 2, 3, 9), (32, 3, 3, 3, 2, 2, 32),
     (64, 5, 5, 5, 2, 2, 48), (256, 5, 5, 5, 2, 2, 256)]
@@ -83,6 +105,31 @@ def test_two_site_matvec_f32_operator(hip, a, wl, wm, wr, d1, d2, b):
     got32 = kernels.apply_two_site_fused(_t(th, hip, torch.float32), *op32)
     assert got32.dtype == torch.float32
     assert _rel(got32, ref) < 2e-5
+
+
+@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES[3:] + [(7, 20, 20, 20, 2, 3, 9)])
+def test_two_site_matvec_bf16_operator(hip, a, wl, wm, wr, d1, d2, b):
+    """BF16 operator (LT, RT bf16; S float32) with float64 theta/out: theta and the mixed
+    intermediate are rounded to bf16 (8-bit mantissa), so the result is within a few parts in
+    1e3 of the exact one; the exact layouts of LT/RT are the bf16-rounded permutations."""
+    print(f"two_site bf16 {(a, wl, wm, wr, d1, d2, b)}", flush=True)
+    rng = np.random.default_rng(23 + a)
+    L, R = rng.standard_normal((a, wl, a)), rng.standard_normal((b, wr, b))
+    W1, W2 = rng.standard_normal((wl, wm, d1, d1)), rng.standard_normal((wm, wr, d2, d2))
+    th = rng.standard_normal((a, d1, d2, b))
+    ref = D.apply_two_site_fused(th, *D.fused_two_site(L, W1, W2, R))
+    op = kernels.fused_two_site(_t(L, hip), _t(W1, hip), _t(W2, hip), _t(R, hip),
+                                dtype=torch.bfloat16)
+    LT, S, RT = op
+    assert (LT.dtype, S.dtype, RT.dtype) == (torch.bfloat16, torch.float32, torch.bfloat16)
+    Lb = torch.as_tensor(L).to(torch.bfloat16).permute(2, 0, 1)
+    Rb = torch.as_tensor(R).to(torch.bfloat16).permute(1, 2, 0)
+    assert torch.equal(LT.cpu(), Lb) and torch.equal(RT.cpu(), Rb)
+    got = kernels.apply_two_site_fused(_t(th, hip), *op)
+    assert got.dtype == torch.float64
+    assert _rel(got, ref) < 1.5e-2
+    with pytest.raises(TypeError):
+        kernels.apply_two_site_fused(_t(th, hip, torch.float32), *op)
 
 
 @pytest.mark.parametrize("a,d,c,wl,wr", [(1, 2, 2, 1, 5), (2, 2, 4, 5, 5), (7, 2, 9, 3, 4),

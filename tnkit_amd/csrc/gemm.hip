@@ -123,6 +123,60 @@ int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t st
   }
 }
 
+namespace {
+template <typename TA, typename TB, typename TO, int BM, int BN, int WM, int WN, int VA, int VB>
+int launch_bf16_v(const GemmArgs& g, int batch, hipStream_t stream) {
+  const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
+  dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(batch));
+  hipLaunchKernelGGL((gemm_tn_bf16_kernel<TA, TB, TO, BM, BN, WM, WN, VA, VB>), grid,
+                     dim3(WM * WN * 64), 0, stream, g);
+  TNH_CHECK_LAUNCH("gemm_tn_bf16");
+  return TNH_OK;
+}
+
+template <typename TA, typename TB, typename TO, int BM, int BN, int WM, int WN>
+int launch_bf16_one(const GemmArgs& g, int batch, hipStream_t stream) {
+  const long tiles = ceil_div(g.M, BM) * ceil_div(g.N, BN);
+  if (tiles == 0 || batch == 0) return TNH_OK;
+  TNH_REQUIRE(tiles < (1L << 31) && batch <= 65535, TNH_ERR_ARG, "gemm_tn_bf16: grid too large");
+  constexpr int NT = WM * WN * 64;
+  constexpr int VA = 16 / sizeof(TA), VB = 16 / sizeof(TB);
+  constexpr bool can_a = (32 * BM) % (VA * NT) == 0;
+  constexpr bool can_b = (32 * BN) % (VB * NT) == 0;
+  const bool va = can_a && vec_ok<VA, TA>(g.A, g.lda, g.M) && (batch == 1 || g.strideA % VA == 0);
+  const bool vb = can_b && vec_ok<VB, TB>(g.B, g.ldb, g.N) && (batch == 1 || g.strideB % VB == 0);
+  if constexpr (can_a && can_b) {
+    if (va && vb) return launch_bf16_v<TA, TB, TO, BM, BN, WM, WN, VA, VB>(g, batch, stream);
+  }
+  if constexpr (can_a) {
+    if (va) return launch_bf16_v<TA, TB, TO, BM, BN, WM, WN, VA, 1>(g, batch, stream);
+  }
+  if constexpr (can_b) {
+    if (vb) return launch_bf16_v<TA, TB, TO, BM, BN, WM, WN, 1, VB>(g, batch, stream);
+  }
+  return launch_bf16_v<TA, TB, TO, BM, BN, WM, WN, 1, 1>(g, batch, stream);
+}
+}  // namespace
+
+template <typename TA, typename TB, typename TO>
+int launch_gemm_bf16(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint) {
+  TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn_bf16: negative size");
+  int cfg = tile_hint;
+  if (cfg == 0) {
+    const long b = batch > 0 ? batch : 1;
+    cfg = ceil_div(g.M, 128) * ceil_div(g.N, 128) * b >= 512 ? 1 : 2;
+  }
+  switch (cfg) {
+    case 1: return launch_bf16_one<TA, TB, TO, 128, 128, 2, 2>(g, batch, stream);
+    case 2: return launch_bf16_one<TA, TB, TO, 64, 64, 2, 2>(g, batch, stream);
+    default: set_error("gemm_tn_bf16: unknown tile config %d", cfg); return TNH_ERR_ARG;
+  }
+}
+
+template int launch_gemm_bf16<__bf16, __bf16, float>(const GemmArgs&, int, hipStream_t, int);
+template int launch_gemm_bf16<double, __bf16, float>(const GemmArgs&, int, hipStream_t, int);
+template int launch_gemm_bf16<__bf16, __bf16, double>(const GemmArgs&, int, hipStream_t, int);
+
 template int launch_gemm_tn_cplx<double, c128, c128, c128>(const GemmArgs&, int, int, hipStream_t, int);
 template int launch_gemm_tn_cplx<float, c64, c64, c64>(const GemmArgs&, int, int, hipStream_t, int);
 template int launch_gemm_tn_cplx<float, c128, c64, c64>(const GemmArgs&, int, int, hipStream_t, int);
@@ -151,6 +205,7 @@ extern "C" int tnh_gemm_tn(int dtype, int M, int N, int K, const void* A, long l
     case TNH_C128:
       return launch_gemm_tn_cplx<double, c128, c128, c128>(g, batch, 0, s, tile_hint);
     case TNH_C64: return launch_gemm_tn_cplx<float, c64, c64, c64>(g, batch, 0, s, tile_hint);
+    case TNH_BF16: return launch_gemm_bf16<__bf16, __bf16, float>(g, batch, s, tile_hint);
     default: set_error("tnh_gemm_tn: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
   }
 }

@@ -354,6 +354,149 @@ __global__ void __launch_bounds__(WM* WN * 64)
     }
 }
 
+// BF16 K-major GEMM on v_mfma_f32_16x16x32_bf16 (float accumulation): C = A^T B with A (K x M)
+// and B (K x N) stored K-major in TA / TB (double, float or bf16) and rounded to bf16 on the way
+// into LDS. For this MFMA lane l supplies 8 consecutive k of row (or column) l % 16 starting at
+// k = 8 (l / 16), so the LDS tiles are held k-contiguous, As[m][k] and Bs[n][k] with rows of
+// BK = 32 (+8 pad) values, and every fragment is one 16-byte LDS read. Result layout as the f32
+// 16x16x4 MFMA: lane l, item r holds D[4 (l / 16) + r][l % 16].
+template <typename TA, typename TB, typename TO, int BM, int BN, int WM, int WN, int VA, int VB>
+__global__ void __launch_bounds__(WM* WN * 64)
+    gemm_tn_bf16_kernel(GemmArgs g) {
+  constexpr int BK = 32;
+  constexpr int LDK = BK + 8;
+  constexpr int NT = WM * WN * 64;
+  constexpr int FM = BM / WM / 16;
+  constexpr int FN = BN / WN / 16;
+  constexpr int NA = BK * BM / (VA * NT);
+  constexpr int NB = BK * BN / (VB * NT);
+  static_assert(FM >= 1 && FN >= 1 && NA >= 1 && NB >= 1, "bf16 tile shape");
+  static_assert((BK * BM) % (VA * NT) == 0 && (BK * BN) % (VB * NT) == 0, "bf16 tile loads");
+  __shared__ __align__(16) __bf16 As[2][BM * LDK];
+  __shared__ __align__(16) __bf16 Bs[2][BN * LDK];
+
+  const int M = g.M, N = g.N, K = g.K;
+  const int nbm = (M + BM - 1) / BM, nbn = (N + BN - 1) / BN;
+  int tm, tn;
+  tile_of(blockIdx.x, nbm, nbn, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
+  const long bz = blockIdx.y;
+  const TA* __restrict__ A = static_cast<const TA*>(g.A) + bz * g.strideA;
+  const TB* __restrict__ B = static_cast<const TB*>(g.B) + bz * g.strideB;
+  TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN, row16 = lane & 15, kq = lane >> 4;
+
+  f4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4{};
+
+  __bf16 ra[NA * VA], rb[NB * VB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = t + i * NT, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+      const int k = k0 + kr, m = m0 + mc;
+      if constexpr (VA == 1) {
+        ra[i] = (k < K && m < M) ? static_cast<__bf16>(static_cast<float>(A[(long)k * g.lda + m]))
+                                 : static_cast<__bf16>(0.0f);
+      } else {
+        typedef TA vec_t __attribute__((ext_vector_type(VA)));
+        if (k < K && m < M) {
+          const vec_t x = *reinterpret_cast<const vec_t*>(A + (long)k * g.lda + m);
+#pragma unroll
+          for (int v = 0; v < VA; ++v) ra[i * VA + v] = static_cast<__bf16>(static_cast<float>(x[v]));
+        } else {
+#pragma unroll
+          for (int v = 0; v < VA; ++v) ra[i * VA + v] = static_cast<__bf16>(0.0f);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = t + i * NT, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+      const int k = k0 + kr, n = n0 + nc;
+      if constexpr (VB == 1) {
+        rb[i] = (k < K && n < N) ? static_cast<__bf16>(static_cast<float>(B[(long)k * g.ldb + n]))
+                                 : static_cast<__bf16>(0.0f);
+      } else {
+        typedef TB vec_t __attribute__((ext_vector_type(VB)));
+        if (k < K && n < N) {
+          const vec_t x = *reinterpret_cast<const vec_t*>(B + (long)k * g.ldb + n);
+#pragma unroll
+          for (int v = 0; v < VB; ++v) rb[i * VB + v] = static_cast<__bf16>(static_cast<float>(x[v]));
+        } else {
+#pragma unroll
+          for (int v = 0; v < VB; ++v) rb[i * VB + v] = static_cast<__bf16>(0.0f);
+        }
+      }
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = t + i * NT, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+#pragma unroll
+      for (int v = 0; v < VA; ++v) As[buf][(mc + v) * LDK + kr] = ra[i * VA + v];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int e = t + i * NT, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+#pragma unroll
+      for (int v = 0; v < VB; ++v) Bs[buf][(nc + v) * LDK + kr] = rb[i * VB + v];
+    }
+  };
+  const int nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    bf8 af[FM], bfr[FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+      af[i] = *reinterpret_cast<const bf8*>(&As[cur][(wm * (BM / WM) + i * 16 + row16) * LDK + kq * 8]);
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+      bfr[j] = *reinterpret_cast<const bf8*>(&Bs[cur][(wn * (BN / WN) + j * 16 + row16) * LDK + kq * 8]);
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * (BN / WN) + j * 16 + row16;
+      if (n >= N) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * (BM / WM) + i * 16 + 4 * kq + r;
+        if (m < M) {
+          TO* dst = C + (long)m * g.ldc + n;
+          if (g.accumulate)
+            *dst = static_cast<TO>(static_cast<float>(*dst) + acc[i][j][r]);
+          else
+            *dst = static_cast<TO>(acc[i][j][r]);
+        }
+      }
+    }
+  }
+}
+
+// Host launcher of the bf16 GEMM: tile 1 = 128x128 (4 waves), 2 = 64x64 (4 waves); 0 = automatic.
+template <typename TA, typename TB, typename TO>
+int launch_gemm_bf16(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint = 0);
+
 template <typename TR, typename TA, typename TB, typename TO>
 int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t stream,
                         int tile_hint = 0);

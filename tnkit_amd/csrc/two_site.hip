@@ -498,6 +498,55 @@ int right_env(int batch, int c, int d, int b, int wl, int wr, const void* R, lon
 
 using namespace tnh;
 
+// The BF16 operator: LT and RT in bf16 (the two GEMMs' B operands), S in float (the MPO mix runs
+// in float); L and R arrive in bf16, W1 and W2 in float.
+int fused_two_site_bf16(int batch, int a, int wl, int wm, int wr, int d1, int d2, int b,
+                        const void* L, long sL, const void* W1, long sW1, const void* W2, long sW2,
+                        const void* R, long sR, void* LT, long sLT, void* S, long sS, void* RT,
+                        long sRT, hipStream_t st) {
+  TNH_TRY(permute3<__bf16>(batch, a, a, wl, 1, (long)wl * a, a, L, sL, LT, sLT, st));
+  TNH_TRY(permute3<__bf16>(batch, wr, b, b, b, 1, (long)wr * b, R, sR, RT, sRT, st));
+  const int total = d1 * d2 * wr * wl * d1 * d2;
+  hipLaunchKernelGGL(build_s_kernel<float>, dim3(grid_for(total), 1, batch), dim3(256), 0, st, wl,
+                     wm, wr, d1, d2, static_cast<const float*>(W1), sW1,
+                     static_cast<const float*>(W2), sW2, static_cast<float*>(S), sS);
+  TNH_CHECK_LAUNCH("build_s");
+  return TNH_OK;
+}
+
+// The BF16 matvec: GEMM1 on bf16 matrix cores (theta rounded to bf16 on the way into LDS, float
+// accumulation, X1 in float), the MPO mix in float writing X2 in bf16, GEMM3 on bf16 matrix cores
+// into the io type.
+template <typename TIO>
+int apply_two_site_bf16(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* LT,
+                        long sLT, const void* S, long sS, const void* RT, long sRT,
+                        const void* theta, long sth, void* out, long sout, void* ws,
+                        size_t ws_bytes, hipStream_t st, int tile1, int tile3, int stages) {
+  const long P = a, Q = a, B = b, dd = (long)d1 * d2;
+  const long nx1 = dd * B * P * wl, nx2 = (long)wr * B * P * dd;
+  Arena ar(ws, ws_bytes);
+  float* X1 = ar.take<float>(nx1 * batch);
+  __bf16* X2 = ar.take<__bf16>(nx2 * batch);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused (bf16): workspace %zu < %zu bytes",
+              ws_bytes, ar.used);
+  if (stages & 1) {
+    GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
+               dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+    TNH_TRY((launch_gemm_bf16<TIO, __bf16, float>(g, batch, st, tile1)));
+  }
+  if (stages & 2) {
+    MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
+              P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd, sS, nx1, nx2};
+    TNH_TRY((mix<float, float, __bf16>(m, batch, S, X1, X2, st)));
+  }
+  if (stages & 4) {
+    GemmArgs g{static_cast<int>(P * dd), static_cast<int>(B), static_cast<int>(wr * B), X2, P * dd,
+               nx2, RT, B, sRT, out, B, sout, 0};
+    TNH_TRY((launch_gemm_bf16<__bf16, __bf16, TIO>(g, batch, st, tile3)));
+  }
+  return TNH_OK;
+}
+
 extern "C" {
 
 int tnh_fused_two_site_batched(int dtype, int batch, int a, int wl, int wm, int wr, int d1, int d2,
@@ -516,6 +565,9 @@ int tnh_fused_two_site_batched(int dtype, int batch, int a, int wl, int wm, int 
     case TNH_F32: return TNH_FTS(float);
     case TNH_C128: return TNH_FTS(c128);
     case TNH_C64: return TNH_FTS(c64);
+    case TNH_BF16:
+      return fused_two_site_bf16(batch, a, wl, wm, wr, d1, d2, b, L, sL, W1, sW1, W2, sW2, R, sR,
+                                 LT, sLT, S, sS, RT, sRT, st);
     default: set_error("tnh_fused_two_site: unsupported dtype %d", dtype); return TNH_ERR_UNSUPPORTED;
   }
 #undef TNH_FTS
@@ -529,8 +581,10 @@ int tnh_fused_two_site(int dtype, int a, int wl, int wm, int wr, int d1, int d2,
 }
 
 size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr) {
-  const size_t e = dsize(dtype);
   const size_t dd = (size_t)d1 * d2;
+  if (dtype == TNH_BF16)  // X1 in float, X2 in bf16
+    return align_up(dd * b * a * wl * 4, 256) + align_up((size_t)wr * b * a * dd * 2, 256) + 256;
+  const size_t e = dsize(dtype);
   return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + 256;
 }
 
@@ -554,6 +608,10 @@ int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1
   if (dtype == TNH_C64 && io_dtype == TNH_C64) return TNH_A2S(c64, c64);
   if (dtype == TNH_C64 && io_dtype == TNH_C128) return TNH_A2S(c64, c128);
 #undef TNH_A2S
+  if (dtype == TNH_BF16 && io_dtype == TNH_F64)
+    return apply_two_site_bf16<double>(batch, a, d1, d2, b, wl, wr, LT, sLT, S, sS, RT, sRT,
+                                       theta, sth, out, sout, ws, ws_bytes, st, tile1, tile3,
+                                       stages);
   set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
   return TNH_ERR_UNSUPPORTED;
 }

@@ -50,18 +50,20 @@ def stream(device=None) -> int:
 
 def gemm_tn(A: torch.Tensor, B: torch.Tensor, out: Optional[torch.Tensor] = None,
             accumulate: bool = False, tile: int = 0) -> torch.Tensor:
-    """out[m, n] (+)= sum_k A[k, m] B[k, n] for 2-D contiguous A (K, M), B (K, N)."""
-    _need(A, "A", 2)
-    _need(B, "B", 2)
+    """out[m, n] (+)= sum_k A[k, m] B[k, n] for 2-D contiguous A (K, M), B (K, N). bfloat16
+    A and B run on the bf16 matrix cores with float32 accumulation and a float32 out."""
+    _need(A, "A", 2, _ANY + (torch.bfloat16,))
+    _need(B, "B", 2, _ANY + (torch.bfloat16,))
     if A.dtype != B.dtype or A.shape[0] != B.shape[0]:
         raise ValueError(f"gemm_tn: A {tuple(A.shape)} {A.dtype} and B {tuple(B.shape)} {B.dtype}")
     K, M = A.shape
     N = B.shape[1]
+    odt = torch.float32 if A.dtype == torch.bfloat16 else A.dtype
     if out is None:
-        out = torch.empty((M, N), dtype=A.dtype, device=A.device)
+        out = torch.empty((M, N), dtype=odt, device=A.device)
         accumulate = False
     _need(out, "out", 2)
-    if tuple(out.shape) != (M, N) or out.dtype != A.dtype:
+    if tuple(out.shape) != (M, N) or out.dtype != odt:
         raise ValueError(f"gemm_tn: out {tuple(out.shape)} {out.dtype}, expected {(M, N)}")
     lib = _lib.load()
     check(lib.tnh_gemm_tn(dcode(A.dtype), M, N, K, ptr(A), M, ptr(B), N, ptr(out), N, 1, 0, 0, 0,
@@ -79,7 +81,8 @@ def fused_two_site(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch
                                                                  torch.Tensor]:
     """(LT, S, RT) of one two-site problem (tnkit/_dmrg.py:172-180) in the device layouts
     LT (q, p, wl), S (d1 d2 wr, wl d1 d2), RT (wr, b, b'). `dtype` (float32) builds the operator
-    of the low-precision matvec from float64 inputs."""
+    of the low-precision matvec from float64 inputs; `dtype=torch.bfloat16` builds the BF16
+    operator (LT and RT in bfloat16, S in float32)."""
     for t, n in ((L, "L"), (W1, "W1"), (W2, "W2"), (R, "R")):
         _need(t, n, 3 if n in ("L", "R") else 4)
     a, wl, a2 = L.shape
@@ -91,12 +94,15 @@ def fused_two_site(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch
         raise ValueError(f"fused_two_site: inconsistent shapes L{tuple(L.shape)} "
                          f"W1{tuple(W1.shape)} W2{tuple(W2.shape)} R{tuple(R.shape)}")
     dt = dtype or L.dtype
+    if dt == torch.bfloat16 and L.dtype.is_complex:
+        raise TypeError("fused_two_site: the bfloat16 operator is real")
     if dt != L.dtype:
-        L, R = L.to(dt), R.to(dt)
-    W1, W2 = W1.to(dt).contiguous(), W2.to(dt).contiguous()
+        L, R = L.to(dt).contiguous(), R.to(dt).contiguous()
+    sdt = torch.float32 if dt == torch.bfloat16 else dt
+    W1, W2 = W1.to(sdt).contiguous(), W2.to(sdt).contiguous()
     dev = L.device
     LT = torch.empty((a, a, wl), dtype=dt, device=dev)
-    S = torch.empty((d1 * d2 * wr, wl * d1 * d2), dtype=dt, device=dev)
+    S = torch.empty((d1 * d2 * wr, wl * d1 * d2), dtype=sdt, device=dev)
     RT = torch.empty((wr, b, b), dtype=dt, device=dev)
     check(_lib.load().tnh_fused_two_site(dcode(dt), a, wl, wm, wr, d1, d2, b, ptr(L), ptr(W1),
                                          ptr(W2), ptr(R), ptr(LT), ptr(S), ptr(RT), stream(dev)),
@@ -109,20 +115,26 @@ def apply_two_site_fused(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor,
                          tiles: Tuple[int, int] = (0, 0), ws: Optional[torch.Tensor] = None,
                          stages: int = 7) -> torch.Tensor:
     """H_eff theta (tnkit/_dmrg.py:183-189) for theta (a, d1, d2, b). The operator's dtype sets
-    the arithmetic; theta/out may be float64 around a float32 operator (mixed precision)."""
+    the arithmetic; theta/out may be float64 around a float32 operator (mixed precision) or
+    around the bfloat16 operator of `fused_two_site(..., dtype=torch.bfloat16)`."""
     _need(theta, "theta", 4)
-    _need(LT, "LT", 3)
+    _need(LT, "LT", 3, _ANY + (torch.bfloat16,))
     _need(S, "S", 2)
-    _need(RT, "RT", 3)
+    _need(RT, "RT", 3, _ANY + (torch.bfloat16,))
     q, d1, d2, b = theta.shape
     a, p, wl = LT.shape
     wr, bb, bp = RT.shape
     if not (a == q and p == q and bb == b and bp == b and S.shape == (d1 * d2 * wr, wl * d1 * d2)):
         raise ValueError(f"apply_two_site_fused: theta{tuple(theta.shape)} LT{tuple(LT.shape)} "
                          f"S{tuple(S.shape)} RT{tuple(RT.shape)} do not match")
-    if not (LT.dtype == S.dtype == RT.dtype):
+    if LT.dtype == torch.bfloat16:
+        if not (RT.dtype == torch.bfloat16 and S.dtype == torch.float32
+                and theta.dtype == torch.float64):
+            raise TypeError("apply_two_site_fused: the bfloat16 operator takes LT, RT bfloat16, "
+                            "S float32 and a float64 theta")
+    elif not (LT.dtype == S.dtype == RT.dtype):
         raise TypeError("apply_two_site_fused: LT, S, RT must share a dtype")
-    if theta.dtype != LT.dtype and _LOW.get(theta.dtype) != LT.dtype:
+    elif theta.dtype != LT.dtype and _LOW.get(theta.dtype) != LT.dtype:
         raise TypeError(f"apply_two_site_fused: theta {theta.dtype} with operator {LT.dtype}")
     if out is None:
         out = torch.empty_like(theta)
