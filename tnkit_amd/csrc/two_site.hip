@@ -205,21 +205,89 @@ struct StageMax {  // register budget of the staged kernel: 32 real or 16 comple
   static constexpr int value = real_of<T>::cplx ? 16 : kStageMax;
 };
 
-__device__ __forceinline__ void stage_decode(int mode, int L, int n, int n2, int& j, int& t) {
-  const int n1 = n / n2;
+// Unsigned division by a divisor fixed for the whole launch: q / d = umulhi(q, ceil(2^32 / d)),
+// exact whenever q * d < 2^32 (the rounding excess of the multiplier, below d, then stays below
+// one unit in the last place). The staging decode divides indices below kStageMax * kStageG by
+// divisors up to kStageMax, far inside that range; a general integer division costs tens of VALU
+// instructions per element, which made the staging loops instruction-bound.
+struct FastDiv {
+  unsigned d, m;
+  __device__ explicit FastDiv(unsigned d_)
+      : d(d_), m(d_ > 1 ? static_cast<unsigned>((0x100000000ull + d_ - 1) / d_) : 0u) {}
+  __device__ __forceinline__ unsigned div(unsigned q) const { return d > 1 ? __umulhi(q, m) : q; }
+};
+
+// (j, t) of position L of the staging order: j = j1 * n2 + j2 is the in-group index (j1 < n1,
+// j2 < n2), t the group within the workgroup.
+__device__ __forceinline__ void stage_decode(int mode, unsigned L, const FastDiv& n1,
+                                             const FastDiv& n2, int& j, int& t) {
   if (mode == 1) {  // L = (j2 * G + t) * n1 + j1
-    const int j1 = L % n1;
-    const int r = L / n1;
-    t = r % kStageG;
-    j = j1 * n2 + r / kStageG;
+    const unsigned r = n1.div(L);
+    const int j1 = static_cast<int>(L - r * n1.d);
+    t = static_cast<int>(r % kStageG);
+    j = j1 * static_cast<int>(n2.d) + static_cast<int>(r / kStageG);
   } else if (mode == 2) {  // L = (j1 * G + t) * n2 + j2
-    const int j2 = L % n2;
-    const int r = L / n2;
-    t = r % kStageG;
-    j = (r / kStageG) * n2 + j2;
+    const unsigned r = n2.div(L);
+    const int j2 = static_cast<int>(L - r * n2.d);
+    t = static_cast<int>(r % kStageG);
+    j = static_cast<int>(r / kStageG) * static_cast<int>(n2.d) + j2;
   } else {  // L = j * G + t
-    j = L / kStageG;
-    t = L % kStageG;
+    j = static_cast<int>(L / kStageG);
+    t = static_cast<int>(L % kStageG);
+  }
+}
+
+// Copies the inputs of the kStageG groups g20.. at one g1 into LDS rows buf[ji][t] in the order
+// in which they lie in memory. All of a thread's loads are issued before any LDS store, so up to
+// kMax are in flight. Offsets within the workgroup's window fit 32 bits (checked on the host).
+template <int kMax, typename TC, typename TI>
+__device__ __forceinline__ void stage_in(const MixArgs& a, int mode, const TI* __restrict__ in,
+                                         long ib, long g20, TC* buf) {
+  const FastDiv n2(static_cast<unsigned>(a.nji2));
+  const FastDiv n1(static_cast<unsigned>(a.nji / a.nji2));
+  const TI* base = in + ib + g20 * a.is2;
+  const int is2 = static_cast<int>(a.is2), s1 = static_cast<int>(a.ijs1),
+            s2 = static_cast<int>(a.ijs2);
+  const int tmax = static_cast<int>(a.G2 - g20 < kStageG ? a.G2 - g20 : kStageG);
+  TC v[kMax];
+  int dst[kMax];
+#pragma unroll
+  for (int i = 0; i < kMax; ++i) {
+    v[i] = TC(0);
+    dst[i] = -1;
+    if (i < a.nji) {
+      int ji, t;
+      stage_decode(mode, threadIdx.x + i * kStageG, n1, n2, ji, t);
+      dst[i] = ji * kStageLd + t;
+      if (t < tmax) {
+        const int ji1 = static_cast<int>(n2.div(ji)), ji2 = ji - ji1 * a.nji2;
+        v[i] = static_cast<TC>(base[t * is2 + ji1 * s1 + ji2 * s2]);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kMax; ++i)
+    if (dst[i] >= 0) buf[dst[i]] = v[i];
+}
+
+// Writes the LDS rows buf[jo][t] of the kStageG groups g20.. at one g1 out in memory order.
+template <typename TC, typename TO>
+__device__ __forceinline__ void stage_out(const MixArgs& a, int mode, const TC* buf,
+                                          TO* __restrict__ out, long ob, long g20) {
+  const FastDiv n2(static_cast<unsigned>(a.njo2));
+  const FastDiv n1(static_cast<unsigned>(a.njo / a.njo2));
+  TO* base = out + ob + g20 * a.os2;
+  const int os2 = static_cast<int>(a.os2), s1 = static_cast<int>(a.ojs1),
+            s2 = static_cast<int>(a.ojs2);
+  const int tmax = static_cast<int>(a.G2 - g20 < kStageG ? a.G2 - g20 : kStageG);
+  const int nout = a.njo * kStageG;
+  for (int L = threadIdx.x; L < nout; L += blockDim.x) {
+    int jo, tt;
+    stage_decode(mode, L, n1, n2, jo, tt);
+    if (tt < tmax) {
+      const int jo1 = static_cast<int>(n2.div(jo)), jo2 = jo - jo1 * a.njo2;
+      base[tt * os2 + jo1 * s1 + jo2 * s2] = static_cast<TO>(buf[jo * kStageLd + tt]);
+    }
   }
 }
 
@@ -240,31 +308,7 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   for (int k = threadIdx.x; k < nm; k += blockDim.x) ms[k] = Mx[k];
   const long g1 = blockIdx.y;
   const long g20 = (long)blockIdx.x * kStageG;
-  const long ib = g1 * a.is1;
-  // all of a thread's loads are issued before any LDS store, so up to kStageMax are in flight
-  {
-    TC v[kMax];
-    int dst[kMax];
-#pragma unroll
-    for (int i = 0; i < kMax; ++i) {
-      v[i] = TC(0);
-      dst[i] = -1;
-      if (i < a.nji) {
-        const int L = threadIdx.x + i * kStageG;
-        int ji, t;
-        stage_decode(in_mode, L, a.nji, a.nji2, ji, t);
-        dst[i] = ji * kStageLd + t;
-        const long g2 = g20 + t;
-        if (g2 < a.G2) {
-          const int ji1 = ji / a.nji2, ji2 = ji - ji1 * a.nji2;
-          v[i] = static_cast<TC>(in[ib + g2 * a.is2 + ji1 * a.ijs1 + ji2 * a.ijs2]);
-        }
-      }
-    }
-#pragma unroll
-    for (int i = 0; i < kMax; ++i)
-      if (dst[i] >= 0) buf[dst[i]] = v[i];
-  }
+  stage_in<kMax>(a, in_mode, in, g1 * a.is1, g20, buf);
   __syncthreads();
   TC acc[kMax];
 #pragma unroll
@@ -285,17 +329,76 @@ __global__ void __launch_bounds__(256) mix_staged_kernel(MixArgs a, int in_mode,
   for (int o = 0; o < kMax; ++o)
     if (o < a.njo) buf[o * kStageLd + t] = acc[o];
   __syncthreads();
-  const long ob = g1 * a.os1;
-  const int nout = a.njo * kStageG;
-  for (int L = threadIdx.x; L < nout; L += blockDim.x) {
-    int jo, tt;
-    stage_decode(out_mode, L, a.njo, a.njo2, jo, tt);
-    const long g2 = g20 + tt;
-    if (g2 < a.G2) {
-      const int jo1 = jo / a.njo2, jo2 = jo - jo1 * a.njo2;
-      out[ob + g2 * a.os2 + jo1 * a.ojs1 + jo2 * a.ojs2] = static_cast<TO>(buf[jo * kStageLd + tt]);
+  stage_out(a, out_mode, buf, out, g1 * a.os1, g20);
+}
+
+// The staged mix on the matrix cores (real types). The mix of the kStageG groups of a workgroup
+// is the product Out^T (groups x njo) = In^T (groups x nji) * Mx^T (nji x njo): each wave takes
+// 64 groups as four 16-row MFMA tiles, K = nji in steps of 4 (at most 8), N = njo in one or two
+// 16-column tiles. The Mx^T fragments are loaded once per lane from global memory (the operator
+// is a few hundred entries and stays in cache); In^T fragments are read from the LDS rows that
+// stage_in filled, and the results go back to LDS rows for stage_out. A scalar FMA form spends
+// several VALU instructions per operator entry and group; here a wave issues nks * nns MFMAs per
+// 16 groups, so the kernel is bound by its HBM streams.
+template <typename TC, typename TI, typename TO>
+__global__ void __launch_bounds__(256) mix_mfma_kernel(MixArgs a, int in_mode, int out_mode,
+                                                       const TC* __restrict__ Mx,
+                                                       const TI* __restrict__ in,
+                                                       TO* __restrict__ out) {
+  using MF = Mfma<TC>;
+  using acc_t = typename MF::acc_t;
+  Mx += blockIdx.z * a.bm;
+  in += blockIdx.z * a.bin;
+  out += blockIdx.z * a.bout;
+  extern __shared__ __align__(16) unsigned char stage_lds[];
+  TC* buf = reinterpret_cast<TC*>(stage_lds);
+  const long g1 = blockIdx.y;
+  const long g20 = (long)blockIdx.x * kStageG;
+  stage_in<kStageMax>(a, in_mode, in, g1 * a.is1, g20, buf);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, kq = lane >> 4;
+  const int nji = a.nji, njo = a.njo;
+  const int nks = (nji + 3) >> 2, nns = (njo + 15) >> 4;
+  TC bfr[8][2];
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks)
+#pragma unroll
+    for (int ns = 0; ns < 2; ++ns) {
+      const int k = 4 * ks + kq, jo = 16 * ns + l16;
+      bfr[ks][ns] = (k < nji && jo < njo) ? Mx[jo * nji + k] : TC(0);
+    }
+  __syncthreads();
+  acc_t acc[4][2];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int ns = 0; ns < 2; ++ns) acc[mt][ns] = acc_t{};
+  const TC* col = buf + 64 * wave + l16;
+#pragma unroll
+  for (int ks = 0; ks < 8; ++ks) {
+    if (ks < nks) {
+      const int k = 4 * ks + kq;
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const TC x = k < nji ? col[k * kStageLd + 16 * mt] : TC(0);
+        acc[mt][0] = MF::run(x, bfr[ks][0], acc[mt][0]);
+        if (nns > 1) acc[mt][1] = MF::run(x, bfr[ks][1], acc[mt][1]);
+      }
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int ns = 0; ns < 2; ++ns) {
+      const int jo = 16 * ns + l16;
+      if (ns < nns && jo < njo) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          buf[jo * kStageLd + 64 * wave + 16 * mt + MF::row(kq, r)] = acc[mt][ns][r];
+      }
+    }
+  __syncthreads();
+  stage_out(a, out_mode, buf, out, g1 * a.os1, g20);
 }
 
 namespace {
@@ -337,9 +440,25 @@ int mix(const MixArgs& a, int batch, const void* Mx, const void* in, void* out, 
   const int rows = a.nji > a.njo ? a.nji : a.njo;
   const size_t lds = (((size_t)a.njo * a.nji + 1) / 2 * 2 + (size_t)rows * kStageLd) * sizeof(TC);
   constexpr int kMax = StageMax<TC>::value;
-  if (a.nji <= kMax && a.njo <= kMax && im >= 0 && om >= 0 && a.G1 <= 65535 && lds <= 65536) {
+  // the staged kernels index within one workgroup's window in 32 bits
+  const long win_in = (long)(kStageG - 1) * a.is2 + (long)(a.nji / a.nji2 - 1) * a.ijs1 +
+                      (long)(a.nji2 - 1) * a.ijs2;
+  const long win_out = (long)(kStageG - 1) * a.os2 + (long)(a.njo / a.njo2 - 1) * a.ojs1 +
+                       (long)(a.njo2 - 1) * a.ojs2;
+  const bool win_ok = win_in < (1L << 31) && win_out < (1L << 31) && a.is2 >= 0 &&
+                      a.ijs1 >= 0 && a.ijs2 >= 0 && a.os2 >= 0 && a.ojs1 >= 0 && a.ojs2 >= 0;
+  if (a.nji <= kMax && a.njo <= kMax && im >= 0 && om >= 0 && a.G1 <= 65535 && lds <= 65536 &&
+      win_ok) {
     dim3 grid(static_cast<unsigned>((a.G2 + kStageG - 1) / kStageG), static_cast<unsigned>(a.G1),
               batch);
+    if constexpr (!real_of<TC>::cplx) {
+      const size_t lds_mfma = (size_t)rows * kStageLd * sizeof(TC);
+      hipLaunchKernelGGL((mix_mfma_kernel<TC, TI, TO>), grid, dim3(kStageG), lds_mfma, st, a, im,
+                         om, static_cast<const TC*>(Mx), static_cast<const TI*>(in),
+                         static_cast<TO*>(out));
+      TNH_CHECK_LAUNCH("mix_mfma");
+      return TNH_OK;
+    }
     hipLaunchKernelGGL((mix_staged_kernel<TC, TI, TO>), grid, dim3(kStageG), lds, st, a, im, om,
                        static_cast<const TC*>(Mx), static_cast<const TI*>(in),
                        static_cast<TO*>(out));
