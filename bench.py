@@ -54,6 +54,26 @@ def stage_bytes(a, b, d1, d2, wl, wr, e_op, e_io):
             4: x2 + wr * b * b * e_op + a * d1 * d2 * b * e_io}
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "matvec_traffic.json")
+
+
+def measured_traffic(kernel_label, chi, form):
+    """HBM bytes per launch of a kernel, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by what the counter reads for a copy
+    kernel of known size). None when no summary covers this kernel at this chi."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None, None
+    for row in table.get("rows", []):
+        if row["chi"] == chi and row["form"] == form and row["stage"] in kernel_label:
+            return row["fetch_bytes"] + row["write_bytes"], os.path.relpath(TRAFFIC_FILE,
+                                                                             os.getcwd())
+    return None, None
+
+
 def event_time(fn, reps, stream):
     """Average ms of `fn` over `reps` calls, with HIP events on `stream`."""
     import torch
@@ -640,7 +660,7 @@ def main():
         stage_ms[st] = event_time(fn, max(5, args.steps // 2), stream)
     dom = max(stage_ms, key=stage_ms.get)
     names = {1: "GEMM1 theta x L (gemm_tn_kernel<float,double,float,float>)",
-             2: "MPO mix (mix_kernel<float,float,float>)",
+             2: "MPO mix (mix_mfma_kernel<float,float,float>)",
              4: "GEMM3 x R (gemm_tn_kernel<float,float,float,double>)"}
     if dom == 2:
         achieved = sbytes[2] / (stage_ms[2] * 1e-3) / 1e9
@@ -651,6 +671,7 @@ def main():
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK["f32"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK["f32"], "traffic": None}
     roof["kernel"] = names[dom]
+    roof["traffic"], roof["traffic_source"] = measured_traffic(names[dom], a, "mixed")
     roof["per_stage_ms"] = {names[k]: round(v, 4) for k, v in stage_ms.items()}
 
     # ---- local ground state: mixed vs float64 -------------------------------------------

@@ -167,7 +167,7 @@ int tnh_dot(long n, const double* x, const double* y, double* out, void* ws, siz
 int tnh_scale_by(long n, const double* x, const double* div, double mul, double* y,
                  tnh_stream_t stream);
 
-/* dst = (dst_dtype) src, for F64 <-> F32. */
+/* dst = (dst_dtype) src, for F64 <-> F32 and F64 -> BF16 (round to nearest even). */
 int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst,
                 tnh_stream_t stream);
 

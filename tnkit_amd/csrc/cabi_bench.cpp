@@ -2,7 +2,7 @@
 // of the XXZ bulk problem, or a plain device copy for counter calibration. Used for rocprofv3
 // counter runs and as the example of a C caller of include/tnkit_hip.h.
 //
-//   cabi_bench mixed|fp64 [chi=1024] [reps=10]     H_eff theta, D = 5, d = 2, random L, R, theta
+//   cabi_bench mixed|fp64|bf16 [chi=1024] [reps=10] H_eff theta, D = 5, d = 2, random L, R, theta
 //   cabi_bench copy [MB=256] [reps=3]               y = 1.0 * x over MB megabytes (tnh_scale_by)
 #include <hip/hip_runtime.h>
 
@@ -78,8 +78,9 @@ int main(int argc, char** argv) {
   }
 
   const bool mixed = strcmp(mode, "mixed") == 0;
-  if (!mixed && strcmp(mode, "fp64") != 0) {
-    fprintf(stderr, "usage: cabi_bench mixed|fp64 [chi] [reps] | copy [MB] [reps]\n");
+  const bool bf16 = strcmp(mode, "bf16") == 0;
+  if (!mixed && !bf16 && strcmp(mode, "fp64") != 0) {
+    fprintf(stderr, "usage: cabi_bench mixed|fp64|bf16 [chi] [reps] | copy [MB] [reps]\n");
     return 1;
   }
   const int chi = argc > 2 ? atoi(argv[2]) : 1024;
@@ -109,9 +110,17 @@ int main(int argc, char** argv) {
     fprintf(stderr, "device allocation failed\n");
     return 2;
   }
-  const int dt = mixed ? TNH_F32 : TNH_F64;
-  const size_t e = mixed ? 4 : 8;
+  const int dt = mixed ? TNH_F32 : (bf16 ? TNH_BF16 : TNH_F64);
+  const size_t e = mixed ? 4 : (bf16 ? 2 : 8);
   void *Lo = L, *Ro = R, *Wo = Wd;
+  if (bf16) {  // L, R in bf16, W (and the built S) in float32
+    HIP_OK(hipMalloc(&Lo, nL * 2));
+    HIP_OK(hipMalloc(&Ro, nL * 2));
+    HIP_OK(hipMalloc(&Wo, W.size() * 4));
+    TNH_OK_(tnh_convert(TNH_BF16, TNH_F64, nL, L, Lo, st));
+    TNH_OK_(tnh_convert(TNH_BF16, TNH_F64, nL, R, Ro, st));
+    TNH_OK_(tnh_convert(TNH_F32, TNH_F64, (long)W.size(), Wd, Wo, st));
+  }
   if (mixed) {
     HIP_OK(hipMalloc(&Lo, nL * 4));
     HIP_OK(hipMalloc(&Ro, nL * 4));
@@ -123,7 +132,7 @@ int main(int argc, char** argv) {
   void *LT, *S, *RT, *out, *ws;
   HIP_OK(hipMalloc(&LT, nL * e));
   HIP_OK(hipMalloc(&RT, nL * e));
-  HIP_OK(hipMalloc(&S, (size_t)d * d * D * D * d * d * e));
+  HIP_OK(hipMalloc(&S, (size_t)d * d * D * D * d * d * (bf16 ? 4 : e)));
   HIP_OK(hipMalloc(&out, nth * 8));
   const size_t wsb = tnh_two_site_ws_bytes(dt, chi, d, d, chi, D, D);
   HIP_OK(hipMalloc(&ws, wsb));

@@ -324,6 +324,9 @@ int tnh_convert(int dst_dtype, int src_dtype, long n, const void* src, void* dst
   } else if (dst_dtype == TNH_F64 && src_dtype == TNH_F32) {
     hipLaunchKernelGGL((convert_kernel<double, float>), dim3(nblocks_stream(n)), dim3(256), 0, st, n,
                        static_cast<const float*>(src), static_cast<double*>(dst));
+  } else if (dst_dtype == TNH_BF16 && src_dtype == TNH_F64) {  // one rounding, to nearest even
+    hipLaunchKernelGGL((convert_kernel<__bf16, double>), dim3(nblocks_stream(n)), dim3(256), 0, st,
+                       n, static_cast<const double*>(src), static_cast<__bf16*>(dst));
   } else {
     set_error("tnh_convert: unsupported pair (%d <- %d)", dst_dtype, src_dtype);
     return TNH_ERR_UNSUPPORTED;
