@@ -23,39 +23,76 @@ namespace {
 constexpr int kGatherThreads = 256;
 constexpr int kGatherChunk = 2048;  // output elements per workgroup
 
+// A workgroup owns output chunk [base, base + kGatherChunk); runs are sorted by destination and
+// do not overlap, and clo/chi bound the runs that may touch the chunk. The descriptors of those
+// runs are first copied into LDS (when there are at most kRunsLds of them, the usual case: runs
+// shorter than a few dozen elements go to the tile kernel instead), then each thread takes 8
+// elements 256 apart, finds each one's run by binary search in LDS, and issues all 8 loads
+// before its first store. Searching in global memory instead puts three dependent global loads
+// in front of every element. Chunks touched by more runs search in global memory.
+constexpr int kRunsLds = 256;
+constexpr int kPerThread = kGatherChunk / kGatherThreads;
+
 template <typename TS, typename TD>
 __global__ void __launch_bounds__(kGatherThreads)
     gather_runs_kernel(const TS* __restrict__ src, TD* __restrict__ dst, long n,
                        const long* __restrict__ sbeg, const long* __restrict__ sstep,
                        const long* __restrict__ dbeg, const long* __restrict__ len,
                        const int* __restrict__ clo, const int* __restrict__ chi, int fill) {
+  __shared__ long s_d0[kRunsLds], s_d1[kRunsLds], s_s0[kRunsLds], s_st[kRunsLds];
   const long base = static_cast<long>(blockIdx.x) * kGatherChunk;
   const int lo = clo[blockIdx.x];
   const int hi = chi[blockIdx.x];  // runs [lo, hi) may overlap this chunk
-  int r = lo;
-  for (int i = 0; i < kGatherChunk / kGatherThreads; ++i) {
-    const long j = base + threadIdx.x + static_cast<long>(i) * kGatherThreads;
-    if (j >= n) break;
-    // the last run in [r, hi) starting at or before j (j grows, so r only moves forward)
-    int a = r, b = hi;
-    while (b - a > 1) {
-      const int mid = (a + b) >> 1;
-      if (dbeg[mid] <= j)
-        a = mid;
-      else
-        b = mid;
+  const int nr = hi - lo;
+  const int t = threadIdx.x;
+  TD v[kPerThread];
+  bool hit[kPerThread];
+  if (nr <= kRunsLds) {
+    for (int k = t; k < nr; k += kGatherThreads) {
+      const long d0 = dbeg[lo + k];
+      s_d0[k] = d0;
+      s_d1[k] = d0 + len[lo + k];
+      s_s0[k] = sbeg[lo + k];
+      s_st[k] = sstep[lo + k];
     }
-    r = a;
-    bool hit = false;
-    TD v = TD(0);
-    if (r < hi) {
-      const long d0 = dbeg[r];
-      if (d0 <= j && j < d0 + len[r]) {
-        v = static_cast<TD>(src[sbeg[r] + (j - d0) * sstep[r]]);
-        hit = true;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const long j = base + t + static_cast<long>(i) * kGatherThreads;
+      int a = 0, b = nr;  // last run with d0 <= j
+      while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (s_d0[mid] <= j)
+          a = mid;
+        else
+          b = mid;
       }
+      hit[i] = j < n && a < nr && s_d0[a] <= j && j < s_d1[a];
+      v[i] = hit[i] ? static_cast<TD>(src[s_s0[a] + (j - s_d0[a]) * s_st[a]]) : TD(0);
     }
-    if (hit || fill) dst[j] = v;
+  } else {
+    int r = lo;
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i) {
+      const long j = base + t + static_cast<long>(i) * kGatherThreads;
+      int a = r, b = hi;  // j grows with i, so r only moves forward
+      while (b - a > 1) {
+        const int mid = (a + b) >> 1;
+        if (dbeg[mid] <= j)
+          a = mid;
+        else
+          b = mid;
+      }
+      r = a;
+      const long d0 = a < hi ? dbeg[a] : 0;
+      hit[i] = j < n && a < hi && d0 <= j && j < d0 + len[a];
+      v[i] = hit[i] ? static_cast<TD>(src[sbeg[a] + (j - d0) * sstep[a]]) : TD(0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    const long j = base + t + static_cast<long>(i) * kGatherThreads;
+    if (j < n && (hit[i] || fill)) dst[j] = v[i];
   }
 }
 
@@ -90,24 +127,41 @@ __global__ void __launch_bounds__(256)
   const long src0 = tl[0], srow = tl[1], scol = tl[2], dst0 = tl[3], drow = tl[4];
   const int nrows = static_cast<int>(tl[5]), ncols = static_cast<int>(tl[6]);
   const int t = threadIdx.x;
+  // fixed trip counts (a tile has at most 64 x 64 = 16 x 256 elements), fully unrolled, so a
+  // thread's 16 loads are all issued before the first store
+  constexpr int kPer = kTile * kTile / 256;
   if (scol == 1) {
     const int n = nrows * ncols;
-    for (int e = t; e < n; e += 256) {
+    TD v[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * 256;
       const int i = e / ncols, j = e - i * ncols;
-      dst[dst0 + static_cast<long>(i) * drow + j] =
-          static_cast<TD>(src[src0 + static_cast<long>(i) * srow + j]);
+      v[q] = e < n ? static_cast<TD>(src[src0 + static_cast<long>(i) * srow + j]) : TD(0);
+    }
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int e = t + q * 256;
+      const int i = e / ncols, j = e - i * ncols;
+      if (e < n) dst[dst0 + static_cast<long>(i) * drow + j] = v[q];
     }
     return;
   }
   const int i = t & 63;
-  for (int j = t >> 6; j < ncols; j += 4)
-    if (i < nrows)
-      buf[j][i] = static_cast<TD>(src[src0 + static_cast<long>(i) * srow +
-                                      static_cast<long>(j) * scol]);
+  const TS* s = src + src0 + static_cast<long>(i) * srow;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int j = (t >> 6) + 4 * q;
+    if (i < nrows && j < ncols) buf[j][i] = static_cast<TD>(s[static_cast<long>(j) * scol]);
+  }
   __syncthreads();
   const int jj = t & 63;
-  for (int ii = t >> 6; ii < nrows; ii += 4)
-    if (jj < ncols) dst[dst0 + static_cast<long>(ii) * drow + jj] = buf[jj][ii];
+  TD* d = dst + dst0 + jj;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int ii = (t >> 6) + 4 * q;
+    if (ii < nrows && jj < ncols) d[static_cast<long>(ii) * drow] = buf[jj][ii];
+  }
 }
 
 template <typename TS, typename TD>
