@@ -153,6 +153,70 @@ def right_canonicalize(M):
 # --------------------------------------------------------------------------------------------
 
 
+def _tridiag_lowest(alphas, betas, k):
+    """Lowest eigenpair of the k x k symmetric tridiagonal (diag alphas[:k], off-diag
+    betas[:k-1]) by dense eigh (tnkit/eigensolver.py:352-371 uses LAPACK's tridiagonal
+    routine; the pair is the same to rounding)."""
+    d = np.asarray(alphas[:k], dtype=np.float64)
+    if k == 1:
+        return float(d[0]), np.array([1.0])
+    e = np.asarray(betas[:k - 1], dtype=np.float64)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    vals, vecs = np.linalg.eigh(T)
+    return float(vals[0]), vecs[:, 0]
+
+
+def lanczos_ground(matvec, v0, tol=1e-8, max_iter=200, reorth="full", reorth_period=5,
+                   stop="residual"):
+    """Lowest eigenpair by Lanczos with an unbounded basis. tnkit/eigensolver.py:374-504:
+    w = H v - beta_prev v_prev; alpha = <v, w>; w -= alpha v; modified Gram-Schmidt against
+    every stored vector when reorth == 'full' (or every `reorth_period` steps for 'periodic');
+    stop on |beta y_last| < tol ('residual') or |e - e_prev| < tol ('energy'); breakdown at
+    beta < 1e-12 max(1, |alpha|); the vector is assembled once at the end."""
+    n0 = float(np.linalg.norm(v0))
+    if not np.isfinite(n0) or n0 < 1e-300:
+        return 0.0, v0, {"iters": 0, "converged": False}
+    v = v0 / n0
+    V = [v]
+    alphas, betas = [], []
+    beta_prev = 0.0
+    v_prev = np.zeros_like(v)
+    prev_e = None
+    converged = False
+    e = 0.0
+    it = 0
+    for it in range(max_iter):
+        w = matvec(v) - beta_prev * v_prev
+        alpha = float(np.vdot(v, w).real)
+        alphas.append(alpha)
+        w = w - alpha * v
+        if reorth == "full" or (reorth == "periodic" and it % reorth_period == 0):
+            for u in V:
+                w = w - np.vdot(u, w) * u
+        e, y = _tridiag_lowest(alphas, betas, len(alphas))
+        beta = float(np.linalg.norm(w))
+        breakdown = (not np.isfinite(beta) or not np.isfinite(alpha)
+                     or beta < 1e-12 * max(1.0, abs(alpha)))
+        hit = (abs(beta) * abs(float(y[-1])) < tol) if stop == "residual" else \
+            (prev_e is not None and abs(e - prev_e) < tol)
+        if breakdown or hit:
+            converged = True
+            it += 1
+            break
+        prev_e = e
+        betas.append(beta)
+        v_prev, beta_prev = v, beta
+        v = w / beta
+        V.append(v)
+    else:
+        it = max_iter
+    _, ground = _tridiag_lowest(alphas, betas, len(alphas))
+    x = np.zeros_like(V[0])
+    for c, vj in zip(ground[:len(V)], V):
+        x = x + float(c) * vj
+    return e, x / np.linalg.norm(x), {"iters": it, "converged": converged}
+
+
 def lanczos_ground_restart(matvec, v0, tol=1e-8, max_iter=200, m=40, k=10,
                            dtype=np.float64):
     """Lowest eigenpair by thick-restart Lanczos (Wu & Simon 2000) with CGS2 reorthogonalisation
@@ -358,11 +422,11 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
          trunc: str = "svd", seed: int = 0, max_iter: Optional[int] = None,
          restart_m: int = 40, restart_k: int = 10, mixed_tol_lo: float = 1e-4,
          init_mps: Optional[List[np.ndarray]] = None, dtype=np.float64,
-         max_steps: Optional[int] = None, canonical: str = "svd"
-         ) -> Tuple[float, List[np.ndarray], Dict]:
+         max_steps: Optional[int] = None, canonical: str = "svd", reorth: str = "full",
+         eig_stop: str = "residual") -> Tuple[float, List[np.ndarray], Dict]:
     """Dense two-site DMRG, the sweep of tnkit/_dmrg.py:560-799 restricted to the dense
     path: warm-started thick-restart Lanczos (or the mixed two-stage form with a float32
-    matvec), SVD or Gram truncation, S renormalised, environments updated after each cut.
+    matvec, or the unbounded-basis Lanczos of eig='lanczos'), SVD or Gram truncation, S renormalised, environments updated after each cut.
 
     `max_steps` (oracle-only) stops after that many local steps, for timing samples.
     `canonical` 'lq' starts from the LQ right-canonical form (the same state in another gauge).
@@ -420,6 +484,9 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
         elif eig == "lanczos-restart":
             e, x, _ = lanczos_ground_restart(mv, theta0.reshape(-1), tol=eig_tol, max_iter=mi,
                                              m=restart_m, k=restart_k, dtype=dtype)
+        elif eig == "lanczos":
+            e, x, _ = lanczos_ground(mv, theta0.reshape(-1), tol=eig_tol, max_iter=mi,
+                                     reorth=reorth, stop=eig_stop)
         else:
             raise ValueError(f"oracle eig {eig!r} not restated")
         x = x.reshape(shape)

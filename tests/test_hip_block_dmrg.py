@@ -132,3 +132,14 @@ def test_block_dmrg_gram_truncation_exact_in_sector(hip, q):
                         n_sweeps=run["n_sweeps"], tol=0.0, rtol=0.0, seed=run["seed"],
                         trunc="gram")
     assert abs(E - float.fromhex(run["energy"])) < 1e-8
+
+
+@pytest.mark.parametrize("stop", ["residual", "energy"])
+def test_block_dmrg_lanczos_exact_in_sector(hip, stop):
+    """eig='lanczos' (unbounded basis, full reorthogonalisation) on the flat block buffer:
+    N = 10 at the exact bond reaches the sector's exact ground energy."""
+    N = 10
+    mpo, _ = BM.to_block_mpo(xxz_mpo(N), [[1, -1]] * N, hip)
+    E, _, info = dmrg(mpo, 32, charge=2, expand=0, n_sweeps=4, tol=0.0, rtol=0.0, seed=5,
+                      eig="lanczos", eig_stop=stop, eig_tol=1e-11 if stop == "energy" else 1e-9)
+    assert abs(E - _sector_ground(N, 2)) < 1e-9

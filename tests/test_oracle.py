@@ -19,6 +19,8 @@ from conftest import GOLDEN
 
 with open(os.path.join(GOLDEN, "golden.json")) as f:
     GOLD = json.load(f)
+with open(os.path.join(GOLDEN, "golden_lanczos.json")) as f:
+    GOLD_LZ = json.load(f)
 VEC = np.load(os.path.join(GOLDEN, "vectors.npz"), allow_pickle=False)
 
 
@@ -380,3 +382,74 @@ def test_product_free_fermion_matches_oracle():
         < 1e-10
     assert abs(F.tfim_ground_energy(N, g0) - X.tfim_free_fermion_energy(N, g0)) < 1e-10
     assert abs(F.tfim_ground_energy(32, 1.0) - X.tfim_free_fermion_energy(32, 1.0)) < 1e-12
+
+
+# --------------------------------------------------------------------------------------------
+# unbounded-basis Lanczos (eig='lanczos')
+# --------------------------------------------------------------------------------------------
+
+def _lz_matrix():
+    run = GOLD_LZ["lanczos_ground"]
+    rng = np.random.default_rng(run["seed"])
+    A = rng.standard_normal((run["n"], run["n"]))
+    return 0.5 * (A + A.T), rng.standard_normal(run["n"])
+
+
+@pytest.mark.parametrize("case", GOLD_LZ["lanczos_ground"]["runs"],
+                         ids=lambda c: f"{c['reorth']}-{c['stop']}-{c['max_iter']}")
+def test_lanczos_ground_matches_tnkit(case):
+    """tnkit's lanczos_ground on a fixed symmetric matrix: the same iteration count, converged
+    flag, Ritz value (1e-12) and overlap with the start vector; a converged Ritz value is also
+    within the residual tolerance of eigh's lowest eigenvalue."""
+    A, v0 = _lz_matrix()
+    e, x, info = D.lanczos_ground(lambda v: A @ v, v0.copy(), tol=case["tol"],
+                                  max_iter=case["max_iter"], reorth=case["reorth"],
+                                  stop=case["stop"])
+    assert info["iters"] == case["iters"] and info["converged"] == case["converged"]
+    assert abs(e - float.fromhex(case["e"])) < 1e-12
+    assert abs(abs(x @ v0) - float.fromhex(case["x_dot_v0"])) < 1e-10
+    if case["converged"] and case["stop"] == "residual":
+        assert abs(e - np.linalg.eigvalsh(A)[0]) < 1e-9
+
+
+def test_lanczos_ground_invariants():
+    """On a diagonal operator with known spectrum the full-reorth Lanczos finds the lowest
+    eigenvalue and its unit vector; a start vector that is an eigenvector breaks down after one
+    step with that eigenvalue."""
+    lam = np.linspace(-3.0, 5.0, 300)
+    rng = np.random.default_rng(1)
+    e, x, info = D.lanczos_ground(lambda v: lam * v, rng.standard_normal(300), tol=1e-10,
+                                  max_iter=300)
+    assert info["converged"] and abs(e + 3.0) < 1e-9 and abs(abs(x[0]) - 1) < 1e-8
+    v = np.zeros(300)
+    v[7] = 2.0
+    e, x, info = D.lanczos_ground(lambda u: lam * u, v, tol=1e-10, max_iter=50)
+    assert info["iters"] == 1 and abs(e - lam[7]) < 1e-15
+
+
+@pytest.mark.parametrize("run", GOLD_LZ["dmrg"], ids=[r["name"] for r in GOLD_LZ["dmrg"]])
+def test_dmrg_lanczos_matches_tnkit(run):
+    """eig='lanczos' DMRG (with eig_stop='energy' on the TFIM run): per-sweep energies to
+    1e-12, the exact matvec count and the final bonds of tnkit's run."""
+    E, M, info = D.dmrg(_model(run["model"]), run["chi"], **run["kwargs"])
+    ref = [float.fromhex(h) for h in run["energies"]]
+    assert max(abs(a - b) for a, b in zip(info["energies"], ref)) < 1e-12
+    assert info["matvec_calls"] == run["matvec_calls"]
+    assert [int(t.shape[2]) for t in M] == run["bonds"]
+
+
+def test_eig_option_rules():
+    """The product's option rules (tnkit/_dmrg.py:603-611, tnkit/eigensolver.py:27-58):
+    reorth='periodic' raises with eig_stop='residual' and warns with 'energy'; an unknown
+    eig_stop raises."""
+    from types import SimpleNamespace
+    from tnkit_amd.eigensolver import check_eig_options
+    with pytest.raises(ValueError):
+        check_eig_options(SimpleNamespace(eig="lanczos-restart", reorth="periodic",
+                                          eig_stop="residual"))
+    with pytest.warns(RuntimeWarning):
+        check_eig_options(SimpleNamespace(eig="lanczos-restart", reorth="periodic",
+                                          eig_stop="energy"))
+    with pytest.raises(ValueError):
+        check_eig_options(SimpleNamespace(eig="lanczos", reorth="full", eig_stop="value"))
+    check_eig_options(SimpleNamespace(eig="lanczos", reorth="periodic", eig_stop="residual"))

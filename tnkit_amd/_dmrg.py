@@ -6,7 +6,7 @@ Tensor axes follow tnkit (tnkit/_dmrg.py:7-11):
 
 The front door `dmrg(mpo, chi, options=None, **kwargs)` and `DMRGOptions` keep tnkit's names,
 defaults and return values (E, MPS, info). What is served: dense real MPOs, two-site steps,
-eig in {'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram'}. Other options of tnkit
+eig in {'lanczos', 'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram'}. Other options of tnkit
 raise ValueError naming what is not provided.
 
 'lanczos-mixed' here runs stage 1 on a float32 operator on the float32 matrix cores (real, not
@@ -24,7 +24,8 @@ import torch
 
 from . import kernels
 from .backend import HipBackend, get_backend
-from .eigensolver import lanczos_ground_mixed, lanczos_ground_restart
+from .eigensolver import (check_eig_options, lanczos_ground, lanczos_ground_mixed,
+                          lanczos_ground_restart)
 from .linalg import gram_truncation
 from .mps import MPS
 
@@ -186,19 +187,17 @@ def _check(o: DMRGOptions, mpo) -> None:
         raise ValueError("block MPOs are not served by the dense HIP sweep")
     if o.sites != 2:
         raise ValueError(f"sites must be 2 on the HIP backend, got {o.sites!r}")
-    if o.eig not in ("lanczos-restart", "lanczos-mixed"):
-        raise ValueError(f"unknown eig solver {o.eig!r} for the HIP backend; expected one of "
-                         "['lanczos-mixed', 'lanczos-restart']")
+    if o.eig not in ("lanczos", "lanczos-restart", "lanczos-mixed"):
+        raise ValueError(f"unknown eig solver {o.eig!r}; expected one of "
+                         "['lanczos', 'lanczos-mixed', 'lanczos-restart']")
     if o.trunc not in ("svd", "gram"):
         raise ValueError(f"unknown trunc {o.trunc!r} for the HIP backend; expected 'svd' or 'gram'")
     if o.charge is not None:
         raise ValueError("charge= needs a block MPO")
     if o.expand not in (None, 0):
         raise ValueError("subspace expansion is not served by the HIP backend (expand=0)")
-    if o.eig_stop != "residual":
-        raise ValueError("eig_stop must be 'residual' on the HIP backend")
-    if o.reorth == "periodic":
-        raise ValueError("reorth='periodic' with stop='residual' is not supported")
+    check_eig_options(o)
+
 
 
 class DMRGEngine:
@@ -287,6 +286,9 @@ class DMRGEngine:
             e, x, inf = lanczos_ground_mixed(matvec_lo, matvec, v0, be, tol=cur_tol, max_iter=mi,
                                              m=o.restart_m, k=o.restart_k, tol_lo=o.mixed_tol_lo,
                                              counts=self.mixed_counts)
+        elif o.eig == "lanczos":
+            e, x, inf = lanczos_ground(matvec, v0, be, tol=cur_tol, max_iter=mi, reorth=o.reorth,
+                                       stop=o.eig_stop)
         else:
             e, x, inf = lanczos_ground_restart(matvec, v0, be, tol=cur_tol, max_iter=mi,
                                                m=o.restart_m, k=o.restart_k,

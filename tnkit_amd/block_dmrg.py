@@ -18,7 +18,8 @@ import torch
 from . import block_linalg as BL
 from . import block_mps as BM
 from .block_ops import BlockTensor, BlockTwoSite, einsum
-from .eigensolver import lanczos_ground_mixed, lanczos_ground_restart
+from .eigensolver import (check_eig_options, lanczos_ground, lanczos_ground_mixed,
+                          lanczos_ground_restart)
 from .mps import MPS
 
 
@@ -38,9 +39,10 @@ def right_canonicalize(M: List[BlockTensor]) -> List[BlockTensor]:
 def _check(o, mpo) -> None:
     if o.sites != 2:
         raise ValueError(f"sites must be 2 on the HIP backend, got {o.sites!r}")
-    if o.eig not in ("lanczos-restart", "lanczos-mixed"):
-        raise ValueError(f"unknown eig solver {o.eig!r} for the HIP block sweep; expected one of "
-                         "['lanczos-mixed', 'lanczos-restart']")
+    if o.eig not in ("lanczos", "lanczos-restart", "lanczos-mixed"):
+        raise ValueError(f"unknown eig solver {o.eig!r}; expected one of "
+                         "['lanczos', 'lanczos-mixed', 'lanczos-restart']")
+    check_eig_options(o)
     if o.trunc not in ("svd", "gram"):
         raise ValueError(f"unknown trunc {o.trunc!r}; expected 'svd' or 'gram'")
     if o.charge is None:
@@ -100,6 +102,9 @@ class BlockDMRGEngine:
             e, x, _ = lanczos_ground_mixed(matvec_lo, matvec, v0.clone(), tol=cur_tol,
                                            max_iter=mi, m=o.restart_m, k=o.restart_k,
                                            tol_lo=o.mixed_tol_lo, counts=self.mixed_counts)
+        elif o.eig == "lanczos":
+            e, x, _ = lanczos_ground(matvec, v0.clone(), tol=cur_tol, max_iter=mi,
+                                     reorth=o.reorth, stop=o.eig_stop)
         else:
             e, x, _ = lanczos_ground_restart(matvec, v0.clone(), tol=cur_tol, max_iter=mi,
                                              m=o.restart_m, k=o.restart_k)

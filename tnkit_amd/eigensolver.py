@@ -10,10 +10,174 @@ in tnkit.
 """
 from __future__ import annotations
 
+import time
+import warnings
+
 import numpy as np
 import torch
 
 from . import kernels
+
+try:
+    from scipy.linalg import eigh_tridiagonal as _eigh_tridiagonal
+except ImportError:  # pragma: no cover
+    _eigh_tridiagonal = None
+
+
+def check_reorth_periodic(reorth, stop) -> None:
+    """tnkit's guard on reorth='periodic' (tnkit/eigensolver.py:27-58): it raises with
+    stop='residual' and warns with stop='energy'; other reorth values pass."""
+    if reorth != "periodic":
+        return
+    if stop == "residual":
+        raise ValueError("reorth='periodic' with stop='residual' is not supported: periodic "
+                         "reorthogonalisation loses basis orthogonality on long local solves, "
+                         "and the residual rule then iterates against a tridiagonal that no "
+                         "longer describes the basis. Use reorth='full'.")
+    warnings.warn("reorth='periodic' loses orthogonality on long local solves (large bond "
+                  "dimension near a critical point) and can stall DMRG above the ground state. "
+                  "Use reorth='full'.", RuntimeWarning, stacklevel=3)
+
+
+def check_eig_options(o) -> None:
+    """tnkit's rules for eig_stop and reorth (tnkit/_dmrg.py:603-611): 'lanczos' takes both;
+    the other solvers ignore them, but reorth='periodic' is still refused with
+    eig_stop='residual' (and warned about with 'energy')."""
+    if o.eig_stop not in ("residual", "energy"):
+        raise ValueError(f"eig_stop must be 'residual' or 'energy', got {o.eig_stop!r}")
+    if o.eig != "lanczos":
+        check_reorth_periodic(o.reorth, o.eig_stop)
+
+
+def _tridiag_ground(alphas, betas, k, vector=True):
+    """(lowest eigenvalue, its eigenvector or None) of the k x k symmetric tridiagonal with
+    diagonal alphas[:k] and off-diagonal betas[:k-1] (tnkit/eigensolver.py:61-92, 352-371)."""
+    d = np.asarray(alphas[:k], dtype=np.float64)
+    if k == 1:
+        return float(d[0]), (np.array([1.0]) if vector else None)
+    e = np.asarray(betas[:k - 1], dtype=np.float64)
+    if _eigh_tridiagonal is not None:
+        try:
+            if vector:
+                vals, vecs = _eigh_tridiagonal(d, e, select="i", select_range=(0, 0))
+                return float(vals[0]), vecs[:, 0]
+            vals = _eigh_tridiagonal(d, e, eigvals_only=True, select="i", select_range=(0, 0))
+            return float(vals[0]), None
+        except Exception:  # pragma: no cover - LAPACK stebz/stein failure: dense fallback
+            pass
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    vals, vecs = np.linalg.eigh(T)
+    return float(vals[0]), (vecs[:, 0] if vector else None)
+
+
+_CHUNK = 64  # basis rows per device buffer of lanczos_ground (the orthogonalisation window)
+
+
+def lanczos_ground(matvec, v0, ops=None, tol=1e-8, max_iter=200, reorth="full",
+                   reorth_period=5, timers=None, stop="residual"):
+    """Lowest eigenpair by Lanczos with an unbounded basis (tnkit/eigensolver.py:374-504):
+    the three-term recurrence, reorthogonalisation against the whole basis ('full'), every
+    `reorth_period` steps ('periodic') or never ('none'), and tnkit's stopping rules: the Ritz
+    residual |beta y_last| < tol ('residual') or |e - e_prev| < tol ('energy'); breakdown at
+    beta < 1e-12 max(1, |alpha|). Returns (e, x, {'iters', 'converged'}).
+
+    The basis lives on the device in buffers of 64 rows, allocated as it grows. A full
+    reorthogonalisation is two classical Gram-Schmidt passes per buffer (`kernels.krylov_orth`),
+    older buffers first; the call on the current buffer also subtracts beta_prev v_prev and
+    alpha v, and returns (alpha, beta) in one host read. tnkit subtracts one vector at a time
+    (modified Gram-Schmidt); the two agree to rounding."""
+    if reorth is True:
+        reorth = "full"
+    elif reorth is False:
+        reorth = "none"
+    if reorth not in ("full", "periodic", "none"):
+        raise ValueError(f"reorth must be 'full', 'periodic' or 'none', got {reorth!r}")
+    if stop not in ("residual", "energy"):
+        raise ValueError(f"stop must be 'residual' or 'energy', got {stop!r}")
+    check_reorth_periodic(reorth, stop)
+    if v0.dtype != torch.float64:
+        raise TypeError("lanczos_ground (hip): float64 vectors only")
+    _t = time.perf_counter if timers is not None else None
+    dev = v0.device
+    v0 = v0.reshape(-1).contiguous()
+    n = v0.numel()
+    kw = kernels.KrylovWorkspace(n, dev)
+    n0 = float(kernels.norm(v0, kw).item())
+    if not np.isfinite(n0) or n0 < 1e-300:
+        return 0.0, v0, {"iters": 0, "converged": False}
+    chunks = []
+
+    def row(i):
+        c = i // _CHUNK
+        while len(chunks) <= c:
+            chunks.append(torch.empty((_CHUNK, n), dtype=torch.float64, device=dev))
+        return chunks[c][i % _CHUNK]
+
+    kernels.scale_by(v0, None, 1.0 / n0, row(0))
+    alphas, betas = [], []
+    beta_prev = 0.0
+    prev_e = None
+    converged = False
+    e = 0.0
+    it = 0
+    for it in range(max_iter):
+        if _t:
+            t0 = _t()
+        w = matvec(row(it))
+        if w.dtype != torch.float64 or not w.is_contiguous():
+            w = w.to(torch.float64).contiguous()
+        w = w.reshape(-1)
+        c, jj = divmod(it, _CHUNK)
+        do_reorth = reorth == "full" or (reorth == "periodic" and it % reorth_period == 0)
+        if _t:
+            timers["matvec"] += _t() - t0
+            t0 = _t()
+        if do_reorth:
+            if jj == 0 and it > 0:   # v_prev closes the previous buffer
+                kernels.lincomb(chunks[c - 1][_CHUNK - 1:], [-beta_prev], w, beta=1.0)
+            for cc in range(c):
+                kernels.krylov_orth(chunks[cc], _CHUNK, w, 0, [], 0, kw)
+            sub, base = ([beta_prev], jj - 1) if jj >= 1 else ([], 0)
+            kernels.krylov_orth(chunks[c], jj + 1, w, jj, sub, base, kw)
+            alpha, beta = (float(x) for x in kw.scal.cpu().tolist())
+        else:
+            if it > 0:
+                kernels.lincomb(row(it - 1).unsqueeze(0), [-beta_prev], w, beta=1.0)
+            alpha = float(kernels.dot(row(it), w, kw).item())
+            kernels.lincomb(row(it).unsqueeze(0), [-alpha], w, beta=1.0)
+            beta = float(kernels.norm(w, kw).item())
+        alphas.append(alpha)
+        if _t:
+            timers["reorth"] += _t() - t0
+            t0 = _t()
+        k = len(alphas)
+        e, y = _tridiag_ground(alphas, betas, k, vector=(stop == "residual"))
+        if _t:
+            timers["tridiag"] += _t() - t0
+        breakdown = (not np.isfinite(beta) or not np.isfinite(alpha)
+                     or beta < 1e-12 * max(1.0, abs(alpha)))
+        if stop == "residual":
+            hit = abs(beta) * abs(float(y[-1])) < tol
+        else:
+            hit = prev_e is not None and abs(e - prev_e) < tol
+        if breakdown or hit:
+            converged = True
+            it += 1
+            break
+        prev_e = e
+        betas.append(beta)
+        beta_prev = beta
+        kernels.scale_by(w, None, 1.0 / beta, row(it + 1))
+    else:
+        it = max_iter
+    _, ground = _tridiag_ground(alphas, betas, len(alphas))
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    for c in range(-(-len(ground) // _CHUNK)):
+        coef = [float(g) for g in ground[c * _CHUNK:(c + 1) * _CHUNK]]
+        kernels.lincomb(chunks[c][:len(coef)], coef, x, beta=1.0)
+    nx = float(kernels.norm(x, kw).item())
+    kernels.scale_by(x, None, 1.0 / nx, x)
+    return e, x, {"iters": it, "converged": converged}
 
 
 def lanczos_ground_restart(matvec, v0, ops=None, tol=1e-8, max_iter=200, m=40, k=10,
