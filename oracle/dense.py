@@ -332,6 +332,71 @@ def lanczos_ground_mixed(matvec_lo, matvec_hi, v0, tol=1e-8, max_iter=200, m=40,
                   "n_lo": c["lo"], "n_hi": c["hi"], "e_lo": e_lo}
 
 
+def _tridiag_matrix(alphas, betas):
+    j = len(alphas)
+    T = np.diag(np.asarray(alphas, dtype=np.float64))
+    if j > 1:
+        off = np.asarray(betas[:j - 1], dtype=np.float64)
+        T += np.diag(off, 1) + np.diag(off, -1)
+    return T
+
+
+def lanczos_expm(matvec, v, tau, m=30, tol=1e-12, adaptive=True, info=None, est_ref_j=2):
+    """exp(tau H) v by an adaptive Krylov space with full reorthogonalisation
+    (tnkit/eigensolver.py:230-349): stop at breakdown (beta < tol) or when Saad's first
+    neglected term |beta| |tau| |exp(tau T)[j-1, 0]| beta0 drops below tol (j >= 2); `info`
+    gets krylov_dim, err_est, beta1, est_ref (the same estimate frozen at j = est_ref_j),
+    est_ref_j and capped."""
+    import scipy.linalg
+    beta0 = float(np.linalg.norm(v))
+    if not np.isfinite(beta0) or beta0 < 1e-300:
+        return v
+    V = [v / beta0]
+    alphas, betas = [], []
+    beta = 0.0
+    v_prev = np.zeros_like(v)
+    est, est_ref, est_ref_at, beta1 = float("inf"), float("nan"), 0, float("nan")
+    capped = False
+    for k in range(m):
+        w = matvec(V[-1]) - beta * v_prev
+        alpha = float(np.vdot(V[-1], w).real)
+        alphas.append(alpha)
+        w = w - alpha * V[-1]
+        for u in V:
+            w = w - np.vdot(u, w) * u
+        beta = float(np.linalg.norm(w))
+        if k == 0:
+            beta1 = beta
+        if beta < tol:
+            if est_ref_at == 0:
+                est_ref, est_ref_at = 0.0, k + 1
+            break
+        jj = len(alphas)
+        want_ref = info is not None and est_ref_at == 0 and jj >= est_ref_j
+        stop = False
+        if (adaptive and jj >= 2) or want_ref:
+            Et = scipy.linalg.expm(tau * _tridiag_matrix(alphas, betas))
+            est = abs(beta) * abs(tau) * abs(Et[jj - 1, 0]) * beta0
+            if want_ref:
+                est_ref, est_ref_at = est, jj
+            stop = bool(adaptive and jj >= 2 and est < tol)
+        betas.append(beta)
+        v_prev = V[-1]
+        V.append(w / beta)
+        if stop:
+            break
+    else:
+        capped = bool(adaptive)
+    c = beta0 * scipy.linalg.expm(tau * _tridiag_matrix(alphas, betas))[:, 0]
+    if info is not None:
+        info.update(krylov_dim=len(alphas), err_est=est, beta1=beta1, est_ref=est_ref,
+                    est_ref_j=est_ref_at, capped=capped)
+    out = np.zeros_like(V[0], dtype=np.complex128)
+    for coeff, vj in zip(c, V):
+        out = out + complex(coeff) * vj
+    return out
+
+
 def lanczos_expm_fixed(matvec, v, tau, m):
     """exp(tau H) v from m Krylov steps with CGS2, alpha read from the first projection and a
     clamped division instead of a breakdown branch; exponential of the tridiagonal by its

@@ -21,6 +21,8 @@ with open(os.path.join(GOLDEN, "golden.json")) as f:
     GOLD = json.load(f)
 with open(os.path.join(GOLDEN, "golden_lanczos.json")) as f:
     GOLD_LZ = json.load(f)
+with open(os.path.join(GOLDEN, "golden_expm.json")) as f:
+    GOLD_EX = json.load(f)
 VEC = np.load(os.path.join(GOLDEN, "vectors.npz"), allow_pickle=False)
 
 
@@ -453,3 +455,40 @@ def test_eig_option_rules():
     with pytest.raises(ValueError):
         check_eig_options(SimpleNamespace(eig="lanczos", reorth="full", eig_stop="value"))
     check_eig_options(SimpleNamespace(eig="lanczos", reorth="periodic", eig_stop="residual"))
+
+
+# --------------------------------------------------------------------------------------------
+# adaptive Krylov exponential
+# --------------------------------------------------------------------------------------------
+
+def expm_case(c):
+    """(matvec, start vector, dense matrix) of one recorded lanczos_expm case."""
+    if c.get("diag"):
+        lam = np.arange(1.0, 41.0)
+        v0 = np.zeros(40, dtype=np.complex128)
+        v0[[2, 9, 30]] = [1.0, 2.0, -1.0]
+        return np.diag(lam), v0
+    rng = np.random.default_rng(c["seed"])
+    A = rng.standard_normal((c["n"], c["n"]))
+    A = 0.5 * (A + A.T)
+    return A, rng.standard_normal(c["n"]) + 1j * rng.standard_normal(c["n"])
+
+
+@pytest.mark.parametrize("c", GOLD_EX["lanczos_expm"], ids=lambda c: f"dt{c['dt']}-m{c['m']}")
+def test_lanczos_expm_matches_tnkit(c):
+    """tnkit's adaptive exponential: the same Krylov dimension, cap flag, frozen estimate and
+    output vector (1e-13); a converged run is also exp(tau A) v from scipy to 1e-9."""
+    A, v0 = expm_case(c)
+    info = {}
+    out = D.lanczos_expm(lambda v: A @ v, v0, -1j * c["dt"], m=c["m"], tol=c["tol"],
+                         adaptive=c["adaptive"], info=info)
+    ref = np.array([float.fromhex(x) for x in c["out_re"]]) + \
+        1j * np.array([float.fromhex(x) for x in c["out_im"]])
+    assert info["krylov_dim"] == c["krylov_dim"] and info["capped"] == c["capped"]
+    assert info["est_ref_j"] == c["est_ref_j"]
+    assert abs(info["est_ref"] - float.fromhex(c["est_ref"])) <= 1e-12 * abs(
+        float.fromhex(c["est_ref"])) + 1e-300
+    assert np.abs(out - ref).max() < 1e-13
+    if not c["capped"] and c["adaptive"]:
+        exact = scipy.linalg.expm(-1j * c["dt"] * A) @ v0
+        assert np.abs(out - exact).max() < 1e-9

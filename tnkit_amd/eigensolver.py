@@ -302,3 +302,122 @@ def lanczos_ground_mixed(matvec_lo, matvec_hi, v0, ops=None, tol=1e-8, max_iter=
     return e_hi, x_hi, {"iters": i_lo["iters"] + i_hi["iters"], "converged": i_hi["converged"],
                         "restarts": i_lo["restarts"] + i_hi["restarts"], "n_lo": c1["n"],
                         "n_hi": c2["n"], "e_lo": e_lo}
+
+
+# ---------------------------------------------------------------------------------------------
+# adaptive Krylov exponential
+# ---------------------------------------------------------------------------------------------
+
+KRYLOV_MAX = 32  # basis rows the device step keeps in registers (tnh_expm_step_c)
+
+
+def _tridiag_matrix(alphas, betas):
+    j = len(alphas)
+    T = np.diag(np.asarray(alphas, dtype=np.float64))
+    if j > 1:
+        off = np.asarray(betas[:j - 1], dtype=np.float64)
+        T += np.diag(off, 1) + np.diag(off, -1)
+    return T
+
+
+class ExpmMonitor:
+    """The host side of the adaptive Krylov exponential (tnkit/eigensolver.py:230-287): the
+    breakdown test beta < tol, Saad's a-posteriori estimate |beta| |tau| |exp(tau T)[j-1, 0]|
+    beta0 as the adaptive stop (from j = 2), the same estimate frozen at j = est_ref_j for `info`,
+    and the final coefficients beta0 exp(tau T) e_1. `adaptive=None` follows the environment
+    variable TNKIT_EXPM_ADAPTIVE (anything but '0' means on), as tnkit does."""
+
+    def __init__(self, tau, beta0, tol, adaptive, info, est_ref_j):
+        if adaptive is None:
+            import os
+            adaptive = os.environ.get("TNKIT_EXPM_ADAPTIVE", "1") != "0"
+        self.tau, self.beta0, self.tol, self.adaptive = tau, beta0, tol, adaptive
+        self.info, self.est_ref_j = info, est_ref_j
+        self.est = float("inf")
+        self.est_ref, self.est_ref_at, self.beta1 = float("nan"), 0, float("nan")
+
+    def check(self, k, alphas, betas, beta):
+        """After step k: 'breakdown' (stop, beta unused), 'converged' (stop after this vector)
+        or None."""
+        from scipy.linalg import expm
+        if k == 0:
+            self.beta1 = beta
+        if beta < self.tol:
+            if self.info is not None and self.est_ref_at == 0:
+                self.est_ref, self.est_ref_at = 0.0, k + 1
+            return "breakdown"
+        jj = len(alphas)
+        want_ref = self.info is not None and self.est_ref_at == 0 and jj >= self.est_ref_j
+        if (self.adaptive and jj >= 2) or want_ref:
+            Et = expm(self.tau * _tridiag_matrix(alphas, betas))
+            self.est = abs(beta) * abs(self.tau) * abs(Et[jj - 1, 0]) * self.beta0
+            if want_ref:
+                self.est_ref, self.est_ref_at = self.est, jj
+            if self.adaptive and jj >= 2 and self.est < self.tol:
+                return "converged"
+        return None
+
+    def finish(self, alphas, betas, capped):
+        from scipy.linalg import expm
+        c = self.beta0 * expm(self.tau * _tridiag_matrix(alphas, betas))[:, 0]
+        if self.info is not None:
+            self.info.update(krylov_dim=len(alphas), err_est=self.est, beta1=self.beta1,
+                             est_ref=self.est_ref, est_ref_j=self.est_ref_at, capped=capped)
+        return c
+
+
+def lanczos_expm_flat(matvec, x: torch.Tensor, tau, m=30, tol=1e-12, adaptive=None, info=None,
+                      est_ref_j=2) -> torch.Tensor:
+    """exp(tau H) x for a flat complex128 device vector x (any float/complex dtype is promoted),
+    `matvec` mapping flat complex128 device vectors to the same. Each step is one
+    `tnh_expm_step_c` (three-term subtraction, two Gram-Schmidt passes, alpha, beta and the next
+    basis vector on the device) and one host read of (alpha, beta) for the monitor."""
+    if m > KRYLOV_MAX:
+        raise ValueError(f"lanczos_expm: m = {m} above the device Krylov window {KRYLOV_MAX}")
+    x = x.reshape(-1).to(torch.complex128)
+    n = x.numel()
+    beta0 = float(torch.linalg.vector_norm(x).item())
+    if not np.isfinite(beta0) or beta0 < 1e-300:
+        return x
+    mon = ExpmMonitor(tau, beta0, tol, adaptive, info, est_ref_j)
+    dev = x.device
+    V = torch.zeros((1, m, n), dtype=torch.complex128, device=dev)
+    ab = torch.zeros((1, 2 * m), dtype=torch.float64, device=dev)
+    ws = kernels.expm_ws(1, dev)
+    V[0, 0] = x / beta0
+    alphas, betas = [], []
+    capped = False
+    for k in range(m):
+        w = matvec(V[0, k]).reshape(1, n).to(torch.complex128).contiguous()
+        kernels.expm_step_c(V, w, ab, k, ws)
+        alpha, beta = ab[0, [k, m + k]].tolist()
+        alphas.append(alpha)
+        stop = mon.check(k, alphas, betas, beta)
+        if stop == "breakdown":
+            break
+        betas.append(beta)
+        if stop:
+            break
+    else:
+        capped = bool(mon.adaptive)
+    c = mon.finish(alphas, betas, capped)
+    out = torch.empty((1, n), dtype=torch.complex128, device=dev)
+    kernels.lincomb_c(V, torch.as_tensor(c[None, :]).to(dev), out, len(alphas))
+    return out.reshape(-1)
+
+
+def lanczos_expm(matvec, v, tau, ops=None, m=30, tol=1e-12, adaptive=None, info=None,
+                 est_ref_j=2):
+    """Krylov approximation of exp(tau H) v with full reorthogonalisation and tnkit's adaptive
+    stop and diagnostics (tnkit/eigensolver.py:290-349; `info` keys krylov_dim, err_est, beta1,
+    est_ref, est_ref_j, capped). `v` is a device tensor of any shape (matvec sees that shape) or
+    a BlockTensor, which goes to `block_lanczos.lanczos_expm` as in tnkit. m <= 32."""
+    from .block_ops import BlockTensor
+    if isinstance(v, BlockTensor):
+        from . import block_lanczos
+        return block_lanczos.lanczos_expm(matvec, v, tau, ops, m=m, tol=tol, adaptive=adaptive,
+                                          info=info, est_ref_j=est_ref_j)
+    shape = tuple(v.shape)
+    out = lanczos_expm_flat(lambda x: matvec(x.reshape(shape)).reshape(-1), v, tau, m=m, tol=tol,
+                            adaptive=adaptive, info=info, est_ref_j=est_ref_j)
+    return out.reshape(shape)
