@@ -443,6 +443,41 @@ def truncate_svd(A, chi):
     return U[:, :keep], S[:keep], Vh[:keep], float(np.sum(S[keep:] ** 2))
 
 
+def randomized_svd(A, chi, rng, oversample=8, power_iters=2, mixed=False):
+    """Rank-chi factorisation by the Halko-Martinsson-Tropp range finder
+    (tnkit/linalg.py:81-135): a Gaussian sketch of ell = chi + oversample columns drawn from
+    `rng` on the host (real for real A), `power_iters` rounds of QR-orthonormalised subspace
+    iteration, then the SVD of the ell x n projection. `mixed` runs the range finder in single
+    precision and the projection and its SVD in double. Below min(m, n) <= ell it is the full
+    SVD. Returns (U, S, Vh, absolute discarded weight)."""
+    m, n = A.shape
+    if min(m, n) <= chi + oversample:
+        U, S, Vh = np.linalg.svd(A, full_matrices=False)
+        keep = min(chi, S.shape[0])
+        return U[:, :keep], S[:keep], Vh[:keep], float(np.sum(S[keep:] ** 2))
+    ell = int(min(chi + oversample, n))
+    Om = rng.standard_normal((n, ell))
+    if np.iscomplexobj(A):
+        Om = Om + 1j * rng.standard_normal((n, ell))
+    if mixed:
+        low = np.complex64 if np.iscomplexobj(A) else np.float32
+        As = A.astype(low)
+        Q, _ = np.linalg.qr(As @ Om.astype(low))
+        for _ in range(power_iters):
+            Q, _ = np.linalg.qr(As.conj().T @ Q)
+            Q, _ = np.linalg.qr(As @ Q)
+        Q = Q.astype(A.dtype)
+    else:
+        Q, _ = np.linalg.qr(A @ Om)
+        for _ in range(power_iters):
+            Q, _ = np.linalg.qr(A.conj().T @ Q)
+            Q, _ = np.linalg.qr(A @ Q)
+    Ub, S, Vh = np.linalg.svd(Q.conj().T @ A, full_matrices=False)
+    U = Q @ Ub
+    keep = min(chi, S.shape[0])
+    return U[:, :keep], S[:keep], Vh[:keep], float(np.sum(S[keep:] ** 2))
+
+
 def _gram_left(A, chi):
     """Top-chi left factorisation A ~= Q C from the smaller Gram matrix.  tnkit/linalg.py:138-159"""
     m, n = A.shape
@@ -516,6 +551,7 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
         R[i] = update_right_env(R[i + 1], M[i], mpo[i])
     energies, sweep_s = [], []
     total_disc = 0.0
+    rng_trunc = np.random.default_rng(seed + 1)     # tnkit/_dmrg.py:613
     counts = {"matvec": 0}
     mixed_counts = {"lo": 0, "hi": 0}
     steps = [0]
@@ -561,6 +597,9 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
             U, Sv, Vh, disc = truncate_svd(mat, chi)
         elif trunc == "gram":
             U, Sv, Vh, disc = truncate_gram(mat, chi, "left" if direction == "right" else "right")
+        elif trunc in ("randomized", "randomized-mixed"):
+            U, Sv, Vh, disc = randomized_svd(mat, chi, rng_trunc,
+                                             mixed=trunc == "randomized-mixed")
         else:
             raise ValueError(f"oracle trunc {trunc!r} not restated")
         Sv = Sv / np.linalg.norm(Sv)

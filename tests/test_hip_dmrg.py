@@ -23,6 +23,8 @@ with open(os.path.join(GOLDEN, "golden.json")) as f:
     GOLD = json.load(f)
 with open(os.path.join(GOLDEN, "golden_lanczos.json")) as f:
     GOLD_LZ = json.load(f)
+with open(os.path.join(GOLDEN, "golden_rsvd.json")) as f:
+    GOLD_RS = json.load(f)
 
 
 def _model(m):
@@ -165,3 +167,40 @@ def test_lanczos_ground_long_basis(hip):
     e_ref, _, info_ref = D.lanczos_ground(lambda v: lam * v, v0.copy(), tol=1e-14, max_iter=200)
     assert info["iters"] == info_ref["iters"] == 200
     assert abs(e - e_ref) < 1e-10 and e >= lam[0] - 1e-12
+
+
+@pytest.mark.parametrize("r", GOLD_RS["randomized_svd"], ids=lambda r: f"{r['m']}x{r['n']}")
+def test_randomized_svd_device_vs_tnkit(hip, r):
+    """The device range finder with tnkit's generator: tnkit's singular values and discarded
+    weight (1e-12; 1e-6 relative for the float32 range finder), an orthonormal U, and the
+    reconstruction error of tnkit's factorisation."""
+    from tnkit_amd.linalg import randomized_svd_device
+    rng = np.random.default_rng(r["seed"])
+    m, n = r["m"], r["n"]
+    U0, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+    V0, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+    A = (U0 * np.exp(-0.3 * np.arange(min(m, n)))) @ V0.T
+    U, S, Vh, disc = randomized_svd_device(torch.as_tensor(A).to(hip), r["chi"],
+                                           rng=np.random.default_rng(r["seed"] + 10),
+                                           mixed=r["mixed"])
+    Sref = np.array([float.fromhex(x) for x in r["S"]])
+    tol = 1e-6 if r["mixed"] else 1e-12
+    assert np.abs(S.cpu().numpy() - Sref).max() < tol * Sref[0]
+    assert abs(disc - float.fromhex(r["disc"])) < tol * max(Sref[0] ** 2, 1.0)
+    Un = U.cpu().numpy()
+    assert np.abs(Un.T @ Un - np.eye(Un.shape[1])).max() < (1e-6 if r["mixed"] else 1e-12)
+    rec = np.abs((Un * S.cpu().numpy()) @ Vh.cpu().numpy() - A).max()
+    assert abs(rec - float.fromhex(r["recon_err"])) < (1e-4 if r["mixed"] else 1e-6) * rec + 1e-12
+
+
+@pytest.mark.parametrize("run", GOLD_RS["dmrg"], ids=[r["name"] for r in GOLD_RS["dmrg"]])
+def test_dmrg_randomized_matches_tnkit(hip, run):
+    """trunc='randomized' / 'randomized-mixed' on the HIP backend against tnkit's trajectory
+    with the same seed (the sketches are drawn from the same host generator)."""
+    print("dmrg", run["name"], flush=True)
+    Ehip, M, info = H.dmrg(_model(run["model"]), run["chi"], backend="hip-real", **run["kwargs"])
+    ref = [float.fromhex(h) for h in run["energies"]]
+    for a, b in zip(info["energies"], ref):
+        assert abs(a - b) <= 1e-9 * max(1.0, abs(b))
+    assert abs(info["matvec_calls"] - run["matvec_calls"]) <= 0.02 * run["matvec_calls"]
+    assert [int(t.shape[2]) for t in M] == run["bonds"]

@@ -23,6 +23,8 @@ with open(os.path.join(GOLDEN, "golden_lanczos.json")) as f:
     GOLD_LZ = json.load(f)
 with open(os.path.join(GOLDEN, "golden_expm.json")) as f:
     GOLD_EX = json.load(f)
+with open(os.path.join(GOLDEN, "golden_rsvd.json")) as f:
+    GOLD_RS = json.load(f)
 VEC = np.load(os.path.join(GOLDEN, "vectors.npz"), allow_pickle=False)
 
 
@@ -492,3 +494,47 @@ def test_lanczos_expm_matches_tnkit(c):
     if not c["capped"] and c["adaptive"]:
         exact = scipy.linalg.expm(-1j * c["dt"] * A) @ v0
         assert np.abs(out - exact).max() < 1e-9
+
+
+# --------------------------------------------------------------------------------------------
+# randomized range-finder truncation
+# --------------------------------------------------------------------------------------------
+
+def rsvd_matrix(r):
+    rng = np.random.default_rng(r["seed"])
+    m, n = r["m"], r["n"]
+    U, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+    V, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+    return (U * np.exp(-0.3 * np.arange(min(m, n)))) @ V.T
+
+
+@pytest.mark.parametrize("r", GOLD_RS["randomized_svd"], ids=lambda r: f"{r['m']}x{r['n']}")
+def test_randomized_svd_matches_tnkit(r):
+    """tnkit's randomized_svd_device with the same generator: singular values, discarded weight
+    and reconstruction error to 1e-13; and, against the exact SVD, the kept values are the
+    leading ones to the range finder's accuracy."""
+    A = rsvd_matrix(r)
+    U, S, Vh, disc = D.randomized_svd(A, r["chi"], np.random.default_rng(r["seed"] + 10),
+                                      mixed=r["mixed"])
+    Sref = np.array([float.fromhex(x) for x in r["S"]])
+    assert np.abs(S - Sref).max() < 1e-13
+    assert abs(disc - float.fromhex(r["disc"])) < 1e-13
+    assert abs(np.abs((U * S) @ Vh - A).max() - float.fromhex(r["recon_err"])) < 1e-13
+    Sx = np.linalg.svd(A, compute_uv=False)
+    assert np.abs(S - Sx[:r["chi"]]).max() < 1e-4 * Sx[0]
+    # the float32 range finder leaves Q orthonormal to float32 rounding only
+    assert np.abs(U.T @ U - np.eye(U.shape[1])).max() < (1e-6 if r["mixed"] else 1e-12)
+
+
+@pytest.mark.parametrize("run", GOLD_RS["dmrg"], ids=[r["name"] for r in GOLD_RS["dmrg"]])
+def test_dmrg_randomized_matches_tnkit(run):
+    """trunc='randomized' (and the float32 range finder of 'randomized-mixed') DMRG: per-sweep
+    energies, matvec counts, bonds and discarded weight of tnkit's run with the same seed."""
+    E, M, info = D.dmrg(_model(run["model"]), run["chi"], **run["kwargs"])
+    ref = [float.fromhex(h) for h in run["energies"]]
+    mixed = run["kwargs"]["trunc"] == "randomized-mixed"
+    assert max(abs(a - b) for a, b in zip(info["energies"], ref)) < (1e-10 if mixed else 1e-12)
+    assert abs(info["matvec_calls"] - run["matvec_calls"]) <= (2 if mixed else 0)
+    assert [int(t.shape[2]) for t in M] == run["bonds"]
+    dw = float.fromhex(run["discarded_weight"])
+    assert abs(info["discarded_weight"] - dw) < 1e-4 * dw

@@ -6,7 +6,7 @@ Tensor axes follow tnkit (tnkit/_dmrg.py:7-11):
 
 The front door `dmrg(mpo, chi, options=None, **kwargs)` and `DMRGOptions` keep tnkit's names,
 defaults and return values (E, MPS, info). What is served: dense real MPOs, two-site steps,
-eig in {'lanczos', 'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram'}. Other options of tnkit
+eig in {'lanczos', 'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram', 'randomized', 'randomized-mixed'}. Other options of tnkit
 raise ValueError naming what is not provided.
 
 'lanczos-mixed' here runs stage 1 on a float32 operator on the float32 matrix cores (real, not
@@ -26,7 +26,7 @@ from . import kernels
 from .backend import HipBackend, get_backend
 from .eigensolver import (check_eig_options, lanczos_ground, lanczos_ground_mixed,
                           lanczos_ground_restart)
-from .linalg import gram_truncation
+from .linalg import gram_truncation, randomized_svd_device
 from .mps import MPS
 
 
@@ -190,8 +190,9 @@ def _check(o: DMRGOptions, mpo) -> None:
     if o.eig not in ("lanczos", "lanczos-restart", "lanczos-mixed"):
         raise ValueError(f"unknown eig solver {o.eig!r}; expected one of "
                          "['lanczos', 'lanczos-mixed', 'lanczos-restart']")
-    if o.trunc not in ("svd", "gram"):
-        raise ValueError(f"unknown trunc {o.trunc!r} for the HIP backend; expected 'svd' or 'gram'")
+    if o.trunc not in ("svd", "gram", "randomized", "randomized-mixed"):
+        raise ValueError(f"unknown trunc {o.trunc!r} for the HIP backend; expected 'svd', "
+                         "'gram', 'randomized' or 'randomized-mixed'")
     if o.charge is not None:
         raise ValueError("charge= needs a block MPO")
     if o.expand not in (None, 0):
@@ -318,8 +319,11 @@ class DMRGEngine:
             keep = min(chi, int(S.shape[0]))
             disc = float(torch.sum(S[keep:] ** 2).item())
             U, S, Vh = U[:, :keep], S[:keep], Vh[:keep]
-        else:
+        elif o.trunc == "gram":
             U, S, Vh, disc = gram_truncation(mat, chi, "left" if direction == "right" else "right")
+        else:
+            U, S, Vh, disc = randomized_svd_device(mat, chi, be, rng=self.rng,
+                                                   mixed=o.trunc == "randomized-mixed")
         S = S / torch.linalg.norm(S)
         if direction == "right":
             M[i] = U.reshape(a, s1, -1).contiguous()
