@@ -713,6 +713,76 @@ def tdvp2_evolve(mps, mpo, chi, dt, n_steps, krylov_m=20, measure=None, svd_tol=
     return M, rec
 
 
+def apply_zero_site(C, L, R):
+    """Bond H_eff C (tnkit/_dmrg.py:202-213): C (r, b) -> sum L[p,w,q] C[q,b'] R[b,w,b']."""
+    return np.einsum("pwq,qc,bwc->pb", L, C, R, optimize=True)
+
+
+def tdvp1_evolve(mps, mpo, dt, n_steps, fixed_m=None, krylov_m=30, measure=None,
+                 meas_every=1):
+    """One-site TDVP (tnkit/tdvp.py:804-852 and the sweep of :1188-1360): right-canonicalise,
+    then per step a rightward sweep (evolve site i by exp(-i H1 dt/2), QR it, evolve the bond
+    by exp(+i H0 dt/2), absorb it into site i+1) and the mirrored leftward sweep (LQ via the
+    QR of the transpose). The Krylov solver is the adaptive `lanczos_expm` capped at
+    `krylov_m`, or `lanczos_expm_fixed` with `fixed_m` steps. Bond dimensions never change.
+    Returns (M, records) with records (t, measure(M)) at t = 0 and every `meas_every` steps."""
+    N = len(mps)
+    M = right_canonicalize([np.asarray(m, dtype=np.complex128) for m in mps])
+    W = [np.asarray(w, dtype=np.complex128) for w in mpo]
+    L, R = [None] * (N + 1), [None] * (N + 1)
+    L[0] = np.ones((1, 1, 1), dtype=np.complex128)
+    R[N] = np.ones((1, 1, 1), dtype=np.complex128)
+    for i in range(N - 1, 0, -1):
+        R[i] = update_right_env(R[i + 1], M[i], W[i])
+
+    def ex(mv, x, tau):
+        if fixed_m:
+            return lanczos_expm_fixed(mv, x.reshape(-1), tau, fixed_m).reshape(x.shape)
+        return lanczos_expm(mv, x.reshape(-1), tau, m=krylov_m).reshape(x.shape)
+
+    records = [(0.0, measure(M))] if measure is not None else []
+    for step in range(n_steps):
+        for i in range(N):
+            a, s, b = M[i].shape
+            mv1 = lambda x: apply_one_site(x.reshape(a, s, b), L[i], W[i], R[i + 1]).reshape(-1)  # noqa: E731,E501
+            M[i] = ex(mv1, M[i], -0.5j * dt)
+            if i == N - 1:
+                continue
+            Q, C = np.linalg.qr(M[i].reshape(a * s, b))
+            r = C.shape[0]
+            M[i] = Q.reshape(a, s, r)
+            L[i + 1] = update_left_env(L[i], M[i], W[i])
+            mv0 = lambda x: apply_zero_site(x.reshape(r, b), L[i + 1], R[i + 1]).reshape(-1)  # noqa: E731,E501
+            C = ex(mv0, C, 0.5j * dt)
+            M[i + 1] = np.einsum("rb,bsc->rsc", C, M[i + 1], optimize=True)
+        for i in range(N - 1, -1, -1):
+            a, s, b = M[i].shape
+            mv1 = lambda x: apply_one_site(x.reshape(a, s, b), L[i], W[i], R[i + 1]).reshape(-1)  # noqa: E731,E501
+            M[i] = ex(mv1, M[i], -0.5j * dt)
+            if i == 0:
+                continue
+            Qt, Ct = np.linalg.qr(np.einsum("asb->sba", M[i]).reshape(s * b, a))
+            r = Ct.shape[0]
+            M[i] = np.einsum("sbr->rsb", Qt.reshape(s, b, r))
+            R[i] = update_right_env(R[i + 1], M[i], W[i])
+            C = Ct.T
+            mv0 = lambda x: apply_zero_site(x.reshape(a, r), L[i], R[i]).reshape(-1)  # noqa: E731,E501
+            C = ex(mv0, C, 0.5j * dt)
+            M[i - 1] = np.einsum("csa,ar->csr", M[i - 1], C, optimize=True)
+        if measure is not None and (step + 1) % meas_every == 0:
+            records.append(((step + 1) * dt, measure(M)))
+    return M, records
+
+
+def mps_dense(M):
+    """The state vector of an MPS (site 0 slowest)."""
+    v = np.ones((1, 1), dtype=np.result_type(*[np.asarray(t).dtype for t in M]))
+    for t in M:
+        t = np.asarray(t)
+        v = np.einsum("xa,asb->xsb", v, t).reshape(-1, t.shape[2])
+    return v.reshape(-1)
+
+
 def local_expectations(M, op):
     """<op_i> of a dense MPS: right-canonicalise a copy, sweep the norm environment from the
     left (tnkit/observables.py:112-147)."""

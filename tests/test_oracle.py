@@ -538,3 +538,33 @@ def test_dmrg_randomized_matches_tnkit(run):
     assert [int(t.shape[2]) for t in M] == run["bonds"]
     dw = float.fromhex(run["discarded_weight"])
     assert abs(info["discarded_weight"] - dw) < 1e-4 * dw
+
+
+# --------------------------------------------------------------------------------------------
+# one-site TDVP
+# --------------------------------------------------------------------------------------------
+
+with open(os.path.join(GOLDEN, "golden_tdvp1.json")) as f:
+    GOLD_T1 = json.load(f)
+
+
+def _tdvp1_problem(n=10, chi=6, seed=2026):
+    rng = np.random.default_rng(seed)
+    dims = [1] + [min(chi, 2 ** min(i + 1, n - i - 1)) for i in range(n - 1)] + [1]
+    return [(rng.normal(size=(dims[i], 2, dims[i + 1]))
+             + 1j * rng.normal(size=(dims[i], 2, dims[i + 1]))) / 10.0 for i in range(n)]
+
+
+@pytest.mark.parametrize("run", GOLD_T1["runs"], ids=[r["name"] for r in GOLD_T1["runs"]])
+def test_tdvp1_matches_tnkit(run):
+    """tnkit's tdvp1_evolve (adaptive and fixed Krylov dimension 8): <sigma^z_i> per step and
+    the final state vector to 1e-13."""
+    SZ = np.diag([1.0, -1.0])
+    M, rec = D.tdvp1_evolve(_tdvp1_problem(), tfim_mpo(10, 1.0, 0.1), run["dt"],
+                            run["n_steps"], fixed_m=run["kwargs"].get("fast_expm"),
+                            measure=lambda M: D.local_expectations(M, SZ).real)
+    ref_z = np.array([[float.fromhex(x) for x in z] for z in run["z"]])
+    assert np.abs(np.array([z for _, z in rec]) - ref_z).max() < 1e-13
+    psi = np.array([float.fromhex(x) for x in run["psi_re"]]) + \
+        1j * np.array([float.fromhex(x) for x in run["psi_im"]])
+    assert np.abs(D.mps_dense(M) - psi).max() < 1e-13
