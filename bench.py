@@ -659,7 +659,7 @@ def main():
         fn()
         stage_ms[st] = event_time(fn, max(5, args.steps // 2), stream)
     dom = max(stage_ms, key=stage_ms.get)
-    names = {1: "GEMM1 theta x L (gemm_tn_kernel<float,double,float,float>)",
+    names = {1: "GEMM1 theta x L (narrow_kernel + gemm_tn_kernel<float,float,float,float>)",
              2: "MPO mix (mix_mfma_kernel<float,float,float>)",
              4: "GEMM3 x R (gemm_tn_kernel<float,float,float,double>)"}
     if dom == 2:

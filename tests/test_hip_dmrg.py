@@ -113,7 +113,7 @@ def test_backend_contract(hip):
     assert float(be.norm(v)) == pytest.approx(np.sqrt(30.0))
     assert float(be.vdot(v, v)) == pytest.approx(30.0)
     with pytest.raises(ValueError):
-        H.dmrg(tfim_mpo(4, 1.0), 4, backend="hip-real", trunc="randomized")
+        H.dmrg(tfim_mpo(4, 1.0), 4, backend="hip-real", trunc="cutn")
 
 
 @pytest.mark.parametrize("case", GOLD_LZ["lanczos_ground"]["runs"],

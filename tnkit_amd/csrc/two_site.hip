@@ -17,6 +17,8 @@
 // taken inside the first GEMM of each environment update. Every entry has a batched form: k
 // independent problems of one shape with element strides between them (the legs of the batched
 // time evolution), one launch per kernel for all k.
+#include <type_traits>
+
 #include "gemm_tn.h"
 
 namespace tnh {
@@ -515,6 +517,23 @@ int fused_two_site(int batch, int a, int wl, int wm, int wr, int d1, int d2, int
 
 // stages: bit 0 GEMM1, bit 1 S-mix, bit 2 GEMM3 (7 = the whole matvec). A partial run reads and
 // writes the workspace as the whole one does; it exists to time one stage at a time.
+// theta (float64) -> float32 ahead of the mixed matvec's GEMM1: the GEMM then stages 4-byte
+// operands (its tile loads and LDS stores are those of the all-float32 kernel), for one pass
+// over theta (12 bytes per element).
+__global__ void __launch_bounds__(256) narrow_kernel(long n, const double* __restrict__ x,
+                                                     float* __restrict__ y) {
+  const long stride = (long)gridDim.x * 256 * 2;
+  for (long i = ((long)blockIdx.x * 256 + threadIdx.x) * 2; i < n; i += stride) {
+    if (i + 1 < n) {
+      const double2 v = *reinterpret_cast<const double2*>(x + i);
+      *reinterpret_cast<float2*>(y + i) = make_float2(static_cast<float>(v.x),
+                                                      static_cast<float>(v.y));
+    } else {
+      y[i] = static_cast<float>(x[i]);
+    }
+  }
+}
+
 template <typename TC, typename TIO>
 int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* LT,
                    long sLT, const void* S, long sS, const void* RT, long sRT, const void* theta,
@@ -525,12 +544,36 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   Arena ar(ws, ws_bytes);
   TC* X1 = ar.take<TC>(nx1 * batch);
   TC* X2 = ar.take<TC>(nx2 * batch);
+  // the mixed form narrows theta first when its batches are packed (the workspace query
+  // reserves room for it)
+  const long nth = Q * dd * B;
+  constexpr bool kNarrow = std::is_same<TC, float>::value && std::is_same<TIO, double>::value;
+  float* Th = nullptr;
+  if constexpr (kNarrow) {
+    if (batch == 1 || sth == nth) Th = ar.take<float>(nth * batch);
+  }
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
   if (stages & 1) {  // GEMM1: M = d1 d2 b, N = p wl, K = q
-    GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
-               dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
-    TNH_TRY((gemm_any<TC, TIO, TC, TC>(g, batch, 0, st, tile1)));
+    if constexpr (kNarrow) {
+      if (Th != nullptr) {
+        const long n = nth * batch;
+        long g = (n / 2 + 255) / 256;
+        if (g > 4096) g = 4096;
+        if (g < 1) g = 1;
+        hipLaunchKernelGGL(narrow_kernel, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, n,
+                           static_cast<const double*>(theta), Th);
+        TNH_CHECK_LAUNCH("narrow");
+        GemmArgs g1{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), Th,
+                    dd * B, nth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+        TNH_TRY((gemm_any<TC, TC, TC, TC>(g1, batch, 0, st, tile1)));
+      }
+    }
+    if (Th == nullptr) {
+      GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
+                 dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+      TNH_TRY((gemm_any<TC, TIO, TC, TC>(g, batch, 0, st, tile1)));
+    }
   }
   if (stages & 2) {  // S-mix
     MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
@@ -704,7 +747,10 @@ size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, in
   if (dtype == TNH_BF16)  // X1 in float, X2 in bf16
     return align_up(dd * b * a * wl * 4, 256) + align_up((size_t)wr * b * a * dd * 2, 256) + 256;
   const size_t e = dsize(dtype);
-  return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + 256;
+  // float32: room for theta narrowed to float32 (used when the vectors are float64)
+  const size_t th = dtype == TNH_F32 ? align_up((size_t)a * dd * b * 4, 256) : 0;
+  return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + th +
+         256;
 }
 
 int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1, int d2, int b,
