@@ -149,6 +149,14 @@ int tnh_krylov_orth(long n, int nlive, const double* V, long ldv, double* w, int
                     int sub_base, const double* sub_coef, double* scal, void* ws, size_t ws_bytes,
                     tnh_stream_t stream);
 
+/* y += weight * sum_{r<k} <V[r], x> V[r], float64, k <= 64, coefficients reduced on the device
+ * (no host read). x may alias y. With weight = penalty and V the block vectors of earlier states
+ * it is the orthogonality penalty of tnkit/dmrg_excited.py:709-720; with weight = -1 and V
+ * orthonormal it is the projector off span(V) of tnkit/dmrg_excited.py:384-397. The workspace
+ * is tnh_krylov_ws_bytes. */
+int tnh_rank_update(long n, int k, const double* V, long ldv, const double* x, double weight,
+                    double* y, void* ws, size_t ws_bytes, tnh_stream_t stream);
+
 /* out = beta * out + sum_{r<nrows} coef[r] V[r]  (coef HOST, nrows < 128): the Ritz vector
  * reconstruction of tnkit/eigensolver.py:670-673 and the Krylov combination of
  * tnkit/eigensolver.py:212. */

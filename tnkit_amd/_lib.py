@@ -51,6 +51,7 @@ SIGNATURES = {
     "tnh_krylov_ws_bytes": (c_size_t, [c_long, c_int]),
     "tnh_krylov_orth": (c_int, [c_long, c_int, _P, c_long, _P, c_int, c_int, c_int, _PD, _P, _P,
                                 c_size_t, _P]),
+    "tnh_rank_update": (c_int, [c_long, c_int, _P, c_long, _P, c_double, _P, _P, c_size_t, _P]),
     "tnh_lincomb": (c_int, [c_long, c_int, _P, c_long, _PD, c_double, _P, _P]),
     "tnh_norm": (c_int, [c_long, _P, _P, _P, c_size_t, _P]),
     "tnh_dot": (c_int, [c_long, _P, _P, _P, _P, c_size_t, _P]),

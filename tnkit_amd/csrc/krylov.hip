@@ -79,18 +79,36 @@ __global__ void __launch_bounds__(256)
     if (mode == 0) {
 #pragma unroll
       for (int k = 0; k < NV; ++k) acc[k] += v[k] * wi;
-    } else {
+    } else if (mode == 1) {
       acc[0] += wi * wi;
     }
   }
+  if (mode == 2) return;  // subtraction only (uniform over the block, before any barrier)
   block_partials<NV>(acc, mode == 0 ? nv : 1, part, mode == 0 ? nv : 1);
 }
 
+// partial dots <V[k], x> for k < nv (x is only read)
+template <int NV>
+__global__ void __launch_bounds__(256)
+    dots_kernel(long n, int nv, const double* __restrict__ V, long ldv, const double* __restrict__ x,
+                double* __restrict__ part) {
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    const double xi = x[i];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      if (k < nv) acc[k] += V[(long)k * ldv + i] * xi;
+  }
+  block_partials<NV>(acc, nv, part, nv);
+}
+
 // out[k] = sum_b part[b*ld + k]  for k < ncol, in block order. mode 0: plain; 1: sqrt (norms);
-// 2: alpha -- scal[0] = sum and coef[k] = (k == jsel) ? sum : 0 for k < ncoef.
+// 2: alpha -- scal[0] = sum and coef[k] = (k == jsel) ? sum : 0 for k < ncoef; 3: scale * sum.
 __global__ void __launch_bounds__(256)
     reduce_kernel(int nb, int ncol, const double* __restrict__ part, int ld, double* __restrict__ out,
-                  int mode, int jsel, int ncoef, double* __restrict__ coef) {
+                  int mode, int jsel, int ncoef, double* __restrict__ coef, double scale) {
   __shared__ double red[4];
   const int k = blockIdx.x;
   double s = 0.0;
@@ -101,6 +119,7 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) {
     double tot = red[0] + red[1] + red[2] + red[3];
     if (mode == 1) tot = sqrt(tot);
+    if (mode == 3) tot *= scale;
     out[k] = tot;
     if (mode == 2)
       for (int r = 0; r < ncoef; ++r) coef[r] = (r == jsel) ? tot : 0.0;
@@ -198,10 +217,29 @@ int pass_any(long n, int nv, const double* V, long ldv, double* w, const double*
   return TNH_ERR_ARG;
 }
 
+template <int NV>
+int dots(long n, int nv, const double* V, long ldv, const double* x, double* part, int nb,
+         hipStream_t st) {
+  hipLaunchKernelGGL(dots_kernel<NV>, dim3(nb), dim3(256), 0, st, n, nv, V, ldv, x, part);
+  TNH_CHECK_LAUNCH("dots");
+  return TNH_OK;
+}
+
+int dots_any(long n, int nv, const double* V, long ldv, const double* x, double* part, int nb,
+             hipStream_t st) {
+  if (nv <= 4) return dots<4>(n, nv, V, ldv, x, part, nb, st);
+  if (nv <= 8) return dots<8>(n, nv, V, ldv, x, part, nb, st);
+  if (nv <= 16) return dots<16>(n, nv, V, ldv, x, part, nb, st);
+  if (nv <= 32) return dots<32>(n, nv, V, ldv, x, part, nb, st);
+  if (nv <= 64) return dots<64>(n, nv, V, ldv, x, part, nb, st);
+  set_error("rank_update: at most 64 vectors are supported, got %d", nv);
+  return TNH_ERR_ARG;
+}
+
 int reduce(int nb, int ncol, const double* part, int ld, double* out, int mode, int jsel, int ncoef,
-           double* coef, hipStream_t st) {
+           double* coef, hipStream_t st, double scale = 1.0) {
   hipLaunchKernelGGL(reduce_kernel, dim3(ncol), dim3(256), 0, st, nb, ncol, part, ld, out, mode, jsel,
-                     ncoef, coef);
+                     ncoef, coef, scale);
   TNH_CHECK_LAUNCH("reduce");
   return TNH_OK;
 }
@@ -260,6 +298,24 @@ int tnh_krylov_orth(long n, int nlive, const double* V, long ldv, double* w, int
   TNH_TRY(pass_any(n, nlive, V, ldv, w, c2, 1, part, nb, st));
   TNH_TRY(reduce(nb, 1, part, 1, scal + 1, 1, 0, 0, nullptr, st));
   return TNH_OK;
+}
+
+int tnh_rank_update(long n, int k, const double* V, long ldv, const double* x, double weight,
+                    double* y, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(n > 0 && ldv >= n, TNH_ERR_ARG, "tnh_rank_update: bad n=%ld ldv=%ld", n, ldv);
+  TNH_REQUIRE(k >= 0 && k <= 64, TNH_ERR_ARG, "tnh_rank_update: need 0 <= k <= 64, got %d", k);
+  if (k == 0) return TNH_OK;
+  Arena ar(ws, ws_bytes);
+  double* part = ar.take<double>(1024 * 64);
+  double* c = ar.take<double>(64);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_rank_update: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nb = nblocks(n);
+  // c = -weight * V x  (every coefficient is complete before y is touched, so x may alias y)
+  TNH_TRY(dots_any(n, k, V, ldv, x, part, nb, st));
+  TNH_TRY(reduce(nb, k, part, k, c, 3, 0, 0, nullptr, st, -weight));
+  // y -= c V
+  return pass_any(n, k, V, ldv, y, c, 2, part, nb, st);
 }
 
 int tnh_lincomb(long n, int nrows, const double* V, long ldv, const double* coef, double beta,

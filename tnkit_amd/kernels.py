@@ -235,6 +235,25 @@ def krylov_orth(V: torch.Tensor, nlive: int, w: torch.Tensor, j: int, sub_coef: 
     del keep
 
 
+def rank_update(V: torch.Tensor, x: torch.Tensor, weight: float, y: torch.Tensor,
+                kw: Optional[KrylovWorkspace] = None) -> torch.Tensor:
+    """In place: y += weight * sum_r <V[r], x> V[r] for the k <= 64 rows of V (float64; x may be
+    y itself). The orthogonality penalty and the projector of excited-state DMRG."""
+    _need(V, "V", 2, (torch.float64,))
+    _need(x, "x", 1, (torch.float64,))
+    _need(y, "y", 1, (torch.float64,))
+    n = x.numel()
+    if V.shape[1] != n or y.numel() != n:
+        raise ValueError(f"rank_update: V {tuple(V.shape)}, x {n}, y {y.numel()} do not match")
+    if V.shape[0] > 64:
+        raise ValueError(f"rank_update: at most 64 vectors, got {V.shape[0]}")
+    kw = kw or KrylovWorkspace(n, x.device)
+    check(_lib.load().tnh_rank_update(n, V.shape[0], ptr(V), V.stride(0), ptr(x), float(weight),
+                                      ptr(y), ptr(kw.ws), kw.ws.numel(), stream(x.device)),
+          "rank_update")
+    return y
+
+
 def lincomb(V: torch.Tensor, coef: Sequence[float], out: torch.Tensor, beta: float = 0.0) -> torch.Tensor:
     """out = beta * out + sum_r coef[r] V[r] (coefficients from the host)."""
     _need(V, "V", 2, (torch.float64,))
