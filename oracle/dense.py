@@ -523,11 +523,15 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
          restart_m: int = 40, restart_k: int = 10, mixed_tol_lo: float = 1e-4,
          init_mps: Optional[List[np.ndarray]] = None, dtype=np.float64,
          max_steps: Optional[int] = None, canonical: str = "svd", reorth: str = "full",
-         eig_stop: str = "residual") -> Tuple[float, List[np.ndarray], Dict]:
+         eig_stop: str = "residual", expand: int = 0, expand_alpha: float = 3e-3,
+         expand_decay: float = 0.3) -> Tuple[float, List[np.ndarray], Dict]:
     """Dense two-site DMRG, the sweep of tnkit/_dmrg.py:560-799 restricted to the dense
     path: warm-started thick-restart Lanczos (or the mixed two-stage form with a float32
     matvec, or the unbounded-basis Lanczos of eig='lanczos'), SVD or Gram truncation, S renormalised, environments updated after each cut.
 
+    `expand` > 0 runs that many opening sweeps with local subspace expansion at
+    alpha = expand_alpha * expand_decay**sweep (tnkit/_dmrg.py:700-705, :780; the split is
+    `oracle.expansion.enriched_split`), and a sweep with expansion never ends the run.
     `max_steps` (oracle-only) stops after that many local steps, for timing samples.
     `canonical` 'lq' starts from the LQ right-canonical form (the same state in another gauge).
     Returns (E, M, info) with tnkit's info keys 'sweeps', 'energies', 'discarded_weight',
@@ -556,7 +560,7 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
     mixed_counts = {"lo": 0, "hi": 0}
     steps = [0]
 
-    def solve(i, direction):
+    def solve(i, direction, alpha=0.0):
         nonlocal total_disc
         W1, W2 = mpo[i], mpo[i + 1]
         theta0 = np.einsum("asc,ctb->astb", M[i], M[i + 1], optimize=True)
@@ -592,6 +596,19 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
             raise ValueError(f"oracle eig {eig!r} not restated")
         x = x.reshape(shape)
         a, s1, s2, b = shape
+        if alpha:
+            from .expansion import enriched_split, expansion_term
+            P = expansion_term(x, L[i], W1, W2, R[i + 2], direction == "right")
+            A, C, disc = enriched_split(x, P, alpha, chi, direction == "right")
+            if direction == "right":
+                M[i], M[i + 1] = A, C / np.linalg.norm(C)
+                L[i + 1] = update_left_env(L[i], M[i], W1)
+            else:
+                M[i + 1], M[i] = A, C / np.linalg.norm(C)
+                R[i + 1] = update_right_env(R[i + 2], M[i + 1], W2)
+            total_disc += disc
+            steps[0] += 1
+            return e
         mat = x.reshape(a * s1, s2 * b)
         if trunc == "svd":
             U, Sv, Vh, disc = truncate_svd(mat, chi)
@@ -618,15 +635,18 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
     prev = None
     E = None
     sweep = -1
+    alphas = []
     for sweep in range(n_sweeps):
         t0 = time.perf_counter()
+        alpha = expand_alpha * expand_decay ** sweep if sweep < expand else 0.0
+        alphas.append(alpha)
         for i in range(0, N - 1):
-            E = solve(i, "right")
+            E = solve(i, "right", alpha)
             if max_steps is not None and steps[0] >= max_steps:
                 break
         else:
             for i in range(N - 2, -1, -1):
-                E = solve(i, "left")
+                E = solve(i, "left", alpha)
                 if max_steps is not None and steps[0] >= max_steps:
                     break
         energies.append(E)
@@ -635,10 +655,10 @@ def dmrg(mpo: Sequence[np.ndarray], chi: int, n_sweeps: int = 8, tol: float = 1e
             break
         if prev is not None and abs(E - prev) < max(tol, rtol * abs(E)):
             break
-        prev = E
+        prev = None if alpha else E
     info = {"sweeps": sweep + 1, "energies": energies, "discarded_weight": total_disc,
             "matvec_calls": counts["matvec"], "sweep_s": sweep_s, "mixed_counts": mixed_counts,
-            "steps": steps[0]}
+            "steps": steps[0], "expand": expand, "expand_alphas": alphas}
     return E, M, info
 
 

@@ -6,8 +6,9 @@ Tensor axes follow tnkit (tnkit/_dmrg.py:7-11):
 
 The front door `dmrg(mpo, chi, options=None, **kwargs)` and `DMRGOptions` keep tnkit's names,
 defaults and return values (E, MPS, info). What is served: dense real MPOs, two-site steps,
-eig in {'lanczos', 'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram', 'randomized', 'randomized-mixed'}. Other options of tnkit
-raise ValueError naming what is not provided.
+eig in {'lanczos', 'lanczos-restart', 'lanczos-mixed'}, trunc in {'svd', 'gram', 'randomized',
+'randomized-mixed'}, and subspace expansion in the opening sweeps (expand, expand_alpha,
+expand_decay). Other options of tnkit raise ValueError naming what is not provided.
 
 'lanczos-mixed' here runs stage 1 on a float32 operator on the float32 matrix cores (real, not
 tnkit's complex64 cast; tnkit/backend.py:554-563) with float64 Krylov vectors, and stage 2 in
@@ -26,6 +27,7 @@ from . import kernels
 from .backend import HipBackend, get_backend
 from .eigensolver import (check_eig_options, lanczos_ground, lanczos_ground_mixed,
                           lanczos_ground_restart)
+from .expansion import enriched_split, expansion_term
 from .linalg import gram_truncation, randomized_svd_device
 from .mps import MPS
 
@@ -195,9 +197,29 @@ def _check(o: DMRGOptions, mpo) -> None:
                          "'gram', 'randomized' or 'randomized-mixed'")
     if o.charge is not None:
         raise ValueError("charge= needs a block MPO")
-    if o.expand not in (None, 0):
-        raise ValueError("subspace expansion is not served by the HIP backend (expand=0)")
     check_eig_options(o)
+
+
+BLOCK_EXPAND = 4    # opening expansion sweeps on block MPOs when expand=None (tnkit/_dmrg.py:45)
+
+
+def expand_sweeps(o, block: bool) -> int:
+    """The number of opening sweeps with subspace expansion, with tnkit's rules and errors
+    (tnkit/_dmrg.py:579-590): None means min(4, n_sweeps - 2) on block MPOs and 0 on dense ones;
+    the last sweep is always without expansion."""
+    expand = o.expand
+    if expand is not None and (isinstance(expand, bool)
+                               or not isinstance(expand, (int, np.integer))):
+        raise TypeError(f"expand must be an int or None, got {expand!r}")
+    if expand is None:
+        expand = min(BLOCK_EXPAND, max(0, o.n_sweeps - 2)) if block else 0
+    if not 0 <= expand < o.n_sweeps:
+        raise ValueError(f"expand must be in [0, n_sweeps), so that the last sweep is unbiased; "
+                         f"got expand={expand!r}, n_sweeps={o.n_sweeps!r}")
+    if expand and not (o.expand_alpha > 0 and 0 < o.expand_decay <= 1):
+        raise ValueError("expand needs expand_alpha > 0 and 0 < expand_decay <= 1 "
+                         "(pass expand=0 to turn expansion off)")
+    return int(expand)
 
 
 
@@ -300,7 +322,9 @@ class DMRGEngine:
         self.last_eig_info = dict(inf, matvecs=count["n"])
         return e, x.reshape(shape)
 
-    def solve_block(self, i: int, direction: str, cur_tol: float) -> float:
+    def solve_block(self, i: int, direction: str, cur_tol: float, alpha: float = 0.0) -> float:
+        """One two-site step; alpha > 0 splits the centre with subspace expansion
+        (tnkit/_dmrg.py:700-705) instead of the truncation."""
         o, be, M, L, R, chi = self.o, self.be, self.M, self.L, self.R, self.chi
         W1, W2 = self.mpo[i], self.mpo[i + 1]
         a, s1 = M[i].shape[0], M[i].shape[1]
@@ -313,6 +337,17 @@ class DMRGEngine:
         e, x = self.eigsolve(theta0, op, op_lo, cur_tol)
         del op, op_lo
         t0 = time.perf_counter()
+        if alpha:
+            right = direction == "right"
+            P = expansion_term(x, L[i], W1, W2, R[i + 2], right)
+            A, C, disc = enriched_split(x, P, alpha, chi, be, right)
+            C = (C / torch.linalg.norm(C)).contiguous()
+            M[i], M[i + 1] = (A, C) if right else (C, A)
+            be.sync()
+            self.phase["trunc"] += time.perf_counter() - t0
+            self.total_disc += disc
+            self._env(i, direction)
+            return e
         mat = x.reshape(a * s1, s2 * b)
         if o.trunc == "svd":
             U, S, Vh = be.svd(mat)
@@ -334,21 +369,24 @@ class DMRGEngine:
         be.sync()
         self.phase["trunc"] += time.perf_counter() - t0
         self.total_disc += disc
-        t0 = time.perf_counter()
-        if direction == "right":
-            L[i + 1] = update_left_env(L[i], M[i], W1)
-        else:
-            R[i + 1] = update_right_env(R[i + 2], M[i + 1], W2)
-        be.sync()
-        self.phase["env"] += time.perf_counter() - t0
+        self._env(i, direction)
         return e
 
-    def sweep(self, cur_tol: float) -> float:
+    def _env(self, i: int, direction: str) -> None:
+        t0 = time.perf_counter()
+        if direction == "right":
+            self.L[i + 1] = update_left_env(self.L[i], self.M[i], self.mpo[i])
+        else:
+            self.R[i + 1] = update_right_env(self.R[i + 2], self.M[i + 1], self.mpo[i + 1])
+        self.be.sync()
+        self.phase["env"] += time.perf_counter() - t0
+
+    def sweep(self, cur_tol: float, alpha: float = 0.0) -> float:
         E = None
         for i in range(0, self.N - 1):
-            E = self.solve_block(i, "right", cur_tol)
+            E = self.solve_block(i, "right", cur_tol, alpha)
         for i in range(self.N - 2, -1, -1):
-            E = self.solve_block(i, "left", cur_tol)
+            E = self.solve_block(i, "left", cur_tol, alpha)
         return E
 
 
@@ -356,10 +394,12 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
     if getattr(mpo[0], "is_block", False):          # U(1) block tensors: the block sweep
         from .block_dmrg import block_dmrg
         return block_dmrg(mpo, chi, o)
+    expand = expand_sweeps(o, False)
     eng = DMRGEngine(mpo, chi, o)
     be = eng.be
     energies: List[float] = []
     sweep_s: List[float] = []
+    alphas: List[float] = []
     prev_E = None
     E = None
     sweep = -1
@@ -367,7 +407,9 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
         t_sweep = time.perf_counter()
         cur_tol = (o.eig_tol_warmup if (o.eig_tol_warmup is not None and sweep < o.warmup_sweeps)
                    else o.eig_tol)
-        E = eng.sweep(cur_tol)
+        alpha = o.expand_alpha * o.expand_decay ** sweep if sweep < expand else 0.0
+        alphas.append(alpha)
+        E = eng.sweep(cur_tol, alpha)
         energies.append(E)
         be.sync()
         sweep_s.append(time.perf_counter() - t_sweep)
@@ -375,9 +417,9 @@ def _dmrg(mpo, chi: int, o: DMRGOptions):
             print(f"  sweep {sweep + 1:2d}: E = {E:.12f}  (chi={chi}, trunc={o.trunc}, {be.name})")
         if prev_E is not None and abs(E - prev_E) < max(o.tol, o.rtol * abs(E)):
             break
-        prev_E = E
+        prev_E = None if alpha else E
     info = {"sweeps": sweep + 1, "energies": energies, "backend": be.name, "blocks": False,
             "eig": o.eig, "discarded_weight": eng.total_disc, "matvec_calls": eng.matvec_calls,
-            "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2, "expand": 0,
-            "expand_alphas": [0.0] * (sweep + 1), "sweep_s": sweep_s}
+            "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2,
+            "expand": expand, "expand_alphas": alphas, "sweep_s": sweep_s}
     return E, MPS(eng.M, center=0), info

@@ -1,11 +1,13 @@
 """Two-site DMRG on U(1) block MPOs and MPSs, on the HIP device.
 
-The sweep of tnkit/_dmrg.py:560-799 taking its `block` branches, with subspace expansion off
-(expand=0): a random block MPS (`block_mps.random_block_mps`) made right-canonical by an LQ per
+The sweep of tnkit/_dmrg.py:560-799 taking its `block` branches: a random block MPS (`block_mps.random_block_mps`) made right-canonical by an LQ per
 block (tnkit/_dmrg.py:295-306), then per bond theta = M[i] M[i+1], the lowest eigenpair of the
 block H_eff (`block_ops.BlockTwoSite`, all sector GEMMs of each contraction in one grouped
 launch) by thick-restart or mixed-precision Lanczos on the flat canonical buffer
 (tnkit/_dmrg.py:638-644), the truncated SVD across sectors, and the block environment update.
+Subspace expansion runs in the opening sweeps as in tnkit (by default min(4, n_sweeps - 2) of
+them on block MPOs): the centre is split by `expansion.enriched_split`, which fills charge
+sectors the bond lacks.
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ import torch
 from . import block_linalg as BL
 from . import block_mps as BM
 from .block_ops import BlockTensor, BlockTwoSite, einsum
+from .expansion import enriched_split, expansion_term
+from ._dmrg import expand_sweeps
 from .eigensolver import (check_eig_options, lanczos_ground, lanczos_ground_mixed,
                           lanczos_ground_restart)
 from .mps import MPS
@@ -47,8 +51,6 @@ def _check(o, mpo) -> None:
         raise ValueError(f"unknown trunc {o.trunc!r}; expected 'svd' or 'gram'")
     if o.charge is None:
         raise ValueError("block MPOs need charge= (the total charge of the state)")
-    if o.expand not in (None, 0):
-        raise ValueError("subspace expansion is not served by the HIP block sweep (expand=0)")
     if o.init_mps is not None and not getattr(o.init_mps[0], "is_block", False):
         raise ValueError("init_mps for a block MPO must hold block tensors")
     if any(W.dtype != torch.float64 for W in mpo):
@@ -113,7 +115,7 @@ class BlockDMRGEngine:
         self.matvec_calls += count["n"]
         return e, BlockTensor(x0.struct, x.reshape(-1).contiguous())
 
-    def solve_block(self, i: int, direction: str, cur_tol: float) -> float:
+    def solve_block(self, i: int, direction: str, cur_tol: float, alpha: float = 0.0) -> float:
         o, M, L, R = self.o, self.M, self.L, self.R
         W1, W2 = self.mpo[i], self.mpo[i + 1]
         theta0 = einsum("asc,ctb->astb", M[i], M[i + 1])
@@ -125,6 +127,17 @@ class BlockDMRGEngine:
         e, x = self.eigsolve(theta0, op, op_lo, cur_tol)
         del op, op_lo
         t0 = time.perf_counter()
+        if alpha:
+            right = direction == "right"
+            P = expansion_term(x, L[i], W1, W2, R[i + 2], right)
+            A, C, disc = enriched_split(x, P, alpha, self.chi, None, right)
+            C = C.with_data(C.data / BL.norm(C))
+            M[i], M[i + 1] = (A, C) if right else (C, A)
+            torch.cuda.synchronize()
+            self.phase["trunc"] += time.perf_counter() - t0
+            self.total_disc += disc
+            self._env(i, direction)
+            return e
         mat, rec = BL.fuse(x, ((0, 1), (2, 3)))
         if o.trunc == "gram":
             U, S, Vh, disc = BL.truncated_gram(mat, self.chi,
@@ -141,35 +154,42 @@ class BlockDMRGEngine:
         torch.cuda.synchronize()
         self.phase["trunc"] += time.perf_counter() - t0
         self.total_disc += disc
-        t0 = time.perf_counter()
-        if direction == "right":
-            L[i + 1] = BM.update_left_env(L[i], M[i], W1)
-        else:
-            R[i + 1] = BM.update_right_env(R[i + 2], M[i + 1], W2)
-        torch.cuda.synchronize()
-        self.phase["env"] += time.perf_counter() - t0
+        self._env(i, direction)
         return e
 
-    def sweep(self, cur_tol: float) -> float:
+    def _env(self, i: int, direction: str) -> None:
+        t0 = time.perf_counter()
+        if direction == "right":
+            self.L[i + 1] = BM.update_left_env(self.L[i], self.M[i], self.mpo[i])
+        else:
+            self.R[i + 1] = BM.update_right_env(self.R[i + 2], self.M[i + 1], self.mpo[i + 1])
+        torch.cuda.synchronize()
+        self.phase["env"] += time.perf_counter() - t0
+
+    def sweep(self, cur_tol: float, alpha: float = 0.0) -> float:
         E = None
         for i in range(0, self.N - 1):
-            E = self.solve_block(i, "right", cur_tol)
+            E = self.solve_block(i, "right", cur_tol, alpha)
         for i in range(self.N - 2, -1, -1):
-            E = self.solve_block(i, "left", cur_tol)
+            E = self.solve_block(i, "left", cur_tol, alpha)
         return E
 
 
 def block_dmrg(mpo: Sequence[BlockTensor], chi: int, o) -> tuple:
     """The sweep loop of tnkit/_dmrg.py:767-799 on block tensors; returns (E, MPS, info)."""
+    expand = expand_sweeps(o, True)
     eng = BlockDMRGEngine(mpo, chi, o)
     energies: List[float] = []
     sweep_s: List[float] = []
+    alphas: List[float] = []
     prev_E, E, sweep = None, None, -1
     for sweep in range(o.n_sweeps):
         t0 = time.perf_counter()
         cur_tol = (o.eig_tol_warmup if (o.eig_tol_warmup is not None and sweep < o.warmup_sweeps)
                    else o.eig_tol)
-        E = eng.sweep(cur_tol)
+        alpha = o.expand_alpha * o.expand_decay ** sweep if sweep < expand else 0.0
+        alphas.append(alpha)
+        E = eng.sweep(cur_tol, alpha)
         energies.append(E)
         torch.cuda.synchronize()
         sweep_s.append(time.perf_counter() - t0)
@@ -177,10 +197,10 @@ def block_dmrg(mpo: Sequence[BlockTensor], chi: int, o) -> tuple:
             print(f"  sweep {sweep + 1:2d}: E = {E:.12f}  (chi={chi}, blocks, hip)")
         if prev_E is not None and abs(E - prev_E) < max(o.tol, o.rtol * abs(E)):
             break
-        prev_E = E
+        prev_E = None if alpha else E
     info: Dict[str, Any] = {
         "sweeps": sweep + 1, "energies": energies, "backend": "hip-real", "blocks": True,
         "eig": o.eig, "discarded_weight": eng.total_disc, "matvec_calls": eng.matvec_calls,
-        "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2, "expand": 0,
-        "expand_alphas": [0.0] * (sweep + 1), "sweep_s": sweep_s}
+        "phase_time": eng.phase, "mixed_counts": eng.mixed_counts, "sites": 2,
+        "expand": expand, "expand_alphas": alphas, "sweep_s": sweep_s}
     return E, MPS(eng.M, center=0), info

@@ -88,9 +88,11 @@ def _svd(M: torch.Tensor, method: str):
     return U[:, o], S, Q.t()
 
 
-def truncated_svd(m: BlockTensor, chi: int, method: str = "svd"):
+def truncated_svd(m: BlockTensor, chi: int, method: str = "svd", zero_tol: float = 0.0):
     """m ~ U diag(S) Vh keeping the chi largest singular values across all sectors
-    (tnkit/blocks.py:1380-1462 with svd_tol = zero_tol = degenerate_tol = 0, bucket = 1).
+    (tnkit/blocks.py:1380-1462 with svd_tol = degenerate_tol = 0, bucket = 1). zero_tol > 0
+    drops singular values at or below zero_tol times the largest before the cap (at least one
+    is kept).
     Returns (U, S, Vh, disc): U with legs (m.legs[0], k-), Vh with (k+, m.legs[1]), S a device
     vector over the new leg in sector order, disc the discarded weight sum(sigma_dropped^2).
     The new leg has one sector per block with kept values, labelled by the row sector's charge,
@@ -107,7 +109,10 @@ def truncated_svd(m: BlockTensor, chi: int, method: str = "svd"):
     s_host = torch.cat([p[3] for p in parts]).cpu().numpy().astype(np.float64)
     owner = np.concatenate([np.full(len(p[3]), j) for j, p in enumerate(parts)])
     order = np.lexsort((np.arange(s_host.size), -s_host))
-    k = min(int(chi), s_host.size)
+    live = s_host.size
+    if zero_tol > 0:
+        live = max(1, int((s_host > zero_tol * s_host.max()).sum()))
+    k = min(int(chi), live)
     kept = np.bincount(owner[order[:k]], minlength=len(parts))
     disc = float((s_host[order[k:]] ** 2).sum())
     label = (lambda r: lr.charges[r]) if lr.sign > 0 else (lambda r: -lr.charges[r])

@@ -215,14 +215,14 @@ class _ExcitedEngine(DMRGEngine):
         self.t_eig = 0.0
         self._V: Optional[torch.Tensor] = None
 
-    def solve_block(self, i: int, direction: str, cur_tol: float) -> float:
+    def solve_block(self, i: int, direction: str, cur_tol: float, alpha: float = 0.0) -> float:
         if self.prev:
             self._V = torch.stack([_block_vector(self.ELs[j][i], P[i], P[i + 1],
                                                  self.ERs[j][i + 2])
                                    for j, P in enumerate(self.prev)]).contiguous()
         else:
             self._V = None
-        e = super().solve_block(i, direction, cur_tol)
+        e = super().solve_block(i, direction, cur_tol, alpha)
         for j, P in enumerate(self.prev):
             if direction == "right":
                 self.ELs[j][i + 1] = _grow_EL(self.ELs[j][i], self.M[i], P[i])
