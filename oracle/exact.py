@@ -119,3 +119,24 @@ def tfim_quench_z(g0, g1, times):
         G = R @ G0 @ R.T
         out.append(-np.array([G[2 * j, 2 * j + 1] for j in range(N)]))
     return np.array(out)
+
+
+def tfim_block_entropies(g):
+    """Entanglement entropy (nats) of the first l sites, l = 1..N-1, of the free-fermion ground
+    state of `tfim_majorana_matrix(g)`: the reduced state is Gaussian with covariance
+    Gamma[:2l, :2l], whose i Gamma has eigenvalues +-nu_k, and S = sum_k h((1 + nu_k) / 2) with
+    h(p) = -p ln p - (1 - p) ln(1 - p) (Peschel's formula for a block at the open end)."""
+    G = tfim_ground_covariance(g)
+    N = G.shape[0] // 2
+    out = []
+    for l in range(1, N):
+        nu = np.linalg.eigvalsh(1j * G[:2 * l, :2 * l])
+        nu = np.clip(nu[nu > 0], 0.0, 1.0)
+        p = (1 + nu) / 2
+        q = 1 - p
+        s = 0.0
+        for a in np.concatenate([p, q]):
+            if a > 1e-16:
+                s -= a * np.log(a)
+        out.append(s)
+    return np.array(out)

@@ -49,8 +49,9 @@ def _check(o, mpo) -> None:
     check_eig_options(o)
     if o.trunc not in ("svd", "gram"):
         raise ValueError(f"unknown trunc {o.trunc!r}; expected 'svd' or 'gram'")
-    if o.charge is None:
-        raise ValueError("block MPOs need charge= (the total charge of the state)")
+    if (o.charge is None) == (o.init_mps is None):
+        raise ValueError("a block MPO needs exactly one of charge= (the total charge of the state "
+                         "sought) and a block init_mps (which fixes the charge)")
     if o.init_mps is not None and not getattr(o.init_mps[0], "is_block", False):
         raise ValueError("init_mps for a block MPO must hold block tensors")
     if any(W.dtype != torch.float64 for W in mpo):
