@@ -68,7 +68,7 @@ def measured_traffic(kernel_label, chi, form):
     except (OSError, ValueError):
         return None, None
     for row in table.get("rows", []):
-        if row["chi"] == chi and row["form"] == form and row["stage"] in kernel_label:
+        if row["chi"] == chi and row["form"] == form and kernel_label.startswith(row["stage"]):
             return row["fetch_bytes"] + row["write_bytes"], os.path.relpath(TRAFFIC_FILE,
                                                                              os.getcwd())
     return None, None
@@ -661,7 +661,7 @@ def main():
     dom = max(stage_ms, key=stage_ms.get)
     names = {1: "GEMM1 theta x L (narrow_kernel + gemm_tn_kernel<float,float,float,float>)",
              2: "MPO mix (mix_mfma_kernel<float,float,float>)",
-             4: "GEMM3 x R (gemm_tn_kernel<float,float,float,double>)"}
+             4: "GEMM3 x R (gemm_dma_kernel<float,double,32>)"}
     if dom == 2:
         achieved = sbytes[2] / (stage_ms[2] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",

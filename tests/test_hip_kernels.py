@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from oracle import dense as D
-from tnkit_amd import kernels
+from tnkit_amd import _lib, kernels
 from tnkit_amd.models import xxz_mpo
 
 pytestmark = pytest.mark.gpu
@@ -39,6 +39,32 @@ def test_gemm_tn(hip, M, N, K, tile, dtype):
                           tile=tile)
     if M * N:
         assert _rel(acc, 2 * ref) < tol
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 16), (128, 256, 48), (384, 128, 80), (256, 384, 96),
+                                   (256, 256, 64), (256, 256, 40)])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_gemm_tn_direct_to_lds(hip, M, N, K, dtype):
+    """Shapes that take the direct-to-LDS kernel (M, N multiples of 128, K of its step: one
+    stage, a full ring, a ring that wraps), batched with strides and accumulating, through the C
+    ABI; K = 40 in float32 falls back to the staged kernel."""
+    print(f"gemm_tn dma M={M} N={N} K={K} {dtype}", flush=True)
+    rng = np.random.default_rng(M + 3 * N + 7 * K)
+    nb = 3
+    A = rng.standard_normal((nb, K, M))
+    B = rng.standard_normal((nb, K, N))
+    ref = np.einsum("bkm,bkn->bmn", A, B)
+    Ad, Bd = _t(A, hip, dtype), _t(B, hip, dtype)
+    C = torch.full((nb, M, N), 0.5, dtype=dtype, device=hip)
+    lib = _lib.load()
+    for acc in (0, 1):
+        rc = lib.tnh_gemm_tn(_lib.dcode(dtype), M, N, K, Ad.data_ptr(), M, Bd.data_ptr(), N,
+                             C.data_ptr(), N, nb, K * M, K * N, M * N, acc, 0,
+                             kernels.stream(hip))
+        assert rc == 0, lib.tnh_last_error()
+        want = ref if acc == 0 else 2 * ref
+        tol = 1e-13 if dtype == torch.float64 else 2e-6 * np.sqrt(K)
+        assert _rel(C, want) < tol
 
 
 @pytest.mark.parametrize("M,N,K,tile", [(1, 1, 1, 0), (17, 33, 9, 0), (128, 128, 128, 1),

@@ -1,5 +1,6 @@
 // Tile selection and launch of the K-major MFMA GEMM, plus the C-ABI entry tnh_gemm_tn.
 #include <cstdint>
+#include <type_traits>
 
 #include "gemm_tn.h"
 
@@ -49,6 +50,40 @@ int launch_one(const GemmArgs& g, int batch, hipStream_t stream) {
   return launch_v<TC, TA, TB, TO, BM, BN, BK, WM, WN, 1, 1>(g, batch, stream);
 }
 
+// The direct-to-LDS kernel (gemm_dma_kernel) when the operands are stored in the compute type
+// and the shape and strides fit it: float64 always (0.74 vs 0.79 ms on the chi = 1024 matvec
+// GEMMs), float32 when there are at most two 128 x 128 tiles per CU (a long K, as the third
+// matvec GEMM: 0.36 vs 0.43 ms); with more tiles the staged kernel keeps up and leads slightly.
+template <typename TC, typename TA, typename TB, typename TO>
+bool try_dma(const GemmArgs& g, int batch, hipStream_t stream, int& rc) {
+  if constexpr (std::is_same<TA, TC>::value && std::is_same<TB, TC>::value &&
+                (std::is_same<TC, float>::value || std::is_same<TC, double>::value)) {
+    constexpr int EPC = 16 / sizeof(TC);
+    constexpr int BK = std::is_same<TC, float>::value ? 32 : 16;
+    if (batch < 1 || g.M % 128 || g.N % 128 || g.K < BK || g.K % BK) return false;
+    if (g.lda % EPC || g.ldb % EPC || reinterpret_cast<uintptr_t>(g.A) % 16 ||
+        reinterpret_cast<uintptr_t>(g.B) % 16)
+      return false;
+    if (batch > 1 && (g.strideA % EPC || g.strideB % EPC)) return false;
+    const long tiles = (long)(g.M / 128) * (g.N / 128);
+    if (std::is_same<TC, float>::value && tiles * batch > 512) return false;
+    if (tiles >= (1L << 31) || batch > 65535) return false;
+    hipLaunchKernelGGL((gemm_dma_kernel<TC, TO, BK>), dim3(static_cast<unsigned>(tiles), batch),
+                       dim3(256), 0, stream, g);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error("gemm_tn (dma): launch failed: %s", hipGetErrorString(e));
+      rc = TNH_ERR_HIP;
+    } else {
+      rc = TNH_OK;
+    }
+    return true;
+  } else {
+    (void)g, (void)batch, (void)stream, (void)rc;
+    return false;
+  }
+}
+
 }  // namespace
 
 // Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64, K step 16 doubles / 32 floats),
@@ -60,6 +95,8 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
   TNH_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0, TNH_ERR_ARG, "gemm_tn: negative size");
   int cfg = tile_hint;
   if (cfg == 0) {
+    int rc = TNH_OK;
+    if (g.M > 0 && g.N > 0 && try_dma<TC, TA, TB, TO>(g, batch, stream, rc)) return rc;
     const long b = batch > 0 ? batch : 1;
     const long t128 = ceil_div(g.M, 128) * ceil_div(g.N, 128) * b;
     const long t64 = ceil_div(g.M, 64) * ceil_div(g.N, 64) * b;

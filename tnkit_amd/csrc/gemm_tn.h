@@ -15,6 +15,8 @@
 // float on the way into LDS, C stored in double from a float accumulator), which is how the
 // mixed-precision matvec takes float64 Krylov vectors without a separate conversion pass.
 #pragma once
+#include <type_traits>
+
 #include "cplx.h"
 #include "tnh_common.h"
 
@@ -215,6 +217,147 @@ __global__ void __launch_bounds__(WM* WN * 64)
       }
     }
   }
+}
+
+// ---- direct-to-LDS variant --------------------------------------------------------------------
+// Float or double operands stored in the compute type (no conversion on the way in). Tiles go
+// from HBM straight into LDS with global_load_lds_dwordx4 (lane i of a wave writes bytes
+// [16 i, 16 i + 16) after the wave's LDS base), so no VGPRs stage them and no ds_write is issued.
+// Three LDS stages of BK rows form a ring: the tile two steps ahead is in flight while one is
+// computed. Stages are separate arrays so that the compiler can tell a DMA into one stage from
+// fragment reads of another and inserts no wait between them; the waits on the tile being
+// consumed are explicit vmcnt counts before the workgroup barrier. Rows are unpadded (the DMA
+// writes contiguous 1 KB per wave) and rotated by 16 elements per (row mod 4), so the four k rows
+// of one fragment read fall in distinct bank ranges. 128 x 128 tiles, 4 waves of 64 x 64.
+// Requires M, N % 128 == 0, K % BK == 0, 16-byte aligned operands with lda, ldb and batch strides
+// multiples of a 16-byte chunk (the launcher checks).
+typedef __attribute__((address_space(3))) void* lds_void_t;
+
+template <typename T, int BK>
+struct DmaStages {
+  T a0[BK * 128], a1[BK * 128], a2[BK * 128];
+  T b0[BK * 128], b1[BK * 128], b2[BK * 128];
+};
+
+template <int S, typename SM>
+__device__ __forceinline__ auto* dma_stage_a(SM& m) {
+  if constexpr (S == 0) return m.a0;
+  else if constexpr (S == 1) return m.a1;
+  else return m.a2;
+}
+
+template <int S, typename SM>
+__device__ __forceinline__ auto* dma_stage_b(SM& m) {
+  if constexpr (S == 0) return m.b0;
+  else if constexpr (S == 1) return m.b1;
+  else return m.b2;
+}
+
+// s_waitcnt vmcnt(NW), the other counters left alone (gfx9 encoding)
+template <int NW>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(NW >= 0 && NW < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((NW & 15) | ((NW >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+template <typename T, typename TO, int BK>
+__global__ void __launch_bounds__(256) gemm_dma_kernel(GemmArgs g) {
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  constexpr int BM = 128, FM = 4, FN = 4;
+  constexpr int EPC = 16 / sizeof(T);  // elements per 16-byte chunk
+  constexpr int CPR = BM / EPC;        // chunks per tile row
+  constexpr int NJ = BK * CPR / 256;   // chunks per thread per operand tile
+  constexpr int ROT = 16 / EPC;        // the 16-element row rotation, in chunks
+  static_assert(NJ * 256 == BK * CPR && BK % 4 == 0, "dma tile shape");
+  __shared__ __align__(16) DmaStages<T, BK> sm;
+  const int M = g.M, N = g.N, K = g.K;
+  int tm, tn;
+  tile_of(blockIdx.x, M / BM, N / BM, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BM;
+  const long bz = blockIdx.y;
+  const T* __restrict__ A = static_cast<const T*>(g.A) + bz * g.strideA;
+  const T* __restrict__ B = static_cast<const T*>(g.B) + bz * g.strideB;
+  TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1, row16 = lane & 15, kq = lane >> 4;
+  // element offsets, within a tile, of this thread's chunks (chunk c of the tile lands at LDS
+  // position c; it holds the chunk of its row rotated back by ROT * (row mod 4))
+  long goffA[NJ], goffB[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int c = j * 256 + t;
+    const int r = c / CPR, p = c % CPR;
+    const int q = (p - ROT * (r & 3)) & (CPR - 1);
+    goffA[j] = (long)r * g.lda + m0 + EPC * q;
+    goffB[j] = (long)r * g.ldb + n0 + EPC * q;
+  }
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+  const int nk = K / BK;
+  const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * 64 + row16 + 16 * kq;
+  auto issue = [&](int kt, T* as, T* bs) {
+    const T* a = A + (long)kt * BK * g.lda;
+    const T* b = B + (long)kt * BK * g.ldb;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
+                                       (lds_void_t)(as + (j * 256 + wave * 64) * EPC), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
+                                       (lds_void_t)(bs + (j * 256 + wave * 64) * EPC), 16, 0, 0);
+    }
+  };
+  auto compute = [&](const T* as, const T* bs) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * BM + ((aoff + i * 16) & (BM - 1))];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + kq) * BM + ((boff + j * 16) & (BM - 1))];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+    }
+  };
+  // one step of the ring on stage S: tile kt landed (at most the next tile's loads still in
+  // flight), barrier, prefetch tile kt + 2 into the stage read two steps ago, compute tile kt
+  auto step = [&](auto stage, int kt) {
+    constexpr int S = decltype(stage)::value;
+    if (kt + 1 < nk)
+      wait_vmcnt<2 * NJ>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (kt + 2 < nk) issue(kt + 2, dma_stage_a<(S + 2) % 3>(sm), dma_stage_b<(S + 2) % 3>(sm));
+    compute(dma_stage_a<S>(sm), dma_stage_b<S>(sm));
+  };
+  if (nk > 0) issue(0, sm.a0, sm.b0);
+  if (nk > 1) issue(1, sm.a1, sm.b1);
+  for (int kt = 0; kt < nk; kt += 3) {
+    step(std::integral_constant<int, 0>{}, kt);
+    if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
+    if (kt + 2 < nk) step(std::integral_constant<int, 2>{}, kt + 2);
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * 64 + j * 16 + row16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 64 + i * 16 + MF::row(kq, r);
+        TO* dst = C + (long)m * g.ldc + n;
+        if (g.accumulate)
+          *dst = static_cast<TO>(static_cast<T>(*dst) + acc[i][j][r]);
+        else
+          *dst = static_cast<TO>(acc[i][j][r]);
+      }
+    }
 }
 
 // Complex K-major GEMM: C[m][n] = sum_k A[k][m] * B'[k][n] with B' = B or conj(B), complex
