@@ -114,9 +114,12 @@ def test_two_site_matvec_f64(hip, a, wl, wm, wr, d1, d2, b):
     assert np.abs(S - Sref).max() <= 1e-13 * max(1, np.abs(Sref).max())
 
 
-@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES[3:])
+@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES[3:] + [(64, 5, 4, 3, 2, 2, 96),
+                                                              (96, 3, 7, 8, 2, 2, 64)])
 def test_two_site_matvec_f32_operator(hip, a, wl, wm, wr, d1, d2, b):
-    """float32 operator with float64 input/output: error at float32 rounding of the sums."""
+    """float32 operator with float64 input/output: error at float32 rounding of the sums.
+    Shapes with a, b multiples of 32 and wl in (3, 5) take GEMM1 with the MPO mix in its
+    epilogue, also with wr != wl."""
     print(f"two_site f32 {(a, wl, wm, wr, d1, d2, b)}", flush=True)
     rng = np.random.default_rng(11 + a)
     L, R = rng.standard_normal((a, wl, a)), rng.standard_normal((b, wr, b))

@@ -534,6 +534,210 @@ __global__ void __launch_bounds__(256) narrow_kernel(long n, const double* __res
   }
 }
 
+
+// ---- GEMM1 with the MPO mix in its epilogue (float32) -----------------------------------------
+// For d1 = d2 = 2 the first two stages of the matvec fuse: a GEMM1 output tile of 128 rows
+// (the four (i, j) blocks of 32 consecutive b) by 32 WL columns (32 consecutive p, all WL
+// channels w) holds, for each of its 32 x 32 (b, p) pairs, the 4 WL inputs of the MPO mix, so
+// the tile is mixed in the epilogue and written straight to X2[r][(b p)][(s t)]; X1 never goes
+// to HBM (it was 84 MB written and read again in float32 at chi = 1024). The main loop is the
+// staged K-major loop of gemm_tn_kernel with the A rows gathered from the four (i, j)
+// segments. The epilogue runs in two halves of 16 p (the two wave columns), through an LDS tile
+// that aliases the main loop's buffers: acc -> LDS, then each thread mixes two (b, p) pairs with
+// the 4 WR x 4 WL operator S and stores 4 (s, t) values per channel r.
+struct Gemm1MixArgs {
+  int K, B, P, WR;
+  const void* A;   // theta in the compute type: K rows of 4 B (i j b)
+  long lda, sA;
+  const void* Bm;  // LT: K rows of P WL (p w)
+  long ldb, sB;
+  const void* S;   // (s t r) x (w i j), row-major, 4 WR x 4 WL
+  long sS;
+  void* X2;        // (r, b, p, s t)
+  long sX2;
+};
+
+template <typename T, int WL>
+__global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  constexpr int BM = 128, BN = 32 * WL, BK = 16, PAD = 16;
+  constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
+  constexpr int FM = 4, FN = WL;                    // 2 x 2 waves of 64 x 16 WL
+  constexpr int VA = 16 / sizeof(T);                // A: 16-byte vectors
+  constexpr int VB = 8 / sizeof(T);                 // B: 8-byte vectors (BN is a multiple of 32)
+  constexpr int NA = BK * BM / (VA * 256), NB = BK * BN / (VB * 256);
+  static_assert(NA * VA * 256 == BK * BM && NB * VB * 256 == BK * BN, "gemm1_mix tile loads");
+  constexpr int ELD = 16 * WL + 1;                  // epilogue tile row (one half), odd stride
+  constexpr int MAIN = 2 * BK * (LDA_S + LDB_S);
+  constexpr int EPI = BM * ELD;
+  __shared__ __align__(16) T lds[MAIN > EPI ? MAIN : EPI];
+  __shared__ T Ss[4 * 8 * 4 * WL];
+  T* As0 = lds;
+  T* Bs0 = lds + 2 * BK * LDA_S;
+
+  const int B = g.B, P = g.P, K = g.K, WR = g.WR;
+  const int nbm = B / 32, nbn = P / 32;
+  int tm, tn;
+  tile_of(blockIdx.x, nbm, nbn, tm, tn);
+  const int b0 = tm * 32, p0 = tn * 32;
+  const long bz = blockIdx.y;
+  const T* __restrict__ A = static_cast<const T*>(g.A) + bz * g.sA;
+  const T* __restrict__ Bm = static_cast<const T*>(g.Bm) + bz * g.sB;
+  const T* __restrict__ Sg = static_cast<const T*>(g.S) + bz * g.sS;
+  T* __restrict__ X2 = static_cast<T*>(g.X2) + bz * g.sX2;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave >> 1, wn = wave & 1, row16 = lane & 15, kq = lane >> 4;
+  for (int e = t; e < 4 * WR * 4 * WL; e += 256) Ss[e] = Sg[e];
+
+  typedef T va_t __attribute__((ext_vector_type(VA)));
+  typedef T vb_t __attribute__((ext_vector_type(VB)));
+  va_t ra[NA];
+  vb_t rb[NB];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int e = t + v * 256, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+      const int k = k0 + kr;
+      // columns mc .. mc + VA - 1 of the tile: segment ij = mc / 32, b = b0 + mc % 32
+      if (k < K)
+        ra[v] = *reinterpret_cast<const va_t*>(A + (long)k * g.lda + (long)(mc >> 5) * B + b0 +
+                                              (mc & 31));
+      else
+        ra[v] = va_t{};
+    }
+#pragma unroll
+    for (int v = 0; v < NB; ++v) {
+      const int e = t + v * 256, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+      const int k = k0 + kr;
+      if (k < K)
+        rb[v] = *reinterpret_cast<const vb_t*>(Bm + (long)k * g.ldb + (long)p0 * WL + nc);
+      else
+        rb[v] = vb_t{};
+    }
+  };
+  auto store = [&](int buf) {
+    T* as = As0 + buf * BK * LDA_S;
+    T* bs = Bs0 + buf * BK * LDB_S;
+#pragma unroll
+    for (int v = 0; v < NA; ++v) {
+      const int e = t + v * 256, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+      *reinterpret_cast<va_t*>(as + kr * LDA_S + mc) = ra[v];
+    }
+#pragma unroll
+    for (int v = 0; v < NB; ++v) {
+      const int e = t + v * 256, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+      *reinterpret_cast<vb_t*>(bs + kr * LDB_S + nc) = rb[v];
+    }
+  };
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+  const int nk = (K + BK - 1) / BK;
+  if (nk > 0) {
+    load(0);
+    store(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    const T* as = As0 + cur * BK * LDA_S;
+    const T* bs = Bs0 + cur * BK * LDB_S;
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * LDA_S + wm * 64 + i * 16 + row16];
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+        bf[j] = bs[(kk + kq) * LDB_S + wn * 16 * WL + j * 16 + row16];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+    }
+    if (kt + 1 < nk) store(cur ^ 1);
+    __syncthreads();
+  }
+
+  // epilogue: two halves of 16 p (wave column h), each through the LDS tile E[128][16 WL]
+  T* E = lds;
+  const long plane = (long)B * P * 4;  // X2 stride between channels r
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (wn == h) {
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            E[(wm * 64 + i * 16 + MF::row(kq, r)) * ELD + j * 16 + row16] = acc[i][j][r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int pair = t + q * 256;          // (bb, pp), pp fastest: 32 b x 16 p
+      const int bb = pair >> 4, pp = pair & 15;
+      T x[WL][4];
+#pragma unroll
+      for (int w = 0; w < WL; ++w)
+#pragma unroll
+        for (int ij = 0; ij < 4; ++ij) x[w][ij] = E[(ij * 32 + bb) * ELD + pp * WL + w];
+      T* dst = X2 + ((long)(b0 + bb) * P + p0 + 16 * h + pp) * 4;
+      for (int r = 0; r < WR; ++r) {
+        T y[4];
+#pragma unroll
+        for (int st = 0; st < 4; ++st) {
+          const T* srow = Ss + (st * WR + r) * (4 * WL);
+          T s = T(0);
+#pragma unroll
+          for (int w = 0; w < WL; ++w)
+#pragma unroll
+            for (int ij = 0; ij < 4; ++ij) s += srow[w * 4 + ij] * x[w][ij];
+          y[st] = s;
+        }
+        T* o = dst + r * plane;
+        o[0] = y[0];
+        o[1] = y[1];
+        o[2] = y[2];
+        o[3] = y[3];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+bool try_gemm1_mix(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* theta,
+                   long sth, const void* LT, long sLT, const void* S, long sS, void* X2, long sX2,
+                   hipStream_t st, int& rc) {
+  if (d1 != 2 || d2 != 2 || (wl != 3 && wl != 5) || wr < 1 || wr > 8) return false;
+  if (b % 32 || a % 32 || a < 32 || b < 32 || batch < 1 || batch > 65535) return false;
+  constexpr int VA = 16 / sizeof(T), VB = 8 / sizeof(T);
+  if (reinterpret_cast<uintptr_t>(theta) % 16 || reinterpret_cast<uintptr_t>(LT) % 8 ||
+      (batch > 1 && (sth % VA || sLT % VB)))
+    return false;
+  Gemm1MixArgs g{a, b, a, wr, theta, 4L * b, sth, LT, (long)a * wl, sLT, S, sS, X2, sX2};
+  const dim3 grid(static_cast<unsigned>((b / 32) * (a / 32)), static_cast<unsigned>(batch));
+  if (wl == 5)
+    hipLaunchKernelGGL((gemm1_mix_kernel<T, 5>), grid, dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((gemm1_mix_kernel<T, 3>), grid, dim3(256), 0, st, g);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("gemm1_mix: launch failed: %s", hipGetErrorString(e));
+    rc = TNH_ERR_HIP;
+  } else {
+    rc = TNH_OK;
+  }
+  return true;
+}
+
 template <typename TC, typename TIO>
 int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* LT,
                    long sLT, const void* S, long sS, const void* RT, long sRT, const void* theta,
@@ -554,7 +758,36 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   }
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
-  if (stages & 1) {  // GEMM1: M = d1 d2 b, N = p wl, K = q
+  // GEMM1 and the MPO mix in one kernel when both run here and the shape fits it
+  bool fused = false;
+  if ((stages & 3) == 3 && tile1 == 0) {
+    // float32 only: the float64 form needs a wave per SIMD for its registers and loses to
+    // the direct-to-LDS GEMM1 plus the separate mix (0.99 vs 0.89 ms at chi = 1024)
+    if constexpr (std::is_same<TC, float>::value && std::is_same<TIO, float>::value) {
+      int rc = TNH_OK;
+      fused = try_gemm1_mix<TC>(batch, a, d1, d2, b, wl, wr, theta, sth, LT, sLT, S, sS, X2,
+                                nx2, st, rc);
+      if (fused && rc != TNH_OK) return rc;
+    } else if constexpr (kNarrow) {
+      if (Th != nullptr) {
+        const long n = nth * batch;
+        long gsz = (n / 2 + 255) / 256;
+        if (gsz > 4096) gsz = 4096;
+        if (gsz < 1) gsz = 1;
+        int rc = TNH_OK;
+        if (d1 == 2 && d2 == 2 && (wl == 3 || wl == 5) && wr <= 8 && a % 32 == 0 &&
+            b % 32 == 0) {
+          hipLaunchKernelGGL(narrow_kernel, dim3(static_cast<unsigned>(gsz)), dim3(256), 0, st,
+                             n, static_cast<const double*>(theta), Th);
+          TNH_CHECK_LAUNCH("narrow");
+          fused = try_gemm1_mix<float>(batch, a, d1, d2, b, wl, wr, Th, nth, LT, sLT, S, sS, X2,
+                                       nx2, st, rc);
+          if (fused && rc != TNH_OK) return rc;
+        }
+      }
+    }
+  }
+  if ((stages & 1) && !fused) {  // GEMM1: M = d1 d2 b, N = p wl, K = q
     if constexpr (kNarrow) {
       if (Th != nullptr) {
         const long n = nth * batch;
@@ -575,7 +808,7 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
       TNH_TRY((gemm_any<TC, TIO, TC, TC>(g, batch, 0, st, tile1)));
     }
   }
-  if (stages & 2) {  // S-mix
+  if ((stages & 2) && !fused) {  // S-mix
     MixArgs m{B, P, static_cast<int>(dd * wr), wr, static_cast<int>(wl * dd), static_cast<int>(dd),
               P * wl, wl, 1, B * P * wl, P * dd, dd, 1, B * P * dd, sS, nx1, nx2};
     TNH_TRY((mix<TC, TC, TC>(m, batch, S, X1, X2, st)));
