@@ -91,7 +91,9 @@ def test_gemm_tn_bf16(hip, M, N, K, tile):
 
 SHAPES = [  # (a, wl, wm, wr, d1, d2, b) -- tnkit/tests/test_two_site_fused.py plus bulk sizes
     (1, 1, 5, 5, 2, 2, 2), (4, 5, 5, 1, 2, 2, 1), (7, 20, 20, 20, 2, 3, 9), (32, 3, 3, 3, 2, 2, 32),
-    (64, 5, 5, 5, 2, 2, 48), (256, 5, 5, 5, 2, 2, 256)]
+    (64, 5, 5, 5, 2, 2, 48), (256, 5, 5, 5, 2, 2, 256),
+    # a, b multiples of 32 with wl in (3, 5): GEMM1 with the MPO mix in its epilogue, wr != wl
+    (64, 5, 4, 3, 2, 2, 96), (96, 3, 7, 8, 2, 2, 64)]
 
 
 @pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES)
@@ -114,8 +116,7 @@ def test_two_site_matvec_f64(hip, a, wl, wm, wr, d1, d2, b):
     assert np.abs(S - Sref).max() <= 1e-13 * max(1, np.abs(Sref).max())
 
 
-@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES[3:] + [(64, 5, 4, 3, 2, 2, 96),
-                                                              (96, 3, 7, 8, 2, 2, 64)])
+@pytest.mark.parametrize("a,wl,wm,wr,d1,d2,b", SHAPES[3:])
 def test_two_site_matvec_f32_operator(hip, a, wl, wm, wr, d1, d2, b):
     """float32 operator with float64 input/output: error at float32 rounding of the sums.
     Shapes with a, b multiples of 32 and wl in (3, 5) take GEMM1 with the MPO mix in its

@@ -557,17 +557,19 @@ struct Gemm1MixArgs {
   long sX2;
 };
 
-template <typename T, int WL>
-__global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
+template <typename T, int WL, int NWM>
+__global__ void __launch_bounds__(NWM * 128, NWM == 2 ? 3 : 1)
+    gemm1_mix_kernel(Gemm1MixArgs g) {
+  constexpr int NT = NWM * 128;                     // NWM x 2 waves
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
   constexpr int BM = 128, BN = 32 * WL, BK = 16, PAD = 16;
   constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
-  constexpr int FM = 4, FN = WL;                    // 2 x 2 waves of 64 x 16 WL
+  constexpr int FM = 128 / NWM / 16, FN = WL;       // waves of (128 / NWM) x 16 WL
   constexpr int VA = 16 / sizeof(T);                // A: 16-byte vectors
   constexpr int VB = 8 / sizeof(T);                 // B: 8-byte vectors (BN is a multiple of 32)
-  constexpr int NA = BK * BM / (VA * 256), NB = BK * BN / (VB * 256);
-  static_assert(NA * VA * 256 == BK * BM && NB * VB * 256 == BK * BN, "gemm1_mix tile loads");
+  constexpr int NA = BK * BM / (VA * NT), NB = BK * BN / (VB * NT);
+  static_assert(NA * VA * NT == BK * BM && NB * VB * NT == BK * BN, "gemm1_mix tile loads");
   constexpr int ELD = 16 * WL + 1;                  // epilogue tile row (one half), odd stride
   constexpr int MAIN = 2 * BK * (LDA_S + LDB_S);
   constexpr int EPI = BM * ELD;
@@ -588,7 +590,7 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
   T* __restrict__ X2 = static_cast<T*>(g.X2) + bz * g.sX2;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1, row16 = lane & 15, kq = lane >> 4;
-  for (int e = t; e < 4 * WR * 4 * WL; e += 256) Ss[e] = Sg[e];
+  for (int e = t; e < 4 * WR * 4 * WL; e += NT) Ss[e] = Sg[e];
 
   typedef T va_t __attribute__((ext_vector_type(VA)));
   typedef T vb_t __attribute__((ext_vector_type(VB)));
@@ -597,7 +599,7 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
   auto load = [&](int k0) {
 #pragma unroll
     for (int v = 0; v < NA; ++v) {
-      const int e = t + v * 256, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+      const int e = t + v * NT, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
       const int k = k0 + kr;
       // columns mc .. mc + VA - 1 of the tile: segment ij = mc / 32, b = b0 + mc % 32
       if (k < K)
@@ -608,7 +610,7 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
     }
 #pragma unroll
     for (int v = 0; v < NB; ++v) {
-      const int e = t + v * 256, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+      const int e = t + v * NT, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
       const int k = k0 + kr;
       if (k < K)
         rb[v] = *reinterpret_cast<const vb_t*>(Bm + (long)k * g.ldb + (long)p0 * WL + nc);
@@ -621,12 +623,12 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
     T* bs = Bs0 + buf * BK * LDB_S;
 #pragma unroll
     for (int v = 0; v < NA; ++v) {
-      const int e = t + v * 256, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
+      const int e = t + v * NT, kr = e / (BM / VA), mc = (e % (BM / VA)) * VA;
       *reinterpret_cast<va_t*>(as + kr * LDA_S + mc) = ra[v];
     }
 #pragma unroll
     for (int v = 0; v < NB; ++v) {
-      const int e = t + v * 256, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
+      const int e = t + v * NT, kr = e / (BN / VB), nc = (e % (BN / VB)) * VB;
       *reinterpret_cast<vb_t*>(bs + kr * LDB_S + nc) = rb[v];
     }
   };
@@ -651,7 +653,7 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
     for (int kk = 0; kk < BK; kk += 4) {
       T af[FM], bf[FN];
 #pragma unroll
-      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * LDA_S + wm * 64 + i * 16 + row16];
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * LDA_S + wm * (BM / NWM) + i * 16 + row16];
 #pragma unroll
       for (int j = 0; j < FN; ++j)
         bf[j] = bs[(kk + kq) * LDB_S + wn * 16 * WL + j * 16 + row16];
@@ -676,12 +678,12 @@ __global__ void __launch_bounds__(256, 3) gemm1_mix_kernel(Gemm1MixArgs g) {
         for (int j = 0; j < FN; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            E[(wm * 64 + i * 16 + MF::row(kq, r)) * ELD + j * 16 + row16] = acc[i][j][r];
+            E[(wm * (BM / NWM) + i * 16 + MF::row(kq, r)) * ELD + j * 16 + row16] = acc[i][j][r];
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int pair = t + q * 256;          // (bb, pp), pp fastest: 32 b x 16 p
+    for (int q = 0; q < 512 / NT; ++q) {
+      const int pair = t + q * NT;           // (bb, pp), pp fastest: 32 b x 16 p
       const int bb = pair >> 4, pp = pair & 15;
       T x[WL][4];
 #pragma unroll
@@ -724,10 +726,13 @@ bool try_gemm1_mix(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
     return false;
   Gemm1MixArgs g{a, b, a, wr, theta, 4L * b, sth, LT, (long)a * wl, sLT, S, sS, X2, sX2};
   const dim3 grid(static_cast<unsigned>((b / 32) * (a / 32)), static_cast<unsigned>(batch));
+  // float32: 4 waves of 64 x 16 wl; float64: 8 waves of 32 x 16 wl (half the accumulator
+  // registers per lane, so two waves per SIMD fit)
+  constexpr int NWM = sizeof(T) == 4 ? 2 : 4;
   if (wl == 5)
-    hipLaunchKernelGGL((gemm1_mix_kernel<T, 5>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm1_mix_kernel<T, 5, NWM>), grid, dim3(NWM * 128), 0, st, g);
   else
-    hipLaunchKernelGGL((gemm1_mix_kernel<T, 3>), grid, dim3(256), 0, st, g);
+    hipLaunchKernelGGL((gemm1_mix_kernel<T, 3, NWM>), grid, dim3(NWM * 128), 0, st, g);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("gemm1_mix: launch failed: %s", hipGetErrorString(e));
@@ -761,9 +766,7 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   // GEMM1 and the MPO mix in one kernel when both run here and the shape fits it
   bool fused = false;
   if ((stages & 3) == 3 && tile1 == 0) {
-    // float32 only: the float64 form needs a wave per SIMD for its registers and loses to
-    // the direct-to-LDS GEMM1 plus the separate mix (0.99 vs 0.89 ms at chi = 1024)
-    if constexpr (std::is_same<TC, float>::value && std::is_same<TIO, float>::value) {
+    if constexpr (std::is_same<TC, TIO>::value && !real_of<TC>::cplx) {
       int rc = TNH_OK;
       fused = try_gemm1_mix<TC>(batch, a, d1, d2, b, wl, wr, theta, sth, LT, sLT, S, sS, X2,
                                 nx2, st, rc);
