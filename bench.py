@@ -551,6 +551,8 @@ def main():
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
     ap.add_argument("--no-block", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the chi=1024 sweep-time measurement (Heisenberg N=64, 3 sweeps)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -719,6 +721,16 @@ def main():
     setup_s = time.perf_counter() - t_setup
     block = {} if args.no_block else block_bench(max(5, args.steps // 2), args.warmup,
                                                  local=not args.no_local)
+    # DMRG sweep time at chi = 1024: tnkit's per-sweep cost study (Heisenberg N = 64, 3 sweeps
+    # from a random state at full chi, mean of sweeps 2-3), float64 against the mixed eigensolver
+    sweep = {}
+    if not args.no_sweep:
+        runs = sweep_cost(64, [args.chi], ["dense:lanczos-restart", "dense:lanczos-mixed",
+                                           "u1:lanczos-restart"])
+        sweep = {"workload": f"Heisenberg chain N=64, chi={args.chi}, 3 sweeps from random, "
+                             "s/sweep = mean of sweeps 2-3, trunc=gram",
+                 "reference_h100_3g_s_per_sweep": {"dense": 38.4, "u1": 15.7},
+                 "runs": runs}
 
     res = {
         "metric": METRIC, "value": fl / (ms32 * 1e-3) / 1e12, "unit": "TFLOP/s", "n_gpus": 1,
@@ -746,6 +758,7 @@ def main():
         "reference_h100_mig_fp64_tflops": 5.67,
         "roofline": roof,
         "block_u1": block,
+        "sweep_chi1024": sweep,
         "setup_s": setup_s,
     }
     if not args.no_cpu:
