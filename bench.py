@@ -726,7 +726,7 @@ def main():
     sweep = {}
     if not args.no_sweep:
         runs = sweep_cost(64, [args.chi], ["dense:lanczos-restart", "dense:lanczos-mixed",
-                                           "u1:lanczos-restart"])
+                                           "u1:lanczos-restart"], log=sys.stderr)
         sweep = {"workload": f"Heisenberg chain N=64, chi={args.chi}, 3 sweeps from random, "
                              "s/sweep = mean of sweeps 2-3, trunc=gram",
                  "reference_h100_3g_s_per_sweep": {"dense": 38.4, "u1": 15.7},

@@ -225,6 +225,19 @@ int tnh_grouped_gemm(int dtype, const long* desc, long ntiles, const int* tiles,
 int tnh_grouped_small(int dtype, const long* desc, long nchunks, const int* chunks, const void* A,
                       const void* B, void* C, tnh_stream_t stream);
 
+/* Batched eigendecomposition of small Hermitian (TNH_C128) or real symmetric (TNH_F64) matrices,
+ * 1 <= n <= 64: the eigh of tnkit's Gram truncation (tnkit/linalg.py:149, batched over the legs
+ * in tnkit/batch_tdvp.py:572-599). Matrix b is A + b*sA with row i at i*lda (both triangles read),
+ * plus add[b*sadd + j] on the diagonal when add is not NULL (real, may be NULL). Writes the
+ * eigenvalues ascending to w + b*sw and the eigenvectors as columns, Z[b*sZ + i*ldz + j] =
+ * component i of eigenvector j; Z may alias A. One workgroup per matrix, cyclic Jacobi in LDS;
+ * a matrix stops after a sweep without rotations or with off-diagonal norm <= tol * ||A||_F.
+ * sweeps (device ints, may be NULL): sweeps run per matrix, negative when max_sweeps ended it
+ * unconverged. */
+int tnh_eigh_jacobi(int dtype, long batch, int n, const void* A, long lda, long sA,
+                    const double* add, long sadd, double* w, long sw, void* Z, long ldz, long sZ,
+                    int max_sweeps, double tol, int* sweeps, tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,105 @@
+"""The batched small Hermitian eigensolver (`tnh_eigh_jacobi`, one workgroup per matrix, Jacobi in
+LDS) against numpy's eigh: eigenvalues to 1e-13 of the spectral radius, A Z = Z diag(w) and
+Z^H Z = 1 to 1e-12, on Gram matrices of the shapes the truncation meets (low rank, zero rows
+with the negative diagonal shifts of the padded legs, degenerate spectra, eigenvalues spread
+over 24 decades), complex128 and float64."""
+import numpy as np
+import pytest
+import torch
+
+from tnkit_amd import kernels, tdvp
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(n, cplx, rng):
+    def rnd(*s):
+        x = rng.standard_normal(s)
+        return x + 1j * rng.standard_normal(s) if cplx else x
+
+    out = []
+    X = rnd(n, 2 * n)
+    out.append(("psd", X @ X.conj().T, None))
+    r = max(1, n // 5)
+    X = rnd(n, r)
+    G = X @ X.conj().T
+    add = np.zeros(n)
+    z = max(0, n - 2 * r)
+    if z:
+        G[n - z:, :] = 0
+        G[:, n - z:] = 0
+        add[n - z:] = -(1.0 + np.arange(n - z, n) / n)
+    out.append(("lowrank+pad", G, add))
+    Q, _ = np.linalg.qr(rnd(n, n))
+    vals = np.repeat(np.arange(1.0, 5.0), (n + 3) // 4)[:n]
+    out.append(("degenerate", (Q * vals) @ Q.conj().T, None))
+    out.append(("spread", (Q * np.logspace(0, -24, n)) @ Q.conj().T, None))
+    out.append(("zero+shift", np.zeros((n, n)), -(1.0 + np.arange(n) / n)))
+    out.append(("diagonal", np.diag(rng.standard_normal(n)), None))
+    return [(name, 0.5 * (G + G.conj().T), add) for name, G, add in out]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 50, 63, 64])
+@pytest.mark.parametrize("cplx", [True, False])
+def test_eigh_jacobi_against_numpy(hip, n, cplx):
+    print(f"eigh_jacobi n={n} complex={cplx}", flush=True)
+    rng = np.random.default_rng(100 * n + cplx)
+    cases = _cases(n, cplx, rng)
+    dt = torch.complex128 if cplx else torch.float64
+    G = torch.as_tensor(np.stack([c[1] for c in cases])).to(hip, dt)
+    add = torch.as_tensor(np.stack([c[2] if c[2] is not None else np.zeros(n) for c in cases]),
+                          dtype=torch.float64, device=hip)
+    w, Z, sw = kernels.eigh_jacobi(G, add)
+    w, Z, sw = w.cpu().numpy(), Z.cpu().numpy(), sw.cpu().numpy()
+    for b, (name, A, a) in enumerate(cases):
+        A = A + np.diag(a if a is not None else np.zeros(n))
+        ref = np.linalg.eigvalsh(A)
+        scale = max(np.abs(ref).max(), 1e-300)
+        assert sw[b] > 0, (name, sw[b])
+        assert np.all(np.diff(w[b]) >= 0), name
+        assert np.abs(w[b] - ref).max() <= 1e-13 * scale, name
+        assert np.abs(A @ Z[b] - Z[b] * w[b]).max() <= 1e-12 * scale, name
+        assert np.abs(Z[b].conj().T @ Z[b] - np.eye(n)).max() <= 1e-12, name
+
+
+def test_eigh_jacobi_reports_unconverged(hip):
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((2, 30, 30))
+    G = torch.as_tensor(X @ X.transpose(0, 2, 1), device=hip)
+    _, _, sw = kernels.eigh_jacobi(G, max_sweeps=1)
+    assert sw.tolist() == [-1, -1]
+    w, _, sw = kernels.eigh_jacobi(G)
+    assert bool((sw > 1).all())
+    assert np.abs(w.cpu().numpy() - np.linalg.eigvalsh(X @ X.transpose(0, 2, 1))).max() < 1e-11
+
+
+def test_eigh_jacobi_refuses_large_and_wrong_dtype(hip):
+    with pytest.raises(ValueError):
+        kernels.eigh_jacobi(torch.zeros((1, 65, 65), dtype=torch.float64, device=hip))
+    with pytest.raises(TypeError):
+        kernels.eigh_jacobi(torch.zeros((1, 4, 4), dtype=torch.float32, device=hip))
+
+
+@pytest.mark.parametrize("iso", ["left", "right"])
+def test_gram_cut_same_with_jacobi_and_vendor(hip, iso, monkeypatch):
+    """The truncation of padded legs keeps the same product U S Vh and Schmidt values whichever
+    eigensolver runs (the legs' spectra are non-degenerate at the cut)."""
+    rng = np.random.default_rng(7)
+    k, m, n = 6, 40, 24
+    mat = rng.standard_normal((k, m, n)) + 1j * rng.standard_normal((k, m, n))
+    mat[1, :, 10:] = 0                      # a leg padded to the widest bond
+    mat[2, 30:, :] = 0
+    t = torch.as_tensor(mat, device=hip)
+    caps = np.array([8, 12, 5, 24, 3, 16])
+    ceil = np.full(k, 24)
+    out = {}
+    for flag in (True, False):
+        monkeypatch.setattr(tdvp, "_JACOBI", flag)
+        U, S, Vh, ranks, disc = tdvp._gram_cut(t, caps, ceil, 0.0, iso)
+        out[flag] = (torch.matmul(U * S[:, None, :].to(U.dtype), Vh).cpu().numpy(),
+                     S.cpu().numpy(), ranks, disc)
+    a, b = out[True], out[False]
+    assert np.array_equal(a[2], b[2])
+    assert np.abs(a[1] - b[1]).max() < 1e-11 * np.abs(b[1]).max()
+    assert np.abs(a[0] - b[0]).max() < 1e-10 * np.abs(b[0]).max()
+    assert np.abs(np.asarray(a[3]) - np.asarray(b[3])).max() < 1e-12

@@ -180,10 +180,10 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
         mats.append((M, Mt))
         grams.append(kernels.gemm_tn(Mt, Mt) if iso == "left" else kernels.gemm_tn(M, M))
     sizes = [g.shape[0] for g in grams]
-    # batched eigh per power-of-two size class
+    # batched eigh per power-of-two size class (classes up to 64 run on the Jacobi kernel)
     cls = {}
     for j, n in enumerate(sizes):
-        cls.setdefault(max(128, 1 << (int(n) - 1).bit_length()), []).append(j)
+        cls.setdefault(max(16, 1 << (int(n) - 1).bit_length()), []).append(j)
     w_all = [None] * len(grams)
     v_all = [None] * len(grams)
     for nmax, idx in cls.items():

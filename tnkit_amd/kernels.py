@@ -254,6 +254,34 @@ def rank_update(V: torch.Tensor, x: torch.Tensor, weight: float, y: torch.Tensor
     return y
 
 
+EIGH_JACOBI_MAX_N = 64
+
+
+def eigh_jacobi(G: torch.Tensor, add: Optional[torch.Tensor] = None, max_sweeps: int = 40,
+                tol: float = 1e-16) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Batched eigh of the Hermitian (k, n, n) complex128 or symmetric float64 G + diag(add),
+    n <= 64, on the device: (w ascending (k, n) float64, Z (k, n, n) with eigenvectors as
+    columns, sweeps (k,) int32, negative where a matrix did not converge in max_sweeps)."""
+    _need(G, "G", 3, (torch.complex128, torch.float64))
+    k, n, n2 = G.shape
+    if n != n2 or not 1 <= n <= EIGH_JACOBI_MAX_N:
+        raise ValueError(f"eigh_jacobi: square matrices of order 1..{EIGH_JACOBI_MAX_N}, got "
+                         f"{tuple(G.shape)}")
+    if add is not None:
+        _need(add, "add", 2, (torch.float64,))
+        if tuple(add.shape) != (k, n):
+            raise ValueError(f"eigh_jacobi: add {tuple(add.shape)} does not match G {tuple(G.shape)}")
+    w = torch.empty((k, n), dtype=torch.float64, device=G.device)
+    Z = torch.empty_like(G)
+    sw = torch.empty(k, dtype=torch.int32, device=G.device)
+    if k:
+        check(_lib.load().tnh_eigh_jacobi(dcode(G.dtype), k, n, ptr(G), n, n * n,
+                                          ptr(add) if add is not None else None, n, ptr(w), n,
+                                          ptr(Z), n, n * n, int(max_sweeps), float(tol), ptr(sw),
+                                          stream(G.device)), "eigh_jacobi")
+    return w, Z, sw
+
+
 def lincomb(V: torch.Tensor, coef: Sequence[float], out: torch.Tensor, beta: float = 0.0) -> torch.Tensor:
     """out = beta * out + sum_r coef[r] V[r] (coefficients from the host)."""
     _need(V, "V", 2, (torch.float64,))

@@ -222,10 +222,29 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
 
 def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None):
     """Batched eigh (ascending) of the Hermitian positive semidefinite G + diag(add) on the
-    device. rocSOLVER's batched Hermitian eigensolver returns non-finite eigenvectors for some
-    complex Gram matrices of low rank in wide padded bonds; those legs are decomposed instead by
-    an SVD of the unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it returns
-    ascending like eigh. Raises if a leg stays non-finite."""
+    device. Orders up to 64 go to the one-workgroup-per-matrix Jacobi kernel
+    (`kernels.eigh_jacobi`); larger ones, and any matrix the kernel leaves unconverged, to
+    rocSOLVER's batched eigh. That one returns non-finite eigenvectors for some complex Gram
+    matrices of low rank in wide padded bonds; those legs are decomposed instead by an SVD of the
+    unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it returns ascending like
+    eigh. Raises if a leg stays non-finite."""
+    if (G.shape[-1] <= kernels.EIGH_JACOBI_MAX_N and G.dtype in (torch.complex128, torch.float64)
+            and _JACOBI):
+        w, Z, sw = kernels.eigh_jacobi(G.contiguous(), None if add is None
+                                       else add.to(torch.float64).contiguous())
+        redo = (sw < 0) | (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
+        if not bool(redo.any()):
+            return w, Z
+        idx = torch.nonzero(redo).flatten()
+        w[idx], Z[idx] = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
+        return w, Z
+    return _eigh_robust_vendor(G, add)
+
+
+_JACOBI = os.environ.get("TNKIT_AMD_JACOBI", "1") != "0"
+
+
+def _eigh_robust_vendor(G: torch.Tensor, add: Optional[torch.Tensor] = None):
     Gs = G if add is None else G + torch.diag_embed(add.to(G.dtype))
     w, Z = torch.linalg.eigh(Gs)
     bad = (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
