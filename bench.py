@@ -671,7 +671,7 @@ def main():
     names = {1: "GEMM1 theta x L, unfused (narrow_kernel + gemm_tn_kernel<float,float,float,float>)",
              2: "MPO mix, unfused (mix_mfma_kernel<float,float,float>)",
              3: "GEMM1+mix theta x L with the MPO mix in the epilogue (narrow_kernel + "
-                "gemm1_mix_kernel<float,5>)",
+                "gemm1_mix_dma_kernel<5>)",
              4: "GEMM3 x R (gemm_dma_kernel<float,double,32>)"}
     dom = max((3, 4), key=stage_ms.get)
     if dom == 2:
