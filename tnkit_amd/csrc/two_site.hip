@@ -714,62 +714,67 @@ __global__ void __launch_bounds__(NWM * 128, NWM == 2 ? 3 : 1)
   }
 }
 
-// The float32 form with direct-to-LDS loads: the main loop of gemm_dma_kernel (tiles go from HBM
+// The same kernel with direct-to-LDS loads: the main loop of gemm_dma_kernel (tiles go from HBM
 // into LDS by global_load_lds_dwordx4, no VGPR staging) on the 128 x 32 WL tile of
-// gemm1_mix_kernel, BK = 32, two stages, so that two workgroups share a CU and one's epilogue
-// overlaps the other's main loop. A rows are rotated by 16 elements per (row mod 4) within their
-// 128 elements, B rows within their 32 WL (as gemm_dma_kernel does, for distinct LDS banks across
+// gemm1_mix_kernel, BK = 32, two stages. float32 runs 4 waves and two workgroups per CU, so that
+// one's epilogue overlaps the other's main loop; float64 runs 8 waves (one workgroup per CU: its
+// stages take 144 KB). A rows are rotated by 16 elements per (row mod 4) within their 128
+// elements, B rows within their 32 WL (as gemm_dma_kernel does, for distinct LDS banks across
 // the four k rows of a fragment read). The epilogue is that of gemm1_mix_kernel.
-template <int WL>
+template <typename T, int WL, int BK>
 struct Gemm1MixDmaSmem {
   struct Stages {
-    float a0[32 * 128], a1[32 * 128];
-    float b0[32 * 32 * WL], b1[32 * 32 * WL];
+    T a0[BK * 128], a1[BK * 128];
+    T b0[BK * 32 * WL], b1[BK * 32 * WL];
   };
   union {
     Stages st;
-    float E[128 * (16 * WL + 1)];
+    T E[128 * (16 * WL + 1)];
   };
 };
 
-template <int WL>
-__global__ void __launch_bounds__(256, 2) gemm1_mix_dma_kernel(Gemm1MixArgs g) {
-  using MF = Mfma<float>;
+template <typename T, int WL, int NWM>
+__global__ void __launch_bounds__(NWM * 128, NWM == 2 ? 2 : 1)
+    gemm1_mix_dma_kernel(Gemm1MixArgs g) {
+  using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
+  constexpr int NT = NWM * 128;
   constexpr int BM = 128, BN = 32 * WL, BK = 32;
-  constexpr int CPA = BM / 4, CPB = BN / 4;                    // 16-byte chunks per k row
-  constexpr int NJA = BK * CPA / 256, NJB = BK * CPB / 256;    // chunks per thread per tile
-  static_assert(NJA * 256 == BK * CPA && NJB * 256 == BK * CPB, "gemm1_mix_dma tile loads");
-  constexpr int FM = 4, FN = WL, ELD = 16 * WL + 1;
-  __shared__ __align__(16) Gemm1MixDmaSmem<WL> sm;
-  __shared__ float Ss[4 * 8 * 4 * WL];
+  constexpr int EPC = 16 / sizeof(T), ROT = 16 / EPC;         // elements per chunk; rotation
+  constexpr int CPA = BM / EPC, CPB = BN / EPC;               // 16-byte chunks per k row
+  constexpr int NJA = BK * CPA / NT, NJB = BK * CPB / NT;     // chunks per thread per tile
+  static_assert(NJA * NT == BK * CPA && NJB * NT == BK * CPB, "gemm1_mix_dma tile loads");
+  static_assert(3 * ROT <= CPB, "gemm1_mix_dma rotation");
+  constexpr int FM = BM / NWM / 16, FN = WL, ELD = 16 * WL + 1;
+  __shared__ __align__(16) Gemm1MixDmaSmem<T, WL, BK> sm;
+  __shared__ T Ss[4 * 8 * 4 * WL];
 
   const int B = g.B, P = g.P, K = g.K, WR = g.WR;
   int tm, tn;
   tile_of(blockIdx.x, B / 32, P / 32, tm, tn);
   const int b0 = tm * 32, p0 = tn * 32;
   const long bz = blockIdx.y;
-  const float* __restrict__ A = static_cast<const float*>(g.A) + bz * g.sA;
-  const float* __restrict__ Bm = static_cast<const float*>(g.Bm) + bz * g.sB;
-  const float* __restrict__ Sg = static_cast<const float*>(g.S) + bz * g.sS;
-  float* __restrict__ X2 = static_cast<float*>(g.X2) + bz * g.sX2;
+  const T* __restrict__ A = static_cast<const T*>(g.A) + bz * g.sA;
+  const T* __restrict__ Bm = static_cast<const T*>(g.Bm) + bz * g.sB;
+  const T* __restrict__ Sg = static_cast<const T*>(g.S) + bz * g.sS;
+  T* __restrict__ X2 = static_cast<T*>(g.X2) + bz * g.sX2;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1, row16 = lane & 15, kq = lane >> 4;
-  for (int e = t; e < 4 * WR * 4 * WL; e += 256) Ss[e] = Sg[e];
+  for (int e = t; e < 4 * WR * 4 * WL; e += NT) Ss[e] = Sg[e];
 
   // element offsets within a K block of this thread's chunks: chunk c lands at LDS position c
-  // and holds the chunk of its row rotated back by 4 chunks per (row mod 4)
+  // and holds the chunk of its row rotated back by ROT chunks per (row mod 4)
   long goffA[NJA], goffB[NJB];
 #pragma unroll
   for (int j = 0; j < NJA; ++j) {
-    const int c = j * 256 + t, r = c / CPA, p = c % CPA;
-    const int mc = 4 * ((p - 4 * (r & 3)) & (CPA - 1));
+    const int c = j * NT + t, r = c / CPA, p = c % CPA;
+    const int mc = EPC * ((p - ROT * (r & 3)) & (CPA - 1));
     goffA[j] = (long)r * g.lda + (long)(mc >> 5) * B + b0 + (mc & 31);
   }
 #pragma unroll
   for (int j = 0; j < NJB; ++j) {
-    const int c = j * 256 + t, r = c / CPB, p = c % CPB;
-    const int nc = 4 * ((p - 4 * (r & 3) + CPB) % CPB);
+    const int c = j * NT + t, r = c / CPB, p = c % CPB;
+    const int nc = EPC * ((p - ROT * (r & 3) + CPB) % CPB);
     goffB[j] = (long)r * g.ldb + (long)p0 * WL + nc;
   }
   acc_t acc[FM][FN];
@@ -778,23 +783,23 @@ __global__ void __launch_bounds__(256, 2) gemm1_mix_dma_kernel(Gemm1MixArgs g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
   const int nk = K / BK;
-  const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * 16 * WL + row16 + 16 * kq;
-  auto issue = [&](int kt, float* as, float* bs) __attribute__((always_inline)) {
-    const float* a = A + (long)kt * BK * g.lda;
-    const float* b = Bm + (long)kt * BK * g.ldb;
+  const int aoff = wm * (BM / NWM) + row16 + 16 * kq, boff = wn * 16 * WL + row16 + 16 * kq;
+  auto issue = [&](int kt, T* as, T* bs) __attribute__((always_inline)) {
+    const T* a = A + (long)kt * BK * g.lda;
+    const T* b = Bm + (long)kt * BK * g.ldb;
 #pragma unroll
     for (int j = 0; j < NJA; ++j)
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
-                                       (lds_void_t)(as + (j * 256 + wave * 64) * 4), 16, 0, 0);
+                                       (lds_void_t)(as + (j * NT + wave * 64) * EPC), 16, 0, 0);
 #pragma unroll
     for (int j = 0; j < NJB; ++j)
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
-                                       (lds_void_t)(bs + (j * 256 + wave * 64) * 4), 16, 0, 0);
+                                       (lds_void_t)(bs + (j * NT + wave * 64) * EPC), 16, 0, 0);
   };
-  auto compute = [&](const float* as, const float* bs) __attribute__((always_inline)) {
+  auto compute = [&](const T* as, const T* bs) __attribute__((always_inline)) {
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      float af[FM], bf[FN];
+      T af[FM], bf[FN];
 #pragma unroll
       for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * BM + ((aoff + i * 16) & (BM - 1))];
 #pragma unroll
@@ -833,7 +838,7 @@ __global__ void __launch_bounds__(256, 2) gemm1_mix_dma_kernel(Gemm1MixArgs g) {
   __syncthreads();
 
   // epilogue: two halves of 16 p (wave column h), each through the LDS tile E[128][16 WL]
-  float* E = sm.E;
+  T* E = sm.E;
   const long plane = (long)B * P * 4;  // X2 stride between channels r
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -844,32 +849,38 @@ __global__ void __launch_bounds__(256, 2) gemm1_mix_dma_kernel(Gemm1MixArgs g) {
         for (int j = 0; j < FN; ++j)
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            E[(wm * 64 + i * 16 + MF::row(kq, r)) * ELD + j * 16 + row16] = acc[i][j][r];
+            E[(wm * (BM / NWM) + i * 16 + MF::row(kq, r)) * ELD + j * 16 + row16] = acc[i][j][r];
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int pair = t + q * 256;          // (bb, pp), pp fastest: 32 b x 16 p
+    for (int q = 0; q < 512 / NT; ++q) {
+      const int pair = t + q * NT;           // (bb, pp), pp fastest: 32 b x 16 p
       const int bb = pair >> 4, pp = pair & 15;
-      float x[WL][4];
+      T x[WL][4];
 #pragma unroll
       for (int w = 0; w < WL; ++w)
 #pragma unroll
         for (int ij = 0; ij < 4; ++ij) x[w][ij] = E[(ij * 32 + bb) * ELD + pp * WL + w];
-      float* dst = X2 + ((long)(b0 + bb) * P + p0 + 16 * h + pp) * 4;
+      T* dst = X2 + ((long)(b0 + bb) * P + p0 + 16 * h + pp) * 4;
       for (int r = 0; r < WR; ++r) {
-        float y[4];
+        T y[4];
 #pragma unroll
         for (int st = 0; st < 4; ++st) {
-          const float* srow = Ss + (st * WR + r) * (4 * WL);
-          float s = 0.0f;
+          const T* srow = Ss + (st * WR + r) * (4 * WL);
+          T s = T(0);
 #pragma unroll
           for (int w = 0; w < WL; ++w)
 #pragma unroll
             for (int ij = 0; ij < 4; ++ij) s += srow[w * 4 + ij] * x[w][ij];
           y[st] = s;
         }
-        *reinterpret_cast<float4*>(dst + r * plane) = make_float4(y[0], y[1], y[2], y[3]);
+        T* o = dst + r * plane;
+        if constexpr (sizeof(T) == 4) {
+          *reinterpret_cast<float4*>(o) = make_float4(y[0], y[1], y[2], y[3]);
+        } else {
+          *reinterpret_cast<double2*>(o) = make_double2(y[0], y[1]);
+          *reinterpret_cast<double2*>(o + 2) = make_double2(y[2], y[3]);
+        }
       }
     }
     __syncthreads();
@@ -888,14 +899,17 @@ bool try_gemm1_mix(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
     return false;
   Gemm1MixArgs g{a, b, a, wr, theta, 4L * b, sth, LT, (long)a * wl, sLT, S, sS, X2, sX2};
   const dim3 grid(static_cast<unsigned>((b / 32) * (a / 32)), static_cast<unsigned>(batch));
-  if constexpr (std::is_same<T, float>::value) {
-    // direct-to-LDS form: 16-byte chunks of LT rows (a wl floats, a % 32 == 0), X2 in float4s
+  {
+    // direct-to-LDS form: 16-byte chunks of LT rows (a wl elements, a % 32 == 0), X2 in 16-byte
+    // stores; float32 4 waves of 64 x 16 wl, float64 8 waves of 32 x 16 wl
+    constexpr int NWD = sizeof(T) == 4 ? 2 : 4;
+    constexpr int EPC = 16 / sizeof(T);
     if (reinterpret_cast<uintptr_t>(LT) % 16 == 0 && reinterpret_cast<uintptr_t>(X2) % 16 == 0 &&
-        (batch == 1 || (sLT % 4 == 0 && sX2 % 4 == 0))) {
+        (batch == 1 || (sth % EPC == 0 && sLT % EPC == 0 && sX2 % EPC == 0))) {
       if (wl == 5)
-        hipLaunchKernelGGL(gemm1_mix_dma_kernel<5>, grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((gemm1_mix_dma_kernel<T, 5, NWD>), grid, dim3(NWD * 128), 0, st, g);
       else
-        hipLaunchKernelGGL(gemm1_mix_dma_kernel<3>, grid, dim3(256), 0, st, g);
+        hipLaunchKernelGGL((gemm1_mix_dma_kernel<T, 3, NWD>), grid, dim3(NWD * 128), 0, st, g);
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) {
         set_error("gemm1_mix_dma: launch failed: %s", hipGetErrorString(e));
