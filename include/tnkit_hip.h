@@ -238,6 +238,20 @@ int tnh_eigh_jacobi(int dtype, long batch, int n, const void* A, long lda, long 
                     const double* add, long sadd, double* w, long sw, void* Z, long ldz, long sZ,
                     int max_sweeps, double tol, int* sweeps, tnh_stream_t stream);
 
+/* Thin SVD A = U diag(S) Vh of small float64 (TNH_F64) or complex128 (TNH_C128) matrices with
+ * min(m, n) <= 64 and tnh_svd_jacobi_lds_bytes(dtype, m, n) <= 162816: the economy SVD of the
+ * backends (tnkit/backend.py:441-447, :784-800) that the 'svd' truncation calls
+ * (tnkit/_dmrg.py:116-155). Matrix b is A + b*sA with row i at i*lda; with k = min(m, n) it
+ * writes S + b*sS (k values, descending), U + b*sU (m x k, row stride ldu) and Vh + b*sV
+ * (k x n, row stride ldv). One workgroup per matrix, one-sided Jacobi in LDS; exactly zero
+ * singular values get an orthonormal completion, so U and Vh are always isometries. A matrix
+ * stops after a sweep without rotations (|b_p^H b_q| <= tol |b_p| |b_q| for every column pair).
+ * sweeps (device ints, may be NULL): sweeps run per matrix, negative when max_sweeps ended it. */
+size_t tnh_svd_jacobi_lds_bytes(int dtype, int m, int n);
+int tnh_svd_jacobi(int dtype, long batch, int m, int n, const void* A, long lda, long sA, double* S,
+                   long sS, void* U, long ldu, long sU, void* Vh, long ldv, long sV, int max_sweeps,
+                   double tol, int* sweeps, tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -2,7 +2,9 @@
 LDS) against numpy's eigh: eigenvalues to 1e-13 of the spectral radius, A Z = Z diag(w) and
 Z^H Z = 1 to 1e-12, on Gram matrices of the shapes the truncation meets (low rank, zero rows
 with the negative diagonal shifts of the padded legs, degenerate spectra, eigenvalues spread
-over 24 decades), complex128 and float64."""
+over 24 decades), complex128 and float64. The small thin SVD (`tnh_svd_jacobi`, one-sided
+Jacobi) against numpy's SVD on random, rank-deficient, zero, spread and degenerate matrices of both
+orientations."""
 import numpy as np
 import pytest
 import torch
@@ -103,3 +105,68 @@ def test_gram_cut_same_with_jacobi_and_vendor(hip, iso, monkeypatch):
     assert np.abs(a[1] - b[1]).max() < 1e-11 * np.abs(b[1]).max()
     assert np.abs(a[0] - b[0]).max() < 1e-10 * np.abs(b[0]).max()
     assert np.abs(np.asarray(a[3]) - np.asarray(b[3])).max() < 1e-12
+
+
+def _svd_cases(m, n, cplx, rng):
+    def rnd(*s):
+        x = rng.standard_normal(s)
+        return x + 1j * rng.standard_normal(s) if cplx else x
+
+    out = [("random", rnd(m, n))]
+    A = rnd(m, n)
+    if min(m, n) >= 3:                       # exact rank deficiency on the short side
+        if m >= n:
+            A[:, 0] = 0
+            A[:, 2] = 0
+        else:
+            A[0, :] = 0
+            A[2, :] = 0
+    out.append(("zero lines", A))
+    out.append(("rank one", np.outer(rnd(m), rnd(n))))
+    out.append(("zero", np.zeros((m, n)) * (1j if cplx else 1)))
+    r = min(m, n)
+    Q1, _ = np.linalg.qr(rnd(m, r))
+    Q2, _ = np.linalg.qr(rnd(n, r))
+    out.append(("spread", (Q1 * np.logspace(0, -24, r)) @ Q2.conj().T))
+    out.append(("degenerate", (Q1 * np.repeat([1.0, 0.5], (r + 1) // 2)[:r]) @ Q2.conj().T))
+    return out
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (5, 1), (7, 5), (5, 7), (32, 32), (64, 64),
+                                 (80, 64), (64, 80), (200, 16), (16, 200), (64, 33)])
+@pytest.mark.parametrize("cplx", [True, False])
+def test_svd_jacobi_against_numpy(hip, m, n, cplx):
+    """One-sided Jacobi SVD against numpy's: singular values to 1e-13 of the largest, U S Vh = A
+    to 1e-13, U and Vh isometries to 1e-12 (also with exactly zero singular values), one batched
+    launch over all cases."""
+    print(f"svd_jacobi {m}x{n} complex={cplx}", flush=True)
+    rng = np.random.default_rng(7 * m + n + cplx)
+    cases = _svd_cases(m, n, cplx, rng)
+    dt = torch.complex128 if cplx else torch.float64
+    A = torch.as_tensor(np.stack([c[1] for c in cases])).to(hip, dt)
+    U, S, Vh, sw = kernels.svd_jacobi(A)
+    U, S, Vh, sw = U.cpu().numpy(), S.cpu().numpy(), Vh.cpu().numpy(), sw.cpu().numpy()
+    r = min(m, n)
+    for b, (name, X) in enumerate(cases):
+        ref = np.linalg.svd(X, compute_uv=False)
+        scale = max(ref[0], 1e-300)
+        assert sw[b] > 0, (name, sw[b])
+        assert np.all(np.diff(S[b]) <= 0), name
+        assert np.abs(S[b] - ref).max() <= 1e-13 * scale, name
+        assert np.abs((U[b] * S[b]) @ Vh[b] - X).max() <= 1e-13 * scale, name
+        assert np.abs(U[b].conj().T @ U[b] - np.eye(r)).max() <= 1e-12, name
+        assert np.abs(Vh[b] @ Vh[b].conj().T - np.eye(r)).max() <= 1e-12, name
+
+
+def test_thin_svd_routes_and_falls_back(hip):
+    """thin_svd equals torch's SVD on a matrix the kernel takes and on one it does not."""
+    rng = np.random.default_rng(9)
+    for m, n in ((40, 30), (300, 100)):
+        X = rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))
+        U, S, Vh = kernels.thin_svd(torch.as_tensor(X, device=hip))
+        assert kernels.svd_jacobi_fits(m, n, torch.complex128) == (m == 40)
+        ref = np.linalg.svd(X, compute_uv=False)
+        assert np.abs(S.cpu().numpy() - ref).max() < 1e-12 * ref[0]
+        assert np.abs(((U * S.to(U.dtype)) @ Vh).cpu().numpy() - X).max() < 1e-12 * ref[0]
+    with pytest.raises(ValueError):
+        kernels.svd_jacobi(torch.zeros((100, 100), dtype=torch.complex128, device=hip))

@@ -248,3 +248,22 @@ def test_real_mpo_network_matvec(hip):
     op = kernels.fused_two_site(Ld, _t(mpo[3], hip), _t(mpo[4], hip), _t(Rs[5], hip))
     got = kernels.apply_two_site_fused(_t(th, hip), *op)
     assert _rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("wl", [3, 5])
+@pytest.mark.parametrize("low", [False, True])
+def test_batched_real_fused_matvec(hip, wl, low):
+    """k legs of the real matvec at shapes the fused GEMM1 + mix kernel takes (d = 2, a and b
+    multiples of 32), float64 and float32 operators, against the oracle per leg."""
+    print(f"batched real fused wl={wl} low={low}", flush=True)
+    rng = np.random.default_rng(40 + wl + 2 * low)
+    k, a, wm, wr, d, b = 3, 64, 4, 5, 2, 96
+    L, R = rng.standard_normal((k, a, wl, a)), rng.standard_normal((k, b, wr, b))
+    W1, W2 = rng.standard_normal((k, wl, wm, d, d)), rng.standard_normal((k, wm, wr, d, d))
+    th = rng.standard_normal((k, a, d, d, b))
+    op = kernels.fused_two_site_b(_t(L, hip), _t(W1, hip), _t(W2, hip), _t(R, hip),
+                                  dtype=torch.float32 if low else None)
+    got = kernels.apply_two_site_b(_t(th, hip), *op).cpu().numpy()
+    for j in range(k):
+        ref = D.apply_two_site_fused(th[j], *D.fused_two_site(L[j], W1[j], W2[j], R[j]))
+        assert _rel(got[j], ref) < (2e-5 if low else 1e-12)

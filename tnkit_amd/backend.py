@@ -166,7 +166,8 @@ class HipBackend:
 
     # --- factorisations -----------------------------------------------------------------------
     def svd(self, A):
-        return self.torch.linalg.svd(A, full_matrices=False)
+        from . import kernels
+        return kernels.thin_svd(A)
 
     def qr(self, A):
         return self.torch.linalg.qr(A, mode="reduced")

@@ -63,7 +63,7 @@ def _matrix_blocks(m: BlockTensor):
 
 def _svd(M: torch.Tensor, method: str):
     if method == "svd":
-        return torch.linalg.svd(M, full_matrices=False)
+        return kernels.thin_svd(M)
     if method != "gram":
         raise ValueError(f"unknown truncation {method!r}; expected 'svd' or 'gram'")
     # Gram route (tnkit/blocks.py:1345-1377): eigh of the smaller Gram matrix, then a QR of the

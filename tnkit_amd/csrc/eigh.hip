@@ -251,3 +251,269 @@ extern "C" int tnh_eigh_jacobi(int dtype, long batch, int n, const void* A, long
   return tnh::launch_jacobi<tnh::c128>(batch, n, A, lda, sA, add, sadd, w, sw, Z, ldz, sZ,
                                        max_sweeps, tol, sweeps, st);
 }
+
+namespace tnh {
+
+// ---- thin SVD of small matrices: one-sided (Hestenes) Jacobi -----------------------------------
+// For A (m x n), the work matrix B = A (m >= n) or A^H (m < n), M x N with N = min(m, n) <= 64,
+// sits in LDS column-major with the accumulated right rotations V (N x N). A sweep visits every
+// column pair once in the round-robin rounds of jacobi_eigh_kernel; a group of G lanes (a power
+// of two within one wave) owns one pair of a round: it forms |b_p|^2, |b_q|^2 and b_p^H b_q,
+// and when |b_p^H b_q| > tol |b_p| |b_q| rotates both columns of B and of V by the Jacobi
+// rotation of that 2 x 2 Gram matrix (orthogonalising them). Pairs of a round are disjoint, so a
+// round needs one barrier. At convergence (a sweep without rotations) the column norms are the
+// singular values, the normalised columns the left vectors of B, V its right vectors; columns
+// are orthogonal to tol relative to their own norms, so small singular values keep their
+// relative accuracy. Exactly zero columns (rank deficiency) get an orthonormal completion by
+// two Gram-Schmidt passes over unit vectors, so the returned U and Vh are always isometries.
+template <typename T>
+__device__ __forceinline__ T jshfl_xor(T v, int off);
+template <>
+__device__ __forceinline__ double jshfl_xor<double>(double v, int off) {
+  return __shfl_xor(v, off, 64);
+}
+template <>
+__device__ __forceinline__ c128 jshfl_xor<c128>(c128 v, int off) {
+  return {__shfl_xor(v.re, off, 64), __shfl_xor(v.im, off, 64)};
+}
+
+// LDS of one matrix: W (M x N2), V (N2 x N2) and N2 coefficients in T; N2 singular values,
+// ranks and flags; 64 doubles of reduction scratch
+__host__ __device__ constexpr size_t svd_jacobi_lds(int M, int N2, size_t elem) {
+  return ((size_t)M * N2 + (size_t)N2 * N2 + N2) * elem +
+         (size_t)N2 * (sizeof(double) + 2 * sizeof(int)) + 64 * sizeof(double);
+}
+constexpr size_t kSvdLdsMax = 163840 - 1024;   // 160 KB of LDS, less the static variables
+
+template <typename T>
+__global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
+    int m, int n, const T* __restrict__ A, long lda, long sA, double* __restrict__ S, long sS,
+    T* __restrict__ U, long ldu, long sU, T* __restrict__ Vh, long ldv, long sV, int max_sweeps,
+    double tol, int* __restrict__ sweeps) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const bool tr = m < n;
+  const int M = tr ? n : m, N = tr ? m : n, N2 = N + (N & 1), h = N2 / 2;
+  T* W = reinterpret_cast<T*>(smem_raw);               // M x N2, column j at W + j*M
+  T* V = W + (size_t)M * N2;                           // N2 x N2, column j at V + j*N2
+  T* coef = V + (size_t)N2 * N2;                       // N2 projection coefficients
+  double* sig = reinterpret_cast<double*>(coef + N2);
+  int* rank = reinterpret_cast<int*>(sig + N2);
+  int* valid = rank + N2;
+  double* red = reinterpret_cast<double*>(valid + N2);  // 64 doubles of scratch
+  __shared__ int nrot;
+  const int tid = threadIdx.x;
+  const long b = blockIdx.x;
+  const T* Ab = A + b * sA;
+
+  for (int idx = tid; idx < M * N2; idx += kJacThreads) {
+    const int j = idx / M, i = idx - j * M;
+    T v = T(0.0);
+    if (j < N) v = tr ? jconj(Ab[(long)j * lda + i]) : Ab[(long)i * lda + j];
+    W[idx] = v;
+  }
+  for (int idx = tid; idx < N2 * N2; idx += kJacThreads) {
+    const int j = idx / N2, i = idx - j * N2;
+    V[idx] = T(i == j ? 1.0 : 0.0);
+  }
+  // lanes per pair: the largest power of two <= min(64, 256 / h)
+  int G = 64;
+  while (G * h > kJacThreads) G >>= 1;
+  const int g = tid / G, gl = tid - g * G;
+  const bool active = g < h;
+
+  int sweep = 0;
+  bool done = false;
+  while (!done && sweep < max_sweeps) {
+    if (tid == 0) nrot = 0;
+    for (int r = 0; r < N2 - 1; ++r) {
+      __syncthreads();
+      if (active) {
+        const int pos_q = N2 - 1 - g;
+        const int p = g == 0 ? 0 : 1 + (g - 1 + r) % (N2 - 1);
+        const int q = 1 + (pos_q - 1 + r) % (N2 - 1);
+        T* wp = W + (size_t)p * M;
+        T* wq = W + (size_t)q * M;
+        double a = 0.0, d = 0.0;
+        T gm = T(0.0);
+        for (int i = gl; i < M; i += G) {
+          const T x = wp[i], y = wq[i];
+          a += jabs2(x);
+          d += jabs2(y);
+          gm = gm + jmul(jconj(x), y);
+        }
+        for (int off = G >> 1; off >= 1; off >>= 1) {
+          a += __shfl_xor(a, off, 64);
+          d += __shfl_xor(d, off, 64);
+          gm = gm + jshfl_xor(gm, off);
+        }
+        const double ag2 = jabs2(gm);
+        if (ag2 > 0.0 && ag2 > tol * tol * a * d) {
+          const double ag = sqrt(ag2);
+          const T e = jscale(1.0 / ag, jconj(gm));
+          const double tau = (d - a) / (2.0 * ag);
+          const double at = fabs(tau);
+          double t = at > 1e100 ? 0.5 / at : 1.0 / (at + sqrt(1.0 + tau * tau));
+          if (tau < 0.0) t = -t;
+          const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+          for (int i = gl; i < M; i += G) {
+            const T x = wp[i], y = jmul(wq[i], e);
+            wp[i] = jscale(cs, x) - jscale(sn, y);
+            wq[i] = jscale(sn, x) + jscale(cs, y);
+          }
+          T* vp = V + (size_t)p * N2;
+          T* vq = V + (size_t)q * N2;
+          for (int i = gl; i < N2; i += G) {
+            const T x = vp[i], y = jmul(vq[i], e);
+            vp[i] = jscale(cs, x) - jscale(sn, y);
+            vq[i] = jscale(sn, x) + jscale(cs, y);
+          }
+          if (gl == 0) atomicAdd(&nrot, 1);
+        }
+      }
+    }
+    __syncthreads();
+    ++sweep;
+    done = (nrot == 0);
+    __syncthreads();   // every thread has read nrot before the next sweep clears it
+  }
+
+  // singular values and descending order (ties by index)
+  for (int j = tid; j < N; j += kJacThreads) {
+    double s2 = 0.0;
+    for (int i = 0; i < M; ++i) s2 += jabs2(W[(size_t)j * M + i]);
+    sig[j] = sqrt(s2);
+  }
+  __syncthreads();
+  for (int j = tid; j < N; j += kJacThreads) {
+    const double sj = sig[j];
+    int rk = 0;
+    for (int i = 0; i < N; ++i) rk += (sig[i] > sj) || (sig[i] == sj && i < j);
+    rank[j] = rk;
+    valid[j] = sj > 0.0;
+  }
+  __syncthreads();
+  for (int idx = tid; idx < M * N; idx += kJacThreads) {
+    const int j = idx / M;
+    if (sig[j] > 0.0) W[idx] = jscale(1.0 / sig[j], W[idx]);
+  }
+  __syncthreads();
+  // orthonormal completion of exactly zero columns (uniform loops: sig and valid are in LDS)
+  for (int j = 0; j < N; ++j) {
+    if (sig[j] > 0.0) continue;
+    for (int cand = 0; cand < M; ++cand) {
+      // column j <- e_cand, then two passes of  x <- x - sum_k w_k (w_k^H x)  over valid k
+      for (int i = tid; i < M; i += kJacThreads) W[(size_t)j * M + i] = T(i == cand ? 1.0 : 0.0);
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        // coefficients c_k = w_k^H x over the valid columns, one thread per k
+        for (int k = tid; k < N; k += kJacThreads) {
+          T c = T(0.0);
+          if (valid[k])
+            for (int i = 0; i < M; ++i)
+              c = c + jmul(jconj(W[(size_t)k * M + i]), W[(size_t)j * M + i]);
+          coef[k] = c;
+        }
+        __syncthreads();
+        for (int i = tid; i < M; i += kJacThreads) {
+          T x = W[(size_t)j * M + i];
+          for (int k = 0; k < N; ++k)
+            if (valid[k]) x = x - jmul(W[(size_t)k * M + i], coef[k]);
+          W[(size_t)j * M + i] = x;
+        }
+        __syncthreads();
+      }
+      double part = 0.0;
+      for (int i = tid; i < M; i += kJacThreads) part += jabs2(W[(size_t)j * M + i]);
+      const double nrm2 = jac_block_sum(part, red);
+      if (nrm2 > 0.5 / M) {   // some e_cand keeps >= (M - rank) / M >= 1 / M of its norm
+        const double inv = 1.0 / sqrt(nrm2);
+        for (int i = tid; i < M; i += kJacThreads)
+          W[(size_t)j * M + i] = jscale(inv, W[(size_t)j * M + i]);
+        __syncthreads();
+        if (tid == 0) valid[j] = 1;
+        __syncthreads();
+        break;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+
+  // outputs: k = N singular triplets, descending
+  double* Sb = S + b * sS;
+  T* Ub = U + b * sU;
+  T* Vb = Vh + b * sV;
+  for (int j = tid; j < N; j += kJacThreads) Sb[rank[j]] = sig[j];
+  if (!tr) {
+    // U (m x N): column rank[j] = w_j; Vh (N x n): row rank[j] = v_j^H
+    for (int idx = tid; idx < M * N; idx += kJacThreads) {
+      const int j = idx / M, i = idx - j * M;
+      Ub[(long)i * ldu + rank[j]] = W[idx];
+    }
+    for (int idx = tid; idx < N * N; idx += kJacThreads) {
+      const int j = idx / N, c = idx - j * N;
+      Vb[(long)rank[j] * ldv + c] = jconj(V[(size_t)j * N2 + c]);
+    }
+  } else {
+    // A = V S W^H: U (m = N x N): column rank[j] = v_j; Vh (N x n = M): row rank[j] = w_j^H
+    for (int idx = tid; idx < N * N; idx += kJacThreads) {
+      const int j = idx / N, i = idx - j * N;
+      Ub[(long)i * ldu + rank[j]] = V[(size_t)j * N2 + i];
+    }
+    for (int idx = tid; idx < M * N; idx += kJacThreads) {
+      const int j = idx / M, c = idx - j * M;
+      Vb[(long)rank[j] * ldv + c] = jconj(W[idx]);
+    }
+  }
+  if (sweeps && tid == 0) sweeps[b] = done ? sweep : -sweep;
+}
+
+template <typename T>
+int launch_svd_jacobi(long batch, int m, int n, const void* A, long lda, long sA, double* S,
+                      long sS, void* U, long ldu, long sU, void* Vh, long ldv, long sV,
+                      int max_sweeps, double tol, int* sweeps, hipStream_t st) {
+  const int M = m > n ? m : n, N = m < n ? m : n, N2 = N + (N & 1);
+  const size_t smem = svd_jacobi_lds(M, N2, sizeof(T));
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&svd_jacobi_kernel<T>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
+  TNH_REQUIRE(e == hipSuccess, TNH_ERR_HIP, "tnh_svd_jacobi: %zu bytes of LDS: %s", smem,
+              hipGetErrorString(e));
+  hipLaunchKernelGGL(svd_jacobi_kernel<T>, dim3((unsigned)batch), dim3(kJacThreads), smem, st, m,
+                     n, static_cast<const T*>(A), lda, sA, S, sS, static_cast<T*>(U), ldu, sU,
+                     static_cast<T*>(Vh), ldv, sV, max_sweeps, tol, sweeps);
+  TNH_CHECK_LAUNCH("svd_jacobi");
+  return TNH_OK;
+}
+
+}  // namespace tnh
+
+extern "C" size_t tnh_svd_jacobi_lds_bytes(int dtype, int m, int n) {
+  const int M = m > n ? m : n, N = m < n ? m : n;
+  const size_t elem = dtype == TNH_C128 ? 16 : 8;
+  return tnh::svd_jacobi_lds(M, N + (N & 1), elem);
+}
+
+extern "C" int tnh_svd_jacobi(int dtype, long batch, int m, int n, const void* A, long lda,
+                              long sA, double* S, long sS, void* U, long ldu, long sU, void* Vh,
+                              long ldv, long sV, int max_sweeps, double tol, int* sweeps,
+                              tnh_stream_t stream) {
+  TNH_REQUIRE(dtype == TNH_F64 || dtype == TNH_C128, TNH_ERR_UNSUPPORTED,
+              "tnh_svd_jacobi: dtype %d (float64 or complex128 only)", dtype);
+  const int N = m < n ? m : n;
+  TNH_REQUIRE(m >= 1 && n >= 1 && N <= tnh::kJacMax, TNH_ERR_UNSUPPORTED,
+              "tnh_svd_jacobi: min(m, n) = %d outside 1..%d", N, tnh::kJacMax);
+  TNH_REQUIRE(tnh_svd_jacobi_lds_bytes(dtype, m, n) <= tnh::kSvdLdsMax, TNH_ERR_UNSUPPORTED,
+              "tnh_svd_jacobi: %d x %d needs %zu bytes of LDS (at most %zu)", m, n,
+              tnh_svd_jacobi_lds_bytes(dtype, m, n), tnh::kSvdLdsMax);
+  TNH_REQUIRE(batch >= 0 && batch < (1L << 31) && lda >= n && ldu >= N && ldv >= n && sA >= 0 &&
+                  sS >= 0 && sU >= 0 && sV >= 0 && max_sweeps >= 1 && tol > 0.0,
+              TNH_ERR_ARG, "tnh_svd_jacobi: bad sizes batch=%ld m=%d n=%d lda=%ld ldu=%ld ldv=%ld",
+              batch, m, n, lda, ldu, ldv);
+  if (batch == 0) return TNH_OK;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == TNH_F64)
+    return tnh::launch_svd_jacobi<double>(batch, m, n, A, lda, sA, S, sS, U, ldu, sU, Vh, ldv, sV,
+                                          max_sweeps, tol, sweeps, st);
+  return tnh::launch_svd_jacobi<tnh::c128>(batch, m, n, A, lda, sA, S, sS, U, ldu, sU, Vh, ldv,
+                                           sV, max_sweeps, tol, sweeps, st);
+}

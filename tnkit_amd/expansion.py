@@ -18,13 +18,14 @@ from typing import Tuple
 import torch
 
 from . import block_linalg as BL
+from . import kernels
 from .block_ops import BlockTensor, einsum as beinsum
 
 
 def _top(mat: torch.Tensor, k: int, rel_tol: float):
     """U, S, Vh of the k largest singular values above rel_tol times the largest, at least one
     (tnkit/expansion.py:35-50, dense branch)."""
-    U, S, Vh = torch.linalg.svd(mat, full_matrices=False)
+    U, S, Vh = kernels.thin_svd(mat)
     s = S.cpu()
     n = min(int(k), max(1, int((s > rel_tol * s[0]).sum())))
     return U[:, :n], S[:n], Vh[:n]

@@ -7,6 +7,7 @@ mismatch raises ValueError/TypeError instead of reaching a kernel.
 from __future__ import annotations
 
 import ctypes
+import os
 from typing import Optional, Sequence, Tuple
 
 import torch
@@ -280,6 +281,60 @@ def eigh_jacobi(G: torch.Tensor, add: Optional[torch.Tensor] = None, max_sweeps:
                                           ptr(Z), n, n * n, int(max_sweeps), float(tol), ptr(sw),
                                           stream(G.device)), "eigh_jacobi")
     return w, Z, sw
+
+
+SVD_JACOBI_LDS_MAX = 163840 - 1024
+
+
+def svd_jacobi_fits(m: int, n: int, dtype: torch.dtype) -> bool:
+    """Whether `svd_jacobi` takes an m x n matrix of this dtype (min(m, n) <= 64 and the work
+    matrix with its rotations in one workgroup's LDS)."""
+    if dtype not in (torch.float64, torch.complex128) or min(m, n) < 1 or min(m, n) > 64:
+        return False
+    return _lib.load().tnh_svd_jacobi_lds_bytes(dcode(dtype), m, n) <= SVD_JACOBI_LDS_MAX
+
+
+def svd_jacobi(A: torch.Tensor, max_sweeps: int = 60, tol: Optional[float] = None
+               ) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor, torch.Tensor]:
+    """Thin SVD of an (m, n) or batched (k, m, n) float64 / complex128 matrix on the device by
+    one-sided Jacobi: (U (.., m, r), S (.., r) descending, Vh (.., r, n), sweeps), r = min(m, n),
+    sweeps negative where a matrix did not converge in max_sweeps. U and Vh are isometries
+    (exactly zero singular values get an orthonormal completion)."""
+    two = A.dim() == 2
+    Ab = A.unsqueeze(0) if two else A
+    _need(Ab, "A", 3, (torch.float64, torch.complex128))
+    k, m, n = Ab.shape
+    if not svd_jacobi_fits(m, n, A.dtype):
+        raise ValueError(f"svd_jacobi: {m} x {n} {A.dtype} does not fit one workgroup")
+    r = min(m, n)
+    tol = 4e-16 * max(m, n) ** 0.5 if tol is None else float(tol)
+    S = torch.empty((k, r), dtype=torch.float64, device=A.device)
+    U = torch.empty((k, m, r), dtype=A.dtype, device=A.device)
+    Vh = torch.empty((k, r, n), dtype=A.dtype, device=A.device)
+    sw = torch.empty(k, dtype=torch.int32, device=A.device)
+    if k:
+        check(_lib.load().tnh_svd_jacobi(dcode(A.dtype), k, m, n, ptr(Ab), n, m * n, ptr(S), r,
+                                         ptr(U), r, m * r, ptr(Vh), n, r * n, int(max_sweeps),
+                                         tol, ptr(sw), stream(A.device)), "svd_jacobi")
+    if two:
+        return U[0], S[0], Vh[0], sw
+    return U, S, Vh, sw
+
+
+def thin_svd(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """torch.linalg.svd(A, full_matrices=False) for a device matrix: the Jacobi kernel when the
+    matrix fits it (a small SVD there costs tens of microseconds against about a millisecond in
+    rocSOLVER), rocSOLVER otherwise or for a matrix the kernel leaves unconverged."""
+    if (A.is_cuda and A.dim() in (2, 3) and _JACOBI_SVD
+            and svd_jacobi_fits(int(A.shape[-2]), int(A.shape[-1]), A.dtype)):
+        Ac = A.resolve_conj().contiguous()
+        U, S, Vh, sw = svd_jacobi(Ac)
+        if not bool((sw <= 0).any()):
+            return U, S, Vh
+    return torch.linalg.svd(A, full_matrices=False)
+
+
+_JACOBI_SVD = os.environ.get("TNKIT_AMD_JACOBI", "1") != "0"
 
 
 def lincomb(V: torch.Tensor, coef: Sequence[float], out: torch.Tensor, beta: float = 0.0) -> torch.Tensor:

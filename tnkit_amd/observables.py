@@ -88,7 +88,7 @@ def _spectra(M: List[torch.Tensor], upto: int) -> List[np.ndarray]:
     for i in range(upto):
         A = M[i] if carry is None else torch.tensordot(carry, M[i], dims=([1], [0]))
         Dl, d, Dr = A.shape
-        _, S, Vh = torch.linalg.svd(A.reshape(Dl * d, Dr), full_matrices=False)
+        _, S, Vh = kernels.thin_svd(A.reshape(Dl * d, Dr))
         out.append(S)
         carry = S.to(Vh.dtype)[:, None] * Vh
     sizes = [int(s.numel()) for s in out]

@@ -75,7 +75,7 @@ def two_site_step(M, L, R, W, i: int, cap: int, dt: float, direction: str, solve
     th = (M[i].reshape(a * s1, c) @ M[i + 1].reshape(c, s2 * b)).reshape(a, s1, s2, b)
     mv = _matvec(kernels.fused_two_site(L[i], W[i], W[i + 1], R[i + 2]), (a, s1, s2, b))
     th = solve(mv, th.contiguous(), -0.5j * dt).reshape(a * s1, s2 * b)
-    U, S, Vh = torch.linalg.svd(th, full_matrices=False)
+    U, S, Vh = kernels.thin_svd(th)
     s_host = S.cpu().numpy().astype(np.float64)
     tot = float(np.sum(s_host ** 2)) or 1.0
     k, pinned, disc = cut(s_host, cap, svd_tol, degenerate_tol, 0.0, tot)
