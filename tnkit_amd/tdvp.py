@@ -120,14 +120,45 @@ def _krylov_graph(key, op, make_mv, k: int, m: int, n: int, dev):
     return entry
 
 
+def _expm_tridiag(ab: torch.Tensor, beta0: torch.Tensor, tau: complex, m: int):
+    """Coefficients c (k, m) of exp(tau T_leg) beta0 e_0 in the Krylov basis and each leg's Saad
+    estimate |beta_m tau [e^{tau T}]_{m-1,0}| beta0, from the k tridiagonals T (alphas then betas
+    in `ab`), all on the device: one batched eigendecomposition of the k tridiagonals
+    (tnkit/batch_tdvp.py:164-168, 'eigh' on the device; the Jacobi kernel up to m = 64). A
+    tridiagonal the kernel leaves unconverged gets an infinite estimate, which the per-step
+    residual check turns into a larger m or an error."""
+    k = ab.shape[0]
+    al, be = ab[:, :m], ab[:, m:2 * m]
+    T = torch.diag_embed(al)
+    if m > 1:
+        off = be[:, :m - 1]
+        T = T + torch.diag_embed(off, 1) + torch.diag_embed(off, -1)
+    T = T.contiguous()
+    if m <= kernels.EIGH_JACOBI_MAX_N and _JACOBI:
+        lam, Z, sw = kernels.eigh_jacobi(T)
+        ok = sw > 0
+    else:
+        lam, Z = torch.linalg.eigh(T)
+        ok = torch.ones(k, dtype=torch.bool, device=T.device)
+    Zc = Z.to(torch.complex128)
+    ph = torch.exp(tau * lam.to(torch.complex128))
+    e0 = torch.matmul(Zc, (ph * Zc[:, 0, :]).unsqueeze(-1))[..., 0]      # [e^{tau T}]_{:, 0}
+    c = beta0.to(torch.complex128)[:, None] * e0
+    est = be[:, m - 1] * abs(tau) * e0[:, m - 1].abs() * beta0
+    est = torch.where(ok, est, torch.full_like(est, float("inf")))
+    return c, est
+
+
 def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
-                               resid: Optional[np.ndarray] = None, graph=None) -> torch.Tensor:
+                               resid=None, graph=None) -> torch.Tensor:
     """exp(tau H_leg) v_leg for k legs at once, from a fixed m Krylov steps
     (tnkit/batch_tdvp.py:247-324, tnkit/eigensolver.py:95-217): CGS2 with alpha from the first
     projection, clamped division instead of a breakdown branch, exp of the k tridiagonals by
-    their eigendecompositions. `v` is (k, n) complex128. `resid` (length k) receives each leg's
-    Saad a-posteriori error estimate |beta_m tau [e^{tau T}]_{m-1,0}| beta0. `graph` =
-    (key, operator tensors, make_mv) lets the Krylov loop run as a captured hipGraph."""
+    their eigendecompositions on the device, so a solve reads nothing back. `v` is (k, n)
+    complex128. `resid` (length k: a device float64 tensor, or a numpy array, which costs a host
+    read) is raised to each leg's Saad a-posteriori error estimate
+    |beta_m tau [e^{tau T}]_{m-1,0}| beta0. `graph` = (key, operator tensors, make_mv) lets the
+    Krylov loop run as a captured hipGraph."""
     k, n = v.shape
     m = max(1, min(int(m), n))
     dev = v.device
@@ -150,22 +181,13 @@ def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
         for j in range(m):
             w = matvec_b(V[:, j]).reshape(k, n).to(torch.complex128).contiguous()
             kernels.expm_step_c(V, w, ab, j, ws)
-    abh = ab.cpu().numpy()
-    b0 = beta0.cpu().numpy()
-    al, be = abh[:, :m], abh[:, m:]
-    T = np.zeros((k, m, m))
-    idx = np.arange(m)
-    T[:, idx, idx] = al
-    if m > 1:
-        T[:, idx[:-1], idx[1:]] = be[:, :m - 1]
-        T[:, idx[1:], idx[:-1]] = be[:, :m - 1]
-    lam, Z = np.linalg.eigh(T)
-    E = np.einsum("kij,kj,klj->kil", Z, np.exp(tau * lam), Z)   # (k, m, m)
-    c = b0[:, None] * E[:, :, 0]
-    if resid is not None:
-        resid[:] = np.maximum(resid, np.abs(be[:, m - 1] * tau * E[:, m - 1, 0]) * b0)
+    c, est = _expm_tridiag(ab, beta0, tau, m)
+    if isinstance(resid, torch.Tensor):
+        torch.maximum(resid, est, out=resid)
+    elif resid is not None:
+        resid[:] = np.maximum(resid, est.cpu().numpy())
     out = torch.empty((k, n), dtype=torch.complex128, device=dev)
-    kernels.lincomb_c(V, torch.as_tensor(c).to(dev), out, m)
+    kernels.lincomb_c(V, c, out, m)
     return out
 
 
@@ -220,25 +242,38 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
     return M
 
 
-def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None):
+def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None, deferred: bool = False):
     """Batched eigh (ascending) of the Hermitian positive semidefinite G + diag(add) on the
     device. Orders up to 64 go to the one-workgroup-per-matrix Jacobi kernel
     (`kernels.eigh_jacobi`); larger ones, and any matrix the kernel leaves unconverged, to
     rocSOLVER's batched eigh. That one returns non-finite eigenvectors for some complex Gram
     matrices of low rank in wide padded bonds; those legs are decomposed instead by an SVD of the
     unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it returns ascending like
-    eigh. Raises if a leg stays non-finite."""
+    eigh. Raises if a leg stays non-finite. With `deferred`, a Jacobi result is returned with
+    the device mask of legs to redo, (w, Z, redo), and the caller, which reads w back anyway,
+    finishes with `_eigh_redo`; otherwise (w, Z) after the check here."""
     if (G.shape[-1] <= kernels.EIGH_JACOBI_MAX_N and G.dtype in (torch.complex128, torch.float64)
             and _JACOBI):
         w, Z, sw = kernels.eigh_jacobi(G.contiguous(), None if add is None
                                        else add.to(torch.float64).contiguous())
         redo = (sw < 0) | (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
+        if deferred:
+            return w, Z, redo
         if not bool(redo.any()):
             return w, Z
         idx = torch.nonzero(redo).flatten()
         w[idx], Z[idx] = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
         return w, Z
-    return _eigh_robust_vendor(G, add)
+    w, Z = _eigh_robust_vendor(G, add)
+    return (w, Z, None) if deferred else (w, Z)
+
+
+def _eigh_redo(G: torch.Tensor, add: Optional[torch.Tensor], w: torch.Tensor, Z: torch.Tensor,
+               redo: np.ndarray) -> None:
+    """Replace in place the legs of a deferred `_eigh_robust` that the Jacobi kernel left
+    unconverged or non-finite (host mask) by the vendor eigensolver's."""
+    idx = torch.as_tensor(np.nonzero(redo)[0], device=G.device)
+    w[idx], Z[idx] = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
 
 
 _JACOBI = os.environ.get("TNKIT_AMD_JACOBI", "1") != "0"
@@ -282,11 +317,20 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     r_ = G.shape[1]
     dg = torch.diagonal(G, dim1=1, dim2=2).real
     shift = -(1.0 + torch.arange(r_, device=G.device, dtype=torch.float64) / r_)
-    w, Z = _eigh_robust(G, torch.where(dg == 0, shift[None, :], torch.zeros_like(dg)))
+    add = torch.where(dg == 0, shift[None, :], torch.zeros_like(dg))
+    w, Z, redo = _eigh_robust(G, add, deferred=True)
+    # one host read: the spectra, and which legs the Jacobi kernel asks to redo
+    both = torch.cat([w, redo.to(w.dtype)[:, None]], 1) if redo is not None else w
+    both = both.cpu().numpy()
+    if redo is not None and both[:, -1].any():
+        _eigh_redo(G, add, w, Z, both[:, -1] > 0)
+        both = w.cpu().numpy()
+    elif redo is not None:
+        both = both[:, :-1]
     w = torch.flip(w, dims=[1]).clamp(min=0.0)
     Z = torch.flip(Z, dims=[2])
     r = min(m, n)
-    wh = w.cpu().numpy()
+    wh = np.clip(both[:, ::-1], 0.0, None)
     tot = wh.sum(axis=1)
     tot = np.where(tot > 0, tot, 1.0)
     tail = np.cumsum(wh[:, ::-1], axis=1)[:, ::-1]
@@ -312,7 +356,7 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
         # kept columns come first, so their Q and R do not see the fill, and the fill's rows of
         # R meet the zero rows of Z^H below.
         AZ = torch.matmul(mat, Zk)
-        if bool((~mk).any()):
+        if bool((ranks < nk).any()):                   # host ranks: no device read
             gen = torch.Generator(device=mat.device)
             gen.manual_seed(12345)
             fill = torch.randn(AZ.shape, generator=gen, device=mat.device, dtype=torch.float64)
@@ -365,7 +409,7 @@ def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool):
 
 
 def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, moving: str,
-          resid: np.ndarray, low: bool) -> np.ndarray:
+          resid: torch.Tensor, low: bool) -> np.ndarray:
     k, d = Bt.k, Bt.d
     A, Bm = Bt.M[i], Bt.M[i + 1]
     a, b = A.shape[1], Bm.shape[3]
@@ -472,11 +516,12 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
     disc_tot = np.zeros(k)
     resid_max = np.zeros(k)
     for step in range(n_steps):
-        resid = np.zeros(k)
+        resid_d = torch.zeros(k, dtype=torch.float64, device=dev)
         for i in range(0, N - 1):
-            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "right", resid, o.low_precision)
+            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "right", resid_d, o.low_precision)
         for i in range(N - 2, -1, -1):
-            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "left", resid, o.low_precision)
+            disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "left", resid_d, o.low_precision)
+        resid = resid_d.cpu().numpy()                 # the step's one read of the residuals
         resid_max = np.maximum(resid_max, resid)
         bad = np.nonzero(resid > o.krylov_tol)[0]
         if len(bad):
