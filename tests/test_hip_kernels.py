@@ -55,16 +55,17 @@ def test_gemm_tn_direct_to_lds(hip, M, N, K, dtype):
     B = rng.standard_normal((nb, K, N))
     ref = np.einsum("bkm,bkn->bmn", A, B)
     Ad, Bd = _t(A, hip, dtype), _t(B, hip, dtype)
-    C = torch.full((nb, M, N), 0.5, dtype=dtype, device=hip)
     lib = _lib.load()
-    for acc in (0, 1):
-        rc = lib.tnh_gemm_tn(_lib.dcode(dtype), M, N, K, Ad.data_ptr(), M, Bd.data_ptr(), N,
-                             C.data_ptr(), N, nb, K * M, K * N, M * N, acc, 0,
-                             kernels.stream(hip))
-        assert rc == 0, lib.tnh_last_error()
-        want = ref if acc == 0 else 2 * ref
-        tol = 1e-13 if dtype == torch.float64 else 2e-6 * np.sqrt(K)
-        assert _rel(C, want) < tol
+    for tile in (0, 7):                       # 7: the 8-wave form (the float64 default)
+        C = torch.full((nb, M, N), 0.5, dtype=dtype, device=hip)
+        for acc in (0, 1):
+            rc = lib.tnh_gemm_tn(_lib.dcode(dtype), M, N, K, Ad.data_ptr(), M, Bd.data_ptr(), N,
+                                 C.data_ptr(), N, nb, K * M, K * N, M * N, acc, tile,
+                                 kernels.stream(hip))
+            assert rc == 0, lib.tnh_last_error()
+            want = ref if acc == 0 else 2 * ref
+            tol = 1e-13 if dtype == torch.float64 else 2e-6 * np.sqrt(K)
+            assert _rel(C, want) < tol
 
 
 @pytest.mark.parametrize("M,N,K,tile", [(1, 1, 1, 0), (17, 33, 9, 0), (128, 128, 128, 1),

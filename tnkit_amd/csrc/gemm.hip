@@ -54,7 +54,7 @@ int launch_one(const GemmArgs& g, int batch, hipStream_t stream) {
 // and the shape and strides fit it: float64 always (0.74 vs 0.79 ms on the chi = 1024 matvec
 // GEMMs), float32 when there are at most two 128 x 128 tiles per CU (a long K, as the third
 // matvec GEMM: 0.36 vs 0.43 ms); with more tiles the staged kernel keeps up and leads slightly.
-template <typename TC, typename TA, typename TB, typename TO>
+template <typename TC, typename TA, typename TB, typename TO, int WNC = 2>
 bool try_dma(const GemmArgs& g, int batch, hipStream_t stream, int& rc) {
   if constexpr (std::is_same<TA, TC>::value && std::is_same<TB, TC>::value &&
                 (std::is_same<TC, float>::value || std::is_same<TC, double>::value)) {
@@ -68,8 +68,8 @@ bool try_dma(const GemmArgs& g, int batch, hipStream_t stream, int& rc) {
     const long tiles = (long)(g.M / 128) * (g.N / 128);
     if (std::is_same<TC, float>::value && tiles * batch > 512) return false;
     if (tiles >= (1L << 31) || batch > 65535) return false;
-    hipLaunchKernelGGL((gemm_dma_kernel<TC, TO, BK>), dim3(static_cast<unsigned>(tiles), batch),
-                       dim3(256), 0, stream, g);
+    hipLaunchKernelGGL((gemm_dma_kernel<TC, TO, BK, WNC>),
+                       dim3(static_cast<unsigned>(tiles), batch), dim3(WNC * 128), 0, stream, g);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       set_error("gemm_tn (dma): launch failed: %s", hipGetErrorString(e));
@@ -88,7 +88,8 @@ bool try_dma(const GemmArgs& g, int batch, hipStream_t stream, int& rc) {
 
 // Tile shapes (tile_hint): 1 = 128x128 (4 waves of 64x64, K step 16 doubles / 32 floats),
 // 2 = 64x64, 3 = 32x32, 4 = 16x128 (skinny M such as the thick-restart rotation),
-// 5 = 128x128 with the short K step (8 doubles / 16 floats), 6 = 256x128 (8 waves).
+// 5 = 128x128 with the short K step (8 doubles / 16 floats), 6 = 256x128 (8 waves),
+// 7 = the direct-to-LDS 128x128 kernel with 8 waves of 64x32 (when the operands fit it).
 // 0 picks by dtype and tile count (below), and the skinny tile when M <= 16.
 template <typename TC, typename TA, typename TB, typename TO>
 int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hint) {
@@ -96,7 +97,10 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
   int cfg = tile_hint;
   if (cfg == 0) {
     int rc = TNH_OK;
-    if (g.M > 0 && g.N > 0 && try_dma<TC, TA, TB, TO>(g, batch, stream, rc)) return rc;
+    // float64: 8 waves of 64 x 32 (two per SIMD; 0.64 vs 0.70 ms on the chi = 1024 GEMM3);
+    // float32: 4 waves of 64 x 64 (8 waves measured no faster)
+    constexpr int kWnc = sizeof(TC) == 8 ? 4 : 2;
+    if (g.M > 0 && g.N > 0 && try_dma<TC, TA, TB, TO, kWnc>(g, batch, stream, rc)) return rc;
     const long b = batch > 0 ? batch : 1;
     const long t128 = ceil_div(g.M, 128) * ceil_div(g.N, 128) * b;
     const long t64 = ceil_div(g.M, 64) * ceil_div(g.N, 64) * b;
@@ -122,6 +126,11 @@ int launch_gemm_tn(const GemmArgs& g, int batch, hipStream_t stream, int tile_hi
     case 4: return launch_one<TC, TA, TB, TO, 16, 128, 16, 1, 4>(g, batch, stream);
     case 5: return launch_one<TC, TA, TB, TO, 128, 128, BKs, 2, 2>(g, batch, stream);
     case 6: return launch_one<TC, TA, TB, TO, 256, 128, BKs, 4, 2>(g, batch, stream);
+    case 7: {
+      int rc = TNH_OK;
+      if (g.M > 0 && g.N > 0 && try_dma<TC, TA, TB, TO, 4>(g, batch, stream, rc)) return rc;
+      return launch_one<TC, TA, TB, TO, 128, 128, BKb, 2, 2>(g, batch, stream);
+    }
     default: set_error("gemm_tn: unknown tile config %d", cfg); return TNH_ERR_ARG;
   }
 }

@@ -228,7 +228,8 @@ __global__ void __launch_bounds__(WM* WN * 64)
 // fragment reads of another and inserts no wait between them; the waits on the tile being
 // consumed are explicit vmcnt counts before the workgroup barrier. Rows are unpadded (the DMA
 // writes contiguous 1 KB per wave) and rotated by 16 elements per (row mod 4), so the four k rows
-// of one fragment read fall in distinct bank ranges. 128 x 128 tiles, 4 waves of 64 x 64.
+// of one fragment read fall in distinct bank ranges. 128 x 128 tiles, 2 x WNC waves of
+// 64 x (128 / WNC) (WNC = 2: 4 waves of 64 x 64; WNC = 4: 8 waves of 64 x 32, two per SIMD).
 // Requires M, N % 128 == 0, K % BK == 0, 16-byte aligned operands with lda, ldb and batch strides
 // multiples of a 16-byte chunk (the launcher checks).
 typedef __attribute__((address_space(3))) void* lds_void_t;
@@ -260,16 +261,17 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((NW & 15) | ((NW >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
 
-template <typename T, typename TO, int BK>
-__global__ void __launch_bounds__(256) gemm_dma_kernel(GemmArgs g) {
+template <typename T, typename TO, int BK, int WNC = 2>
+__global__ void __launch_bounds__(WNC * 128) gemm_dma_kernel(GemmArgs g) {
   using MF = Mfma<T>;
   using acc_t = typename MF::acc_t;
-  constexpr int BM = 128, FM = 4, FN = 4;
+  constexpr int NT = WNC * 128;        // 2 x WNC waves of 64 x (128 / WNC)
+  constexpr int BM = 128, FM = 4, FN = 128 / WNC / 16, WCOL = 128 / WNC;
   constexpr int EPC = 16 / sizeof(T);  // elements per 16-byte chunk
   constexpr int CPR = BM / EPC;        // chunks per tile row
-  constexpr int NJ = BK * CPR / 256;   // chunks per thread per operand tile
+  constexpr int NJ = BK * CPR / NT;    // chunks per thread per operand tile
   constexpr int ROT = 16 / EPC;        // the 16-element row rotation, in chunks
-  static_assert(NJ * 256 == BK * CPR && BK % 4 == 0, "dma tile shape");
+  static_assert(NJ * NT == BK * CPR && BK % 4 == 0, "dma tile shape");
   __shared__ __align__(16) DmaStages<T, BK> sm;
   const int M = g.M, N = g.N, K = g.K;
   int tm, tn;
@@ -280,13 +282,13 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(GemmArgs g) {
   const T* __restrict__ B = static_cast<const T*>(g.B) + bz * g.strideB;
   TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = wave >> 1, wn = wave & 1, row16 = lane & 15, kq = lane >> 4;
+  const int wm = wave / WNC, wn = wave % WNC, row16 = lane & 15, kq = lane >> 4;
   // element offsets, within a tile, of this thread's chunks (chunk c of the tile lands at LDS
   // position c; it holds the chunk of its row rotated back by ROT * (row mod 4))
   long goffA[NJ], goffB[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int c = j * 256 + t;
+    const int c = j * NT + t;
     const int r = c / CPR, p = c % CPR;
     const int q = (p - ROT * (r & 3)) & (CPR - 1);
     goffA[j] = (long)r * g.lda + m0 + EPC * q;
@@ -298,16 +300,16 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(GemmArgs g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
   const int nk = K / BK;
-  const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * 64 + row16 + 16 * kq;
+  const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * WCOL + row16 + 16 * kq;
   auto issue = [&](int kt, T* as, T* bs) {
     const T* a = A + (long)kt * BK * g.lda;
     const T* b = B + (long)kt * BK * g.ldb;
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
-                                       (lds_void_t)(as + (j * 256 + wave * 64) * EPC), 16, 0, 0);
+                                       (lds_void_t)(as + (j * NT + wave * 64) * EPC), 16, 0, 0);
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
-                                       (lds_void_t)(bs + (j * 256 + wave * 64) * EPC), 16, 0, 0);
+                                       (lds_void_t)(bs + (j * NT + wave * 64) * EPC), 16, 0, 0);
     }
   };
   auto compute = [&](const T* as, const T* bs) {
@@ -347,7 +349,7 @@ __global__ void __launch_bounds__(256) gemm_dma_kernel(GemmArgs g) {
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-      const int n = n0 + wn * 64 + j * 16 + row16;
+      const int n = n0 + wn * WCOL + j * 16 + row16;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * 64 + i * 16 + MF::row(kq, r);
