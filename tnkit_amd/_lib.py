@@ -124,6 +124,18 @@ def ptr(t) -> int:
 
 
 def stream_of(device=None) -> int:
-    """The current torch (HIP) stream handle on `device`."""
+    """The current torch (HIP) stream handle on `device` (the capture stream inside a graph
+    capture). torch's raw-stream query skips building a Stream object, which costs several
+    microseconds per launch on the latency-bound small-bond paths."""
     import torch
-    return torch.cuda.current_stream(device).cuda_stream
+    raw = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+    if raw is None:
+        return torch.cuda.current_stream(device).cuda_stream
+    if device is None:
+        idx = torch.cuda.current_device()
+    elif isinstance(device, int):
+        idx = device
+    else:
+        dev = device if isinstance(device, torch.device) else torch.device(device)
+        idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    return raw(idx)
