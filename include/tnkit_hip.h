@@ -233,7 +233,7 @@ int tnh_grouped_small(int dtype, const long* desc, long nchunks, const int* chun
  * component i of eigenvector j; Z may alias A. One workgroup per matrix, cyclic Jacobi in LDS;
  * a matrix stops after a sweep without rotations or with off-diagonal norm <= tol * ||A||_F.
  * sweeps (device ints, may be NULL): sweeps run per matrix, negative when max_sweeps ended it
- * unconverged. */
+ * unconverged or the input was not finite. */
 int tnh_eigh_jacobi(int dtype, long batch, int n, const void* A, long lda, long sA,
                     const double* add, long sadd, double* w, long sw, void* Z, long ldz, long sZ,
                     int max_sweeps, double tol, int* sweeps, tnh_stream_t stream);
@@ -246,7 +246,8 @@ int tnh_eigh_jacobi(int dtype, long batch, int n, const void* A, long lda, long 
  * (k x n, row stride ldv). One workgroup per matrix, one-sided Jacobi in LDS; exactly zero
  * singular values get an orthonormal completion, so U and Vh are always isometries. A matrix
  * stops after a sweep without rotations (|b_p^H b_q| <= tol |b_p| |b_q| for every column pair).
- * sweeps (device ints, may be NULL): sweeps run per matrix, negative when max_sweeps ended it. */
+ * sweeps (device ints, may be NULL): sweeps run per matrix, negative when max_sweeps ended it
+ * or the input was not finite. */
 size_t tnh_svd_jacobi_lds_bytes(int dtype, int m, int n);
 int tnh_svd_jacobi(int dtype, long batch, int m, int n, const void* A, long lda, long sA, double* S,
                    long sS, void* U, long ldu, long sU, void* Vh, long ldv, long sV, int max_sweeps,

@@ -170,3 +170,24 @@ def test_thin_svd_routes_and_falls_back(hip):
         assert np.abs(((U * S.to(U.dtype)) @ Vh).cpu().numpy() - X).max() < 1e-12 * ref[0]
     with pytest.raises(ValueError):
         kernels.svd_jacobi(torch.zeros((100, 100), dtype=torch.complex128, device=hip))
+
+
+def test_jacobi_kernels_keep_good_matrices_and_flag_nan(hip):
+    """A NaN in one matrix of a batch leaves the others exact and shows up in that matrix's
+    outputs (never finite garbage), which is what thin_svd and the truncation check before
+    deferring to the vendor routines."""
+    rng = np.random.default_rng(11)
+    X = rng.standard_normal((2, 6, 6))
+    G = X @ X.transpose(0, 2, 1)
+    G[1, 2, 3] = G[1, 3, 2] = np.nan
+    w, Z, sw = kernels.eigh_jacobi(torch.as_tensor(G, device=hip))
+    w, Z, sw = w.cpu().numpy(), Z.cpu().numpy(), sw.cpu().numpy()
+    assert np.abs(w[0] - np.linalg.eigvalsh(G[0])).max() < 1e-12 * np.abs(w[0]).max()
+    assert sw[0] > 0
+    assert sw[1] < 0                           # reported, so the caller redoes it
+    A = rng.standard_normal((2, 7, 5))
+    A[1, 0, 0] = np.nan
+    U, S, Vh, sw = kernels.svd_jacobi(torch.as_tensor(A, device=hip))
+    S = S.cpu().numpy()
+    assert np.abs(S[0] - np.linalg.svd(A[0], compute_uv=False)).max() < 1e-13 * S[0, 0]
+    assert np.isnan(S[1]).any() and sw.cpu().numpy()[1] < 0

@@ -89,8 +89,11 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
   const double lim2 = tol * tol * nrm2;
 
   int sweep = 0;
+  // a non-finite matrix is reported unconverged at once: its NaN entries would never rotate and
+  // the sweeps would end "converged" on a partly diagonalised matrix
+  bool bad = !isfinite(nrm2);
   bool done = (n2 <= 1);
-  while (!done && sweep < max_sweeps) {
+  while (!done && !bad && sweep < max_sweeps) {
     if (tid == 0) sh.nrot = 0;
     for (int r = 0; r < n2 - 1; ++r) {
       __syncthreads();
@@ -184,18 +187,22 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
       if (i != j) off += jabs2(sh.A[i * L + j]);
     }
     const double off2 = jac_block_sum(off, sh.red);   // ends with a barrier: nrot is final
-    done = (sh.nrot == 0) || (off2 <= lim2);
+    bad = !isfinite(off2);
+    done = !bad && ((sh.nrot == 0) || (off2 <= lim2));
     __syncthreads();   // every thread has read nrot before the next sweep clears it
   }
   __syncthreads();
 
-  // ascending order, ties by index
+  // ascending order, ties by index; a NaN sorts last (a total order, so the ranks stay a
+  // permutation and every output is written; the caller sees the NaN)
   if (tid < n) {
     const double wj = jre(sh.A[tid * L + tid]);
+    const double kj = isnan(wj) ? INFINITY : wj;
     int rk = 0;
     for (int i = 0; i < n; ++i) {
       const double wi = jre(sh.A[i * L + i]);
-      rk += (wi < wj) || (wi == wj && i < tid);
+      const double ki = isnan(wi) ? INFINITY : wi;
+      rk += (ki < kj) || (ki == kj && i < tid);
     }
     sh.rank[tid] = rk;
     w[b * sw + rk] = wj;
@@ -206,7 +213,7 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
     const int i = idx / n, j = idx - i * n;
     Zb[(long)i * ldz + sh.rank[j]] = sh.V[i * L + j];
   }
-  if (sweeps && tid == 0) sweeps[b] = done ? sweep : -sweep;
+  if (sweeps && tid == 0) sweeps[b] = (done && !bad) ? sweep : -(sweep > 0 ? sweep : 1);
 }
 
 template <typename T>
@@ -386,8 +393,12 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
   __syncthreads();
   for (int j = tid; j < N; j += kJacThreads) {
     const double sj = sig[j];
+    const double kj = isnan(sj) ? -1.0 : sj;   // NaN sorts last (ranks stay a permutation)
     int rk = 0;
-    for (int i = 0; i < N; ++i) rk += (sig[i] > sj) || (sig[i] == sj && i < j);
+    for (int i = 0; i < N; ++i) {
+      const double ki = isnan(sig[i]) ? -1.0 : sig[i];
+      rk += (ki > kj) || (ki == kj && i < j);
+    }
     rank[j] = rk;
     valid[j] = sj > 0.0;
   }
@@ -465,7 +476,11 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
       Vb[(long)rank[j] * ldv + c] = jconj(W[idx]);
     }
   }
-  if (sweeps && tid == 0) sweeps[b] = done ? sweep : -sweep;
+  if (sweeps && tid == 0) {   // non-finite input shows up as a non-finite singular value
+    bool fin = true;
+    for (int j = 0; j < N; ++j) fin = fin && isfinite(sig[j]);
+    sweeps[b] = (done && fin) ? sweep : -(sweep > 0 ? sweep : 1);
+  }
 }
 
 template <typename T>

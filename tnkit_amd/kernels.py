@@ -324,12 +324,13 @@ def svd_jacobi(A: torch.Tensor, max_sweeps: int = 60, tol: Optional[float] = Non
 def thin_svd(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """torch.linalg.svd(A, full_matrices=False) for a device matrix: the Jacobi kernel when the
     matrix fits it (a small SVD there costs tens of microseconds against about a millisecond in
-    rocSOLVER), rocSOLVER otherwise or for a matrix the kernel leaves unconverged."""
+    rocSOLVER), rocSOLVER otherwise or for a matrix the kernel leaves unconverged or non-finite
+    (which then raises or returns what torch does for it)."""
     if (A.is_cuda and A.dim() in (2, 3) and _JACOBI_SVD
             and svd_jacobi_fits(int(A.shape[-2]), int(A.shape[-1]), A.dtype)):
         Ac = A.resolve_conj().contiguous()
         U, S, Vh, sw = svd_jacobi(Ac)
-        if not bool((sw <= 0).any()):
+        if not bool((sw <= 0).any()) and bool(torch.isfinite(S).all()):
             return U, S, Vh
     return torch.linalg.svd(A, full_matrices=False)
 
