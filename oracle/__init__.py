@@ -1,8 +1,9 @@
 """CPU oracle for the two-site effective-Hamiltonian path (TEST INFRASTRUCTURE ONLY).
 
 This package is the oracle the tests check against, never the product. Only `tests/`,
-`__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py` may import it. The product package `tnkit_amd` never
-imports it and has no CPU fallback: its kernels run through `libtnkit_hip.so` or fail.
+`__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py` may import it. The product
+package `tnkit_amd` never imports it and has no CPU fallback: its kernels run through
+`libtnkit_hip.so` or fail.
 
 Contents
 --------
