@@ -10,7 +10,8 @@
 // independent 2x2 blocks (rows of pair k, columns of pair l), one per work item, and V <- V U
 // into independent row pieces. Rotations with |A_pq| <= 1e-16 sqrt(|A_pp A_qq|) are skipped.
 // A solve stops after the first sweep that applies no rotation or ends with an off-diagonal
-// Frobenius norm <= tol * ||A||_F.
+// Frobenius norm <= tol * ||A||_F. Only the blocks k <= l are transformed; their mirrors are
+// written as conjugate transposes.
 //
 // Used for the Gram matrices of the truncation (tnkit/linalg.py:138-159, the batched form of
 // tnkit/batch_tdvp.py:572-599), where rocSOLVER's batched eigh spends milliseconds per call on
@@ -21,7 +22,8 @@
 namespace tnh {
 
 constexpr int kJacMax = 64;
-constexpr int kJacThreads = 256;
+constexpr int kJacThreads = 256;    // the SVD kernel
+constexpr int kEighThreads = 512;   // the eigh kernel: eight waves share a round's block updates
 
 __device__ __forceinline__ double jre(double x) { return x; }
 __device__ __forceinline__ double jre(c128 x) { return x.re; }
@@ -41,11 +43,13 @@ struct JacobiShared {
   T e[kJacMax / 2];
   double c[kJacMax / 2], s[kJacMax / 2], dp[kJacMax / 2], dq[kJacMax / 2];
   int p[kJacMax / 2], q[kJacMax / 2], on[kJacMax / 2];
+  unsigned short kl[kJacMax / 2 * (kJacMax / 2 + 1) / 2];   // block pairs k <= l, (k << 8) | l
   int rank[kJacMax];
-  double red[kJacThreads / kWave];
+  double red[kEighThreads / kWave];
   int nrot;
 };
 
+template <int NT>
 __device__ __forceinline__ double jac_block_sum(double v, double* red) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -54,7 +58,7 @@ __device__ __forceinline__ double jac_block_sum(double v, double* red) {
   __syncthreads();
   double t = 0.0;
 #pragma unroll
-  for (int w = 0; w < kJacThreads / kWave; ++w) t += red[w];
+  for (int w = 0; w < NT / kWave; ++w) t += red[w];
   return t;
 }
 
@@ -62,7 +66,7 @@ __device__ __forceinline__ double jac_block_sum(double v, double* red) {
 // w_b ascending eigenvalues, Z_b[i*ldz + j] = component i of eigenvector j (Z may alias A).
 // sweeps[b] (optional) = sweeps run, negated when max_sweeps ended the solve unconverged.
 template <typename T>
-__global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
+__global__ void __launch_bounds__(kEighThreads) jacobi_eigh_kernel(
     int n, const T* A, long lda, long sA, const double* __restrict__ add, long sadd,
     double* __restrict__ w, long sw, T* Z, long ldz, long sZ, int max_sweeps, double tol,
     int* __restrict__ sweeps) {
@@ -73,19 +77,33 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
   const int n2 = n + (n & 1), h = n2 / 2, L = n2;
   const T* Ab = A + b * sA;
 
+  // the Hermitian part of A (both triangles read, the diagonal real), padded with zeros
   double part = 0.0;
-  for (int idx = tid; idx < n2 * n2; idx += kJacThreads) {
+  for (int idx = tid; idx < n2 * n2; idx += kEighThreads) {
     const int i = idx / n2, j = idx - i * n2;
     T v = T(0.0);
     if (i < n && j < n) {
-      v = Ab[(long)i * lda + j];
-      if (i == j && add) v = v + T(add[b * sadd + i]);
+      if (i == j) {
+        v = T(jre(Ab[(long)i * lda + i]) + (add ? add[b * sadd + i] : 0.0));
+      } else {
+        v = jscale(0.5, Ab[(long)i * lda + j] + jconj(Ab[(long)j * lda + i]));
+      }
     }
     sh.A[i * L + j] = v;
     sh.V[i * L + j] = T(i == j ? 1.0 : 0.0);
     part += jabs2(v);
   }
-  const double nrm2 = jac_block_sum(part, sh.red);
+  // the blocks of one round: pairs k <= l of the h index pairs (the others are their mirrors)
+  const int nkl = h * (h + 1) / 2;
+  for (int it = tid; it < nkl; it += kEighThreads) {
+    int k = 0, rem = it;
+    while (rem >= h - k) {
+      rem -= h - k;
+      ++k;
+    }
+    sh.kl[it] = static_cast<unsigned short>((k << 8) | (k + rem));
+  }
+  const double nrm2 = jac_block_sum<kEighThreads>(part, sh.red);
   const double lim2 = tol * tol * nrm2;
 
   int sweep = 0;
@@ -109,13 +127,16 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
         double cs = 1.0, sn = 0.0, dpv = a, dqv = d;
         T e = T(1.0);
         if (ac2 > 0.0 && ac2 > 1e-32 * fabs(a) * fabs(d)) {
-          const double ac = sqrt(ac2);
-          e = jscale(1.0 / ac, jconj(cpq));
-          const double tau = (d - a) / (2.0 * ac);
+          // few long-latency operations on the critical path: one rsqrt for |c| and the phase,
+          // one sqrt and one division for t, one rsqrt for the cosine
+          const double inv = rsqrt(ac2);
+          const double ac = ac2 * inv;
+          e = jscale(inv, jconj(cpq));
+          const double tau = 0.5 * (d - a) * inv;
           const double at = fabs(tau);
           double t = at > 1e100 ? 0.5 / at : 1.0 / (at + sqrt(1.0 + tau * tau));
           if (tau < 0.0) t = -t;
-          cs = 1.0 / sqrt(1.0 + t * t);
+          cs = rsqrt(1.0 + t * t);
           sn = t * cs;
           dpv = a - t * ac;
           dqv = d + t * ac;
@@ -132,9 +153,10 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
         sh.dq[tid] = dqv;
       }
       __syncthreads();
-      // A <- U^H A U, one 2x2 block (rows of pair k, columns of pair l) per item
-      for (int it = tid; it < h * h; it += kJacThreads) {
-        const int k = it / h, l = it - k * h;
+      // A <- U^H A U, one 2x2 block (rows of pair k, columns of pair l, k <= l) per item; the
+      // block (l, k) is its conjugate transpose, so A stays exactly Hermitian
+      for (int it = tid; it < nkl; it += kEighThreads) {
+        const int k = sh.kl[it] >> 8, l = sh.kl[it] & 255;
         const int onk = sh.on[k], onl = sh.on[l];
         if (!onk && !onl) continue;
         const int p1 = sh.p[k], q1 = sh.q[k], p2 = sh.p[l], q2 = sh.q[l];
@@ -167,9 +189,13 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
         sh.A[p1 * L + q2] = x01;
         sh.A[q1 * L + p2] = x10;
         sh.A[q1 * L + q2] = x11;
+        sh.A[p2 * L + p1] = jconj(x00);
+        sh.A[q2 * L + p1] = jconj(x01);
+        sh.A[p2 * L + q1] = jconj(x10);
+        sh.A[q2 * L + q1] = jconj(x11);
       }
       // V <- V U, one row piece (row i, pair l) per item
-      for (int it = tid; it < n2 * h; it += kJacThreads) {
+      for (int it = tid; it < n2 * h; it += kEighThreads) {
         const int i = it / h, l = it - i * h;
         if (!sh.on[l]) continue;
         const int p2 = sh.p[l], q2 = sh.q[l];
@@ -182,11 +208,11 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
     __syncthreads();
     ++sweep;
     double off = 0.0;
-    for (int idx = tid; idx < n2 * n2; idx += kJacThreads) {
+    for (int idx = tid; idx < n2 * n2; idx += kEighThreads) {
       const int i = idx / n2, j = idx - i * n2;
       if (i != j) off += jabs2(sh.A[i * L + j]);
     }
-    const double off2 = jac_block_sum(off, sh.red);   // ends with a barrier: nrot is final
+    const double off2 = jac_block_sum<kEighThreads>(off, sh.red);   // ends with a barrier: nrot is final
     bad = !isfinite(off2);
     done = !bad && ((sh.nrot == 0) || (off2 <= lim2));
     __syncthreads();   // every thread has read nrot before the next sweep clears it
@@ -209,7 +235,7 @@ __global__ void __launch_bounds__(kJacThreads) jacobi_eigh_kernel(
   }
   __syncthreads();
   T* Zb = Z + b * sZ;
-  for (int idx = tid; idx < n * n; idx += kJacThreads) {
+  for (int idx = tid; idx < n * n; idx += kEighThreads) {
     const int i = idx / n, j = idx - i * n;
     Zb[(long)i * ldz + sh.rank[j]] = sh.V[i * L + j];
   }
@@ -225,7 +251,7 @@ int launch_jacobi(long batch, int n, const void* A, long lda, long sA, const dou
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   TNH_REQUIRE(e == hipSuccess, TNH_ERR_HIP, "tnh_eigh_jacobi: %zu bytes of LDS: %s", smem,
               hipGetErrorString(e));
-  hipLaunchKernelGGL(jacobi_eigh_kernel<T>, dim3((unsigned)batch), dim3(kJacThreads), smem, st, n,
+  hipLaunchKernelGGL(jacobi_eigh_kernel<T>, dim3((unsigned)batch), dim3(kEighThreads), smem, st, n,
                      static_cast<const T*>(A), lda, sA, add, sadd, w, sw, static_cast<T*>(Z), ldz,
                      sZ, max_sweeps, tol, sweeps);
   TNH_CHECK_LAUNCH("jacobi_eigh");
@@ -355,13 +381,13 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
         }
         const double ag2 = jabs2(gm);
         if (ag2 > 0.0 && ag2 > tol * tol * a * d) {
-          const double ag = sqrt(ag2);
-          const T e = jscale(1.0 / ag, jconj(gm));
-          const double tau = (d - a) / (2.0 * ag);
+          const double inv = rsqrt(ag2);
+          const T e = jscale(inv, jconj(gm));
+          const double tau = 0.5 * (d - a) * inv;
           const double at = fabs(tau);
           double t = at > 1e100 ? 0.5 / at : 1.0 / (at + sqrt(1.0 + tau * tau));
           if (tau < 0.0) t = -t;
-          const double cs = 1.0 / sqrt(1.0 + t * t), sn = t * cs;
+          const double cs = rsqrt(1.0 + t * t), sn = t * cs;
           for (int i = gl; i < M; i += G) {
             const T x = wp[i], y = jmul(wq[i], e);
             wp[i] = jscale(cs, x) - jscale(sn, y);
@@ -435,7 +461,7 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
       }
       double part = 0.0;
       for (int i = tid; i < M; i += kJacThreads) part += jabs2(W[(size_t)j * M + i]);
-      const double nrm2 = jac_block_sum(part, red);
+      const double nrm2 = jac_block_sum<kJacThreads>(part, red);
       if (nrm2 > 0.5 / M) {   // some e_cand keeps >= (M - rank) / M >= 1 / M of its norm
         const double inv = 1.0 / sqrt(nrm2);
         for (int i = tid; i < M; i += kJacThreads)
