@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(kEighThreads) jacobi_eigh_kernel(
           dpv = a - t * ac;
           dqv = d + t * ac;
           on = 1;
-          atomicAdd(&sh.nrot, 1);
+          sh.nrot = 1;   // any rotation this sweep (every writer stores the same value)
         }
         sh.p[tid] = p;
         sh.q[tid] = q;
@@ -400,7 +400,7 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
             vp[i] = jscale(cs, x) - jscale(sn, y);
             vq[i] = jscale(sn, x) + jscale(cs, y);
           }
-          if (gl == 0) atomicAdd(&nrot, 1);
+          if (gl == 0) nrot = 1;   // any rotation this sweep (same value from every writer)
         }
       }
     }
