@@ -372,6 +372,40 @@ def sweep_cost(N, chis, forms, log=None):
     return out
 
 
+def small_linalg(reps=20):
+    """The small factorisations of the truncations on the device: the Jacobi kernels
+    (kernels.eigh_jacobi, kernels.svd_jacobi) against torch.linalg (rocSOLVER) on the shapes the
+    sweeps meet: a batch of 32 complex Gram matrices of order 50 (the batched TDVP at bonds 25)
+    and float64 SVDs of 16 x 64, 32 x 32 and 64 x 64 (two-site matrices at bonds 8-32).
+    Milliseconds per call with HIP events; every result is checked against torch's."""
+    import torch
+    from tnkit_amd import kernels
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    out = {"eigh": [], "svd": []}
+    X = torch.randn((32, 50, 50), dtype=torch.complex128, device=dev, generator=g)
+    G = (X @ X.mH.resolve_conj()).contiguous()
+    w, _, sw = kernels.eigh_jacobi(G)
+    err = float((w - torch.linalg.eigvalsh(G)).abs().max() / w.abs().max())
+    out["eigh"].append({"shape": [32, 50, 50], "dtype": "c128",
+                        "jacobi_ms": event_time(lambda: kernels.eigh_jacobi(G), reps, stream),
+                        "torch_ms": event_time(lambda: torch.linalg.eigh(G), reps, stream),
+                        "max_sweeps": int(sw.max()), "rel_err_eigenvalues": err})
+    for m, n in ((16, 64), (32, 32), (64, 64)):
+        A = torch.randn((m, n), dtype=torch.float64, device=dev, generator=g)
+        _, S, _, sw = kernels.svd_jacobi(A)
+        ref = torch.linalg.svdvals(A)
+        out["svd"].append({"shape": [m, n], "dtype": "f64",
+                           "jacobi_ms": event_time(lambda: kernels.svd_jacobi(A), reps, stream),
+                           "torch_ms": event_time(lambda: torch.linalg.svd(A, full_matrices=False),
+                                                  reps, stream),
+                           "sweeps": int(sw.max()),
+                           "rel_err_singular_values": float((S - ref).abs().max() / ref[0])})
+    return out
+
+
 def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, log=None):
     """Batched time evolution: `legs` TFIM quenches (N sites) from DMRG ground states at chi = 16
     for fields g0 ~ U[1.2, 2.0] (default_rng(1)) to g = 1, two-site TDVP with per-leg bond caps
@@ -553,6 +587,8 @@ def main():
     ap.add_argument("--no-block", action="store_true")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the chi=1024 sweep-time measurement (Heisenberg N=64, 3 sweeps)")
+    ap.add_argument("--small-linalg", action="store_true",
+                    help="only time the Jacobi eigh / thin SVD kernels against torch (rocSOLVER)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -582,6 +618,9 @@ def main():
 
     if args.block:
         print(json.dumps(block_bench(args.steps, args.warmup, local=not args.no_local)))
+        return
+    if args.small_linalg:
+        print(json.dumps(small_linalg(max(5, args.steps))))
         return
     if args.config5:
         print(json.dumps(config5(64 if args.N == 100 else args.N, log=sys.stderr)))
