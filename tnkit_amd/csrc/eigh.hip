@@ -22,7 +22,6 @@
 namespace tnh {
 
 constexpr int kJacMax = 64;
-constexpr int kJacThreads = 256;    // the SVD kernel
 constexpr int kEighThreads = 512;   // the eigh kernel: eight waves share a round's block updates
 
 __device__ __forceinline__ double jre(double x) { return x; }
@@ -318,8 +317,8 @@ __host__ __device__ constexpr size_t svd_jacobi_lds(int M, int N2, size_t elem) 
 }
 constexpr size_t kSvdLdsMax = 163840 - 1024;   // 160 KB of LDS, less the static variables
 
-template <typename T>
-__global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) svd_jacobi_kernel(
     int m, int n, const T* __restrict__ A, long lda, long sA, double* __restrict__ S, long sS,
     T* __restrict__ U, long ldu, long sU, T* __restrict__ Vh, long ldv, long sV, int max_sweeps,
     double tol, int* __restrict__ sweeps) {
@@ -338,19 +337,19 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
   const long b = blockIdx.x;
   const T* Ab = A + b * sA;
 
-  for (int idx = tid; idx < M * N2; idx += kJacThreads) {
+  for (int idx = tid; idx < M * N2; idx += NT) {
     const int j = idx / M, i = idx - j * M;
     T v = T(0.0);
     if (j < N) v = tr ? jconj(Ab[(long)j * lda + i]) : Ab[(long)i * lda + j];
     W[idx] = v;
   }
-  for (int idx = tid; idx < N2 * N2; idx += kJacThreads) {
+  for (int idx = tid; idx < N2 * N2; idx += NT) {
     const int j = idx / N2, i = idx - j * N2;
     V[idx] = T(i == j ? 1.0 : 0.0);
   }
   // lanes per pair: the largest power of two <= min(64, 256 / h)
   int G = 64;
-  while (G * h > kJacThreads) G >>= 1;
+  while (G * h > NT) G >>= 1;
   const int g = tid / G, gl = tid - g * G;
   const bool active = g < h;
 
@@ -411,13 +410,13 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
   }
 
   // singular values and descending order (ties by index)
-  for (int j = tid; j < N; j += kJacThreads) {
+  for (int j = tid; j < N; j += NT) {
     double s2 = 0.0;
     for (int i = 0; i < M; ++i) s2 += jabs2(W[(size_t)j * M + i]);
     sig[j] = sqrt(s2);
   }
   __syncthreads();
-  for (int j = tid; j < N; j += kJacThreads) {
+  for (int j = tid; j < N; j += NT) {
     const double sj = sig[j];
     const double kj = isnan(sj) ? -1.0 : sj;   // NaN sorts last (ranks stay a permutation)
     int rk = 0;
@@ -429,7 +428,7 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
     valid[j] = sj > 0.0;
   }
   __syncthreads();
-  for (int idx = tid; idx < M * N; idx += kJacThreads) {
+  for (int idx = tid; idx < M * N; idx += NT) {
     const int j = idx / M;
     if (sig[j] > 0.0) W[idx] = jscale(1.0 / sig[j], W[idx]);
   }
@@ -439,11 +438,11 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
     if (sig[j] > 0.0) continue;
     for (int cand = 0; cand < M; ++cand) {
       // column j <- e_cand, then two passes of  x <- x - sum_k w_k (w_k^H x)  over valid k
-      for (int i = tid; i < M; i += kJacThreads) W[(size_t)j * M + i] = T(i == cand ? 1.0 : 0.0);
+      for (int i = tid; i < M; i += NT) W[(size_t)j * M + i] = T(i == cand ? 1.0 : 0.0);
       __syncthreads();
       for (int pass = 0; pass < 2; ++pass) {
         // coefficients c_k = w_k^H x over the valid columns, one thread per k
-        for (int k = tid; k < N; k += kJacThreads) {
+        for (int k = tid; k < N; k += NT) {
           T c = T(0.0);
           if (valid[k])
             for (int i = 0; i < M; ++i)
@@ -451,7 +450,7 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
           coef[k] = c;
         }
         __syncthreads();
-        for (int i = tid; i < M; i += kJacThreads) {
+        for (int i = tid; i < M; i += NT) {
           T x = W[(size_t)j * M + i];
           for (int k = 0; k < N; ++k)
             if (valid[k]) x = x - jmul(W[(size_t)k * M + i], coef[k]);
@@ -460,11 +459,11 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
         __syncthreads();
       }
       double part = 0.0;
-      for (int i = tid; i < M; i += kJacThreads) part += jabs2(W[(size_t)j * M + i]);
-      const double nrm2 = jac_block_sum<kJacThreads>(part, red);
+      for (int i = tid; i < M; i += NT) part += jabs2(W[(size_t)j * M + i]);
+      const double nrm2 = jac_block_sum<NT>(part, red);
       if (nrm2 > 0.5 / M) {   // some e_cand keeps >= (M - rank) / M >= 1 / M of its norm
         const double inv = 1.0 / sqrt(nrm2);
-        for (int i = tid; i < M; i += kJacThreads)
+        for (int i = tid; i < M; i += NT)
           W[(size_t)j * M + i] = jscale(inv, W[(size_t)j * M + i]);
         __syncthreads();
         if (tid == 0) valid[j] = 1;
@@ -480,24 +479,24 @@ __global__ void __launch_bounds__(kJacThreads) svd_jacobi_kernel(
   double* Sb = S + b * sS;
   T* Ub = U + b * sU;
   T* Vb = Vh + b * sV;
-  for (int j = tid; j < N; j += kJacThreads) Sb[rank[j]] = sig[j];
+  for (int j = tid; j < N; j += NT) Sb[rank[j]] = sig[j];
   if (!tr) {
     // U (m x N): column rank[j] = w_j; Vh (N x n): row rank[j] = v_j^H
-    for (int idx = tid; idx < M * N; idx += kJacThreads) {
+    for (int idx = tid; idx < M * N; idx += NT) {
       const int j = idx / M, i = idx - j * M;
       Ub[(long)i * ldu + rank[j]] = W[idx];
     }
-    for (int idx = tid; idx < N * N; idx += kJacThreads) {
+    for (int idx = tid; idx < N * N; idx += NT) {
       const int j = idx / N, c = idx - j * N;
       Vb[(long)rank[j] * ldv + c] = jconj(V[(size_t)j * N2 + c]);
     }
   } else {
     // A = V S W^H: U (m = N x N): column rank[j] = v_j; Vh (N x n = M): row rank[j] = w_j^H
-    for (int idx = tid; idx < N * N; idx += kJacThreads) {
+    for (int idx = tid; idx < N * N; idx += NT) {
       const int j = idx / N, i = idx - j * N;
       Ub[(long)i * ldu + rank[j]] = V[(size_t)j * N2 + i];
     }
-    for (int idx = tid; idx < M * N; idx += kJacThreads) {
+    for (int idx = tid; idx < M * N; idx += NT) {
       const int j = idx / M, c = idx - j * M;
       Vb[(long)rank[j] * ldv + c] = jconj(W[idx]);
     }
@@ -515,12 +514,15 @@ int launch_svd_jacobi(long batch, int m, int n, const void* A, long lda, long sA
                       int max_sweeps, double tol, int* sweeps, hipStream_t st) {
   const int M = m > n ? m : n, N = m < n ? m : n, N2 = N + (N & 1);
   const size_t smem = svd_jacobi_lds(M, N2, sizeof(T));
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&svd_jacobi_kernel<T>),
+  // eight waves (16 lanes per column pair) pay off from N > 32; below, four waves
+  auto kern = N > 32 ? &svd_jacobi_kernel<T, 512> : &svd_jacobi_kernel<T, 256>;
+  const int nt = N > 32 ? 512 : 256;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem);
   TNH_REQUIRE(e == hipSuccess, TNH_ERR_HIP, "tnh_svd_jacobi: %zu bytes of LDS: %s", smem,
               hipGetErrorString(e));
-  hipLaunchKernelGGL(svd_jacobi_kernel<T>, dim3((unsigned)batch), dim3(kJacThreads), smem, st, m,
-                     n, static_cast<const T*>(A), lda, sA, S, sS, static_cast<T*>(U), ldu, sU,
+  hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(nt), smem, st, m, n,
+                     static_cast<const T*>(A), lda, sA, S, sS, static_cast<T*>(U), ldu, sU,
                      static_cast<T*>(Vh), ldv, sV, max_sweeps, tol, sweeps);
   TNH_CHECK_LAUNCH("svd_jacobi");
   return TNH_OK;
