@@ -271,3 +271,58 @@ def test_fuse_split_maps_match_tnkit(case):
     assert np.array_equal(gv.expand(), GOLD[f"{case}/split_v/ix"])
     assert np.array_equal(B.leg_positions(U, 1).expand(), GOLD[f"{case}/scale_u/ix"])
     assert np.array_equal(B.leg_positions(Vh, 0).expand(), GOLD[f"{case}/scale_v/ix"])
+
+
+def test_trim_equals_one_state_at_a_time():
+    """The closed-form trim equals taking one state at a time from the largest sector (the
+    first on ties) until the bond holds chi states."""
+    from tnkit_amd.block_mps import _trim
+    rng = np.random.default_rng(5)
+    for _ in range(400):
+        n = int(rng.integers(1, 9))
+        dims = [int(x) for x in rng.integers(1, 12, n)]
+        chi = int(rng.integers(n, sum(dims) + 3))
+        ref = list(dims)
+        while sum(ref) > chi:
+            k = max(range(n), key=lambda i: (ref[i], -i))
+            ref[k] -= 1
+        assert _trim(dims, chi) == ref, (dims, chi)
+
+
+def test_mpo_charges_refuses_charge_mixing_channels():
+    """X X + Y Y written with real Pauli matrices puts two charges on one channel: refused."""
+    from tnkit_amd import block_mps as BM
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.diag([1.0, -1.0])
+    Id = np.eye(2)
+    W = np.zeros((3, 3, 2, 2))
+    W[0, 0], W[0, 1], W[1, 2], W[2, 2], W[0, 2] = Id, X, X, Id, Z
+    mpo = [W[:1], W, W[:, 2:]]
+    with pytest.raises(ValueError, match="right channel 1"):
+        BM.mpo_charges(mpo, [[1, -1]] * 3)
+    # a channel the left end never reaches is dead: charge 0, its entries ignored
+    W2 = np.zeros((3, 3, 2, 2))
+    W2[0, 0], W2[1, 1], W2[0, 1], W2[2, 2] = Id, Id, Z, Id
+    W2[2, 0] = X            # would give channel 0 two charges if channel 2 were live
+    q = BM.mpo_charges([W2[:1], W2, W2[:, 1:2]], [[1, -1]] * 3)
+    assert [c.tolist() for c in q] == [[0], [0, 0, 0], [0, 0, 0], [0]]
+
+
+@pytest.mark.parametrize("groups", [((0, 1), (2, 3)), ((1, 0), (3, 2)), ((2, 0), (1, 3)),
+                                    ((0,), (1, 2, 3)), ((3, 1, 0), (2,))])
+def test_fuse_then_full_split_is_identity(groups):
+    """Fusing any grouping (permuting ones too) and splitting every fused leg back returns each
+    element to its place; a partial split of a leg keeps the other fused leg as it is."""
+    legs = (B.Leg((-2, 0, 2), (2, 3, 1)), B.Leg((-1, 1), (1, 2)), B.Leg((-1, 1), (2, 1)),
+            B.Leg((-3, -1, 1, 3), (1, 2, 2, 1), -1))
+    s = B.canonical(legs, 0)
+    rec = B.fuse_record(legs, 0, groups)
+    f, g1 = B.fuse_map(s, rec)
+    back, g2 = B.split_map(f, rec, (0, 1))
+    assert back.legs == legs and back.size == s.size
+    assert np.array_equal(g1.expand()[g2.expand()], np.arange(s.size))
+    assert sorted(g1.expand().tolist()) == list(range(s.size))
+    if rec.perm == (0, 1, 2, 3):
+        half, g3 = B.split_map(f, rec, (0,))
+        assert half.legs[-1] == rec.fused[1]
+        assert sorted(g3.expand().tolist()) == list(range(f.size))

@@ -9,7 +9,7 @@
 """
 from __future__ import annotations
 
-from typing import List, Optional, Sequence, Tuple
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -19,41 +19,48 @@ from .block_ops import BlockTensor, einsum
 
 
 def _live_channels(mpo: Sequence[np.ndarray], tol: float):
-    """Per MPO bond, the channels on a path of entries above tol from the first tensor's left
-    channels to the last tensor's right channels (tnkit/block_backend.py:147-159)."""
-    nz = [(np.abs(np.asarray(W)) > tol).any(axis=(2, 3)) for W in mpo]
-    fwd = [np.ones(nz[0].shape[0], bool)]
-    for m in nz:
-        fwd.append(m[fwd[-1]].any(0))
-    bwd = [np.ones(nz[-1].shape[1], bool)]
-    for m in reversed(nz):
-        bwd.append(m[:, bwd[-1]].any(1))
-    return [f & b for f, b in zip(fwd, reversed(bwd))]
+    """Per MPO bond (len(mpo) + 1 of them), a boolean mask of the channels that carry part of
+    the operator: those joined by entries above tol both to some left channel of the first
+    tensor and to some right channel of the last (tnkit/block_backend.py:162-173). Reachability
+    is propagated with 0/1 matrix products of the per-site channel adjacency."""
+    adj = [(np.abs(np.asarray(W)) > tol).any(axis=(2, 3)).astype(np.int64) for W in mpo]
+    ahead = [np.ones(adj[0].shape[0], np.int64)]
+    for A in adj:
+        ahead.append((ahead[-1] @ A > 0).astype(np.int64))
+    behind = [np.ones(adj[-1].shape[1], np.int64)]
+    for A in adj[::-1]:
+        behind.insert(0, (A @ behind[0] > 0).astype(np.int64))
+    return [(f > 0) & (b > 0) for f, b in zip(ahead, behind)]
 
 
 def mpo_charges(mpo: Sequence[np.ndarray], site_charges, tol: float = 0.0,
                 left: Optional[Sequence[int]] = None) -> List[np.ndarray]:
-    """The U(1) charge of every channel of every MPO bond (len(mpo) + 1 arrays): q(wr) = q(wl) +
-    q(s) - q(s') over the nonzero live entries; dead channels get charge 0. Raises when a channel
-    would carry two charges (tnkit/block_backend.py:161-208)."""
+    """The U(1) charge of every channel of every MPO bond (len(mpo) + 1 int64 arrays): an entry
+    W[i, j, s, s'] above tol between live channels sends charge q(i) + q(s) - q(s') to right
+    channel j, and every live channel must receive exactly one value; channels that are not
+    live get 0. `left` gives the first bond's charges (default 0). Raises ValueError when a
+    channel would carry two charges (tnkit/block_backend.py:176-208)."""
     live = _live_channels(mpo, tol)
-    out = [np.zeros(np.shape(mpo[0])[0], np.int64) if left is None
-           else np.asarray(left, np.int64)]
-    for n, (W, q) in enumerate(zip(mpo, site_charges)):
-        W, q = np.asarray(W), [int(x) for x in q]
-        a, b, s, t = np.nonzero(np.abs(W) > tol)
-        on = live[n][a] & live[n + 1][b]
-        found = {int(j): set() for j in np.flatnonzero(live[n + 1])}
-        for i, j, x, y in zip(a[on], b[on], s[on], t[on]):
-            found[int(j)].add(int(out[-1][i]) + q[x] - q[y])
-        qr = np.zeros(W.shape[1], np.int64)
-        for j, vals in found.items():
-            if len(vals) != 1:
-                raise ValueError(f"MPO site {n}, right channel {j}: charges {sorted(vals)} (an "
-                                 "operator that mixes charges?)")
-            qr[j] = vals.pop()
-        out.append(qr)
-    return out
+    q = (np.zeros(np.shape(mpo[0])[0], np.int64) if left is None
+         else np.asarray(left, np.int64))
+    charges = [q]
+    for site, (W, qs) in enumerate(zip(mpo, site_charges)):
+        W = np.asarray(W)
+        qs = np.asarray([int(x) for x in qs], np.int64)
+        i, j, s, t = np.nonzero(np.abs(W) > tol)
+        on = live[site][i] & live[site + 1][j]
+        sent = q[i[on]] + qs[s[on]] - qs[t[on]]
+        dest = j[on]
+        nxt = np.zeros(W.shape[1], np.int64)
+        for ch in np.flatnonzero(live[site + 1]):
+            vals = np.unique(sent[dest == ch])
+            if vals.size != 1:
+                raise ValueError(f"MPO site {site}, right channel {ch}: charges "
+                                 f"{vals.tolist()} (an operator that mixes charges?)")
+            nxt[ch] = vals[0]
+        charges.append(nxt)
+        q = nxt
+    return charges
 
 
 def from_dense(x: np.ndarray, legs: Sequence[B.Leg], charge: int = 0,
@@ -95,59 +102,95 @@ def to_block_mpo(mpo: Sequence[np.ndarray], site_charges, device, dtype=torch.fl
     return out, orders
 
 
+def _state_counts(phys: Sequence[B.Leg], first: int, sign: int):
+    """Per prefix of the sites (0..len(phys) of them): (lowest charge, counts) where counts[k]
+    is the number of basis states of those sites whose charges, each times `sign`, add to
+    first + k. Python-int counts (object arrays): they reach 2^N."""
+    lo, cnt = first, np.array([1], dtype=object)
+    out = [(lo, cnt)]
+    for leg in phys:
+        qs = [sign * c for c in leg.charges]
+        hist = np.zeros(max(qs) - min(qs) + 1, dtype=object)
+        hist[:] = 0
+        for c, d in zip(qs, leg.dims):
+            hist[c - min(qs)] += d
+        lo, cnt = lo + min(qs), np.convolve(cnt, hist)
+        out.append((lo, cnt))
+    return out
+
+
+def _count(table, c: int) -> int:
+    lo, cnt = table
+    k = c - lo
+    return int(cnt[k]) if 0 <= k < len(cnt) else 0
+
+
+def _trim(dims: List[int], chi: int) -> List[int]:
+    """Bring sum(dims) down to chi by taking one state at a time from the largest entry (the
+    first one on ties). Closed form: cap every entry at the lowest level T whose capping
+    removes no more than the excess, then take one more state from each of the first r entries
+    left at T, r being what the capping did not remove."""
+    excess = sum(dims) - chi
+    if excess <= 0:
+        return list(dims)
+    removed = lambda level: sum(max(x - level, 0) for x in dims)  # noqa: E731
+    level = max(dims)
+    while removed(level - 1) <= excess:
+        level -= 1
+    r = excess - removed(level)
+    out = []
+    for x in dims:
+        y = min(x, level)
+        if y == level and r > 0:
+            y, r = y - 1, r - 1
+        out.append(y)
+    return out
+
+
 def random_block_mps(mpo: Sequence[BlockTensor], chi: int, charge: int,
                      rng: Optional[np.random.Generator] = None, device=None,
                      dtype=torch.float64) -> List[BlockTensor]:
-    """tnkit's random block MPS (tnkit/block_backend.py:259-311): bond n keeps the sectors
-    reachable from the left and, backwards, from `charge` at the right end, each with states in
-    proportion to w = min(basis states left, basis states right) (at least 1, at most w); with
-    more than chi candidates the largest w are kept; sectors without a kept successor are
-    dropped. Entries are standard normal, drawn from `rng` block tensor by block tensor in
-    canonical order."""
+    """tnkit's random block MPS (tnkit/block_backend.py:259-311), same bond sectors, dimensions
+    and random numbers for the same generator. Bond n offers the sectors that site n - 1's
+    charges carry the kept sectors of bond n - 1 into (bond 0: charge 0), among those from which
+    the right end can still reach `charge`. A sector's weight w is the smaller of its basis-state
+    counts to the left and to the right; the chi heaviest are kept (smaller charge first on
+    ties) and share chi states in proportion to w, at least 1 and at most w each, then trimmed
+    to chi (`_trim`). Sectors that no charge of the next site carries into a kept sector of the
+    next bond are dropped, right to left. Entries are standard normal, one draw of the whole
+    canonical buffer per site tensor, site by site."""
     rng = np.random.default_rng(0) if rng is None else rng
     device = device if device is not None else mpo[0].data.device
     phys = [W.legs[2] for W in mpo]
-    target = int(charge)
-
-    def grow(counts, leg, sign):
-        out = {}
-        for c, n in counts.items():
-            for a, d in zip(leg.charges, leg.dims):
-                c2 = c + (a if sign > 0 else -a)
-                out[c2] = out.get(c2, 0) + n * d
-        return out
-
-    left, right = [{0: 1}], [{target: 1}]
-    for p in phys:
-        left.append(grow(left[-1], p, 1))
-    for p in reversed(phys):
-        right.append(grow(right[-1], p, -1))
-    right.reverse()
-    if target not in left[-1]:
+    N, target = len(phys), int(charge)
+    from_left = _state_counts(phys, 0, 1)
+    from_right = _state_counts(phys[::-1], target, -1)[::-1]
+    if _count(from_left[N], target) == 0:
         raise ValueError(f"no state of these sites has total charge {target}")
-    bonds, cand = [], {0}
-    for n, (lc, rc) in enumerate(zip(left, right)):
-        w = {c: min(lc[c], rc[c]) for c in cand if c in rc}
-        keep = sorted(w, key=lambda c: (-w[c], c))[:chi]
-        tot = sum(w[c] for c in keep)
-        dims = {c: min(w[c], max(1, int(chi * w[c] / tot))) for c in keep}
-        while sum(dims.values()) > chi:
-            dims[max(dims, key=dims.get)] -= 1
-        bonds.append(dims)
-        if n < len(phys):
-            cand = {c + a for c in keep for a in phys[n].charges}
-    for n in range(len(phys) - 1, -1, -1):
-        prv = {c - a for c in bonds[n + 1] for a in phys[n].charges}
-        bonds[n] = {c: d for c, d in bonds[n].items() if c in prv}
-    legs = []
-    for b in bonds:
-        cs = sorted(b)
-        legs.append(B.Leg(tuple(cs), tuple(b[c] for c in cs), 1))
+    bonds: List[Dict[int, int]] = []
+    offered = [0]
+    for n in range(N + 1):
+        weight = {}
+        for c in offered:
+            r = _count(from_right[n], c)
+            if r:
+                weight[c] = min(_count(from_left[n], c), r)
+        kept = sorted(weight, key=lambda c: (-weight[c], c))[:chi]
+        total = sum(weight[c] for c in kept)
+        share = [min(weight[c], max(1, int(chi * weight[c] / total))) for c in kept]
+        bonds.append(dict(zip(kept, _trim(share, chi))))
+        if n < N:
+            offered = sorted({c + a for c in kept for a in phys[n].charges})
+    for n in range(N - 1, -1, -1):
+        nxt = bonds[n + 1]
+        bonds[n] = {c: d for c, d in bonds[n].items()
+                    if any(c + a in nxt for a in phys[n].charges)}
+    legs = [B.Leg(tuple(sorted(b)), tuple(b[c] for c in sorted(b)), 1) for b in bonds]
     out = []
     for n, p in enumerate(phys):
         s = B.canonical((legs[n], p, legs[n + 1].flip()), 0)
-        out.append(BlockTensor(s, torch.from_numpy(rng.standard_normal(s.size)).to(device,
-                                                                                  dtype)))
+        data = torch.from_numpy(rng.standard_normal(s.size)).to(device, dtype)
+        out.append(BlockTensor(s, data))
     return out
 
 

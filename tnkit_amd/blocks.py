@@ -671,77 +671,76 @@ def _row_major(shape) -> List[int]:
     return list(reversed(st))
 
 
+def _gather_of(out: Struct, src_size: int, boxes) -> Gather:
+    """The gather of a list of boxes (src start, shape, src strides, dst start, dst strides),
+    each given in the destination's axis order."""
+    if not boxes:
+        z = np.zeros(0, np.int64)
+        return Gather((z, z, z, z), out.size, src_size)
+    src0, shp, ss, d0, ds = (np.array(x, np.int64) for x in zip(*boxes))
+    runs, tiles = box_moves(src0, shp.reshape(len(boxes), -1), ss.reshape(len(boxes), -1), d0,
+                            ds.reshape(len(boxes), -1))
+    return Gather(runs, out.size, src_size, tiles)
+
+
 @lru_cache(maxsize=4096)
 def fuse_map(s: Struct, rec: FuseRecord) -> Tuple[Struct, Gather]:
     """The fused layout (canonical over rec.fused) and the gather into it from layout s
-    (tnkit/blocks.py:1227-1241)."""
+    (tnkit/blocks.py:1227-1241). Every block of s lands whole in one fused block: group g's
+    sub-sector tuple names the fused sector and the row offset inside it (rec.sectors), and the
+    block's axes of group g are laid out row-major inside the fused row. One box per source
+    block."""
     out = canonical(rec.fused, rec.charge)
-    perm, n = list(rec.perm), len(rec.legs)
-    kb_l, d0_l, ds_l = [], [], []
-    for k, fb in enumerate(out.blocks):
-        ost = out.st[k].tolist()
-        for subs in itertools.product(*(rec.sectors[g][fb[g]] for g in range(len(rec.groups)))):
-            kb = s.index[_compose(n, ((rec.groups[g], t) for g, (t, _, _) in enumerate(subs)))]
-            dims = s.shp[kb].tolist()
-            ds = [r * ost[g] for g, grp in enumerate(rec.groups)
-                  for r in _row_major([dims[a] for a in grp])]
-            kb_l.append(kb)
-            d0_l.append(int(out.off[k]) + sum(off * ost[g] for g, (_, off, _) in enumerate(subs)))
-            ds_l.append(ds)
-    if not kb_l:
-        z = np.zeros(0, np.int64)
-        return out, Gather((z, z, z, z), out.size, s.size)
-    kb = np.array(kb_l, np.int64)
-    runs, tiles = box_moves(s.off[kb], s.shp[kb][:, perm], s.st[kb][:, perm],
-                            np.array(d0_l, np.int64), np.array(ds_l, np.int64).reshape(len(kb), n))
-    return out, Gather(runs, out.size, s.size, tiles)
+    where = [{t: (k, off) for k, items in enumerate(rec.sectors[g]) for t, off, _ in items}
+             for g in range(len(rec.groups))]
+    perm = list(rec.perm)
+    boxes = []
+    for kb, blk in enumerate(s.blocks):
+        hit = [where[g][tuple(blk[a] for a in grp)] for g, grp in enumerate(rec.groups)]
+        kd = out.index[tuple(k for k, _ in hit)]
+        ost = out.st[kd].tolist()
+        dst = [r * ost[g] for g, grp in enumerate(rec.groups)
+               for r in _row_major([int(s.shp[kb][a]) for a in grp])]
+        d0 = int(out.off[kd]) + sum(off * ost[g] for g, (_, off) in enumerate(hit))
+        boxes.append((int(s.off[kb]), s.shp[kb][perm], s.st[kb][perm], d0, dst))
+    return out, _gather_of(out, s.size, boxes)
 
 
 @lru_cache(maxsize=4096)
 def split_map(s: Struct, rec: FuseRecord, todo: Tuple[int, ...]) -> Tuple[Struct, Gather]:
-    """Split the fused legs `todo` of a tensor laid out as s, keeping its other legs; a full split
-    also undoes the fuse's permutation (tnkit/blocks.py:1244-1286). Returns the canonical output
-    layout and the gather into it."""
-    where = {g: {t: (k, off, nn) for k, sec in enumerate(rec.sectors[g]) for t, off, nn in sec}
-             for g in todo}
-    legs, origin = [], []
+    """Split the fused legs `todo` of a tensor laid out as s (its leg g is rec.fused[g] for g in
+    todo), keeping its other legs; a full split also undoes the fuse's permutation
+    (tnkit/blocks.py:1244-1286). Returns the canonical output layout and the gather into it. A
+    source block's fused sector on leg g stacks the sub-blocks of rec.sectors[g]; every choice
+    of one sub-block per split leg is one output block, read at its row offsets with the split
+    axes row-major inside the fused row."""
+    legs = []
     for g in range(len(s.legs)):
-        if g in where:
-            legs += [rec.legs[a] for a in rec.groups[g]]
-            origin += [(g, p) for p in range(len(rec.groups[g]))]
-        else:
-            legs.append(s.legs[g])
-            origin.append((g, None))
+        legs += [rec.legs[a] for a in rec.groups[g]] if g in todo else [s.legs[g]]
     full = len(todo) == len(rec.groups)
-    perm = tuple(int(i) for i in np.argsort(rec.perm)) if full else tuple(range(len(legs)))
-    out = canonical(tuple(legs[i] for i in perm), s.charge)
-    rows = []
-    for b in out.blocks:
-        bg = [0] * len(legs)
-        for j, i in enumerate(perm):
-            bg[i] = b[j]
-        fb, sub = [0] * len(s.legs), {}
-        for i, (g, p) in enumerate(origin):
-            if p is None:
-                fb[g] = bg[i]
-            else:
-                sub.setdefault(g, []).append(bg[i])
-        start = {}
-        for g, t in sub.items():
-            fb[g], start[g], _ = where[g][tuple(t)]
-        kb = s.index[tuple(fb)]
+    order = [int(i) for i in np.argsort(rec.perm)] if full else list(range(len(legs)))
+    out = canonical(tuple(legs[i] for i in order), s.charge)
+    boxes = []
+    for kb, blk in enumerate(s.blocks):
         sst = s.st[kb].tolist()
-        inner = {g: _row_major([legs[i].dims[bg[i]] for i, (h, p) in enumerate(origin)
-                                if h == g and p is not None]) for g in sub}
-        st = [sst[g] if p is None else sst[g] * inner[g][p] for g, p in origin]
-        rows.append((int(s.off[kb]) + sum(start[g] * sst[g] for g in sub), [st[i] for i in perm]))
-    if not rows:
-        z = np.zeros(0, np.int64)
-        return out, Gather((z, z, z, z), out.size, s.size)
-    src0 = np.array([r[0] for r in rows], np.int64)
-    ss = np.array([r[1] for r in rows], np.int64).reshape(len(rows), len(legs))
-    runs, tiles = box_moves(src0, out.shp, ss, out.off, out.st)
-    return out, Gather(runs, out.size, s.size, tiles)
+        for pick in itertools.product(*(rec.sectors[g][blk[g]] for g in todo)):
+            sub = dict(zip(todo, pick))
+            sec, st = [], []
+            src = int(s.off[kb])
+            for g in range(len(s.legs)):
+                if g not in sub:
+                    sec.append(blk[g])
+                    st.append(sst[g])
+                    continue
+                t, off, _ = sub[g]
+                src += off * sst[g]
+                sec += list(t)
+                st += [r * sst[g] for r in _row_major([rec.legs[a].dims[t[i]]
+                                                        for i, a in enumerate(rec.groups[g])])]
+            kd = out.index[tuple(sec[i] for i in order)]
+            boxes.append((src, out.shp[kd], [st[i] for i in order], int(out.off[kd]),
+                          out.st[kd]))
+    return out, _gather_of(out, s.size, boxes)
 
 
 @lru_cache(maxsize=4096)

@@ -10,6 +10,7 @@ in tnkit.
 """
 from __future__ import annotations
 
+import os
 import time
 import warnings
 
@@ -320,50 +321,59 @@ def _tridiag_matrix(alphas, betas):
     return T
 
 
+def _expm_tridiag(tau, alphas, betas) -> np.ndarray:
+    from scipy.linalg import expm
+    return expm(tau * _tridiag_matrix(alphas, betas))
+
+
 class ExpmMonitor:
-    """The host side of the adaptive Krylov exponential (tnkit/eigensolver.py:230-287): the
-    breakdown test beta < tol, Saad's a-posteriori estimate |beta| |tau| |exp(tau T)[j-1, 0]|
-    beta0 as the adaptive stop (from j = 2), the same estimate frozen at j = est_ref_j for `info`,
-    and the final coefficients beta0 exp(tau T) e_1. `adaptive=None` follows the environment
-    variable TNKIT_EXPM_ADAPTIVE (anything but '0' means on), as tnkit does."""
+    """Stop rule and diagnostics of the adaptive Krylov exponential (tnkit/eigensolver.py:230-287).
+
+    After the j-th Lanczos vector (alphas a_1..a_j, betas b_1..b_{j-1}, and the norm b_j of the
+    residual that would become vector j + 1):
+      * b_j < tol means the Krylov space is invariant: stop without adding the residual;
+      * Saad's first-neglected-term estimate e_j = |b_j| |tau| |exp(tau T_j)[j, 1]| beta0 ends an
+        adaptive solve once e_j < tol, from j = 2 on;
+      * for `info`, the same estimate is frozen at the first j >= est_ref_j (0.0 when the space
+        breaks down before), so it keeps measuring the step size.
+    The estimate is computed only when one of the last two needs it. `adaptive=None` reads
+    TNKIT_EXPM_ADAPTIVE (any value but '0' means adaptive). The final coefficients are
+    beta0 exp(tau T) e_1."""
 
     def __init__(self, tau, beta0, tol, adaptive, info, est_ref_j):
         if adaptive is None:
-            import os
             adaptive = os.environ.get("TNKIT_EXPM_ADAPTIVE", "1") != "0"
-        self.tau, self.beta0, self.tol, self.adaptive = tau, beta0, tol, adaptive
-        self.info, self.est_ref_j = info, est_ref_j
-        self.est = float("inf")
-        self.est_ref, self.est_ref_at, self.beta1 = float("nan"), 0, float("nan")
+        self.tau, self.beta0, self.tol, self.adaptive = tau, beta0, tol, bool(adaptive)
+        self.info, self.ref_dim = info, est_ref_j
+        self.record = {"err_est": float("inf"), "beta1": float("nan"),
+                       "est_ref": float("nan"), "est_ref_j": 0}
 
     def check(self, k, alphas, betas, beta):
         """After step k: 'breakdown' (stop, beta unused), 'converged' (stop after this vector)
         or None."""
-        from scipy.linalg import expm
+        rec, j = self.record, len(alphas)
         if k == 0:
-            self.beta1 = beta
+            rec["beta1"] = beta
+        tracking = self.info is not None and rec["est_ref_j"] == 0
         if beta < self.tol:
-            if self.info is not None and self.est_ref_at == 0:
-                self.est_ref, self.est_ref_at = 0.0, k + 1
+            if tracking:
+                rec["est_ref"], rec["est_ref_j"] = 0.0, j
             return "breakdown"
-        jj = len(alphas)
-        want_ref = self.info is not None and self.est_ref_at == 0 and jj >= self.est_ref_j
-        if (self.adaptive and jj >= 2) or want_ref:
-            Et = expm(self.tau * _tridiag_matrix(alphas, betas))
-            self.est = abs(beta) * abs(self.tau) * abs(Et[jj - 1, 0]) * self.beta0
-            if want_ref:
-                self.est_ref, self.est_ref_at = self.est, jj
-            if self.adaptive and jj >= 2 and self.est < self.tol:
-                return "converged"
-        return None
+        freeze = tracking and j >= self.ref_dim
+        may_stop = self.adaptive and j >= 2
+        if not (freeze or may_stop):
+            return None
+        corner = _expm_tridiag(self.tau, alphas, betas)[j - 1, 0]
+        est = abs(beta) * abs(self.tau) * abs(corner) * self.beta0
+        rec["err_est"] = est
+        if freeze:
+            rec["est_ref"], rec["est_ref_j"] = est, j
+        return "converged" if may_stop and est < self.tol else None
 
     def finish(self, alphas, betas, capped):
-        from scipy.linalg import expm
-        c = self.beta0 * expm(self.tau * _tridiag_matrix(alphas, betas))[:, 0]
         if self.info is not None:
-            self.info.update(krylov_dim=len(alphas), err_est=self.est, beta1=self.beta1,
-                             est_ref=self.est_ref, est_ref_j=self.est_ref_at, capped=capped)
-        return c
+            self.info.update(self.record, krylov_dim=len(alphas), capped=capped)
+        return self.beta0 * _expm_tridiag(self.tau, alphas, betas)[:, 0]
 
 
 def lanczos_expm_flat(matvec, x: torch.Tensor, tau, m=30, tol=1e-12, adaptive=None, info=None,
