@@ -84,3 +84,25 @@ def test_product_cut_is_the_oracle_rule():
         st = float(rng.choice([0.0, 1e-3, 0.05]))
         dt = float(rng.choice([0.0, 1e-6, 0.2]))
         assert cut(s, chi, st, dt) == OE.cut(s, chi, st, dt)
+
+
+def test_product_cut_groups_zeros_and_prior_loss():
+    """The same with long runs of near-equal values, zero tails, a prior loss disc0 and an
+    explicit total."""
+    from tnkit_amd.tdvp_evolver import cut
+    rng = np.random.default_rng(11)
+    for _ in range(500):
+        n = int(rng.integers(1, 16))
+        s = np.sort(rng.random(n) ** 3)[::-1]
+        for _g in range(int(rng.integers(0, 3))):
+            a = int(rng.integers(0, n))
+            b = int(rng.integers(a, n))
+            s[a:b + 1] = s[a] * (1 - 1e-9 * np.arange(b - a + 1))
+        if rng.random() < 0.3:
+            s[int(rng.integers(0, n)):] = 0.0
+        chi = int(rng.integers(1, 18))
+        st = float(rng.choice([0.0, 1e-4, 1e-2, 0.3]))
+        dt = float(rng.choice([0.0, 1e-7, 1e-3, 0.5]))
+        d0 = float(rng.choice([0.0, 1e-5]))
+        tot = None if rng.random() < 0.5 else float(np.sum(s ** 2)) * 1.5
+        assert cut(s, chi, st, dt, d0, tot) == OE.cut(s, chi, st, dt, d0, tot)

@@ -390,6 +390,39 @@ def _tower_penalty(energies, penalty):
     return max(penalty, 3.0 * (max(energies) - min(energies)) + penalty)
 
 
+def _tower_level(mpo, chi, n, states, w0, seed, ov_tol, max_retries, warm, deflate, verbose,
+                 kw):
+    """Level n of the tower (tnkit/dmrg_excited.py:1097-1128). Try k = 0, 1, ... runs
+    dmrg_excited with penalty w0 4^k and seed seed + 17 n + 101 k; only try 0 may start from the
+    level below (`warm`). A try whose largest overlap with the levels below is under ov_tol is
+    kept; otherwise the next try follows, and the last try is kept whatever its overlap. Its
+    info gains `penalty`, `attempt` and the matvec counts of every try."""
+    calls: List[int] = []
+    for attempt in range(max_retries):
+        w = w0 * 4.0 ** attempt
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore", DeflationWarning)
+            E, M, info = dmrg_excited(mpo, chi, prev_states=states, penalty=w,
+                                      seed=seed + 17 * n + 101 * attempt,
+                                      init_mps=states[-1] if warm and attempt == 0 else None,
+                                      verbose=False, deflate=deflate, **kw)
+        calls += info["matvecs_per_sweep"]
+        info.update(penalty=w, attempt=attempt)
+        ov = max(info["penalty_overlaps"], default=0.0)
+        if ov < ov_tol:
+            break
+        if verbose and attempt + 1 < max_retries:
+            print(f"  level {n}: overlap {ov:.2e} > {ov_tol:g}; retry with penalty {4.0 * w:g}",
+                  flush=True)
+    info["matvecs_all_attempts"] = calls
+    if verbose:
+        print(f"  level {n}: E = {E:.12f}   sweeps={info['sweeps']:2d} "
+              f"max|<prev|psi>|={ov:.2e}  w={info['penalty']:g}", flush=True)
+    _warn_overlaps(info["penalty_overlaps"], info["penalty"], deflate,
+                   f"dmrg_tower level {n}", " or `max_retries`")
+    return E, M, info
+
+
 def dmrg_tower(mpo, chi, nlevels, penalty=1.0, seed=0, verbose=False, ov_tol=1e-5,
                max_retries=2, warm_chain=True, deflate="penalty", pair_solve=False, **kw):
     """The lowest `nlevels` eigenstates by repeated penalised DMRG with tnkit's adaptive
@@ -407,32 +440,10 @@ def dmrg_tower(mpo, chi, nlevels, penalty=1.0, seed=0, verbose=False, ov_tol=1e-
     energies: List[float] = []
     infos = []
     for n in range(nlevels):
-        w = _tower_penalty(energies, penalty)
-        calls = []
-        for attempt in range(max_retries):
-            warm = warm_chain and deflate == "penalty" and states and attempt == 0
-            init = states[-1] if warm else None
-            with warnings.catch_warnings():
-                warnings.simplefilter("ignore", DeflationWarning)
-                E, M, info = dmrg_excited(mpo, chi, prev_states=states, penalty=w,
-                                          seed=seed + 17 * n + 101 * attempt, init_mps=init,
-                                          verbose=False, deflate=deflate, **kw)
-            calls.extend(info["matvecs_per_sweep"])
-            ov = max(info["penalty_overlaps"], default=0.0)
-            info["penalty"] = w
-            info["attempt"] = attempt
-            if ov < ov_tol:
-                break
-            w *= 4.0
-            if verbose:
-                print(f"  level {n}: overlap {ov:.2e} > {ov_tol:g}; retry with penalty {w:g}",
-                      flush=True)
-        info["matvecs_all_attempts"] = calls
-        if verbose:
-            print(f"  level {n}: E = {E:.12f}   sweeps={info['sweeps']:2d} "
-                  f"max|<prev|psi>|={ov:.2e}  w={w:g}", flush=True)
-        _warn_overlaps(info["penalty_overlaps"], info["penalty"], deflate,
-                       f"dmrg_tower level {n}", " or `max_retries`")
+        E, M, info = _tower_level(mpo, chi, n, states, _tower_penalty(energies, penalty), seed,
+                                  ov_tol, max_retries,
+                                  warm_chain and deflate == "penalty" and bool(states),
+                                  deflate, verbose, kw)
         states.append([t.detach().cpu().numpy() for t in M])
         energies.append(E)
         infos.append(info)

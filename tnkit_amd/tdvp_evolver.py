@@ -34,33 +34,53 @@ from .tdvp1 import C128, _matvec, _solver, one_site_op, step_left, step_right
 METHODS = ("1site", "2site", "12site")
 
 
+def _group_start(s: np.ndarray, k: int, dtol: float) -> int:
+    """Lowest index of the run of values just above position k that lie within dtol (relative
+    to themselves) of s[k]: s[i] - s[k] <= dtol s[i] for every i in it."""
+    ok = (s[:k] - s[k]) <= dtol * s[:k]
+    bad = np.flatnonzero(~ok)
+    return int(bad[-1]) + 1 if bad.size else 0
+
+
+def _group_end(s: np.ndarray, c: int, k: int, dtol: float) -> int:
+    """One past the run of values s[c:k] that lie within dtol (relative to s[c-1]) of s[c-1]."""
+    ok = (s[c - 1] - s[c:k]) <= dtol * s[c - 1]
+    bad = np.flatnonzero(~ok)
+    return c + (int(bad[0]) if bad.size else k - c)
+
+
 def cut(s: np.ndarray, chi: int, svd_tol: float = 0.0, degenerate_tol: float = 0.0,
         disc0: float = 0.0, tot: Optional[float] = None) -> Tuple[int, bool, float]:
-    """(k, pinned, relative dropped weight): where to cut a descending spectrum
-    (tnkit/_dmrg.py:77-113). The cap keeps min(chi, len(s)); degenerate_tol keeps or drops a
-    group of near-equal values whole (pinned when that moved the cap cut below the cap);
-    svd_tol keeps the fewest values whose relative dropped weight stays within it."""
+    """(k, pinned, relative dropped weight) for a descending spectrum s (tnkit/_dmrg.py:77-113).
+
+    * Cap: keep min(chi, len(s)). With degenerate_tol > 0 a cap cut that would split a group of
+      near-equal values (within degenerate_tol, relative to the larger) moves down to the
+      group's start, unless the group starts at s[0] or s[k] is zero; `pinned` says the cap cut
+      moved.
+    * svd_tol > 0: the fewest kept values, at least one, whose relative dropped weight
+      disc0 + sum(s[j:]^2) / tot stays within svd_tol (tail sums accumulated from the small
+      end); that cut moves up to the end of the group of the last kept value, and is taken
+      only when it lies below the cap cut (then nothing is pinned).
+    tot defaults to sum(s^2) (1 for an empty or zero spectrum)."""
     n = len(s)
     tot = tot or float(np.sum(s ** 2)) or 1.0
-    k = min(chi, n)
+    cap = min(chi, n)
+    k = cap
     if degenerate_tol > 0 and k < n and s[k] > 0:
-        j = k
-        while j > 0 and s[j - 1] - s[k] <= degenerate_tol * s[j - 1]:
-            j -= 1
-        k = j or k
-    pinned = k < min(chi, n)
+        k = _group_start(s, k, degenerate_tol) or k
+    pinned = k < cap
     disc = disc0 + float(np.sum(s[k:] ** 2)) / tot
-    if svd_tol > 0.0 and k > 1:
-        tail = np.cumsum((s[:k] ** 2)[::-1])[::-1]
-        drop = tail / tot + disc
-        ok = np.nonzero(drop[1:] <= svd_tol)[0]
-        if len(ok):
-            c = j = int(ok[0]) + 1
-            if degenerate_tol > 0:
-                while j < k and s[c - 1] - s[j] <= degenerate_tol * s[c - 1]:
-                    j += 1
-            if j < k:
-                k, pinned, disc = j, False, float(drop[j])
+    if svd_tol <= 0.0 or k <= 1:
+        return k, pinned, disc
+    loss = np.cumsum((s[:k] ** 2)[::-1])[::-1] / tot + disc   # loss[j]: keep j values
+    within = np.flatnonzero(loss[1:] <= svd_tol)
+    if not within.size:
+        return k, pinned, disc
+    j = int(within[0]) + 1
+    if degenerate_tol > 0:
+        j = _group_end(s, j, k, degenerate_tol)
+    if j < k:
+        return j, False, float(loss[j])
     return k, pinned, disc
 
 
