@@ -69,6 +69,23 @@ int tnh_apply_two_site_fused(int dtype, int io_dtype, int a, int d1, int d2, int
                              void* out, void* ws, size_t ws_bytes, int tile1, int tile3,
                              tnh_stream_t stream);
 
+/* Bytes of the fixed device scratch of tnh_apply_two_site_fused_noalloc (192 MiB per GPU):
+ * enough for a float64 operator up to a = b = 512 and a float32 operator with float64 vectors
+ * up to a = b = 1024, at d = 2 and MPO bonds 5. */
+#define TNH_NOALLOC_SCRATCH_BYTES (192u * 1024u * 1024u)
+
+/* tnh_apply_two_site_fused with no workspace argument and no allocation of any kind. No
+ * counterpart in tnkit: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The workspace is a
+ * file-scope device array of TNH_NOALLOC_SCRATCH_BYTES that each GPU holds from the first call
+ * on it until the process ends; shapes needing more (tnh_two_site_ws_bytes) are refused with
+ * TNH_ERR_WS. All calls on one GPU share that array, so calls that overlap on one GPU (other
+ * streams, other host threads) race: serialise them. */
+int tnh_apply_two_site_fused_noalloc(int dtype, int io_dtype, int a, int d1, int d2, int b,
+                                     int wl, int wr, const void* LT, const void* S,
+                                     const void* RT, const void* theta, void* out, int tile1,
+                                     int tile3, tnh_stream_t stream);
+
 /* tnh_apply_two_site_fused restricted to some of its kernels, for timing one at a time:
  * stages bit 0 = GEMM1 (theta x L), bit 1 = the MPO mix, bit 2 = GEMM3 (x R); 7 = all. No
  * counterpart in tnkit; a profiling hook over the same workspace layout. */

@@ -30,6 +30,7 @@ SIGNATURES = {
     "tnh_two_site_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int, c_int, c_int, c_int]),
     "tnh_apply_two_site_fused": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                          _P, _P, _P, _P, _P, _P, c_size_t, c_int, c_int, _P]),
+    "tnh_apply_two_site_fused_noalloc": (c_int, [c_int] * 8 + [_P] * 5 + [c_int, c_int, _P]),
     "tnh_apply_two_site_stages": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                           _P, _P, _P, _P, _P, _P, c_size_t, c_int, c_int, c_int,
                                           _P]),
