@@ -5,9 +5,10 @@ tnkit has no autograd (its arrays are numpy or plain torch without gradients); t
 are this package's extension for the Hellmann-Feynman study of the field derivatives
 dE/dg_i = <dH/dg_i> = -<Z_i> of the TFIM with per-site fields.
 
-* `HeffMatvec`: y = H_eff theta (tnkit/_dmrg.py:183-189). Backward: theta_bar = H_eff^T y_bar, the
-  same kernels on the transposed operator (bra and ket of L and R swapped, W transposed in its
-  physical indices), so no intermediate is stored.
+* `HeffMatvec`: y = H_eff theta (tnkit/_dmrg.py:183-189). Backward: theta_bar = H_eff^H y_bar, the
+  same kernels on the adjoint operator (bra and ket of L and R swapped, W transposed in its
+  physical indices, all conjugated), so no intermediate is stored; the gradients of L, W1, W2
+  and R are chains of K-major GEMMs with the MPO mix in between.
 * `MPOExpectation`: E(W_0, ..., W_{N-1}) = <psi|H|psi> / <psi|psi> for a fixed MPS, forward by
   the left environment chain (tnkit/_dmrg.py:216-222) keeping only the last environment.
   Backward: the right environments are rebuilt from the right, the left ones recomputed on the
@@ -24,26 +25,101 @@ import torch
 from . import kernels
 
 
-def _transpose_operator(L, W1, W2, R):
-    """The operator of H_eff^T: L and R with bra and ket swapped, W with s and s' swapped."""
-    return (L.permute(2, 1, 0).contiguous(), W1.permute(0, 1, 3, 2).contiguous(),
-            W2.permute(0, 1, 3, 2).contiguous(), R.permute(2, 1, 0).contiguous())
+def _adjoint_operator(L, W1, W2, R):
+    """The operator of H_eff^H: L and R with bra and ket swapped, W with s and s' swapped, all
+    conjugated (conjugation is a no-op on real tensors)."""
+    def cj(t):
+        return t.conj().resolve_conj() if t.is_complex() else t
+    return (cj(L.permute(2, 1, 0)).contiguous(), cj(W1.permute(0, 1, 3, 2)).contiguous(),
+            cj(W2.permute(0, 1, 3, 2)).contiguous(), cj(R.permute(2, 1, 0)).contiguous())
+
+
+def _kmajor(t: torch.Tensor, k: int) -> torch.Tensor:
+    """t (K, M) as given when k == 0, or t (M, K) transposed to (K, M), contiguous."""
+    return (t if k == 0 else t.t()).contiguous()
+
+
+def _operand_grads(z, x, L, W1, W2, R, need):
+    """dF/dL, dF/dW1, dF/dW2, dF/dR of F = sum z . (H_eff(L, W1, W2, R) x), for z = conj(ybar),
+    each None unless `need` asks for it. With W12[w,r,S,S'] = sum_m W1[w,m,s,s'] W2[m,r,t,t']
+    (S = (s t)) the three are
+        dL[p,w,q]  = sum z[p,S,b'] W12[w,r,S,S'] R[b',r,b] x[q,S',b]
+        dR[b',r,b] = sum z[p,S,b'] L[p,w,q] W12[w,r,S,S'] x[q,S',b]
+        dW12       = sum z[p,S,b'] L[p,w,q] R[b',r,b] x[q,S',b]
+    and each is a chain of K-major GEMMs (K = a or b, then the small MPO mix, then K = d^2 a or
+    d^2 b); dW1 and dW2 follow from dW12 by the other factor."""
+    a, d1, d2, b = x.shape
+    wl, wm, wr = W1.shape[0], W1.shape[1], W2.shape[1]
+    S = d1 * d2
+    W12 = torch.einsum("wmsu,mrtv->wrstuv", W1, W2).reshape(wl, wr, S, S)
+    xm, zm = x.reshape(a, S, b), z.reshape(a, S, b)
+    gL = gR = gW12 = None
+    if need[0]:
+        XR = kernels.gemm_tn(_kmajor(xm.reshape(a * S, b), 1),
+                             _kmajor(R.reshape(b * wr, b), 1))           # [(q S')][(b' r)]
+        XR = XR.reshape(a, S, b, wr).permute(1, 3, 0, 2).reshape(S * wr, a * b).contiguous()
+        Wk = W12.permute(3, 1, 0, 2).reshape(S * wr, wl * S).contiguous()  # [(S' r)][(w S)]
+        Y = kernels.gemm_tn(XR, Wk).reshape(a, b, wl, S)                   # [(q b')][(w S)]
+        Yk = Y.permute(3, 1, 0, 2).reshape(S * b, a * wl).contiguous()     # [(S b')][(q w)]
+        gL = kernels.gemm_tn(_kmajor(zm.reshape(a, S * b), 1), Yk)         # [p][(q w)]
+        gL = gL.reshape(a, a, wl).permute(0, 2, 1).contiguous()
+    if need[1] or need[2] or need[3]:
+        LX = kernels.gemm_tn(_kmajor(L.reshape(a * wl, a), 1),
+                             xm.reshape(a, S * b).contiguous())          # [(p w)][(S' b)]
+        LX = LX.reshape(a, wl, S, b)
+        if need[3]:
+            LXk = LX.permute(1, 2, 0, 3).reshape(wl * S, a * b).contiguous()  # [(w S')][(p b)]
+            Wk = W12.permute(0, 3, 2, 1).reshape(wl * S, S * wr).contiguous()  # [(w S')][(S r)]
+            Y2 = kernels.gemm_tn(LXk, Wk).reshape(a, b, S, wr)             # [(p b)][(S r)]
+            Y2k = Y2.permute(0, 2, 3, 1).reshape(a * S, wr * b).contiguous()  # [(p S)][(r b)]
+            gR = kernels.gemm_tn(zm.reshape(a * S, b).contiguous(), Y2k).reshape(b, wr, b)
+        if need[1] or need[2]:
+            ZR = kernels.gemm_tn(_kmajor(zm.reshape(a * S, b), 1),
+                                 R.reshape(b, wr * b).contiguous())        # [(p S)][(r b)]
+            ZRk = ZR.reshape(a, S, wr, b).permute(0, 3, 1, 2).reshape(a * b, S * wr).contiguous()
+            LXk = LX.permute(0, 3, 1, 2).reshape(a * b, wl * S).contiguous()  # [(p b)][(w S')]
+            G = kernels.gemm_tn(LXk, ZRk).reshape(wl, S, S, wr)            # [(w S')][(S r)]
+            gW12 = G.permute(0, 3, 2, 1).reshape(wl, wr, d1, d2, d1, d2)
+    gW1 = gW2 = None
+    if need[1]:
+        gW1 = torch.einsum("wrstuv,mrtv->wmsu", gW12, W2)
+    if need[2]:
+        gW2 = torch.einsum("wrstuv,wmsu->mrtv", gW12, W1)
+    return gL, gW1, gW2, gR
 
 
 class HeffMatvec(torch.autograd.Function):
-    """y = H_eff theta with theta_bar = H_eff^T y_bar in backward (environments fixed)."""
+    """y = H_eff theta (environments and MPO as inputs). Backward: theta_bar = H_eff^H y_bar by
+    the forward kernels on the adjoint operator, and, for the inputs that require them, the
+    gradients of L, W1, W2 and R (torch's convention: the conjugate of dF/dT for
+    F = sum conj(y_bar) y, which is linear in each)."""
 
     @staticmethod
     def forward(ctx, theta, L, W1, W2, R):
-        ctx.save_for_backward(L, W1, W2, R)
+        theta = theta.contiguous()
+        ctx.save_for_backward(theta, L, W1, W2, R)
         op = kernels.fused_two_site(L, W1, W2, R)
-        return kernels.apply_two_site_fused(theta.contiguous(), *op)
+        return kernels.apply_two_site_fused(theta, *op)
 
     @staticmethod
     def backward(ctx, ybar):
-        L, W1, W2, R = ctx.saved_tensors
-        opT = kernels.fused_two_site(*_transpose_operator(L, W1, W2, R))
-        return kernels.apply_two_site_fused(ybar.contiguous(), *opT), None, None, None, None
+        theta, L, W1, W2, R = ctx.saved_tensors
+        ybar = ybar.contiguous()
+        gth = None
+        if ctx.needs_input_grad[0]:
+            opH = kernels.fused_two_site(*_adjoint_operator(L, W1, W2, R))
+            gth = kernels.apply_two_site_fused(ybar, *opH)
+        need = ctx.needs_input_grad[1:]
+        if not any(need):
+            return gth, None, None, None, None
+        z = ybar.conj().resolve_conj() if ybar.is_complex() else ybar
+        gL, gW1, gW2, gR = _operand_grads(z, theta, L.contiguous(), W1.contiguous(),
+                                          W2.contiguous(), R.contiguous(),
+                                          (need[0], need[1], need[2], need[3]))
+        if ybar.is_complex():
+            gL, gW1, gW2, gR = (g.conj().resolve_conj() if g is not None else None
+                                for g in (gL, gW1, gW2, gR))
+        return gth, gL, gW1, gW2, gR
 
 
 def mpo_gradient(L: torch.Tensor, A: torch.Tensor, W: torch.Tensor, R: torch.Tensor) -> torch.Tensor:
@@ -101,7 +177,9 @@ class MPOExpectation(torch.autograd.Function):
         grads = []
         L = torch.ones((1, 1, 1), dtype=dt, device=dev)
         for i in range(N):
-            grads.append(mpo_gradient(L, M[i], W[i], R[i + 1]) * scale)
+            gi = mpo_gradient(L, M[i], W[i], R[i + 1])
+            # torch's gradient of the real E is the conjugate of dE/dW for complex tensors
+            grads.append((gi.conj().resolve_conj() if gi.is_complex() else gi) * scale)
             if i + 1 < N:
                 L = kernels.update_left_env(L, M[i], W[i])
         return (None, *grads)
