@@ -45,3 +45,20 @@ def test_bad_arguments_are_refused_before_any_launch():
     assert b"positive" in lib.tnh_last_error()
     rc = lib.tnh_gemm_tn(0, 8, 8, 8, None, 4, None, 8, None, 8, 1, 0, 0, 0, 0, 0, None)
     assert rc == _lib.ERR_ARG
+
+
+def test_svd_jacobi_refuses_overlapping_output_strides():
+    """A batch whose U (or Vh, S) strides are smaller than one matrix's footprint would have
+    workgroups writing over each other: refused before any launch."""
+    lib = _lib.load()
+    args = dict(m=16, n=8, lda=8, sA=128, S=None, sS=8, U=None, ldu=8, sU=128, Vh=None, ldv=8,
+                sV=64)
+
+    def call(**kw):
+        a = {**args, **kw}
+        return lib.tnh_svd_jacobi(_lib.F64, 4, a["m"], a["n"], None, a["lda"], a["sA"], a["S"],
+                                  a["sS"], a["U"], a["ldu"], a["sU"], a["Vh"], a["ldv"], a["sV"],
+                                  60, 1e-15, None, None)
+    for bad in (dict(sU=100), dict(sV=50), dict(sS=4)):
+        assert call(**bad) == _lib.ERR_ARG
+        assert b"overlap" in lib.tnh_last_error()

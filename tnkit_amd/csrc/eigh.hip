@@ -553,6 +553,11 @@ extern "C" int tnh_svd_jacobi(int dtype, long batch, int m, int n, const void* A
               TNH_ERR_ARG, "tnh_svd_jacobi: bad sizes batch=%ld m=%d n=%d lda=%ld ldu=%ld ldv=%ld",
               batch, m, n, lda, ldu, ldv);
   if (batch == 0) return TNH_OK;
+  // each matrix's outputs must not overlap the next one's (A is only read, so it may repeat)
+  TNH_REQUIRE(batch == 1 || (sU >= (long)(m - 1) * ldu + N && sV >= (long)(N - 1) * ldv + n &&
+                             sS >= N),
+              TNH_ERR_ARG, "tnh_svd_jacobi: output batch strides overlap (sU=%ld sV=%ld sS=%ld)",
+              sU, sV, sS);
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (dtype == TNH_F64)
     return tnh::launch_svd_jacobi<double>(batch, m, n, A, lda, sA, S, sS, U, ldu, sU, Vh, ldv, sV,
