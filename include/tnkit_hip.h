@@ -98,16 +98,60 @@ int tnh_apply_two_site_stages(int dtype, int io_dtype, int a, int d1, int d2, in
 size_t tnh_env_ws_bytes(int dtype, int a, int d, int c, int wl, int wr);
 
 /* update_left_env (tnkit/_dmrg.py:216-222): out (c, wr, c) = L (a, wl, a) grown past A (a, d, c)
- * with W (wl, wr, d, d). Real dtypes (F64, F32). */
+ * with W (wl, wr, d, d). dtypes F64, F32, C128, C64 (complex: the bra is conj(A)). */
 int tnh_update_left_env(int dtype, int a, int d, int c, int wl, int wr, const void* L,
                         const void* A, const void* W, void* out, void* ws, size_t ws_bytes,
                         tnh_stream_t stream);
 
 /* update_right_env (tnkit/_dmrg.py:225-231): out (c, wl, c) = R (b, wr, b) grown past
- * B (c, d, b) with W (wl, wr, d, d). Real dtypes (F64, F32). */
+ * B (c, d, b) with W (wl, wr, d, d). dtypes F64, F32, C128, C64 (complex: the bra is conj(B)). */
 int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const void* R,
                          const void* B, const void* W, void* out, void* ws, size_t ws_bytes,
                          tnh_stream_t stream);
+
+/* ---- backward (no tnkit counterpart: tnkit has no autograd; these are the hand-written
+ * backward kernels of the differentiation study) -------------------------------------------- */
+
+/* Workspace bytes of tnh_matvec2_bwd. */
+size_t tnh_matvec2_bwd_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr);
+
+/* The VJP of apply_two_site_fused (tnkit/_dmrg.py:183-189) with respect to theta:
+ * theta_bar (a, d1, d2, b) = H_eff^H ybar, H_eff built from L (a, wl, a), W1 (wl, wm, d1, d1),
+ * W2 (wm, wr, d2, d2), R (b, wr, b). The adjoint operator (bra and ket of L and R swapped, s and
+ * s' of W swapped, all conjugated) is built in the workspace and applied by the forward kernels.
+ * dtypes F64, F32, C128, C64. */
+int tnh_matvec2_bwd(int dtype, int a, int d1, int d2, int b, int wl, int wm, int wr,
+                    const void* L, const void* W1, const void* W2, const void* R,
+                    const void* ybar, void* theta_bar, void* ws, size_t ws_bytes,
+                    tnh_stream_t stream);
+
+/* Workspace bytes of tnh_env_left_bwd / tnh_env_right_bwd for bonds (a, c). */
+size_t tnh_env_bwd_ws_bytes(int dtype, int a, int d, int c, int wl, int wr);
+
+/* The VJP of update_left_env (tnkit/_dmrg.py:216-222) with respect to L: given the cotangent
+ * Lbar_out (c, wr, c) of L' = L grown past A (a, d, c) with W (wl, wr, d, d), writes
+ * Lbar (a, wl, a) (torch's convention: the conjugate of dF/dL for F = sum conj(Lbar_out) L').
+ * It is the right-environment update of conj(Lbar_out) past A, conjugated. */
+int tnh_env_left_bwd(int dtype, int a, int d, int c, int wl, int wr, const void* Lbar_out,
+                     const void* A, const void* W, void* Lbar, void* ws, size_t ws_bytes,
+                     tnh_stream_t stream);
+
+/* The VJP of update_right_env (tnkit/_dmrg.py:225-231) with respect to R: Rbar_out (c, wl, c),
+ * B (c, d, b), W (wl, wr, d, d) -> Rbar (b, wr, b); the mirror of tnh_env_left_bwd. */
+int tnh_env_right_bwd(int dtype, int c, int d, int b, int wl, int wr, const void* Rbar_out,
+                      const void* B, const void* W, void* Rbar, void* ws, size_t ws_bytes,
+                      tnh_stream_t stream);
+
+/* ---- hipGraph capture of library calls (replaces tnkit/block_graph.py:89-180 and the capture
+ * half of tnkit/cudagraph.py:87-342 for C callers): tnh_graph_begin starts a thread-local
+ * capture on `stream` (not the null stream); every library call issued on it is recorded (the
+ * library allocates nothing and never synchronises, so all of them can be); tnh_graph_end
+ * instantiates the recording into *graph; tnh_graph_launch replays it on a stream;
+ * tnh_graph_destroy frees it. Pointers are baked in: replays read and write the same buffers. */
+int tnh_graph_begin(tnh_stream_t stream);
+int tnh_graph_end(tnh_stream_t stream, void** graph);
+int tnh_graph_launch(void* graph, tnh_stream_t stream);
+int tnh_graph_destroy(void* graph);
 
 /* ---- batched forms: `batch` independent problems of one shape, element strides s* between
  * consecutive problems (the legs of tnkit's batched evolution, tnkit/batch_tdvp.py:211-243).
@@ -186,6 +230,21 @@ int tnh_norm(long n, const double* x, double* out, void* ws, size_t ws_bytes, tn
 /* out[0] = <x, y>  (tnkit NumpyBackend.vdot for real data, backend.py:292-293). */
 int tnh_dot(long n, const double* x, const double* y, double* out, void* ws, size_t ws_bytes,
             tnh_stream_t stream);
+
+/* Workspace bytes of tnh_rows_dot. */
+size_t tnh_rows_ws_bytes(int dtype);
+
+/* out[r] = sum_i V[r*ldv + i] x[i] for r < k (no conjugation): the einsum "kn,n->k" of tnkit's
+ * Krylov projections (tnkit/eigensolver.py:160, :588, :594) as the backend receives it. dtypes
+ * F64, F32, C128, C64; accumulates in float64 / complex128; deterministic. */
+int tnh_rows_dot(int dtype, int k, long n, const void* V, long ldv, const void* x, void* out,
+                 void* ws, size_t ws_bytes, tnh_stream_t stream);
+
+/* out[i] = sum_{r<k} c[r] V[r*ldv + i] with c on the device: the einsums "k,kn->n",
+ * "r,rn->n", "j,jn->n" of tnkit's eigensolvers (tnkit/eigensolver.py:161, :567, :589, :595,
+ * :672). out may not alias V. */
+int tnh_rows_combine(int dtype, int k, long n, const void* c, const void* V, long ldv, void* out,
+                     tnh_stream_t stream);
 
 /* y = x * (mul / max(div[0], 1e-200)); div may be NULL (then y = mul * x). The division guard is
  * tnkit's guard_denom (tnkit/backend.py:699-708). */

@@ -39,6 +39,15 @@ SIGNATURES = {
                                     c_size_t, _P]),
     "tnh_update_right_env": (c_int, [c_int, c_int, c_int, c_int, c_int, c_int, _P, _P, _P, _P, _P,
                                      c_size_t, _P]),
+    "tnh_matvec2_bwd_ws_bytes": (c_size_t, [c_int] * 7),
+    "tnh_matvec2_bwd": (c_int, [c_int] * 8 + [_P] * 7 + [c_size_t, _P]),
+    "tnh_env_bwd_ws_bytes": (c_size_t, [c_int] * 6),
+    "tnh_env_left_bwd": (c_int, [c_int] * 6 + [_P] * 5 + [c_size_t, _P]),
+    "tnh_env_right_bwd": (c_int, [c_int] * 6 + [_P] * 5 + [c_size_t, _P]),
+    "tnh_graph_begin": (c_int, [_P]),
+    "tnh_graph_end": (c_int, [_P, ctypes.POINTER(c_void_p)]),
+    "tnh_graph_launch": (c_int, [_P, _P]),
+    "tnh_graph_destroy": (c_int, [_P]),
     "tnh_fused_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 7 + [_P]),
     "tnh_apply_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 5 + [_P, c_size_t, c_int,
                                                                           c_int, c_int, _P]),
@@ -56,6 +65,9 @@ SIGNATURES = {
     "tnh_lincomb": (c_int, [c_long, c_int, _P, c_long, _PD, c_double, _P, _P]),
     "tnh_norm": (c_int, [c_long, _P, _P, _P, c_size_t, _P]),
     "tnh_dot": (c_int, [c_long, _P, _P, _P, _P, c_size_t, _P]),
+    "tnh_rows_ws_bytes": (c_size_t, [c_int]),
+    "tnh_rows_dot": (c_int, [c_int, c_int, c_long, _P, c_long, _P, _P, _P, c_size_t, _P]),
+    "tnh_rows_combine": (c_int, [c_int, c_int, c_long, _P, _P, c_long, _P, _P]),
     "tnh_scale_by": (c_int, [c_long, _P, _P, c_double, _P, _P]),
     "tnh_convert": (c_int, [c_int, c_int, c_long, _P, _P, _P]),
     "tnh_gather_chunk": (c_int, []),

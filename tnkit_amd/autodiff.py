@@ -25,15 +25,6 @@ import torch
 from . import kernels
 
 
-def _adjoint_operator(L, W1, W2, R):
-    """The operator of H_eff^H: L and R with bra and ket swapped, W with s and s' swapped, all
-    conjugated (conjugation is a no-op on real tensors)."""
-    def cj(t):
-        return t.conj().resolve_conj() if t.is_complex() else t
-    return (cj(L.permute(2, 1, 0)).contiguous(), cj(W1.permute(0, 1, 3, 2)).contiguous(),
-            cj(W2.permute(0, 1, 3, 2)).contiguous(), cj(R.permute(2, 1, 0)).contiguous())
-
-
 def _kmajor(t: torch.Tensor, k: int) -> torch.Tensor:
     """t (K, M) as given when k == 0, or t (M, K) transposed to (K, M), contiguous."""
     return (t if k == 0 else t.t()).contiguous()
@@ -107,8 +98,8 @@ class HeffMatvec(torch.autograd.Function):
         ybar = ybar.contiguous()
         gth = None
         if ctx.needs_input_grad[0]:
-            opH = kernels.fused_two_site(*_adjoint_operator(L, W1, W2, R))
-            gth = kernels.apply_two_site_fused(ybar, *opH)
+            gth = kernels.matvec2_bwd(L.contiguous(), W1.contiguous(), W2.contiguous(),
+                                      R.contiguous(), ybar)
         need = ctx.needs_input_grad[1:]
         if not any(need):
             return gth, None, None, None, None
@@ -120,6 +111,62 @@ class HeffMatvec(torch.autograd.Function):
             gL, gW1, gW2, gR = (g.conj().resolve_conj() if g is not None else None
                                 for g in (gL, gW1, gW2, gR))
         return gth, gL, gW1, gW2, gR
+
+
+def _cj(t: torch.Tensor) -> torch.Tensor:
+    return t.conj().resolve_conj() if t.is_complex() else t
+
+
+class LeftEnvUpdate(torch.autograd.Function):
+    """L' = update_left_env(L, A, W) (tnkit/_dmrg.py:216-222). Backward: the gradient of L by
+    `tnh_env_left_bwd` (a right-environment update of the cotangent) and of W by
+    `mpo_gradient`; A is treated as fixed (its gradient is not provided and raises)."""
+
+    @staticmethod
+    def forward(ctx, L, A, W):
+        ctx.save_for_backward(L, A, W)
+        return kernels.update_left_env(L.contiguous(), A.contiguous(), W.contiguous())
+
+    @staticmethod
+    def backward(ctx, gout):
+        L, A, W = ctx.saved_tensors
+        if ctx.needs_input_grad[1]:
+            raise NotImplementedError("LeftEnvUpdate: the gradient of the MPS tensor is not "
+                                      "provided")
+        gout = gout.contiguous()
+        gL = kernels.env_left_bwd(gout, A.contiguous(), W.contiguous()) \
+            if ctx.needs_input_grad[0] else None
+        gW = None
+        if ctx.needs_input_grad[2]:
+            # dF/dW = sum L conj(A) A conj(gout); torch's gradient is its conjugate
+            gW = _cj(mpo_gradient(L.contiguous(), A.contiguous(), W, _cj(gout)))
+        return gL, None, gW
+
+
+class RightEnvUpdate(torch.autograd.Function):
+    """R' = update_right_env(R, B, W) (tnkit/_dmrg.py:225-231), the mirror of LeftEnvUpdate:
+    the gradient of R by `tnh_env_right_bwd`, of W by `mpo_gradient` on the mirrored network."""
+
+    @staticmethod
+    def forward(ctx, R, B, W):
+        ctx.save_for_backward(R, B, W)
+        return kernels.update_right_env(R.contiguous(), B.contiguous(), W.contiguous())
+
+    @staticmethod
+    def backward(ctx, gout):
+        R, B, W = ctx.saved_tensors
+        if ctx.needs_input_grad[1]:
+            raise NotImplementedError("RightEnvUpdate: the gradient of the MPS tensor is not "
+                                      "provided")
+        gout = gout.contiguous()
+        gR = kernels.env_right_bwd(gout, B.contiguous(), W.contiguous()) \
+            if ctx.needs_input_grad[0] else None
+        gW = None
+        if ctx.needs_input_grad[2]:
+            # R'[u,w,v] = sum R[x,r,y] conj(B[u,s,x]) W[w,r,s,p] B[v,p,y]: dF/dW is the MPO
+            # gradient with the cotangent as the left environment and R as the right one
+            gW = _cj(mpo_gradient(_cj(gout), B.contiguous(), W, R.contiguous()))
+        return gR, None, gW
 
 
 def mpo_gradient(L: torch.Tensor, A: torch.Tensor, W: torch.Tensor, R: torch.Tensor) -> torch.Tensor:

@@ -160,12 +160,96 @@ class HipBackend:
     def norm_axis(self, x, axis=-1):
         return self.torch.linalg.norm(x, dim=axis)
 
-    # --- contraction (plumbing outside the hot path) ------------------------------------------
+    # --- contraction ----------------------------------------------------------------------------
     def einsum(self, sub, *ops):
-        return self.torch.einsum(sub, *ops)
+        """tnkit's equations on libtnkit_hip: two-operand contractions as one MFMA GEMM, matrix-
+        vector projections on the row kernels (`tnkit_amd.contract`)."""
+        from . import contract
+        return contract.einsum(sub, *ops)
+
+    def mv(self, A, x):
+        """A @ x for a (j, n) matrix and a length-n vector: einsum('jn,n->j', A, x)."""
+        from . import contract
+        return contract.einsum("jn,n->j", A, x)
+
+    def vm(self, c, A):
+        """c @ A for a length-j vector and a (j, n) matrix: einsum('j,jn->n', c, A)."""
+        from . import contract
+        return contract.einsum("j,jn->n", c, A)
+
+    # --- small dense helpers of the device Krylov exponential (tnkit/backend.py:592-751) --------
+    def stack(self, xs):
+        torch = self.torch
+        return torch.stack([x if isinstance(x, torch.Tensor)
+                            else torch.as_tensor(x, device=self.device) for x in xs])
+
+    def stack_axis(self, xs, axis=-1):
+        return self.torch.stack(list(xs), dim=axis)
+
+    def tridiag(self, d, e):
+        t = self.torch
+        T = t.diag(d.to(self.dtype))
+        if e is not None and int(e.shape[0]) > 0:
+            e = e.to(self.dtype)
+            T = T + t.diag(e, 1) + t.diag(e, -1)
+        return T
+
+    def guard_denom(self, x, tiny=1e-200):
+        return self.torch.clamp(x, min=tiny)
+
+    def expm(self, A):
+        return self.torch.matrix_exp(A)
+
+    def expm_host(self, A):
+        from scipy.linalg import expm as _expm
+        return _expm(A.detach().cpu().numpy())
+
+    def expm_taylor(self, A, squarings, order=18):
+        """exp(A) by a truncated Taylor series after `squarings` halvings, then squarings: GEMMs
+        only, no host branch (capturable)."""
+        t = self.torch
+        As = A / (2.0 ** int(squarings))
+        E = t.eye(A.shape[-1], dtype=A.dtype, device=A.device)
+        term = E
+        for k in range(1, int(order) + 1):
+            term = (term @ As) / k
+            E = E + term
+        for _ in range(int(squarings)):
+            E = E @ E
+        return E
+
+    def _sym_tridiag(self, d, e):
+        t = self.torch
+        dd = (d.real if t.is_complex(d) else d).to(self.rdtype)
+        T = t.diag_embed(dd)
+        if e is not None and int(e.shape[-1]) > 0:
+            ee = (e.real if t.is_complex(e) else e).to(self.rdtype)
+            T = T + t.diag_embed(ee, offset=1) + t.diag_embed(ee, offset=-1)
+        return T
+
+    def expm_sym_tridiag_eigh_batched(self, d, e, tau):
+        """exp(tau T_n) for a stack of real symmetric tridiagonals (k, m): the eigendecomposition
+        on the batched Jacobi kernel (m <= 64), exp(tau lam) applied, V diag V^T back."""
+        t = self.torch
+        T = self._sym_tridiag(d, e).contiguous()
+        lam, V = self.eigh(T)
+        tdt = t.complex128 if isinstance(tau, complex) or t.is_tensor(tau) and tau.is_complex() \
+            else t.float64
+        cdt = t.promote_types(self.dtype, tdt)
+        ph = t.exp(t.as_tensor(tau, dtype=cdt, device=T.device) * lam.to(cdt))
+        Vc = V.to(cdt)
+        return (Vc * ph.unsqueeze(-2)) @ Vc.transpose(-1, -2)
+
+    def expm_sym_tridiag_eigh(self, d, e, tau):
+        return self.expm_sym_tridiag_eigh_batched(d.unsqueeze(0),
+                                                  None if e is None else e.unsqueeze(0), tau)[0]
 
     # --- factorisations -----------------------------------------------------------------------
     def svd(self, A):
+        from . import kernels
+        return kernels.thin_svd(A)
+
+    def svd_batched(self, A):
         from . import kernels
         return kernels.thin_svd(A)
 
@@ -173,7 +257,17 @@ class HipBackend:
         return self.torch.linalg.qr(A, mode="reduced")
 
     def eigh(self, A):
-        return self.torch.linalg.eigh(A)
+        """Hermitian eigendecomposition (ascending): the batched Jacobi kernel for orders up to
+        64 in float64 / complex128, the vendor solver above or where Jacobi did not converge."""
+        from . import kernels
+        t = self.torch
+        if (A.is_cuda and A.dim() in (2, 3) and A.dtype in (t.float64, t.complex128)
+                and 1 <= A.shape[-1] <= kernels.EIGH_JACOBI_MAX_N):
+            G = (A.unsqueeze(0) if A.dim() == 2 else A).resolve_conj().contiguous()
+            w, Z, sw = kernels.eigh_jacobi(G)
+            if bool((sw > 0).all()):
+                return (w[0], Z[0]) if A.dim() == 2 else (w, Z)
+        return t.linalg.eigh(A)
 
     def sync(self):
         self.torch.cuda.synchronize(self.device)

@@ -30,6 +30,15 @@ namespace tnh {
 // out[j0][j1][j2] = in[j0*s0 + j1*s1 + j2*s2]   (a 3-axis permutation given by input strides);
 // batch element z = blockIdx.z at element offsets z*bin, z*bout
 template <typename T>
+__device__ __forceinline__ T cj(T v) {
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cj(cpx<T> v) {
+  return conj(v);
+}
+
+template <typename T, bool CJ = false>
 __global__ void permute3_kernel(long n0, long n1, long n2, long s0, long s1, long s2,
                                 const T* __restrict__ in, long bin, T* __restrict__ out, long bout) {
   const long total = n0 * n1 * n2;
@@ -41,8 +50,38 @@ __global__ void permute3_kernel(long n0, long n1, long n2, long s0, long s1, lon
     const long r = o / n2;
     const long j1 = r % n1;
     const long j0 = r / n1;
-    out[o] = in[j0 * s0 + j1 * s1 + j2 * s2];
+    const T v = in[j0 * s0 + j1 * s1 + j2 * s2];
+    if constexpr (CJ) {
+      out[o] = cj(v);
+    } else {
+      out[o] = v;
+    }
   }
+}
+
+// S of the adjoint operator: S'[(s t r)][(w i j)] = conj(sum_m W1[w][m][i][s] W2[m][r][j][t])
+template <typename T>
+__global__ void build_s_adjoint_kernel(int wl, int wm, int wr, int d1, int d2,
+                                       const T* __restrict__ W1, const T* __restrict__ W2,
+                                       T* __restrict__ S) {
+  const int ncol = wl * d1 * d2;
+  const int total = d1 * d2 * wr * ncol;
+  for (int o = blockIdx.x * blockDim.x + threadIdx.x; o < total; o += gridDim.x * blockDim.x) {
+    const int col = o % ncol, row = o / ncol;
+    const int j = col % d2, i = (col / d2) % d1, w = col / (d1 * d2);
+    const int r = row % wr, t = (row / wr) % d2, s = row / (wr * d2);
+    T acc = T(0);
+    for (int m = 0; m < wm; ++m)
+      acc += W1[((w * wm + m) * d1 + i) * d1 + s] * W2[((m * wr + r) * d2 + j) * d2 + t];
+    S[o] = cj(acc);
+  }
+}
+
+// x = conj(x) in place (complex element types)
+template <typename T>
+__global__ void conj_kernel(long n, T* __restrict__ x) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = cj(x[i]);
 }
 
 // S[(s t r)][(w i j)] = sum_m W1[w][m][s][i] * W2[m][r][t][j]   (batch element blockIdx.z)
@@ -412,12 +451,12 @@ inline unsigned grid_for(long total, int threads = 256, long cap = 1L << 16) {
   return static_cast<unsigned>(g);
 }
 
-template <typename T>
+template <typename T, bool CJ = false>
 int permute3(int batch, long n0, long n1, long n2, long s0, long s1, long s2, const void* in,
              long bin, void* out, long bout, hipStream_t st) {
   const long total = n0 * n1 * n2;
   if (total == 0 || batch == 0) return TNH_OK;
-  hipLaunchKernelGGL(permute3_kernel<T>, dim3(grid_for(total), 1, batch), dim3(256), 0, st, n0, n1,
+  hipLaunchKernelGGL((permute3_kernel<T, CJ>), dim3(grid_for(total), 1, batch), dim3(256), 0, st, n0, n1,
                      n2, s0, s1, s2, static_cast<const T*>(in), bin, static_cast<T*>(out), bout);
   TNH_CHECK_LAUNCH("permute3");
   return TNH_OK;
@@ -1085,6 +1124,85 @@ int right_env(int batch, int c, int d, int b, int wl, int wr, const void* R, lon
   return TNH_OK;
 }
 
+// ---- backward: the adjoint matvec and the environment VJPs -------------------------------------
+
+// The operator of H_eff^H in the layouts the matvec streams: LT'[q][p][w] = conj(L[q][w][p]),
+// RT'[r][b][b'] = conj(R[b][r][b']), S' from W1, W2 with s and s' swapped, conjugated.
+template <typename T>
+int fused_adjoint(int a, int wl, int wm, int wr, int d1, int d2, int b, const void* L,
+                  const void* W1, const void* W2, const void* R, T* LT, T* S, T* RT,
+                  hipStream_t st) {
+  TNH_TRY((permute3<T, true>(1, a, a, wl, (long)wl * a, 1, a, L, 0, LT, 0, st)));
+  TNH_TRY((permute3<T, true>(1, wr, b, b, b, (long)wr * b, 1, R, 0, RT, 0, st)));
+  const int total = d1 * d2 * wr * wl * d1 * d2;
+  hipLaunchKernelGGL(build_s_adjoint_kernel<T>, dim3(grid_for(total)), dim3(256), 0, st, wl, wm,
+                     wr, d1, d2, static_cast<const T*>(W1), static_cast<const T*>(W2), S);
+  TNH_CHECK_LAUNCH("build_s_adjoint");
+  return TNH_OK;
+}
+
+template <typename T>
+int conj_inplace(long n, T* x, hipStream_t st) {
+  if constexpr (real_of<T>::cplx) {
+    if (n == 0) return TNH_OK;
+    hipLaunchKernelGGL(conj_kernel<T>, dim3(grid_for(n)), dim3(256), 0, st, n, x);
+    TNH_CHECK_LAUNCH("conj");
+  }
+  return TNH_OK;
+}
+
+inline size_t op_bytes(size_t e, int a, int d1, int d2, int b, int wl, int wr) {
+  const size_t dd = (size_t)d1 * d2;
+  return align_up((size_t)a * a * wl * e, 256) + align_up(dd * wr * wl * dd * e, 256) +
+         align_up((size_t)wr * b * b * e, 256);
+}
+
+template <typename T>
+int matvec2_bwd(int a, int d1, int d2, int b, int wl, int wm, int wr, const void* L,
+                const void* W1, const void* W2, const void* R, const void* ybar, void* theta_bar,
+                void* ws, size_t ws_bytes, hipStream_t st) {
+  const size_t dd = (size_t)d1 * d2;
+  Arena ar(ws, ws_bytes);
+  T* LT = ar.take<T>((size_t)a * a * wl);
+  T* S = ar.take<T>(dd * wr * wl * dd);
+  T* RT = ar.take<T>((size_t)wr * b * b);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_matvec2_bwd: workspace %zu < %zu bytes", ws_bytes,
+              ar.used);
+  const size_t used = align_up(ar.used, 256);
+  TNH_TRY(fused_adjoint<T>(a, wl, wm, wr, d1, d2, b, L, W1, W2, R, LT, S, RT, st));
+  return apply_two_site<T, T>(1, a, d1, d2, b, wl, wr, LT, 0, S, 0, RT, 0, ybar, 0, theta_bar, 0,
+                              static_cast<char*>(ws) + used, ws_bytes - used, st, 0, 0, 7);
+}
+
+// VJP of L' = update_left_env(L, A, W) with respect to L: Lbar = conj(R'(conj(Lbar'), A, W)),
+// R' the right-environment update (A in the bra slot conjugated); for real data
+// Lbar = R'(Lbar', A, W). The VJP of the right update is the mirror, through the left update.
+template <typename T>
+int env_bwd(bool left, int a, int d, int c, int wl, int wr, const void* gout, const void* A,
+            const void* W, void* gin, void* ws, size_t ws_bytes, hipStream_t st) {
+  // left: gout (c, wr, c), A (a, d, c), gin (a, wl, a); right: gout (a, wl, a), B (a, d, c),
+  // gin (c, wr, c)
+  const long ng = left ? (long)c * wr * c : (long)a * wl * a;
+  const long ni = left ? (long)a * wl * a : (long)c * wr * c;
+  Arena ar(ws, ws_bytes);
+  T* g = ar.take<T>(ng);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "tnh_env_bwd: workspace %zu < %zu bytes", ws_bytes, ar.used);
+  const size_t used = align_up(ar.used, 256);
+  const void* gsrc = gout;
+  if constexpr (real_of<T>::cplx) {
+    TNH_TRY((permute3<T, true>(1, 1, 1, ng, 0, 0, 1, gout, 0, g, 0, st)));
+    gsrc = g;
+  }
+  void* rest = static_cast<char*>(ws) + used;
+  const size_t rest_bytes = ws_bytes - used;
+  if (left) {
+    TNH_TRY(right_env<T>(1, a, d, c, wl, wr, gsrc, 0, A, 0, W, 0, gin, 0, rest, rest_bytes, st));
+  } else {
+    TNH_TRY(left_env<T>(1, a, d, c, wl, wr, gsrc, 0, A, 0, W, 0, gin, 0, rest, rest_bytes, st));
+  }
+  return conj_inplace<T>(ni, static_cast<T*>(gin), st);
+}
+
 }  // namespace
 }  // namespace tnh
 
@@ -1286,6 +1404,66 @@ int tnh_update_right_env(int dtype, int c, int d, int b, int wl, int wr, const v
                          tnh_stream_t stream) {
   return tnh_update_right_env_batched(dtype, 1, c, d, b, wl, wr, R, 0, B, 0, W, 0, out, 0, ws,
                                       ws_bytes, stream);
+}
+
+size_t tnh_matvec2_bwd_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, int wr) {
+  return op_bytes(dsize(dtype), a, d1, d2, b, wl, wr) + 256 +
+         tnh_two_site_ws_bytes(dtype, a, d1, d2, b, wl, wr);
+}
+
+int tnh_matvec2_bwd(int dtype, int a, int d1, int d2, int b, int wl, int wm, int wr,
+                    const void* L, const void* W1, const void* W2, const void* R,
+                    const void* ybar, void* theta_bar, void* ws, size_t ws_bytes,
+                    tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wm > 0 && wr > 0 && d1 > 0 && d2 > 0, TNH_ERR_ARG,
+              "tnh_matvec2_bwd: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define TNH_BWD(T) \
+  matvec2_bwd<T>(a, d1, d2, b, wl, wm, wr, L, W1, W2, R, ybar, theta_bar, ws, ws_bytes, st)
+  switch (dtype) {
+    case TNH_F64: return TNH_BWD(double);
+    case TNH_F32: return TNH_BWD(float);
+    case TNH_C128: return TNH_BWD(c128);
+    case TNH_C64: return TNH_BWD(c64);
+  }
+#undef TNH_BWD
+  set_error("tnh_matvec2_bwd: unsupported dtype %d", dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+size_t tnh_env_bwd_ws_bytes(int dtype, int a, int d, int c, int wl, int wr) {
+  const size_t e = dsize(dtype);
+  const size_t big = (size_t)a * a * wl > (size_t)c * c * wr ? (size_t)a * a * wl
+                                                             : (size_t)c * c * wr;
+  return align_up(big * e, 256) + 256 + tnh_env_ws_bytes(dtype, a, d, c, wl, wr);
+}
+
+static int env_bwd_any(bool left, int dtype, int a, int d, int c, int wl, int wr,
+                       const void* gout, const void* A, const void* W, void* gin, void* ws,
+                       size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && c > 0 && d > 0 && wl > 0 && wr > 0, TNH_ERR_ARG,
+              "tnh_env_bwd: sizes must be positive");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (dtype) {
+    case TNH_F64: return env_bwd<double>(left, a, d, c, wl, wr, gout, A, W, gin, ws, ws_bytes, st);
+    case TNH_F32: return env_bwd<float>(left, a, d, c, wl, wr, gout, A, W, gin, ws, ws_bytes, st);
+    case TNH_C128: return env_bwd<c128>(left, a, d, c, wl, wr, gout, A, W, gin, ws, ws_bytes, st);
+    case TNH_C64: return env_bwd<c64>(left, a, d, c, wl, wr, gout, A, W, gin, ws, ws_bytes, st);
+  }
+  set_error("tnh_env_bwd: unsupported dtype %d", dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+int tnh_env_left_bwd(int dtype, int a, int d, int c, int wl, int wr, const void* Lbar_out,
+                     const void* A, const void* W, void* Lbar, void* ws, size_t ws_bytes,
+                     tnh_stream_t stream) {
+  return env_bwd_any(true, dtype, a, d, c, wl, wr, Lbar_out, A, W, Lbar, ws, ws_bytes, stream);
+}
+
+int tnh_env_right_bwd(int dtype, int c, int d, int b, int wl, int wr, const void* Rbar_out,
+                      const void* B, const void* W, void* Rbar, void* ws, size_t ws_bytes,
+                      tnh_stream_t stream) {
+  return env_bwd_any(false, dtype, c, d, b, wl, wr, Rbar_out, B, W, Rbar, ws, ws_bytes, stream);
 }
 
 }  // extern "C"

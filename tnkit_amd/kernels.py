@@ -515,3 +515,161 @@ def lincomb_c(V: torch.Tensor, c: torch.Tensor, out: torch.Tensor, nrows: int) -
                                     cr.stride(0), ptr(out), out.stride(0), stream(V.device)),
           "lincomb_c")
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# batched GEMM, row projections, backward entries, graphs
+# ---------------------------------------------------------------------------------------------
+
+
+def gemm_tn_batched(A: torch.Tensor, B: torch.Tensor) -> torch.Tensor:
+    """out[z, m, n] = sum_k A[z, k, m] B[z, k, n] for contiguous A (Z, K, M), B (Z, K, N) of one
+    dtype (f64, f32, c128, c64): `tnh_gemm_tn` batched over z."""
+    _need(A, "A", 3)
+    _need(B, "B", 3)
+    if A.dtype != B.dtype or A.shape[0] != B.shape[0] or A.shape[1] != B.shape[1]:
+        raise ValueError(f"gemm_tn_batched: A {tuple(A.shape)} {A.dtype} and B {tuple(B.shape)} "
+                         f"{B.dtype}")
+    Z, K, M = A.shape
+    N = B.shape[2]
+    out = torch.empty((Z, M, N), dtype=A.dtype, device=A.device)
+    if Z == 0 or M == 0 or N == 0:
+        return out
+    if K == 0:
+        return out.zero_()
+    lib = _lib.load()
+    check(lib.tnh_gemm_tn(dcode(A.dtype), M, N, K, ptr(A), M, ptr(B), N, ptr(out), N, Z, K * M,
+                          K * N, M * N, 0, 0, stream(A.device)), "gemm_tn_batched")
+    return out
+
+
+def rows_dot(V: torch.Tensor, x: torch.Tensor) -> torch.Tensor:
+    """out[r] = sum_i V[r, i] x[i] (no conjugation) for contiguous V (k, n), x (n,)."""
+    _need(V, "V", 2)
+    _need(x, "x", 1)
+    if V.dtype != x.dtype or V.shape[1] != x.shape[0]:
+        raise ValueError(f"rows_dot: V {tuple(V.shape)} {V.dtype}, x {tuple(x.shape)} {x.dtype}")
+    k, n = V.shape
+    out = torch.empty(k, dtype=V.dtype, device=V.device)
+    if k == 0:
+        return out
+    lib = _lib.load()
+    ws = _ws(lib.tnh_rows_ws_bytes(dcode(V.dtype)), V.device)
+    check(lib.tnh_rows_dot(dcode(V.dtype), k, n, ptr(V), n, ptr(x), ptr(out), ptr(ws), ws.numel(),
+                           stream(V.device)), "rows_dot")
+    return out
+
+
+def rows_combine(c: torch.Tensor, V: torch.Tensor) -> torch.Tensor:
+    """out[i] = sum_r c[r] V[r, i] for contiguous c (k,), V (k, n), c on the device."""
+    _need(c, "c", 1)
+    _need(V, "V", 2)
+    if V.dtype != c.dtype or V.shape[0] != c.shape[0]:
+        raise ValueError(f"rows_combine: c {tuple(c.shape)} {c.dtype}, V {tuple(V.shape)} "
+                         f"{V.dtype}")
+    k, n = V.shape
+    out = torch.empty(n, dtype=V.dtype, device=V.device)
+    if k == 0:
+        return out.zero_()
+    check(_lib.load().tnh_rows_combine(dcode(V.dtype), k, n, ptr(c), ptr(V), n, ptr(out),
+                                       stream(V.device)), "rows_combine")
+    return out
+
+
+def matvec2_bwd(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: torch.Tensor,
+                ybar: torch.Tensor) -> torch.Tensor:
+    """theta_bar = H_eff^H ybar for the two-site operator of (L, W1, W2, R), by the library's
+    adjoint-operator build and the forward kernels (`tnh_matvec2_bwd`)."""
+    for t, n, nd in ((L, "L", 3), (W1, "W1", 4), (W2, "W2", 4), (R, "R", 3), (ybar, "ybar", 4)):
+        _need(t, n, nd)
+    a, wl, _ = L.shape
+    _, wm, d1, _ = W1.shape
+    wr, d2 = W2.shape[1], W2.shape[2]
+    b = R.shape[0]
+    if (tuple(ybar.shape) != (a, d1, d2, b) or len({t.dtype for t in (L, W1, W2, R, ybar)}) != 1
+            or W2.shape[0] != wm or R.shape[1] != wr):
+        raise ValueError("matvec2_bwd: shapes or dtypes do not match")
+    lib = _lib.load()
+    dt = dcode(L.dtype)
+    ws = _ws(lib.tnh_matvec2_bwd_ws_bytes(dt, a, d1, d2, b, wl, wr), L.device)
+    out = torch.empty_like(ybar)
+    check(lib.tnh_matvec2_bwd(dt, a, d1, d2, b, wl, wm, wr, ptr(L), ptr(W1), ptr(W2), ptr(R),
+                              ptr(ybar), ptr(out), ptr(ws), ws.numel(), stream(L.device)),
+          "matvec2_bwd")
+    return out
+
+
+def env_left_bwd(Lbar_out: torch.Tensor, A: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """The gradient of L for L' = update_left_env(L, A, W) given the cotangent of L' (torch's
+    convention for complex data), by `tnh_env_left_bwd`."""
+    for t, n, nd in ((Lbar_out, "Lbar_out", 3), (A, "A", 3), (W, "W", 4)):
+        _need(t, n, nd)
+    a, d, c = A.shape
+    wl, wr = W.shape[0], W.shape[1]
+    if tuple(Lbar_out.shape) != (c, wr, c) or len({Lbar_out.dtype, A.dtype, W.dtype}) != 1:
+        raise ValueError("env_left_bwd: shapes or dtypes do not match")
+    lib = _lib.load()
+    dt = dcode(A.dtype)
+    ws = _ws(lib.tnh_env_bwd_ws_bytes(dt, a, d, c, wl, wr), A.device)
+    out = torch.empty((a, wl, a), dtype=A.dtype, device=A.device)
+    check(lib.tnh_env_left_bwd(dt, a, d, c, wl, wr, ptr(Lbar_out), ptr(A), ptr(W), ptr(out),
+                               ptr(ws), ws.numel(), stream(A.device)), "env_left_bwd")
+    return out
+
+
+def env_right_bwd(Rbar_out: torch.Tensor, B: torch.Tensor, W: torch.Tensor) -> torch.Tensor:
+    """The gradient of R for R' = update_right_env(R, B, W) given the cotangent of R'."""
+    for t, n, nd in ((Rbar_out, "Rbar_out", 3), (B, "B", 3), (W, "W", 4)):
+        _need(t, n, nd)
+    c, d, b = B.shape
+    wl, wr = W.shape[0], W.shape[1]
+    if tuple(Rbar_out.shape) != (c, wl, c) or len({Rbar_out.dtype, B.dtype, W.dtype}) != 1:
+        raise ValueError("env_right_bwd: shapes or dtypes do not match")
+    lib = _lib.load()
+    dt = dcode(B.dtype)
+    ws = _ws(lib.tnh_env_bwd_ws_bytes(dt, c, d, b, wl, wr), B.device)
+    out = torch.empty((b, wr, b), dtype=B.dtype, device=B.device)
+    check(lib.tnh_env_right_bwd(dt, c, d, b, wl, wr, ptr(Rbar_out), ptr(B), ptr(W), ptr(out),
+                                ptr(ws), ws.numel(), stream(B.device)), "env_right_bwd")
+    return out
+
+
+class Graph:
+    """A hipGraph of library calls recorded through the C ABI (`tnh_graph_begin/end`):
+
+        g = Graph(torch_stream)
+        with g.capture():
+            ... library calls on torch_stream (no allocation inside) ...
+        g.launch()
+
+    The recording keeps the device pointers it saw; replays read and write those buffers."""
+
+    def __init__(self, torch_stream: "torch.cuda.Stream"):
+        self.stream = torch_stream
+        self.handle = ctypes.c_void_p(None)
+
+    def capture(self):
+        import contextlib
+        lib = _lib.load()
+
+        @contextlib.contextmanager
+        def ctx():
+            with torch.cuda.stream(self.stream):
+                check(lib.tnh_graph_begin(self.stream.cuda_stream), "graph_begin")
+                try:
+                    yield self
+                finally:
+                    h = ctypes.c_void_p(None)
+                    check(lib.tnh_graph_end(self.stream.cuda_stream, ctypes.byref(h)),
+                          "graph_end")
+                    self.handle = h
+        return ctx()
+
+    def launch(self, torch_stream: Optional["torch.cuda.Stream"] = None) -> None:
+        s = torch_stream if torch_stream is not None else self.stream
+        check(_lib.load().tnh_graph_launch(self.handle, s.cuda_stream), "graph_launch")
+
+    def close(self) -> None:
+        if self.handle:
+            check(_lib.load().tnh_graph_destroy(self.handle), "graph_destroy")
+            self.handle = ctypes.c_void_p(None)
