@@ -158,9 +158,59 @@ def cpu_baseline(prob, budget_s=20.0):
             break
     sec = el / n
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return {"value": fl / sec / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"{n} float64 matvecs of the chi={a} XXZ bulk problem with the numpy oracle "
-                      f"({sec * 1e3:.0f} ms each; BLAS: {', '.join(blas) or 'numpy default'})"}
+    out = {"value": fl / sec / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "port",
+           "sample": f"{n} float64 matvecs of the chi={a} XXZ bulk problem with the numpy oracle "
+                     f"({sec * 1e3:.0f} ms each; BLAS: {', '.join(blas) or 'numpy default'})",
+           "cpu_model": _cpu_model(), "blas": blas}
+    try:                                          # the same matvec on one core
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(1):
+            t1 = time.perf_counter()
+            D.apply_two_site_fused(th, *op)
+            s1 = time.perf_counter() - t1
+        out["one_thread"] = {"value": fl / s1 / 1e12, "unit": "TFLOP/s", "cores": 1,
+                             "sample": f"1 float64 matvec ({s1:.2f} s)"}
+    except Exception as exc:  # pragma: no cover - threadpoolctl is in the image
+        out["one_thread"] = {"error": str(exc)}
+    return out
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def config1_cpu_vs_gpu():
+    """Config 1 (TFIM N=32, g=1, chi=32, 10 sweeps, tol=rtol=0, seed 0, float64) in full on the
+    host cores with the numpy oracle and on the device, each against the free-fermion energy."""
+    import torch
+    from oracle import dense as D
+    from oracle import exact as X
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd.models import tfim_mpo
+    mpo = tfim_mpo(32, 1.0)
+    kw = dict(n_sweeps=10, tol=0.0, rtol=0.0, seed=0)
+    e_exact = X.tfim_free_fermion_energy(32, 1.0)
+    t0 = time.perf_counter()
+    e_cpu, _, _ = D.dmrg(mpo, 32, **kw)
+    t_cpu = time.perf_counter() - t0
+    H.dmrg(mpo, 32, n_sweeps=1, tol=0.0, rtol=0.0, seed=0, backend="hip-real")   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e_gpu, _, info = H.dmrg(mpo, 32, backend="hip-real", **kw)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    return {"workload": "TFIM N=32 g=1, two-site DMRG chi=32, 10 sweeps, tol=rtol=0, seed 0",
+            "cpu_s": t_cpu, "gpu_s": t_gpu, "E_cpu": e_cpu, "E_gpu": e_gpu, "E_exact": e_exact,
+            "dE_cpu": abs(e_cpu - e_exact), "dE_gpu": abs(e_gpu - e_exact),
+            "gpu_matvecs": info["matvec_calls"],
+            "note": "launch-bound at chi=32: the CPU is the faster device here"}
 
 
 def local_solves(th, op64, op32, ws, chi, op16=None):
@@ -547,8 +597,10 @@ def config2(N, stages, eig, trunc, seed=0, log=None):
         t0 = time.perf_counter()
         E, mps, info = H.dmrg(mpo, chi, **kw)
         torch.cuda.synchronize()
+        later = info["sweep_s"][1:] or info["sweep_s"]
         rec = {"chi": chi, "sweeps": n, "E": E, "energies": info["energies"],
-               "sweep_s": info["sweep_s"], "wall_s": time.perf_counter() - t0,
+               "sweep_s": info["sweep_s"], "s_per_sweep_after_first": sum(later) / len(later),
+               "wall_s": time.perf_counter() - t0,
                "matvec_calls": info["matvec_calls"], "mixed_counts": info["mixed_counts"],
                "phase_s": {k: round(v, 3) for k, v in info["phase_time"].items()},
                "discarded_weight": info["discarded_weight"],
@@ -566,7 +618,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--N", type=int, default=100)
-    ap.add_argument("--sweep-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-local", action="store_true",
                     help="skip the local ground-state solves (short profiled runs)")
@@ -741,45 +792,26 @@ def main():
         local = local_solves(th, op64, op32, ws, args.chi, op16=op16)
     del op64, op32, op16, ws, out, y32, y64
 
-    # ---- bulk DMRG steps (eigensolve + Gram truncation + environment) ---------------------
-    steps = {}
-    if args.sweep_steps > 0:
-        i0, Nn = prob["i0"], args.N
-        Rs = [None] * (Nn + 1)
-        Rs[Nn] = torch.ones((1, 1, 1), dtype=torch.float64, device=th.device)
-        for j in range(Nn - 1, i0 + 1, -1):
-            Rs[j] = kernels.update_right_env(Rs[j + 1], prob["M"][j], prob["mpo"][j])
-        for form in ("lanczos-restart", "lanczos-mixed"):
-            eng = H.DMRGEngine.from_state(prob["mpo"], args.chi,
-                                          H.DMRGOptions(eig=form, trunc="gram",
-                                                        backend="hip-real"),
-                                          [m.clone() for m in prob["M"]], {i0: L}, Rs)
-            torch.cuda.synchronize()
-            ts = time.perf_counter()
-            for k in range(min(args.sweep_steps, Nn - 2 - i0)):
-                eng.solve_block(i0 + k, "right", 1e-9)
-            torch.cuda.synchronize()
-            el = time.perf_counter() - ts
-            nst = min(args.sweep_steps, Nn - 2 - i0)
-            steps[form] = {"steps": nst, "s_per_step": el / nst,
-                           "sweep_s_extrapolated": el / nst * 2 * (Nn - 1),
-                           "matvecs": eng.matvec_calls,
-                           "phase_s": {k: round(v, 4) for k, v in eng.phase.items()},
-                           "mixed_counts": eng.mixed_counts}
-            del eng
     setup_s = time.perf_counter() - t_setup
     block = {} if args.no_block else block_bench(max(5, args.steps // 2), args.warmup,
                                                  local=not args.no_local)
-    # DMRG sweep time at chi = 1024: tnkit's per-sweep cost study (Heisenberg N = 64, 3 sweeps
-    # from a random state at full chi, mean of sweeps 2-3), float64 against the mixed eigensolver
-    sweep = {}
+    # the sweep half of the metric: config 2's bond ramp on the XXZ chain N = 100, measured in
+    # both precisions (each stage started from the previous state, as tnkit's runner chains them)
+    ramp = {}
     if not args.no_sweep:
-        runs = sweep_cost(64, [args.chi], ["dense:lanczos-restart", "dense:lanczos-mixed",
-                                           "u1:lanczos-restart"], log=sys.stderr)
-        sweep = {"workload": f"Heisenberg chain N=64, chi={args.chi}, 3 sweeps from random, "
-                             "s/sweep = mean of sweeps 2-3, trunc=gram",
-                 "reference_h100_3g_s_per_sweep": {"dense": 38.4, "u1": 15.7},
-                 "runs": runs}
+        stages = [(64, 4), (512, 4), (args.chi, 4)]
+        ramp = {"workload": f"XXZ chain N={args.N}, two-site DMRG, bond ramp {stages} "
+                            "(12 sweeps), trunc=gram, seed 0",
+                "reference_h100_mig_fp64_s_per_sweep_chi1024_N64": 38.4}
+        for eig in ("lanczos-restart", "lanczos-mixed"):
+            ramp[eig] = config2(args.N, stages, eig, "gram", log=sys.stderr)
+        ramp["dE_per_stage"] = [abs(x["E"] - y["E"]) for x, y in
+                                zip(ramp["lanczos-restart"], ramp["lanczos-mixed"])]
+        ramp["s_per_sweep_at_chi"] = {
+            eig: ramp[eig][-1]["s_per_sweep_after_first"] for eig in
+            ("lanczos-restart", "lanczos-mixed")}
+        ramp["sweep_speedup_mixed_vs_fp64"] = (ramp["s_per_sweep_at_chi"]["lanczos-restart"]
+                                               / ramp["s_per_sweep_at_chi"]["lanczos-mixed"])
 
     res = {
         "metric": METRIC, "value": fl / (ms32 * 1e-3) / 1e12, "unit": "TFLOP/s", "n_gpus": 1,
@@ -803,15 +835,15 @@ def main():
                  "note": "explored, not the headline: bf16 operator (LT, RT) and bf16 theta / "
                          "mixed intermediate on the bf16 matrix cores, float accumulation"},
         "local_dE": local,
-        "bulk_steps": steps,
         "reference_h100_mig_fp64_tflops": 5.67,
         "roofline": roof,
         "block_u1": block,
-        "sweep_chi1024": sweep,
+        "config2_ramp": ramp,
         "setup_s": setup_s,
     }
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(prob)
+        res["config1"] = config1_cpu_vs_gpu()
     print(json.dumps(res))
 
 
