@@ -210,3 +210,47 @@ def test_graphed_krylov_equals_eager(hip, monkeypatch):
             assert x.shape == y.shape
             assert float((x - y).abs().max()) <= 1e-12 * max(1.0, float(x.abs().max()))
     T.clear_graphs()
+
+
+def test_ragged_modes_same_trajectory(hip):
+    """tnkit's three ragged strategies on legs of different entry bonds (ground states at chi 3,
+    5, 5, 8) with caps 6, 10, 12, 16 and a discarded-weight budget: 'pad' (fixed widest shapes),
+    'tight' and 'group' give the same <Z_j>(t) to 1e-10, equal single runs, and 'group' makes
+    one batch per entry profile with its records per group."""
+    N, dt, n = 10, 0.05, 3
+    g0s, chis, caps = [1.3, 1.5, 1.7, 1.9], [3, 5, 5, 8], [6, 10, 12, 16]
+    states = [_ground(N, g, x, seed=j) for j, (g, x) in enumerate(zip(g0s, chis))]
+    mpo = tfim_mpo(N, 1.0)
+    meas = lambda ss: [local_expectations(s, Z) for s in ss]  # noqa: E731
+    out = {}
+    for mode in ("pad", "tight", "group"):
+        stats = {}
+        print("ragged", mode, flush=True)
+        legs, rec = T.evolve_batch(states, mpo, dt, n, chi_max=caps, krylov_m=20,
+                                   keep_device=True, measure=meas, stats=stats, ragged=mode,
+                                   svd_tol=1e-10)
+        out[mode] = (legs, rec, stats)
+    _, rec_t, st_t = out["tight"]
+    _, rec_p, st_p = out["pad"]
+    _, rec_g, st_g = out["group"]
+    profiles = [tuple([1] + [int(t.shape[2]) for t in s[:-1]] + [1]) for s in states]
+    assert st_g["n_groups"] == len(set(profiles)) and len(rec_g) == st_g["n_groups"]
+    assert st_p["n_groups"] == 1 and st_p["fill"] < st_t["fill"]
+    groups = T._partition(states, "group")
+    for j in range(len(states)):
+        gi = next(q for q, g in enumerate(groups) if j in g)
+        slot = groups[gi].index(j)
+        for k in range(n + 1):
+            zt = rec_t[k][1][j]
+            assert np.abs(rec_p[k][1][j] - zt).max() < 1e-10
+            assert np.abs(rec_g[gi][k][1][slot] - zt).max() < 1e-10
+        _, r1 = T.tdvp2_evolve(states[j], mpo, caps[j], dt, n, krylov_m=20, svd_tol=1e-10,
+                               measure=lambda m: local_expectations(m, Z))
+        assert max(np.abs(rec_t[k][1][j] - r1[k][1]).max() for k in range(n + 1)) < 1e-10
+    assert st_t["bonds"] == st_p["bonds"]
+
+
+def test_ragged_mode_refused():
+    with pytest.raises(ValueError, match="ragged"):
+        T.evolve_batch([[np.ones((1, 2, 1))]], [np.ones((1, 1, 2, 2))], 0.05, 1, chi_max=4,
+                       ragged="fold")

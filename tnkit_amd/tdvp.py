@@ -2,7 +2,9 @@
 (`evolve_batch`), after tnkit/tdvp.py:555-800 and tnkit/batch_tdvp.py:572-1002.
 
 One implementation serves both: a trajectory is a batch of one. The k legs are stacked on a
-leading axis and zero-padded to the widest bond of each position ('pad' in tnkit's terms); every
+leading axis and zero-padded at each bond position (tnkit's three ragged modes: 'pad' to the
+widest bond the cap allows, 'tight' to the widest rank kept on that step, 'group' per entry bond
+profile); every
 contraction is one batched libtnkit_hip launch for all legs (`kernels.*_b`), the Krylov
 exponential keeps its scalars on the device and reads them back once per solve, and the
 truncation is one batched Gram eigendecomposition with per-leg ranks and masks, so padding
@@ -40,7 +42,8 @@ class BatchKrylovError(RuntimeError):
 @dataclasses.dataclass
 class BatchTDVPOptions:
     """Settings of `evolve_batch` (tnkit/batch_tdvp.py:134-201). Served: evolver '2tdvp',
-    ragged 'pad', expm_where 'eigh'; `chi_max` may be one cap or one per leg."""
+    ragged 'pad' / 'tight' / 'group', expm_where 'eigh'; `chi_max` may be one cap or one per
+    leg."""
 
     backend: Any = "hip"
     evolver: str = "2tdvp"
@@ -202,18 +205,28 @@ def _pad3(t: torch.Tensor, a: int, b: int) -> torch.Tensor:
     return out
 
 
+def widest_profile(N: int, d: int, cap: int) -> List[int]:
+    """The widest each of the N + 1 bonds of an open chain (outer bonds 1) can be under a
+    two-site sweep capped at `cap`: min(cap, d^i, d^(N-i)) (tnkit/batch_tdvp.py:897-911)."""
+    return [int(min(cap, d ** min(i, 40), d ** min(N - i, 40))) for i in range(N + 1)]
+
+
 class _Batch:
     """k legs on a leading axis, padded to the widest bond at each position, with the true
     per-leg ranks (tnkit/batch_tdvp.py:403-496)."""
 
-    def __init__(self, states, mpos, dev, caps):
+    def __init__(self, states, mpos, dev, caps, ragged: str = "tight"):
         self.k = len(states)
         self.N = len(states[0])
         self.d = int(states[0][0].shape[1])
         self.caps = np.asarray(caps, dtype=np.int64)
+        self.ragged = ragged
         ranks = np.array([[1] + [int(t.shape[2]) for t in s[:-1]] + [1] for s in states])
         self.rank = [ranks[:, i] for i in range(self.N + 1)]
         self.dims = [int(ranks[:, i].max()) for i in range(self.N + 1)]
+        if ragged == "pad":                 # fixed shapes: the widest each bond can ever be
+            self.dims = [max(w, x) for w, x in zip(self.dims, widest_profile(
+                self.N, self.d, int(self.caps.max())))]
         self.M = []
         for i in range(self.N):
             self.M.append(torch.stack([_pad3(s[i], self.dims[i], self.dims[i + 1])
@@ -295,7 +308,7 @@ def _eigh_robust_vendor(G: torch.Tensor, add: Optional[torch.Tensor] = None):
 
 
 def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
-              iso: str):
+              iso: str, width: int = 0):
     """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
     the fewest values whose relative dropped weight stays under svd_tol, capped by the leg's
     cap and structural ceiling (tnkit/batch_tdvp.py:572-599). Returns U (k, m, nk), S (k, nk),
@@ -304,7 +317,7 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     k, m, n = mat.shape
     if iso == "right":
         U, S, Vh, ranks, disc = _gram_cut(mat.mH.resolve_conj().contiguous(), ranks_cap, ceiling,
-                                          svd_tol, "left")
+                                          svd_tol, "left", width)
         return (Vh.mH.resolve_conj().contiguous(), S, U.mH.resolve_conj().contiguous(), ranks,
                 disc)
     tall = m > n                                        # the smaller Gram side, as tnkit does
@@ -347,7 +360,7 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
             # zero, and a zero-weight test on them would drop directions an SVD keeps.
             ranks[leg] = lim
         disc[leg] = float(drop[ranks[leg]])
-    nk = int(ranks.max())
+    nk = min(max(int(ranks.max()), int(width)), r)     # stacked width: the ragged mode's
     mk = torch.arange(nk, device=mat.device)[None, :] < torch.as_tensor(ranks, device=mat.device)[:, None]
     Zk = Z[:, :, :nk] * mk[:, None, :]
     if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
@@ -419,8 +432,17 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
     theta = lanczos_expm_fixed_batched(mv, theta.reshape(k, -1), -0.5j * dt, m, resid, gctx)
     mat = theta.reshape(k, a * d, d * b)
     ceiling = np.minimum(Bt.rank[i] * d, Bt.rank[i + 2] * d)
+    # how wide the stacked bond is held (tnkit/batch_tdvp.py:602-628): 'pad' at its fixed
+    # widest, 'tight' at the widest rank any leg kept, 'group' at what the spectrum and the cap
+    # allow; the rank masks carry each leg's own rank in all three
+    if Bt.ragged == "pad":
+        width = Bt.dims[i + 1]
+    elif Bt.ragged == "group":
+        width = int(min(Bt.caps.max(), a * d, d * b))
+    else:
+        width = 0
     U, S, Vh, ranks, disc = _gram_cut(mat, Bt.caps, ceiling, svd_tol,
-                                      "left" if moving == "right" else "right")
+                                      "left" if moving == "right" else "right", width)
     nk = U.shape[2]
     Bt.rank[i + 1] = ranks
     Bt.dims[i + 1] = nk
@@ -461,8 +483,12 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
                  **kwargs: Any) -> Tuple[List[MPS], List[Any]]:
     """Evolve k MPS of one geometry as one batched program (tnkit/batch_tdvp.py:721-857).
 
-    `hams` is one MPO for every leg or a list of k MPOs with the same bond structure. Returns
-    (states, records): each leg trimmed to its own bonds, and [(t, measure(states_t)), ...]."""
+    `hams` is one MPO for every leg or a list of k MPOs with the same bond structure. `ragged`
+    says how legs of different bonds share the stacked shapes: 'pad' holds every bond at the
+    widest the cap allows (fixed shapes), 'tight' at the widest rank any leg kept on that step,
+    'group' steps the legs of each entry bond profile as their own batch. Returns (states,
+    records): each leg trimmed to its own bonds, and [(t, measure(states_t)), ...], one such
+    list per group when 'group' made several."""
     if options is None:
         o = BatchTDVPOptions(**kwargs)
     elif isinstance(options, dict):
@@ -471,8 +497,8 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
         o = dataclasses.replace(options, **kwargs) if kwargs else options
     if o.evolver != "2tdvp":
         raise ValueError(f"evolver {o.evolver!r}: the HIP batch serves '2tdvp'")
-    if o.ragged != "pad":
-        raise ValueError(f"ragged {o.ragged!r}: the HIP batch serves 'pad'")
+    if o.ragged not in RAGGED_MODES:
+        raise ValueError(f"ragged must be one of {RAGGED_MODES}, got {o.ragged!r}")
     if o.chi_max is None:
         raise ValueError("'2tdvp' needs chi_max")
     if not isinstance(o.krylov_m, (int, np.integer)):
@@ -500,7 +526,58 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
     st = [[c(t) for t in s] for s in states]
     if o.canonicalize:
         st = [_right_canonical(s) for s in st]
-    Bt = _Batch(st, [[c(w) for w in h] for h in mpos], dev, caps)
+    wd = {id(h): [c(w) for w in h] for h in mpos}     # one device copy per distinct MPO
+    mp = [wd[id(h)] for h in mpos]
+    groups = _partition(st, o.ragged)
+    out_legs: List[Any] = [None] * k
+    all_records = []
+    merged: Dict[str, Any] = {"krylov_m": int(o.krylov_m), "resid_max": [0.0] * k,
+                              "disc_per_leg": [0.0] * k, "bonds": None,
+                              "groups": [len(g) for g in groups], "n_groups": len(groups),
+                              "ragged": o.ragged, "fill": []}
+    bonds = [[0] * k for _ in range(N + 1)]
+    for g in groups:
+        legs, recs, gst = _run_group([st[n] for n in g], [mp[n] for n in g], caps[g], dt,
+                                     n_steps, o, dev)
+        all_records.append(recs)
+        merged["krylov_m"] = max(merged["krylov_m"], gst["krylov_m"])
+        merged["fill"].append(gst["fill"])
+        for slot, n in enumerate(g):
+            out_legs[n] = legs[slot]
+            merged["resid_max"][n] = gst["resid_max"][slot]
+            merged["disc_per_leg"][n] = gst["disc_per_leg"][slot]
+            for i in range(N + 1):
+                bonds[i][n] = gst["bonds"][i][slot]
+    merged["bonds"] = bonds
+    merged["fill"] = float(np.mean(merged["fill"]))
+    if o.stats is not None:
+        o.stats.update(merged)
+    return out_legs, (all_records[0] if len(groups) == 1 else all_records)
+
+
+RAGGED_MODES = ("pad", "tight", "group")
+
+
+def _bond_profile(s) -> Tuple[int, ...]:
+    return tuple([1] + [int(t.shape[2]) for t in s[:-1]] + [1])
+
+
+def _partition(states, ragged: str) -> List[List[int]]:
+    """Leg indices that step as one batch: every leg for 'pad' and 'tight', the legs of equal
+    entry bond profile for 'group' (in order of first appearance; tnkit/batch_tdvp.py:887-895)."""
+    if ragged != "group":
+        return [list(range(len(states)))]
+    seen: Dict[Tuple[int, ...], List[int]] = {}
+    for n, s in enumerate(states):
+        seen.setdefault(_bond_profile(s), []).append(n)
+    return list(seen.values())
+
+
+def _run_group(st, mpos, caps, dt: float, n_steps: int, o: "BatchTDVPOptions", dev):
+    """Step one batch of legs for n_steps (tnkit/batch_tdvp.py:937-1002): records, the legs
+    trimmed to their own bonds, and this group's statistics."""
+    k, N = len(st), len(st[0])
+    Bt = _Batch(st, mpos, dev, caps, o.ragged)
     for i in range(N - 1, 0, -1):
         Bt.R[i] = kernels.update_right_env_b(Bt.R[i + 1], Bt.M[i], Bt.W[i])
     m = int(o.krylov_m)
@@ -515,6 +592,7 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
         records.append((0.0, o.measure(snapshot())))
     disc_tot = np.zeros(k)
     resid_max = np.zeros(k)
+    fills = []
     for step in range(n_steps):
         resid_d = torch.zeros(k, dtype=torch.float64, device=dev)
         for i in range(0, N - 1):
@@ -523,6 +601,8 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
             disc_tot += _step(Bt, i, dt, m, o.svd_tol, o.renorm, "left", resid_d, o.low_precision)
         resid = resid_d.cpu().numpy()                 # the step's one read of the residuals
         resid_max = np.maximum(resid_max, resid)
+        fills.append(float(np.mean([np.mean(Bt.rank[i] / float(Bt.dims[i]))
+                                    for i in range(N + 1)])))
         bad = np.nonzero(resid > o.krylov_tol)[0]
         if len(bad):
             if m >= m_max:
@@ -534,11 +614,10 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
             records.append(((step + 1) * dt, o.measure(snapshot())))
         if o.progress is not None:
             o.progress(step + 1, (step + 1) * dt)
-    if o.stats is not None:
-        o.stats.update({"krylov_m": m, "resid_max": resid_max.tolist(),
-                        "disc_per_leg": disc_tot.tolist(), "bonds": [list(map(int, r))
-                                                                     for r in Bt.rank]})
-    return [_trim(Bt, j) for j in range(k)], records
+    gst = {"krylov_m": m, "resid_max": resid_max.tolist(), "disc_per_leg": disc_tot.tolist(),
+           "bonds": [list(map(int, r)) for r in Bt.rank],
+           "fill": float(np.mean(fills)) if fills else 1.0}
+    return [_trim(Bt, j) for j in range(k)], records, gst
 
 
 def tdvp2_evolve(mps: Sequence[Any], mpo: Sequence[Any], chi: int, dt: float, n_steps: int,
