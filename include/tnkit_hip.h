@@ -329,6 +329,29 @@ int tnh_svd_jacobi(int dtype, long batch, int m, int n, const void* A, long lda,
                    long sS, void* U, long ldu, long sU, void* Vh, long ldv, long sV, int max_sweeps,
                    double tol, int* sweeps, tnh_stream_t stream);
 
+/* Householder tridiagonalisation A = Q T Q^T of a real symmetric float64 matrix, 1 <= n <= 2048
+ * (the first phase of the Gram eigendecomposition, tnkit/linalg.py:149). One cooperative launch
+ * of ceil(n / 8) workgroups, row i of A held by workgroup i mod G. Row i of A is read at
+ * A + i*lda as given (pass a symmetric matrix). Writes d (n diagonal), e (n - 1 subdiagonal),
+ * tau (n - 1) and Vr (n x n, row j = reflector v_j: zero at and above j, 1 at j + 1), with
+ * Q = H_0 ... H_{n-2}, H_j = I - tau_j v_j v_j^T. status (device int, may be NULL): 0, or 1 when
+ * a workgroup's bounded wait ran out (the result is then invalid). */
+size_t tnh_sytrd_ws_bytes(int n);
+int tnh_sytrd(int n, const double* A, long lda, double* d, double* e, double* tau, double* Vr,
+              int* status, void* ws, size_t ws_bytes, tnh_stream_t stream);
+
+/* All eigenvalues and the k leading eigenvectors of a real symmetric float64 matrix,
+ * 2 <= n <= 2048: the eigh of tnkit's Gram truncation at orders above 64
+ * (tnkit/linalg.py:138-159, `be.eigh(A @ A^H)` at :149; the block per-sector form
+ * tnkit/blocks.py:1345-1377). tnh_sytrd, divide and conquer on the tridiagonal (leaves of order
+ * <= 32 on tnh_eigh_jacobi), and the WY back-transformation of the k vectors on tnh_gemm_tn.
+ * Writes w (n eigenvalues, ascending) and Z (n x k, row stride ldz): column c = eigenvector of
+ * the c-th largest eigenvalue. info (device int, may be NULL): 0; bit 0: a wait of the
+ * tridiagonalisation ran out; bit 1: a leaf eigensolve did not converge. */
+size_t tnh_syev_top_ws_bytes(int n, int k);
+int tnh_syev_top(int n, int k, const double* A, long lda, double* w, double* Z, long ldz,
+                 int* info, void* ws, size_t ws_bytes, tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,3 +62,20 @@ def test_svd_jacobi_refuses_overlapping_output_strides():
     for bad in (dict(sU=100), dict(sV=50), dict(sS=4)):
         assert call(**bad) == _lib.ERR_ARG
         assert b"overlap" in lib.tnh_last_error()
+
+
+def test_syev_refusals_and_sizes_without_a_device():
+    """The symmetric eigensolver's entries refuse bad orders, k and workspaces before any launch,
+    and size their workspace on the host."""
+    lib = _lib.load()
+    assert lib.tnh_syev_top_ws_bytes(2048, 1024) > 3 * 2048 * 2048 * 8
+    assert lib.tnh_syev_top_ws_bytes(1, 1) == 0 and lib.tnh_syev_top_ws_bytes(4096, 8) == 0
+    assert lib.tnh_sytrd_ws_bytes(2048) > 0 and lib.tnh_sytrd_ws_bytes(4096) == 0
+    rc = lib.tnh_syev_top(4096, 8, None, 4096, None, None, 8, None, None, 0, None)
+    assert rc == _lib.ERR_UNSUPPORTED
+    rc = lib.tnh_syev_top(100, 0, None, 100, None, None, 8, None, None, 0, None)
+    assert rc == _lib.ERR_ARG
+    rc = lib.tnh_syev_top(100, 10, 1, 100, 1, 1, 10, None, 1, 16, None)
+    assert rc == _lib.ERR_WS and b"workspace" in lib.tnh_last_error()
+    rc = lib.tnh_sytrd(0, None, 1, None, None, None, None, None, None, 0, None)
+    assert rc == _lib.ERR_UNSUPPORTED

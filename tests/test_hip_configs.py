@@ -10,6 +10,8 @@
 * Config 5: dE/dg_i of the per-site-field TFIM N=64 at chi=256 against -<Z_i> and against
   central differences of the exact free-fermion energy.
 The numpy oracle runs with up to 16 BLAS threads here (threadpoolctl)."""
+import sys
+
 import numpy as np
 import pytest
 import torch
@@ -29,7 +31,8 @@ CPU_THREADS = 16
 
 
 def _say(*a):
-    print(*a, flush=True)
+    # straight to the process's stderr: long tests report progress past pytest's capture
+    print(*a, file=sys.__stderr__, flush=True)
 
 
 def _rel(got, ref):

@@ -81,6 +81,10 @@ SIGNATURES = {
                                c_long, c_long, _P, c_long, c_long, c_int, c_double, _P, _P]),
     "tnh_eigh_jacobi": (c_int, [c_int, c_long, c_int, _P, c_long, c_long, _P, c_long, _P, c_long,
                                 _P, c_long, c_long, c_int, c_double, _P, _P]),
+    "tnh_sytrd_ws_bytes": (c_size_t, [c_int]),
+    "tnh_sytrd": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, _P, _P, c_size_t, _P]),
+    "tnh_syev_top_ws_bytes": (c_size_t, [c_int, c_int]),
+    "tnh_syev_top": (c_int, [c_int, c_int, _P, c_long, _P, _P, c_long, _P, _P, c_size_t, _P]),
 }
 
 

@@ -258,7 +258,9 @@ class HipBackend:
 
     def eigh(self, A):
         """Hermitian eigendecomposition (ascending): the batched Jacobi kernel for orders up to
-        64 in float64 / complex128, the vendor solver above or where Jacobi did not converge."""
+        64 in float64 / complex128, the library's divide-and-conquer solver for real float64
+        matrices of order 65..2048, the vendor solver otherwise or where Jacobi did not
+        converge."""
         from . import kernels
         t = self.torch
         if (A.is_cuda and A.dim() in (2, 3) and A.dtype in (t.float64, t.complex128)
@@ -267,6 +269,12 @@ class HipBackend:
             w, Z, sw = kernels.eigh_jacobi(G)
             if bool((sw > 0).all()):
                 return (w[0], Z[0]) if A.dim() == 2 else (w, Z)
+        if (A.is_cuda and A.dim() == 2 and A.dtype == t.float64
+                and kernels.EIGH_JACOBI_MAX_N < A.shape[-1] <= kernels.SYEV_MAX_N):
+            G = 0.5 * (A + A.t())
+            n = int(A.shape[-1])
+            w, Z = kernels.syev_top(G.contiguous(), n)
+            return w, t.flip(Z, dims=[1])
         return t.linalg.eigh(A)
 
     def sync(self):

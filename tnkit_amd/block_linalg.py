@@ -61,6 +61,18 @@ def _matrix_blocks(m: BlockTensor):
         yield r, c, m.data[o:o + R * C].reshape(R, C)
 
 
+def _eigh_desc(G: torch.Tensor):
+    """(w, V) of one symmetric float64 Gram matrix, both descending: the library's Jacobi
+    kernel (order <= 64) or divide-and-conquer solver (<= 2048), the vendor's above."""
+    G = 0.5 * (G + G.t())
+    n = int(G.shape[0])
+    if n <= kernels.SYEV_MAX_N:
+        w, V = kernels.eigh_top(G.contiguous(), n)
+        return torch.flip(w, (0,)), V
+    w, V = torch.linalg.eigh(G)
+    return torch.flip(w, (0,)), torch.flip(V, (-1,))
+
+
 def _svd(M: torch.Tensor, method: str):
     if method == "svd":
         return kernels.thin_svd(M)
@@ -70,13 +82,11 @@ def _svd(M: torch.Tensor, method: str):
     # projected factor re-orthonormalises the singular vectors
     R, C = M.shape
     if R >= C:
-        w, V = torch.linalg.eigh(kernels.gemm_tn(M.contiguous(), M.contiguous()))
-        V = torch.flip(V, (-1,))
+        _, V = _eigh_desc(kernels.gemm_tn(M.contiguous(), M.contiguous()))
         Q, Rr = torch.linalg.qr(M @ V)
     else:
         Mt = M.t().contiguous()
-        w, U = torch.linalg.eigh(kernels.gemm_tn(Mt, Mt))
-        U = torch.flip(U, (-1,))
+        _, U = _eigh_desc(kernels.gemm_tn(Mt, Mt))
         Q, Rr = torch.linalg.qr(M.t() @ U)
     d = Rr.diagonal()
     S = d.abs()
@@ -157,11 +167,11 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     """`truncated_svd` through the Gram matrix of each block on the `iso` side: 'left' gives
     U = the eigenvectors of M M^T (an exact left isometry) and S Vh = U^T M; 'right' gives
     Vh = the eigenvectors of M^T M (an exact right isometry) and U S = M Vh^T. The per-block
-    Gram matrices (K-major MFMA GEMMs) are padded to a power-of-two size (at least 128: small
-    batched eigensolves are slower than a few larger ones) with distinct values in [-2, -1) on
-    the padding diagonal, which keeps the padding apart from the nonnegative Gram spectrum and
-    free of degeneracy, and diagonalised in a few batched calls; one host read of all eigenvalues makes
-    the global cut (chi largest, as `truncated_svd`). S is taken as the norms of the carried
+    Gram matrices (K-major MFMA GEMMs) of order up to 64 are padded to a power-of-two size (at
+    least 16) with distinct values in [-2, -1) on the padding diagonal, which keeps the padding
+    apart from the nonnegative Gram spectrum and free of degeneracy, and diagonalised in batched
+    Jacobi calls; larger ones go one by one to the library's divide-and-conquer solver. One host
+    read of all eigenvalues makes the global cut (chi largest, as `truncated_svd`). S is taken as the norms of the carried
     factor, as tnkit's dense Gram truncation does (tnkit/linalg.py:138-204). Same return
     values and new-leg rule as `truncated_svd`."""
     if iso not in ("left", "right"):
@@ -187,6 +197,11 @@ def truncated_gram(m: BlockTensor, chi: int, iso: str = "left"):
     w_all = [None] * len(grams)
     v_all = [None] * len(grams)
     for nmax, idx in cls.items():
+        if nmax > kernels.EIGH_JACOBI_MAX_N:
+            # one divide-and-conquer solve per block (the library's solver, no padding)
+            for j in idx:
+                w_all[j], v_all[j] = _eigh_desc(grams[j])
+            continue
         Gp = torch.zeros((len(idx), nmax, nmax), dtype=torch.float64, device=dev)
         diag = (-(1.0 + torch.arange(nmax, dtype=torch.float64, device=dev) / nmax))[None, :] \
             .repeat(len(idx), 1)

@@ -1,0 +1,1175 @@
+// Real symmetric eigensolver for orders up to 2048 (float64): the eigendecomposition behind the
+// Gram truncation (tnkit/linalg.py:138-159) at the sweep's wide bonds, where the Gram matrix of
+// theta (d chi x d chi) reaches order 2048 at chi = 1024.
+//
+// Three phases:
+//  1. Householder tridiagonalisation A = Q T Q^T (lower form, Q = H_0 H_1 ... H_{n-2},
+//     H_j = I - tau_j v_j v_j^T, v_j zero at and above j, v_j[j+1] = 1) in ONE cooperative launch
+//     (sytrd_coop_kernel). Row i of the matrix lives in the LDS of workgroup i mod G for the
+//     whole reduction; a column costs one grid barrier. After the barrier of column j-1 every
+//     workgroup (a) forms w = p - gamma v from the published p = tau A v, (b) applies the rank-2
+//     update A -= v w^T + w v^T to its own rows, (c) rebuilds row j from its pre-update copy
+//     (published by the row's owner before the barrier) with the same rank-2 correction, so
+//     the next reflector is known to every workgroup without a second barrier, then (d) forms
+//     its rows of p for the new reflector and publishes them with its partial v.p. The matrix is
+//     read from LDS only; what crosses workgroups per column is two vectors of n and one partial
+//     per workgroup (double-buffered by column parity).
+//  2. Divide and conquer on T (Cuppen's tearing with Gu-Eisenstat eigenvectors), leaves of
+//     order <= 32 on the batched Jacobi kernel.
+//  3. The wanted eigenvectors of A = Q times those of T, applied as WY blocks of 32 reflectors
+//     (two MFMA GEMMs per block).
+#include <algorithm>
+#include <vector>
+
+#include "tnh_common.h"
+
+namespace tnh {
+
+// ---- grid barrier ---------------------------------------------------------------------------
+// Workgroup g writes the epoch to its own arrival word; workgroup 0 polls all arrival words (one
+// per thread) and then writes the epoch to every workgroup's own release word, on which that
+// workgroup waits. No word is written or polled by more than one workgroup besides the master,
+// so nothing contends (an atomic counter polled by 256 workgroups costs ~11 us per barrier on
+// this part; this form ~2.5 us; polling the exchanged data itself instead of a barrier floods
+// memory and measured slower). Data that crosses workgroups is written with relaxed agent-scope
+// atomics (through to memory) into slots that are written once per launch and read after the
+// barrier with ordinary loads (never cached before, so never stale); every wave waits for its
+// stores before arriving. Waits are bounded: a wait that runs past the spin limit sets *err (the
+// master still releases), and every workgroup leaves at its next barrier, so a broken launch
+// still ends.
+constexpr int kBarStride = 16;          // unsigned words between barrier words (64 bytes)
+constexpr int kBarSpin = 1 << 20;
+constexpr int kBarMaxG = 256;
+constexpr size_t kBarWords = (size_t)2 * kBarMaxG * kBarStride;
+
+struct GridBar {
+  unsigned* mem;    // arrival word of g at g * kBarStride, release word at (kBarMaxG + g) * stride
+  int* err;
+};
+
+__device__ __forceinline__ void st_shared(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ bool grid_sync(const GridBar& b, unsigned epoch, int* flag) {
+  __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
+  __syncthreads();
+  const unsigned G = gridDim.x, g = blockIdx.x;
+  if (threadIdx.x == 0)
+    __hip_atomic_store(b.mem + g * kBarStride, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (g == 0) {
+    int bad = 0;
+    for (unsigned q = threadIdx.x; q < G; q += blockDim.x) {
+      int spins = 0;
+      while (__hip_atomic_load(b.mem + q * kBarStride, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > kBarSpin) {
+          bad = 1;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    if (bad) __hip_atomic_store(b.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (unsigned q = threadIdx.x; q < G; q += blockDim.x)
+      __hip_atomic_store(b.mem + (kBarMaxG + q) * kBarStride, epoch, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (threadIdx.x == 0) {
+    int spins = 0, bad = 0;
+    while (__hip_atomic_load(b.mem + (kBarMaxG + g) * kBarStride, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+      if (++spins > kBarSpin) {
+        bad = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (bad) __hip_atomic_store(b.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    *flag = bad | __hip_atomic_load(b.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  __syncthreads();
+  return *flag == 0;
+}
+
+__global__ void bar_reset_kernel(unsigned* mem, int* err) {
+  for (size_t i = threadIdx.x; i < kBarWords; i += blockDim.x) mem[i] = 0u;
+  if (threadIdx.x == 0) *err = 0;
+}
+
+// ---- phase 1: tridiagonalisation -------------------------------------------------------------
+constexpr int kTrdThreads = 1024;
+constexpr int kTrdCols = 2;                     // columns per thread (l = t + 1024 c)
+constexpr int kTrdRows = 8;                     // rows per workgroup
+constexpr int kSyevMaxN = kTrdThreads * kTrdCols;   // 2048
+constexpr int kTrdWaves = kTrdThreads / 64;
+
+// Wave sum on VALU only: four DPP lane permutations (xor 1 and xor 2 inside quads, half-row and
+// row mirrors) leave every lane with its 16-lane row's sum, and the four row sums are read out of
+// lanes 0, 16, 32 and 48 (readlane) and added in a fixed order, so the total is the same
+// (bitwise) in every lane and every workgroup.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double lane_d(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+
+__device__ __forceinline__ double wsum(double v) {
+  v += dpp_d<0xB1>(v);    // quad_perm [1, 0, 3, 2]
+  v += dpp_d<0x4E>(v);    // quad_perm [2, 3, 0, 1]
+  v += dpp_d<0x141>(v);   // row_half_mirror
+  v += dpp_d<0x140>(v);   // row_mirror
+  return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
+}
+
+struct TrdArgs {
+  int n;
+  int ns;         // slot length of P and Rp (n rounded up, plus a gap)
+  int gs;         // slot length of part
+  const double* A;
+  long lda;
+  double* Vr;     // n x n, row j = v_j
+  double* d;      // n
+  double* e;      // n - 1
+  double* tau;    // n - 1
+  double* P;      // slot j: tau_j A v_j (rows > j)
+  double* Rp;     // slot j: row j before its update by (v_{j-1}, w_{j-1}) (columns >= j)
+  double* part;   // slot j: per-workgroup partial v_j . p_j
+  GridBar bar;
+};
+
+// Sum over the workgroup of one value per thread with a single barrier: red is used by
+// alternate calls in two halves, so a call's writes cannot meet the previous call's reads.
+__device__ __forceinline__ double trd_sum1(double v, double* red, int& par) {
+  v = wsum(v);
+  double* rb = red + kTrdWaves * par;
+  par ^= 1;
+  if ((threadIdx.x & 63) == 0) rb[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < kTrdWaves; ++w) s += rb[w];
+  return s;
+}
+
+// Grid of G workgroups (G <= n, ceil(n / G) <= kTrdRows) of 1024 threads, launched
+// cooperatively. Row i of A lives in the REGISTERS of workgroup i mod G for the whole reduction:
+// thread t holds columns t and t + 1024 of its workgroup's (at most 8) rows, so the rank-2 update
+// and the product A v are register arithmetic and the only on-chip traffic is the reductions.
+// A is read once (row i by workgroup i mod G, as given: the caller passes a symmetric matrix).
+//
+// Column j after the barrier: gamma from the partials, w_{j-1} = p - gamma v (v_{j-1}[j] = 1),
+// row j = its published copy - v w^T - w v^T at row j, the reflector v_j, then one pass over
+// the own rows that applies the rank-2 update of column j-1 and accumulates p_j = tau_j A v_j;
+// the owner of row j+1 publishes it from the same registers.
+__global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
+  __shared__ double red[2 * kTrdWaves];
+  __shared__ double red8[2][kTrdWaves][kTrdRows];
+  __shared__ double own_v[kTrdRows], own_p[kTrdRows], own_vn[kTrdRows];
+  __shared__ double vnext;    // v_{j-1}[j+1], for row j's subdiagonal entry
+  __shared__ int flag;
+  const int n = a.n, G = gridDim.x, g = blockIdx.x, t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int nr = (n - g + G - 1) / G;
+  int par = 0, par8 = 0;
+
+  double x[kTrdRows][kTrdCols];   // the own rows at this thread's columns
+#pragma unroll
+  for (int r = 0; r < kTrdRows; ++r)
+#pragma unroll
+    for (int c = 0; c < kTrdCols; ++c) {
+      const int l = t + kTrdThreads * c;
+      x[r][c] = (r < nr && l < n) ? a.A[(long)(g + r * G) * a.lda + l] : 0.0;
+    }
+  if (n == 1) {
+    if (t == 0) a.d[0] = x[0][0];
+    return;
+  }
+  if (g == 0)
+#pragma unroll
+    for (int c = 0; c < kTrdCols; ++c) {
+      const int l = t + kTrdThreads * c;
+      if (l < n) st_shared(a.Rp + l, x[0][c]);
+    }
+  if (t < kTrdRows) {
+    own_v[t] = 0.0;
+    own_p[t] = 0.0;
+  }
+  unsigned epoch = 1;
+  if (!grid_sync(a.bar, epoch++, &flag)) return;
+
+  double vr[kTrdCols];   // v_{j-1} at this thread's columns
+#pragma unroll
+  for (int c = 0; c < kTrdCols; ++c) vr[c] = 0.0;
+  double tau_prev = 0.0;
+
+  for (int j = 0; j < n - 1; ++j) {
+    const double* Rj = a.Rp + (long)j * a.ns;
+    double R[kTrdCols], wr[kTrdCols];
+    double gamma = 0.0, dj, alpha;
+    if (j > 0) {
+      const double* Pj = a.P + (long)(j - 1) * a.ns;
+      const double* pj = a.part + (long)(j - 1) * a.gs;
+      const double ps = t < G ? pj[t] : 0.0;
+      double Pl[kTrdCols];
+#pragma unroll
+      for (int c = 0; c < kTrdCols; ++c) {
+        const int l = t + kTrdThreads * c;
+        Pl[c] = l >= j && l < n ? Pj[l] : 0.0;
+        R[c] = l >= j && l < n ? Rj[l] : 0.0;
+      }
+      const double pjj = Pj[j], pj1 = Pj[j + 1], rjj = Rj[j], rj1 = Rj[j + 1];
+      gamma = 0.5 * tau_prev * trd_sum1(ps, red, par);
+      const double wj = pjj - gamma, vx = vnext;
+#pragma unroll
+      for (int c = 0; c < kTrdCols; ++c) {
+        wr[c] = Pl[c] - gamma * vr[c];
+        R[c] -= wr[c] + wj * vr[c];
+      }
+      dj = rjj - 2.0 * wj;
+      alpha = rj1 - vx * wj - (pj1 - gamma * vx);
+    } else {
+#pragma unroll
+      for (int c = 0; c < kTrdCols; ++c) {
+        const int l = t + kTrdThreads * c;
+        R[c] = l < n ? Rj[l] : 0.0;
+        wr[c] = 0.0;
+      }
+      dj = Rj[0];
+      alpha = Rj[1];
+    }
+
+    // the reflector of column j: x = row j right of the diagonal
+    double sq = 0.0;
+#pragma unroll
+    for (int c = 0; c < kTrdCols; ++c) {
+      const int l = t + kTrdThreads * c;
+      if (l >= j + 2 && l < n) sq += R[c] * R[c];
+    }
+    const double sigma = trd_sum1(sq, red, par);
+    double beta = alpha, tj = 0.0, scale = 0.0;
+    if (sigma > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
+      tj = (beta - alpha) / beta;
+      scale = 1.0 / (alpha - beta);
+    }
+    double vn[kTrdCols];
+#pragma unroll
+    for (int c = 0; c < kTrdCols; ++c) {
+      const int l = t + kTrdThreads * c;
+      vn[c] = l == j + 1 ? 1.0 : (l >= j + 2 && l < n ? R[c] * scale : 0.0);
+      if (l < n && l >= g && (l - g) % G == 0) {
+        own_vn[(l - g) / G] = vn[c];
+        a.Vr[(long)j * n + l] = vn[c];     // v_j at the own rows; row j of Vr is shared out
+      }
+      if (l == j + 2) vnext = vn[c];
+    }
+    if (g == j % G && t == 0) {
+      a.d[j] = dj;
+      a.e[j] = beta;
+      a.tau[j] = tj;
+    }
+    if (j == n - 2) {   // x has one entry (tau = 0); the last diagonal gets column j-1's update
+      if (g == (n - 1) % G) {
+        const int rl = (n - 1 - g) / G;
+        const double vi = own_v[rl], wi = own_p[rl] - gamma * own_v[rl];
+#pragma unroll
+        for (int r = 0; r < kTrdRows; ++r)
+#pragma unroll
+          for (int c = 0; c < kTrdCols; ++c)
+            if (r == rl && t + kTrdThreads * c == n - 1) a.d[n - 1] = x[r][c] - 2.0 * vi * wi;
+      }
+      break;
+    }
+    __syncthreads();   // own_vn and vnext written
+
+    // one pass over the own rows: update by (v_{j-1}, w_{j-1}) on columns >= j, p_j on rows > j,
+    // row j+1 published by its owner
+    const int rpub = (j + 1 < n - 1 && g == (j + 1) % G) ? (j + 1 - g) / G : -1;
+    double (*rb)[kTrdRows] = red8[par8];
+    par8 ^= 1;
+#pragma unroll
+    for (int r = 0; r < kTrdRows; ++r) {
+      const double vi = own_v[r], wi = own_p[r] - gamma * own_v[r];
+      const bool live = r < nr && g + r * G >= j;
+      double sacc = 0.0;
+#pragma unroll
+      for (int c = 0; c < kTrdCols; ++c) {
+        const int l = t + kTrdThreads * c;
+        if (live && l >= j && l < n) x[r][c] -= vi * wr[c] + wi * vr[c];
+        sacc += x[r][c] * vn[c];
+        if (r == rpub && l > j && l < n) st_shared(a.Rp + (long)(j + 1) * a.ns + l, x[r][c]);
+      }
+      sacc = wsum((r < nr && g + r * G > j) ? sacc : 0.0);
+      if (lane == 0) rb[wv][r] = sacc;
+    }
+    __syncthreads();
+    double partial = 0.0;
+    if (t < nr) {
+      const int i = g + t * G;
+      double p = 0.0;
+      if (i >= j + 1) {
+        double sum = 0.0;
+#pragma unroll
+        for (int w = 0; w < kTrdWaves; ++w) sum += rb[w][t];
+        p = tj * sum;
+        st_shared(a.P + (long)j * a.ns + i, p);
+        partial = own_vn[t] * p;
+      }
+      own_p[t] = p;
+      own_v[t] = own_vn[t];
+    }
+    if (wv == 0) {
+      partial = wsum(partial);
+      if (t == 0) st_shared(a.part + (long)j * a.gs + g, partial);
+    }
+    tau_prev = tj;
+#pragma unroll
+    for (int c = 0; c < kTrdCols; ++c) vr[c] = vn[c];
+    if (!grid_sync(a.bar, epoch++, &flag)) return;
+  }
+}
+
+int trd_grid(int n) { return (n + kTrdRows - 1) / kTrdRows; }
+
+int trd_slot(int n) { return (n + 31) / 32 * 32 + 256; }
+
+size_t trd_ws_bytes(int n) {
+  const int G = trd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
+  size_t b = 0;
+  b += align_up((size_t)n * ns * sizeof(double), 256);      // P slots
+  b += align_up((size_t)n * ns * sizeof(double), 256);      // Rp slots
+  b += align_up((size_t)n * gs * sizeof(double), 256);      // part slots
+  b += align_up(kBarWords * sizeof(unsigned), 256);         // barrier words
+  b += align_up(sizeof(int), 256);                          // err
+  return b;
+}
+
+// Launches the reduction; returns TNH_OK or an error code (cooperative launch refused, spin
+// limit hit). Checking the spin flag needs a host read, so it is left to the caller (err).
+int launch_sytrd(int n, const double* A, long lda, double* Vr, double* d, double* e, double* tau,
+                 Arena& ar, int** err_out, hipStream_t st) {
+  const int G = trd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
+  double* P = ar.take<double>((size_t)n * ns);
+  double* Rp = ar.take<double>((size_t)n * ns);
+  double* part = ar.take<double>((size_t)n * gs);
+  unsigned* bar = ar.take<unsigned>(kBarWords);
+  int* err = ar.take<int>(1);
+  TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "sytrd: workspace too small");
+  *err_out = err;
+  hipLaunchKernelGGL(bar_reset_kernel, dim3(1), dim3(256), 0, st, bar, err);
+  TNH_CHECK_LAUNCH("sytrd barrier reset");
+  const size_t smem = 0;
+  hipError_t he;
+  TrdArgs args{n, ns, gs, A, lda, Vr, d, e, tau, P, Rp, part, GridBar{bar, err}};
+  void* kargs[] = {&args};
+  he = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&sytrd_coop_kernel), dim3(G),
+                                  dim3(kTrdThreads), kargs, (unsigned)smem, st);
+  TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP,
+              "sytrd: cooperative launch of %d workgroups (%zu B LDS each) refused: %s", G, smem,
+              hipGetErrorString(he));
+  return TNH_OK;
+}
+
+// ---- phase 2: divide and conquer on the tridiagonal ------------------------------------------
+// T (order n) is padded to N = s 2^L (s <= 32) with decoupled entries far below its spectrum
+// (zero couplings), so every node of level l has order s 2^l and every launch is uniform. Leaf
+// b holds rows [b s, (b+1) s) with the tearing corrections |e| on its boundary diagonals and is
+// solved by the batched Jacobi kernel. A node [lo, lo + K) with halves of n1 = K / 2 and coupling
+// beta = e[lo + n1 - 1] is diag(T1, T2) + rho u u^T, rho = |beta|, u = (e_last; sign(beta) e_1),
+// i.e. diag(Q1, Q2) (diag(D) + rho z z^T) diag(Q1, Q2)^T with z = (last row of Q1; sign row 1
+// of Q2). Per level:
+//   dc_prep       (workgroup per node) sort D by merging the halves, normalise z, deflate small
+//                 z entries and close pairs (Givens), list the K' remaining entries
+//   dc_secular    (wave per root) bisection for the roots of 1/rho + sum z_i^2 / (D_i - lam),
+//                 as an offset mu from the nearer pole (Gu and Eisenstat's origin shift)
+//   dc_zhat       (wave per entry) the z for which the computed roots are exact (Loewner)
+//   dc_norms      (wave per eigenpair) eigenvector norms and the final ascending positions
+//   dc_rows       (wave per row) M = P R (eigenvectors of the rank-one problem), row by row
+//   dc_rotate     (workgroup per node) the deflation rotations on rows of M, in reverse order
+//   two batched GEMMs: Qt_new = M^T-products with the halves' eigenvector blocks.
+// Eigenvectors are stored as rows: Qt[c][i] = component i of eigenvector c.
+
+constexpr int kDcLeaf = 32;
+constexpr int kDcTerms = kSyevMaxN / kWave;     // secular terms per lane
+constexpr double kEps = 2.220446049250313e-16;
+
+struct DcLevel {
+  int K, n1, N, top;        // node order, half, padded order, 1 on the last level
+  const double* w_in;       // N: the halves' eigenvalues, ascending per half
+  const double* Qt_in;      // N x N
+  const double* e;          // couplings (length N - 1, padded with zeros)
+  double* w_out;            // N: ascending per node
+  double* Mb;               // nodes x K x K
+  // per-level arrays of N (node b at offset b K)
+  int* perm;                // sorted position -> local index
+  double* Ds;               // sorted (rotated) D
+  double* zs;               // sorted normalised z, zero where deflated
+  int* inv;                 // sorted position -> index among the K' kept entries, -1 deflated
+  int* kept;                // kept index -> sorted position
+  int* rot_j;               // rotation r of node b: rows (rot_jl[r], rot_j[r]) of sorted space
+  int* rot_jl;
+  double* rot_c;
+  double* rot_s;
+  int* cnt;                 // per node: [0] K', [1] rotations
+  double* rhop;             // per node: rho' (rho |z|^2)
+  int* dl;                  // deflated sorted positions, in order
+  double* mu;               // per kept root j: offset from its origin
+  double* dor;              // per kept root j: D' at its origin
+  int* ent;                 // final column -> entity (kept roots first, then deflated)
+  double* zh;               // per kept entry: zhat
+  double* nrm;              // per kept root: 1 / |u_j|
+};
+
+__device__ __forceinline__ double wave_bcast_sum(double v) {
+  v = wsum(v);
+  return __shfl(v, 0, 64);
+}
+
+// One workgroup of 1024 threads per node.
+__global__ void __launch_bounds__(1024) dc_prep_kernel(DcLevel L) {
+  extern __shared__ __align__(16) char smem_raw[];
+  const int K = L.K, n1 = L.n1, b = blockIdx.x, t = threadIdx.x;
+  const int lo = b * K;
+  double* D = reinterpret_cast<double*>(smem_raw);    // K (by local index)
+  double* z = D + K;                                  // K
+  double* Ds = z + K;                                 // K (sorted)
+  double* zs = Ds + K;                                // K
+  int* perm = reinterpret_cast<int*>(zs + K);         // K
+  __shared__ double red[16];
+  __shared__ double sc[2];
+  const double beta = L.e[lo + n1 - 1];
+  const double rho = fabs(beta), sg = beta < 0.0 ? -1.0 : 1.0;
+  for (int i = t; i < K; i += blockDim.x) {
+    D[i] = L.w_in[lo + i];
+    z[i] = i < n1 ? L.Qt_in[(long)(lo + i) * L.N + lo + n1 - 1]
+                  : sg * L.Qt_in[(long)(lo + i) * L.N + lo + n1];
+    perm[i] = i;
+  }
+  __syncthreads();
+  // merge of the two ascending halves by rank (ties: first half first); a non-finite input can
+  // make ranks collide, which leaves duplicates but every index in range
+  for (int i = t; i < K; i += blockDim.x) {
+    const double x = D[i];
+    int r;
+    if (i < n1) {      // count of second-half values < x
+      int a = 0, c = K - n1;
+      while (a < c) {
+        const int m = (a + c) >> 1;
+        if (D[n1 + m] < x) a = m + 1; else c = m;
+      }
+      r = i + a;
+    } else {           // count of first-half values <= x
+      int a = 0, c = n1;
+      while (a < c) {
+        const int m = (a + c) >> 1;
+        if (D[m] <= x) a = m + 1; else c = m;
+      }
+      r = (i - n1) + a;
+    }
+    perm[r] = i;
+  }
+  __syncthreads();
+  double zz = 0.0, dm = 0.0;
+  for (int q = t; q < K; q += blockDim.x) {
+    Ds[q] = D[perm[q]];
+    zs[q] = z[perm[q]];
+    zz += zs[q] * zs[q];
+    dm = fmax(dm, fabs(Ds[q]));
+  }
+  // reductions: |z|^2 and max |D|
+  zz = wsum(zz);
+  for (int o = 32; o >= 1; o >>= 1) dm = fmax(dm, __shfl_xor(dm, o, 64));
+  if ((t & 63) == 0) {
+    red[t >> 6] = zz;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    sc[0] = s;
+  }
+  __syncthreads();
+  if ((t & 63) == 0) red[t >> 6] = dm;
+  __syncthreads();
+  if (t == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    sc[1] = m;
+  }
+  __syncthreads();
+  const double zn2 = sc[0], dmax = sc[1];
+  const double rhop = rho * zn2;
+  const double zinv = zn2 > 0.0 ? 1.0 / sqrt(zn2) : 0.0;
+  for (int q = t; q < K; q += blockDim.x) zs[q] *= zinv;
+  int* sinv = perm + K;     // K: sorted position -> kept index, negative when deflated
+  int* skept = sinv + K;    // K: kept index -> sorted position
+  __shared__ int snk, snr;
+  __syncthreads();
+  if (t == 0) {
+    // deflation, sequential in sorted order (tol as in LAPACK's dlaed2); the last kept entry's
+    // z and D ride in registers
+    const double tol = 8.0 * kEps * fmax(dmax, rhop);
+    int nk = 0, nr = 0, jl = -1;
+    double zl = 0.0, dl = 0.0;
+    for (int j = 0; j < K; ++j) {
+      double zj = zs[j], dj = Ds[j];
+      if (rhop * fabs(zj) <= tol) {
+        zs[j] = 0.0;
+        sinv[j] = -1;
+        continue;
+      }
+      // close pair: |(d_j - d_l) c s| <= tol with c = z_j / t, s = -z_l / t, t = |(z_j, z_l)|
+      if (jl >= 0 && fabs((dj - dl) * zj * zl) <= tol * (zj * zj + zl * zl)) {
+        const double tt = sqrt(zj * zj + zl * zl), it = 1.0 / tt;
+        const double c = zj * it, sn = -zl * it;
+        const double a2 = dl * c * c + dj * sn * sn, b2 = dl * sn * sn + dj * c * c;
+        Ds[jl] = a2;
+        Ds[j] = b2;
+        dj = b2;
+        zj = tt;
+        zs[j] = tt;
+        zs[jl] = 0.0;
+        L.rot_jl[lo + nr] = jl;
+        L.rot_j[lo + nr] = j;
+        L.rot_c[lo + nr] = c;
+        L.rot_s[lo + nr] = sn;
+        ++nr;
+        sinv[jl] = -1;     // jl leaves the kept list (it was appended last)
+        --nk;
+      }
+      sinv[j] = nk;
+      skept[nk++] = j;
+      jl = j;
+      zl = zj;
+      dl = dj;
+    }
+    int nd = 0;
+    for (int j = 0; j < K; ++j)
+      if (sinv[j] < 0) {
+        sinv[j] = -1 - nd;
+        skept[nk + nd++] = j;      // deflated positions follow the kept ones
+      }
+    snk = nk;
+    snr = nr;
+  }
+  __syncthreads();
+  const int nk = snk;
+  for (int q = t; q < K; q += blockDim.x) {
+    L.perm[lo + q] = perm[q];
+    L.Ds[lo + q] = Ds[q];
+    L.zs[lo + q] = zs[q];
+    L.inv[lo + q] = sinv[q];
+    if (q < nk) L.kept[lo + q] = skept[q];
+    else L.dl[lo + q - nk] = skept[q];
+  }
+  if (t == 0) {
+    L.cnt[2 * b] = nk;
+    L.cnt[2 * b + 1] = snr;
+    L.rhop[b] = rhop;
+  }
+}
+
+// One wave per kept root j of node blockIdx.y (4 waves per block). The root of
+// f(mu) = 1/rho + sum z_i^2 / (delta_i - mu) (delta_i = D_i - D_origin) is bracketed and refined
+// by the rational model of Bunch, Nielsen and Sorensen: the poles left and right of the interval
+// are each replaced by c + s / (delta - x), matched in value and slope at the current iterate,
+// and the model's root in the bracket is the next iterate (quadratic convergence); a step that
+// leaves the bracket is replaced by bisection, and every evaluation tightens the bracket.
+__global__ void __launch_bounds__(256) dc_secular_kernel(DcLevel L) {
+  const int K = L.K, b = blockIdx.y, lane = threadIdx.x & 63;
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lo = b * K, nk = L.cnt[2 * b];
+  if (j >= nk) return;
+  const int* kept = L.kept + lo;
+  const double invr = 1.0 / L.rhop[b];
+  double dv[kDcTerms], z2[kDcTerms];
+#pragma unroll
+  for (int c = 0; c < kDcTerms; ++c) {
+    const int i = lane + kWave * c;
+    dv[c] = 0.0;
+    z2[c] = 0.0;
+    if (i < nk) {
+      const int q = kept[i];
+      dv[c] = L.Ds[lo + q];
+      z2[c] = L.zs[lo + q] * L.zs[lo + q];
+    }
+  }
+  const double dj = L.Ds[lo + kept[j]];
+  const bool last = j == nk - 1;
+  const double gap = last ? 0.0 : L.Ds[lo + kept[j + 1]] - dj;
+  int o = j;
+  double a, c2;   // bracket of mu
+  if (!last) {
+    double f = 0.0;
+#pragma unroll
+    for (int c = 0; c < kDcTerms; ++c) f += z2[c] / ((dv[c] - dj) - 0.5 * gap);
+    f = invr + wave_bcast_sum(f);
+    if (f >= 0.0) {
+      a = 0.0;
+      c2 = 0.5 * gap;
+    } else {
+      o = j + 1;
+      a = -0.5 * gap;
+      c2 = 0.0;
+    }
+  } else {
+    double zz = 0.0;
+#pragma unroll
+    for (int c = 0; c < kDcTerms; ++c) zz += z2[c];
+    a = 0.0;
+    c2 = wave_bcast_sum(zz) / invr;
+  }
+  const double dorg = o == j ? dj : L.Ds[lo + kept[o]];
+#pragma unroll
+  for (int c = 0; c < kDcTerms; ++c) dv[c] -= dorg;    // pole offsets from the origin
+  const double dl = dj - dorg, dr = last ? 0.0 : dl + gap;   // the interval's poles
+  double mu = 0.5 * (a + c2);
+  for (int it = 0; it < 200; ++it) {
+    // psi (poles <= j), phi (poles > j), and their slopes at mu
+    double ps = 0.0, pd = 0.0, fs = 0.0, fd = 0.0;
+#pragma unroll
+    for (int c = 0; c < kDcTerms; ++c) {
+      const int i = lane + kWave * c;
+      const double inv = 1.0 / (dv[c] - mu);
+      const double tt = z2[c] * inv, td = tt * inv;
+      if (i <= j) {
+        ps += tt;
+        pd += td;
+      } else {
+        fs += tt;
+        fd += td;
+      }
+    }
+    ps = wave_bcast_sum(ps);
+    pd = wave_bcast_sum(pd);
+    fs = wave_bcast_sum(fs);
+    fd = wave_bcast_sum(fd);
+    const double f = invr + ps + fs;
+    if (f > 0.0) c2 = mu; else a = mu;
+    // f is known to within eps * err (dlaed4's bound); below that the root is found
+    const double err = 8.0 * (fs - ps) + 2.0 * invr + 3.0 * fabs(mu) * (pd + fd);
+    if (fabs(f) <= kEps * err || c2 - a <= 2.0 * kEps * fmax(fabs(a), fabs(c2))) break;
+    // model: cc + q / (dl - x) + s / (dr - x) = 0, x = mu + eta
+    const double Dl = dl - mu, Dr = dr - mu;
+    const double q = pd * Dl * Dl;
+    double nx;
+    if (last) {
+      const double cc = invr + ps - pd * Dl;     // cc + q / (Dl - eta) = 0
+      nx = cc > 0.0 ? mu + Dl + q / cc : 0.5 * (a + c2);
+    } else {
+      const double sr = fd * Dr * Dr;
+      const double cc = invr + (ps - pd * Dl) + (fs - fd * Dr);
+      // cc eta^2 - B eta + C = 0, C = Dl Dr f
+      const double B = cc * (Dl + Dr) + q + sr, C = Dl * Dr * f;
+      const double disc = B * B - 4.0 * cc * C;
+      double eta;
+      if (disc < 0.0) eta = 0.5 * (a + c2) - mu;
+      else if (cc == 0.0) eta = B != 0.0 ? C / B : 0.5 * (a + c2) - mu;
+      else eta = 2.0 * C / (B + copysign(sqrt(disc), B));
+      nx = mu + eta;
+    }
+    if (!(nx > a && nx < c2)) nx = 0.5 * (a + c2);
+    if (nx == mu) break;
+    mu = nx;
+  }
+  if (lane == 0) {
+    L.mu[lo + j] = mu;
+    L.dor[lo + j] = dorg;
+  }
+}
+
+// zhat_i (Loewner): zhat_i^2 = -(D_i - lam_i) prod_{j != i} (D_i - lam_j) / (D_i - D_j) / rho',
+// with D_i - lam_j = (D_i - D_origin(j)) - mu_j; the product is kept as mantissa and exponent.
+__global__ void __launch_bounds__(256) dc_zhat_kernel(DcLevel L) {
+  const int K = L.K, b = blockIdx.y, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lo = b * K, nk = L.cnt[2 * b];
+  if (i >= nk) return;
+  const int* kept = L.kept + lo;
+  const double di = L.Ds[lo + kept[i]];
+  double m = 1.0;
+  int ex = 0;
+  for (int j = lane; j < nk; j += kWave) {
+    const double q = (di - L.dor[lo + j]) - L.mu[lo + j];
+    const double f = j == i ? q : q / (di - L.Ds[lo + kept[j]]);
+    int e2;
+    m = frexp(m * f, &e2);
+    ex += e2;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) {
+    const double m2 = __shfl_xor(m, o, 64);
+    const int e2 = __shfl_xor(ex, o, 64);
+    int e3;
+    m = frexp(m * m2, &e3);
+    ex += e2 + e3;
+  }
+  m = __shfl(m, 0, 64);
+  ex = __shfl(ex, 0, 64);
+  if (lane == 0) {
+    double v = -m / L.rhop[b];          // zhat^2 = v 2^ex, v in (0, 1] when the roots interlace
+    if (ex & 1) {
+      v *= 2.0;
+      ex -= 1;
+    }
+    const double zh = v > 0.0 ? ldexp(sqrt(v), ex / 2) : 0.0;
+    L.zh[lo + i] = L.zs[lo + kept[i]] < 0.0 ? -zh : zh;
+  }
+}
+
+// Entity e of node b: a kept root (e < K') or the (e - K')-th deflated entry. Its eigenvector
+// norm (roots), its value and its rank among all K values (ascending; ties by entity) give the
+// final column: rank, or K - 1 - rank on the last level (descending order).
+__global__ void __launch_bounds__(256) dc_norms_kernel(DcLevel L) {
+  const int K = L.K, b = blockIdx.y, lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lo = b * K, nk = L.cnt[2 * b];
+  if (e >= K) return;
+  const int* kept = L.kept + lo;
+  double val;
+  if (e < nk) {
+    const double dor = L.dor[lo + e], mu = L.mu[lo + e];
+    val = dor + mu;
+    double s = 0.0;
+    for (int i = lane; i < nk; i += kWave) {
+      const double u = L.zh[lo + i] / ((L.Ds[lo + kept[i]] - dor) - mu);
+      s += u * u;
+    }
+    s = wave_bcast_sum(s);
+    if (lane == 0) L.nrm[lo + e] = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+  } else {
+    val = L.Ds[lo + L.dl[lo + e - nk]];
+  }
+  int r = 0;
+  for (int f = lane; f < K; f += kWave) {
+    const double vf = f < nk ? L.dor[lo + f] + L.mu[lo + f] : L.Ds[lo + L.dl[lo + f - nk]];
+    r += (vf < val || (vf == val && f < e)) ? 1 : 0;
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
+  if (lane == 0) {
+    L.w_out[lo + r] = val;
+    L.ent[lo + (L.top ? K - 1 - r : r)] = e;
+  }
+}
+
+// Row perm[q] of M for sorted position q: the eigenvector entries of kept row inv[q] (or the unit
+// row of a deflated entry) at every final column < ncol.
+__global__ void __launch_bounds__(256) dc_rows_kernel(DcLevel L, int ncol) {
+  const int K = L.K, b = blockIdx.y, lane = threadIdx.x & 63;
+  const int q = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lo = b * K, nk = L.cnt[2 * b];
+  if (q >= K) return;
+  const int iv = L.inv[lo + q];
+  double* row = L.Mb + (long)b * K * ncol + (long)L.perm[lo + q] * ncol;
+  if (iv < 0) {
+    const int me = nk + (-1 - iv);
+    for (int c = lane; c < ncol; c += kWave) row[c] = L.ent[lo + c] == me ? 1.0 : 0.0;
+    return;
+  }
+  const double zi = L.zh[lo + iv], di = L.Ds[lo + q];
+  for (int c = lane; c < ncol; c += kWave) {
+    const int e = L.ent[lo + c];   // in [0, K) unless a non-finite value broke the ranking
+    row[c] = (e >= 0 && e < nk) ? zi / ((di - L.dor[lo + e]) - L.mu[lo + e]) * L.nrm[lo + e] : 0.0;
+  }
+}
+
+// The deflation rotations of node blockIdx.y on rows of M, last first: one thread per column
+// applies all of them in order, keeping the row that the next rotation of a chain takes in a
+// register, so a chain costs one load and one store per rotation and no barrier.
+__global__ void __launch_bounds__(256) dc_rotate_kernel(DcLevel L, int ncol) {
+  const int K = L.K, b = blockIdx.y, lo = b * K;
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nr = L.cnt[2 * b + 1];
+  if (col >= ncol || nr == 0) return;
+  double* Mc = L.Mb + (long)b * K * ncol + col;
+  int held = -1;
+  double hv = 0.0;
+  for (int r = nr - 1; r >= 0; --r) {
+    const int ra = L.perm[lo + L.rot_jl[lo + r]], rb = L.perm[lo + L.rot_j[lo + r]];
+    const double c = L.rot_c[lo + r], sn = L.rot_s[lo + r];
+    const double x = ra == held ? hv : Mc[(long)ra * ncol];
+    const double y = rb == held ? hv : Mc[(long)rb * ncol];
+    if (held >= 0 && held != ra && held != rb) Mc[(long)held * ncol] = hv;
+    Mc[(long)rb * ncol] = sn * x + c * y;
+    held = ra;
+    hv = c * x - sn * y;
+  }
+  Mc[(long)held * ncol] = hv;
+}
+
+// Padding and leaves. One workgroup: the bound |T| <= max|d| + 2 max|e|, the padded d and e
+// (decoupled entries at -3 bound), then leaf matrices with the tearing corrections.
+__global__ void __launch_bounds__(1024) dc_pad_kernel(int n, int N, const double* d,
+                                                      const double* e, double* dp, double* ep) {
+  __shared__ double red[16];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    m = fmax(m, fabs(d[i]));
+    if (i < n - 1) m = fmax(m, 2.0 * fabs(e[i]));
+  }
+  for (int o = 32; o >= 1; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  double bound = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) bound = fmax(bound, red[w]);
+  const double dummy = bound > 0.0 ? -3.0 * bound : -1.0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    dp[i] = i < n ? d[i] : dummy;
+    if (i < N - 1) ep[i] = i < n - 1 ? e[i] : 0.0;
+  }
+}
+
+__global__ void dc_leaves_kernel(int s, int N, const double* dp, const double* ep, double* A) {
+  const int b = blockIdx.x, lo = b * s;
+  double* Ab = A + (long)b * s * s;
+  for (int idx = threadIdx.x; idx < s * s; idx += blockDim.x) {
+    const int i = idx / s, j = idx - i * s;
+    double v = 0.0;
+    if (i == j) {
+      v = dp[lo + i];
+      if (i == 0 && lo > 0) v -= fabs(ep[lo - 1]);
+      if (i == s - 1 && lo + s < N) v -= fabs(ep[lo + s - 1]);
+    } else if (j == i + 1) {
+      v = ep[lo + i];
+    } else if (i == j + 1) {
+      v = ep[lo + j];
+    }
+    Ab[idx] = v;
+  }
+}
+
+// Qt[lo + c][lo + i] = Z_b[i][c] (leaf eigenvectors as rows).
+__global__ void dc_scatter_leaves_kernel(int s, int N, const double* Z, double* Qt) {
+  const int b = blockIdx.x, lo = b * s;
+  const double* Zb = Z + (long)b * s * s;
+  for (int idx = threadIdx.x; idx < s * s; idx += blockDim.x) {
+    const int c = idx / s, i = idx - c * s;
+    Qt[(long)(lo + c) * N + lo + i] = Zb[i * s + c];
+  }
+}
+
+
+// ---- phase 3: back-transformation ------------------------------------------------------------
+// Z <- Q Z with Q = Q_0 Q_1 ... (blocks of kBtNb reflectors, Q_b = I - V T V^T, T upper
+// triangular from the recurrence T[0:c, c] = -tau_c T[0:c, 0:c] (V[:, 0:c]^T v_c), T[c][c] =
+// tau_c), applied last block first: Y = V^T Z, Y' = -T Y, Z += V Y' (three K-major GEMMs).
+constexpr int kBtNb = 64;
+
+// Vc[i][j] = Vr[j][i] for reflectors j < n - 1, zero for n - 1 <= j < NV (row stride NV).
+__global__ void transpose_kernel(int n, int NV, const double* __restrict__ src,
+                                 double* __restrict__ dst) {
+  __shared__ double tile[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;   // bx: column block of src (i), by: j
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int j = by + r, i = bx + threadIdx.x;
+    tile[r][threadIdx.x] = (j < n - 1 && i < n) ? src[(long)j * n + i] : 0.0;
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int i = bx + r, j = by + threadIdx.x;
+    if (i < n && j < NV) dst[(long)i * NV + j] = tile[threadIdx.x][r];
+  }
+}
+
+// Panel b (one workgroup): from S = V_b^T V_b (nb x nb) and tau, writes A_b = -T^T (row p, col q
+// = -T[q][p]), the left operand of Y' = -T Y in the K-major GEMM.
+__global__ void __launch_bounds__(256) larft_kernel(int nrefl, const double* S, const double* tau,
+                                                    double* negTt) {
+  __shared__ double T[kBtNb][kBtNb + 1];
+  __shared__ double tmp[kBtNb];
+  const int b = blockIdx.x, jb = b * kBtNb, nb = min(kBtNb, nrefl - jb), t = threadIdx.x;
+  const double* Sb = S + (long)b * kBtNb * kBtNb;
+  for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) T[idx / kBtNb][idx % kBtNb] = 0.0;
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    const double tc = tau[jb + c];
+    // tmp[p] = sum_{q < c} T[p][q] S[q][c] for p < c (T upper: q >= p)
+    if (t < c) {
+      double s = 0.0;
+      for (int q = t; q < c; ++q) s += T[t][q] * Sb[q * kBtNb + c];
+      tmp[t] = s;
+    }
+    __syncthreads();
+    if (t < c) T[t][c] = -tc * tmp[t];
+    if (t == 0) T[c][c] = tc;
+    __syncthreads();
+  }
+  double* out = negTt + (long)b * kBtNb * kBtNb;
+  for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) {
+    const int p = idx / kBtNb, q = idx % kBtNb;
+    out[idx] = -T[q][p];
+  }
+}
+
+__global__ void syev_status_kernel(const int* err, const int* sweeps, int nleaf, int* info) {
+  int bad = 0;
+  for (int i = threadIdx.x; i < nleaf; i += blockDim.x) bad |= sweeps[i] < 0;
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) *info = (*err ? 1 : 0) | (bad ? 2 : 0);
+}
+
+struct SyevPlan {
+  int n, k, L, s, N, npan;
+};
+
+SyevPlan syev_plan(int n, int k) {
+  SyevPlan p;
+  p.n = n;
+  p.k = k;
+  p.L = 1;
+  while ((n + (1 << p.L) - 1) >> p.L > kDcLeaf) ++p.L;
+  p.s = (n + (1 << p.L) - 1) >> p.L;
+  p.N = p.s << p.L;
+  p.npan = (n - 1 + kBtNb - 1) / kBtNb;
+  return p;
+}
+
+// Workspace pieces, in carving order (one function, used for both the size and the carve).
+template <typename F>
+void syev_pieces(const SyevPlan& p, F&& f) {
+  const size_t n = p.n, N = p.N, k = p.k, npan = p.npan;
+  f("trd", tnh::trd_ws_bytes(p.n));
+  f("Vr", n * n * 8);
+  f("Vc", n * npan * kBtNb * 8);
+  f("d", n * 8);
+  f("e", n * 8);
+  f("tau", n * 8);
+  f("dp", N * 8);
+  f("ep", N * 8);
+  f("QtA", N * N * 8);
+  f("QtB", N * N * 8);
+  f("Mb", N * N * 8);
+  f("leafA", N * p.s * 8);
+  f("leafZ", N * p.s * 8);
+  f("wA", N * 8);
+  f("wB", N * 8);
+  f("sweeps", (N / p.s) * 4);
+  for (const char* nm : {"perm", "inv", "kept", "dl", "rot_j", "rot_jl", "ent"}) f(nm, N * 4);
+  for (const char* nm : {"Ds", "zs", "rot_c", "rot_s", "mu", "dor", "zh", "nrm"}) f(nm, N * 8);
+  f("cnt", N * 8);
+  f("rhop", N * 8);
+  f("Zw", N * k * 8);
+  f("S", npan * kBtNb * kBtNb * 8);
+  f("negTt", npan * kBtNb * kBtNb * 8);
+  f("Y", kBtNb * k * 8);
+  f("Y2", kBtNb * k * 8);
+  f("info", 256);
+}
+
+size_t syev_ws_bytes(const SyevPlan& p) {
+  size_t tot = 0;
+  syev_pieces(p, [&](const char*, size_t b) { tot += align_up(b, 256); });
+  return tot + 256;
+}
+
+int gemm(int M, int N, int K, const double* A, long lda, const double* B, long ldb, double* C,
+         long ldc, int batch, long sA, long sB, long sC, int acc, hipStream_t st) {
+  return tnh_gemm_tn(TNH_F64, M, N, K, A, lda, B, ldb, C, ldc, batch, sA, sB, sC, acc, 0, st);
+}
+
+int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z, long ldz,
+             void* ws, size_t ws_bytes, int* info, hipStream_t st) {
+  std::vector<char*> ptrs;
+  {
+    char* base = reinterpret_cast<char*>(align_up(reinterpret_cast<size_t>(ws), 256));
+    size_t off = 0;
+    syev_pieces(p, [&](const char*, size_t b) {
+      ptrs.push_back(base + off);
+      off += align_up(b, 256);
+    });
+  }
+  int q = 0;
+  auto next = [&]() { return ptrs[q++]; };
+  char* trd = next();
+  double* Vr = reinterpret_cast<double*>(next());
+  double* Vc = reinterpret_cast<double*>(next());
+  double* d = reinterpret_cast<double*>(next());
+  double* e = reinterpret_cast<double*>(next());
+  double* tau = reinterpret_cast<double*>(next());
+  double* dp = reinterpret_cast<double*>(next());
+  double* ep = reinterpret_cast<double*>(next());
+  double* Qt[2] = {reinterpret_cast<double*>(next()), reinterpret_cast<double*>(next())};
+  double* Mb = reinterpret_cast<double*>(next());
+  double* leafA = reinterpret_cast<double*>(next());
+  double* leafZ = reinterpret_cast<double*>(next());
+  double* wv[2] = {reinterpret_cast<double*>(next()), reinterpret_cast<double*>(next())};
+  int* sweeps = reinterpret_cast<int*>(next());
+  DcLevel Lv;
+  Lv.perm = reinterpret_cast<int*>(next());
+  Lv.inv = reinterpret_cast<int*>(next());
+  Lv.kept = reinterpret_cast<int*>(next());
+  Lv.dl = reinterpret_cast<int*>(next());
+  Lv.rot_j = reinterpret_cast<int*>(next());
+  Lv.rot_jl = reinterpret_cast<int*>(next());
+  Lv.ent = reinterpret_cast<int*>(next());
+  Lv.Ds = reinterpret_cast<double*>(next());
+  Lv.zs = reinterpret_cast<double*>(next());
+  Lv.rot_c = reinterpret_cast<double*>(next());
+  Lv.rot_s = reinterpret_cast<double*>(next());
+  Lv.mu = reinterpret_cast<double*>(next());
+  Lv.dor = reinterpret_cast<double*>(next());
+  Lv.zh = reinterpret_cast<double*>(next());
+  Lv.nrm = reinterpret_cast<double*>(next());
+  Lv.cnt = reinterpret_cast<int*>(next());
+  Lv.rhop = reinterpret_cast<double*>(next());
+  double* Zw = reinterpret_cast<double*>(next());
+  double* S = reinterpret_cast<double*>(next());
+  double* negTt = reinterpret_cast<double*>(next());
+  double* Y = reinterpret_cast<double*>(next());
+  double* Y2 = reinterpret_cast<double*>(next());
+  int* dinfo = reinterpret_cast<int*>(next());
+  (void)dinfo;
+  const int n = p.n, N = p.N, k = p.k, s = p.s;
+
+  // 1. tridiagonalisation
+  Arena tar(trd, trd_ws_bytes(n));
+  int* err = nullptr;
+  int rc = launch_sytrd(n, A, lda, Vr, d, e, tau, tar, &err, st);
+  if (rc != TNH_OK) return rc;
+
+  // 2. divide and conquer
+  hipLaunchKernelGGL(dc_pad_kernel, dim3(1), dim3(1024), 0, st, n, N, d, e, dp, ep);
+  hipLaunchKernelGGL(dc_leaves_kernel, dim3(N / s), dim3(256), 0, st, s, N, dp, ep, leafA);
+  TNH_CHECK_LAUNCH("syev leaves");
+  rc = tnh_eigh_jacobi(TNH_F64, N / s, s, leafA, s, (long)s * s, nullptr, 0, wv[0], s, leafZ, s,
+                       (long)s * s, 60, 2e-16, sweeps, st);
+  if (rc != TNH_OK) return rc;
+  hipLaunchKernelGGL(dc_scatter_leaves_kernel, dim3(N / s), dim3(256), 0, st, s, N, leafZ, Qt[0]);
+  TNH_CHECK_LAUNCH("syev scatter");
+  int cur = 0;
+  Lv.N = N;
+  Lv.e = ep;
+  Lv.Mb = Mb;
+  for (int l = 1; l <= p.L; ++l) {
+    const int K = s << l, nodes = N / K, top = l == p.L;
+    Lv.K = K;
+    Lv.n1 = K / 2;
+    Lv.top = top;
+    Lv.w_in = wv[cur];
+    Lv.w_out = wv[cur ^ 1];
+    Lv.Qt_in = Qt[cur];
+    const size_t psm = (size_t)K * (4 * sizeof(double) + 3 * sizeof(int));
+    hipError_t he = hipFuncSetAttribute(reinterpret_cast<const void*>(&dc_prep_kernel),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)psm);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: prep LDS %zu: %s", psm,
+                hipGetErrorString(he));
+    hipLaunchKernelGGL(dc_prep_kernel, dim3(nodes), dim3(1024), psm, st, Lv);
+    const dim3 wg((K + 3) / 4, nodes);
+    hipLaunchKernelGGL(dc_secular_kernel, wg, dim3(256), 0, st, Lv);
+    hipLaunchKernelGGL(dc_zhat_kernel, wg, dim3(256), 0, st, Lv);
+    hipLaunchKernelGGL(dc_norms_kernel, wg, dim3(256), 0, st, Lv);
+    const int ncol = top ? k : K;
+    hipLaunchKernelGGL(dc_rows_kernel, wg, dim3(256), 0, st, Lv, ncol);
+    hipLaunchKernelGGL(dc_rotate_kernel, dim3((ncol + 255) / 256, nodes), dim3(256), 0, st, Lv,
+                       ncol);
+    TNH_CHECK_LAUNCH("syev merge");
+    const int n1 = K / 2;
+    if (!top) {
+      double* Qo = Qt[cur ^ 1];
+      const long sq = (long)K * (N + 1);
+      rc = gemm(K, n1, n1, Mb, K, Qt[cur], N, Qo, N, nodes, (long)K * K, sq, sq, 0, st);
+      if (rc != TNH_OK) return rc;
+      rc = gemm(K, n1, n1, Mb + (long)n1 * K, K, Qt[cur] + (long)n1 * N + n1, N, Qo + n1, N,
+                nodes, (long)K * K, sq, sq, 0, st);
+      if (rc != TNH_OK) return rc;
+    } else {
+      rc = gemm(n1, k, n1, Qt[cur], N, Mb, k, Zw, k, 1, 0, 0, 0, 0, st);
+      if (rc != TNH_OK) return rc;
+      rc = gemm(n1, k, n1, Qt[cur] + (long)n1 * N + n1, N, Mb + (long)n1 * k, k,
+                Zw + (long)n1 * k, k, 1, 0, 0, 0, 0, st);
+      if (rc != TNH_OK) return rc;
+    }
+    cur ^= 1;
+  }
+  // eigenvalues: the top level's ascending values above the N - n padding entries
+  hipError_t he = hipMemcpyAsync(w, wv[cur] + (N - n), (size_t)n * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st);
+  TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: eigenvalue copy: %s", hipGetErrorString(he));
+
+  // 3. back-transformation of the k vectors (rows 0..n-1 of Zw)
+  const int NV = p.npan * kBtNb;
+  hipLaunchKernelGGL(transpose_kernel, dim3((n + 31) / 32, (NV + 31) / 32), dim3(32, 8), 0, st, n,
+                     NV, Vr, Vc);
+  TNH_CHECK_LAUNCH("syev transpose");
+  const int nrefl = n - 1;
+  rc = gemm(kBtNb, kBtNb, n, Vc, NV, Vc, NV, S, kBtNb, p.npan, kBtNb, kBtNb,
+            (long)kBtNb * kBtNb, 0, st);
+  if (rc != TNH_OK) return rc;
+  hipLaunchKernelGGL(larft_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S, tau, negTt);
+  TNH_CHECK_LAUNCH("syev larft");
+  for (int b = p.npan - 1; b >= 0; --b) {
+    const int jb = b * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
+    rc = gemm(nb, k, rows, Vc + (long)jb * NV + jb, NV, Zw + (long)jb * k, k, Y, k, 1, 0, 0, 0, 0,
+              st);
+    if (rc != TNH_OK) return rc;
+    rc = gemm(nb, k, nb, negTt + (long)b * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, 1, 0, 0, 0, 0, st);
+    if (rc != TNH_OK) return rc;
+    rc = gemm(rows, k, nb, Vr + (long)jb * n + jb, n, Y2, k, Zw + (long)jb * k, k, 1, 0, 0, 0, 1,
+              st);
+    if (rc != TNH_OK) return rc;
+  }
+  he = hipMemcpy2DAsync(Z, (size_t)ldz * sizeof(double), Zw, (size_t)k * sizeof(double),
+                        (size_t)k * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st);
+  TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: vector copy: %s", hipGetErrorString(he));
+  if (info) {
+    hipLaunchKernelGGL(syev_status_kernel, dim3(1), dim3(256), 0, st, err, sweeps, N / s, info);
+    TNH_CHECK_LAUNCH("syev status");
+  }
+  return TNH_OK;
+}
+
+}  // namespace tnh
+
+extern "C" size_t tnh_sytrd_ws_bytes(int n) {
+  return n >= 1 && n <= tnh::kSyevMaxN ? tnh::trd_ws_bytes(n) : 0;
+}
+
+extern "C" int tnh_sytrd(int n, const double* A, long lda, double* d, double* e, double* tau,
+                         double* Vr, int* status, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(n >= 1 && n <= tnh::kSyevMaxN, TNH_ERR_UNSUPPORTED,
+              "tnh_sytrd: n = %d outside 1..%d", n, tnh::kSyevMaxN);
+  TNH_REQUIRE(lda >= n && A && d && Vr && (n == 1 || (e && tau)), TNH_ERR_ARG,
+              "tnh_sytrd: bad arguments (n=%d lda=%ld)", n, lda);
+  TNH_REQUIRE(ws_bytes >= tnh::trd_ws_bytes(n), TNH_ERR_WS,
+              "tnh_sytrd: workspace %zu < %zu bytes", ws_bytes, tnh::trd_ws_bytes(n));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  tnh::Arena ar(ws, ws_bytes);
+  int* err = nullptr;
+  int rc = tnh::launch_sytrd(n, A, lda, Vr, d, e, tau, ar, &err, st);
+  if (rc != TNH_OK) return rc;
+  if (status) {
+    hipError_t he = hipMemcpyAsync(status, err, sizeof(int), hipMemcpyDeviceToDevice, st);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "tnh_sytrd: status copy: %s",
+                hipGetErrorString(he));
+  }
+  return TNH_OK;
+}
+
+
+extern "C" size_t tnh_syev_top_ws_bytes(int n, int k) {
+  if (n < 2 || n > tnh::kSyevMaxN || k < 1 || k > n) return 0;
+  return tnh::syev_ws_bytes(tnh::syev_plan(n, k));
+}
+
+extern "C" int tnh_syev_top(int n, int k, const double* A, long lda, double* w, double* Z,
+                            long ldz, int* info, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(n >= 2 && n <= tnh::kSyevMaxN, TNH_ERR_UNSUPPORTED,
+              "tnh_syev_top: n = %d outside 2..%d", n, tnh::kSyevMaxN);
+  TNH_REQUIRE(k >= 1 && k <= n && lda >= n && ldz >= k && A && w && Z, TNH_ERR_ARG,
+              "tnh_syev_top: bad arguments (n=%d k=%d lda=%ld ldz=%ld)", n, k, lda, ldz);
+  const tnh::SyevPlan p = tnh::syev_plan(n, k);
+  TNH_REQUIRE(ws && ws_bytes >= tnh::syev_ws_bytes(p), TNH_ERR_WS,
+              "tnh_syev_top: workspace %zu < %zu bytes", ws_bytes, tnh::syev_ws_bytes(p));
+  return tnh::syev_top(p, A, lda, w, Z, ldz, ws, ws_bytes, info,
+                       static_cast<hipStream_t>(stream));
+}

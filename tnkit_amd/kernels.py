@@ -283,6 +283,48 @@ def eigh_jacobi(G: torch.Tensor, add: Optional[torch.Tensor] = None, max_sweeps:
     return w, Z, sw
 
 
+SYEV_MAX_N = 2048
+
+
+def syev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Eigendecomposition of one real symmetric (n, n) float64 matrix, 2 <= n <= 2048, on the
+    device by the library's own solver (Householder tridiagonalisation in one cooperative launch,
+    divide and conquer, WY back-transformation): (w (n,) ascending, Z (n, k) whose column c is the
+    eigenvector of the c-th largest eigenvalue). Both triangles of G are read as given, so pass a
+    symmetric matrix. With `check_status` the solver's status word is read back (a host sync) and
+    a failed reduction raises."""
+    _need(G, "G", 2, (torch.float64,))
+    n = int(G.shape[0])
+    if G.shape[1] != n or not 2 <= n <= SYEV_MAX_N:
+        raise ValueError(f"syev_top: square matrices of order 2..{SYEV_MAX_N}, got {tuple(G.shape)}")
+    if not 1 <= int(k) <= n:
+        raise ValueError(f"syev_top: k = {k} outside 1..{n}")
+    lib = _lib.load()
+    w = torch.empty(n, dtype=torch.float64, device=G.device)
+    Z = torch.empty((n, int(k)), dtype=torch.float64, device=G.device)
+    info = torch.zeros(1, dtype=torch.int32, device=G.device)
+    ws = _ws(lib.tnh_syev_top_ws_bytes(n, int(k)), G.device)
+    check(lib.tnh_syev_top(n, int(k), ptr(G), n, ptr(w), ptr(Z), int(k), ptr(info), ptr(ws),
+                           ws.numel(), stream(G.device)), "syev_top")
+    if check_status and int(info.item()) != 0:
+        raise _lib.HipError(f"syev_top: the solver reported status {int(info.item())} "
+                            "(1: a workgroup wait ran out, 2: a leaf did not converge)")
+    return w, Z
+
+
+def eigh_top(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor]:
+    """(w ascending (n,), Z (n, k) for the k largest eigenvalues, descending) of a symmetric
+    float64 (n, n) matrix on the device: the batched Jacobi kernel up to order 64, the library's
+    divide-and-conquer solver up to 2048. Larger orders raise."""
+    n = int(G.shape[0])
+    if n <= EIGH_JACOBI_MAX_N:
+        w, V, sw = eigh_jacobi(G.unsqueeze(0).contiguous())
+        if int(sw[0].item()) < 0:
+            raise _lib.HipError("eigh_top: the Jacobi kernel did not converge")
+        return w[0], torch.flip(V[0][:, n - int(k):], dims=[1])
+    return syev_top(G.contiguous(), int(k))
+
+
 SVD_JACOBI_LDS_MAX = 163840 - 1024
 
 

@@ -1,9 +1,11 @@
 """Truncations on the HIP device: the Gram-matrix truncation (tnkit/linalg.py:138-204) and the
 randomized range-finder SVD (tnkit/linalg.py:81-135).
 
-The Gram matrix is one libtnkit_hip K-major GEMM; the eigendecomposition of the (at most
-2 chi)^2 Gram matrix is torch.linalg.eigh on the device. The range finder's sketch and power
-products are plain dense GEMMs (torch/rocBLAS), its QRs and the small core SVD rocSOLVER.
+The Gram matrix is one libtnkit_hip K-major GEMM; its eigendecomposition (order at most 2 chi)
+is the library's own: the batched Jacobi kernel up to order 64, the divide-and-conquer solver
+(`kernels.syev_top`) up to 2048, which also returns only the kept vectors. The range finder's
+sketch and power products are plain dense GEMMs (torch/rocBLAS), its QRs and the small core SVD
+rocSOLVER.
 """
 from __future__ import annotations
 
@@ -23,10 +25,13 @@ def _gram_left(A: torch.Tensor, chi: int):
     else:
         G = kernels.gemm_tn(A.contiguous(), A.contiguous())   # A^T A
     G = 0.5 * (G + G.t())
-    w, Z = torch.linalg.eigh(G)
     r = min(m, n)
     k = min(int(chi), r)
-    Z = torch.flip(Z[:, r - k:], dims=[1])
+    if r <= kernels.SYEV_MAX_N:
+        w, Z = kernels.eigh_top(G.contiguous(), k)           # Z: the k leading, descending
+    else:
+        w, Z = torch.linalg.eigh(G)
+        Z = torch.flip(Z[:, r - k:], dims=[1])
     wh = torch.flip(w, dims=[0]).cpu().numpy()
     if m <= n:
         return Z, Z.t() @ A, wh, k

@@ -299,11 +299,36 @@ def _eigh_robust_vendor(G: torch.Tensor, add: Optional[torch.Tensor] = None):
     if not bool(bad.any()):
         return w, Z
     idx = torch.nonzero(bad).flatten()
-    U, S, _ = torch.linalg.svd(G[idx])
-    w[idx] = torch.flip(S, dims=[1]).to(w.dtype)
-    Z[idx] = torch.flip(U, dims=[2])
+    w[idx], Z[idx] = _eigh_by_svd(G[idx], None if add is None else add[idx])
     if not (bool(torch.isfinite(Z).all()) and bool(torch.isfinite(w).all())):
         raise FloatingPointError("Hermitian eigendecomposition returned non-finite values")
+    return w, Z
+
+
+def _eigh_by_svd(G: torch.Tensor, add: Optional[torch.Tensor]):
+    """Ascending eigenpairs of a batch (k, r, r) of PSD Gram matrices plus their padding shifts
+    `add` (negative on exactly zero rows, else 0), by one batched SVD: the padded diagonal is
+    lifted to distinct values above the whole spectrum (2 (tr G + 1) + j / r), so the padding's
+    unit vectors come out as the leading singular vectors, apart from the live null directions
+    (a plain SVD of G would mix the two inside its zero singular subspace). Those leading
+    triplets are then given back their negative shift values and everything is sorted
+    ascending, as eigh would return it; for a PSD matrix G = U diag(S) U^H."""
+    k, r, _ = G.shape
+    dev = G.device
+    if add is None:
+        add = torch.zeros((k, r), dtype=torch.float64, device=dev)
+    dead = add != 0
+    tr = torch.diagonal(G, dim1=1, dim2=2).real.sum(1)
+    lift = 2.0 * (tr[:, None] + 1.0) + torch.arange(r, device=dev, dtype=torch.float64)[None, :] / r
+    Gp = G + torch.diag_embed(torch.where(dead, lift, torch.zeros_like(lift)).to(G.dtype))
+    U, S, _ = torch.linalg.svd(Gp)
+    # a leading triplet belongs to the padding: its vector is the unit vector of a dead index
+    col = torch.argmax(U.abs(), dim=1)                                  # (k, r) index per vector
+    is_dead = torch.gather(dead, 1, col)
+    vals = torch.where(is_dead, torch.gather(add.to(torch.float64), 1, col), S.to(torch.float64))
+    order = torch.argsort(vals, dim=1)
+    w = torch.gather(vals, 1, order)
+    Z = torch.gather(U, 2, order[:, None, :].expand(k, r, r))
     return w, Z
 
 
