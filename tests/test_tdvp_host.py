@@ -40,3 +40,51 @@ def test_eigh_by_svd_keeps_padding_out(live, rank):
     assert float(top[dead].abs().max()) <= 1e-12
     ref = np.linalg.eigvalsh(G.numpy()[np.ix_(cols, cols)])
     assert np.abs(np.sort(w.numpy()[-live:]) - ref).max() < 1e-12 * scale
+
+
+def _torch_eigh(G, add=None, deferred=False):
+    Gs = G + (torch.diag_embed(add.to(G.dtype)) if add is not None else 0)
+    w, Z = torch.linalg.eigh(Gs)
+    return (w, Z, None) if deferred else (w, Z)
+
+
+@pytest.mark.parametrize("iso", ["left", "right"])
+@pytest.mark.parametrize("a,b,width", [(12, 10, 16), (12, 30, 16), (30, 10, 16), (30, 30, 50)])
+def test_gram_cut_on_the_live_extent(monkeypatch, iso, a, b, width):
+    """The truncation formed over the indices some leg fills (rows a < max left rank, columns
+    b < max right rank) keeps the same ranks, singular values and product U S Vh as over the
+    whole padded bond, with an exact isometry and zero rows on the padding; the eigensolver is
+    numpy-grade torch eigh here, the batched device solvers are tested on the GPU."""
+    monkeypatch.setattr(T, "_eigh_robust", _torch_eigh)
+    rng = np.random.default_rng(a * 100 + b)
+    k, d = 5, 2
+    ra = np.minimum(np.array([12, 6, 3, 8, 12]), a)
+    rb = np.minimum(np.array([7, 10, 4, 5, 2]), b)
+    mat = np.zeros((k, a * d, d * b), complex)
+    for leg in range(k):
+        shp = (ra[leg] * d, d * rb[leg])
+        X = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        full = np.zeros((a, d, d, b), complex)
+        full[:ra[leg], :, :, :rb[leg]] = X.reshape(ra[leg], d, d, rb[leg])
+        mat[leg] = full.reshape(a * d, d * b)
+    mt = torch.as_tensor(mat)
+    caps = np.array([4, 8, 20, 3, 6])
+    ceil = np.minimum(ra * d, rb * d)
+    rows = torch.arange(int(ra.max()) * d)
+    cols = (torch.arange(d)[:, None] * b + torch.arange(int(rb.max()))[None, :]).reshape(-1)
+    U1, S1, V1, r1, _ = T._gram_cut(mt, caps, ceil, 0.0, iso, width)
+    U2, S2, V2, r2, _ = T._gram_cut(mt, caps, ceil, 0.0, iso, width, (rows, cols))
+    assert U1.shape == U2.shape and V1.shape == V2.shape
+    assert (r1 == r2).all()
+    for leg in range(k):
+        r = int(r1[leg])
+        assert float((S1[leg, :r] - S2[leg, :r]).abs().max()) < 1e-12
+        p1 = (U1[leg, :, :r] * S1[leg, :r]) @ V1[leg, :r]
+        p2 = (U2[leg, :, :r] * S2[leg, :r]) @ V2[leg, :r]
+        assert float((p1 - p2).abs().max()) < 1e-10
+        iso_m = U2[leg, :, :r] if iso == "left" else V2[leg, :r].mH
+        assert float((iso_m.mH @ iso_m - torch.eye(r, dtype=iso_m.dtype)).abs().max()) < 1e-12
+        pad_rows = np.ones(a * d, bool)
+        pad_rows[:ra[leg] * d] = False
+        if iso == "left" and pad_rows.any():            # Householder QR: zero to rounding
+            assert float(U2[leg, torch.as_tensor(pad_rows), :r].abs().max()) <= 1e-13

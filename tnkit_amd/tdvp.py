@@ -333,21 +333,30 @@ def _eigh_by_svd(G: torch.Tensor, add: Optional[torch.Tensor]):
 
 
 def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd_tol: float,
-              iso: str, width: int = 0):
+              iso: str, width: int = 0, live=None):
     """Batched truncation of mat (k, m, n) by the Gram matrix of its smaller side: per-leg rank =
     the fewest values whose relative dropped weight stays under svd_tol, capped by the leg's
     cap and structural ceiling (tnkit/batch_tdvp.py:572-599). Returns U (k, m, nk), S (k, nk),
     Vh (k, nk, n), ranks, relative discarded weight per leg. The `iso` factor is an exact
-    isometry on each leg's kept columns (tnkit/linalg.py:162-204)."""
+    isometry on each leg's kept columns (tnkit/linalg.py:162-204). `live` = (row indices, column
+    indices) of mat that any leg can fill (the union of the legs' unpadded extents), or None:
+    the Gram matrix is formed over those only, so a bond held wider than every leg's rank (the
+    fixed shapes of 'pad') costs the rank, not the width; the other indices are exact zeros of
+    every leg and get zero rows in the factors."""
     k, m, n = mat.shape
     if iso == "right":
+        lv = None if live is None else (live[1], live[0])
         U, S, Vh, ranks, disc = _gram_cut(mat.mH.resolve_conj().contiguous(), ranks_cap, ceiling,
-                                          svd_tol, "left", width)
+                                          svd_tol, "left", width, lv)
         return (Vh.mH.resolve_conj().contiguous(), S, U.mH.resolve_conj().contiguous(), ranks,
                 disc)
     tall = m > n                                        # the smaller Gram side, as tnkit does
-    G = (torch.matmul(mat.mH.resolve_conj().contiguous(), mat) if tall
-         else torch.matmul(mat, mat.mH.resolve_conj().contiguous()))   # (k, r, r)
+    side = None if live is None else live[1 if tall else 0]
+    if side is not None and int(side.numel()) == (n if tall else m):
+        side = None
+    X = mat if side is None else mat.index_select(2 if tall else 1, side)
+    G = (torch.matmul(X.mH.resolve_conj().contiguous(), X) if tall
+         else torch.matmul(X, X.mH.resolve_conj().contiguous()))       # (k, r, r)
     G = 0.5 * (G + G.mH)
     # padded legs leave exact zero rows; a large exactly degenerate zero block makes the batched
     # eigensolver return NaN, so those rows get distinct negative diagonal entries, which keeps
@@ -367,7 +376,7 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
         both = both[:, :-1]
     w = torch.flip(w, dims=[1]).clamp(min=0.0)
     Z = torch.flip(Z, dims=[2])
-    r = min(m, n)
+    r = int(Z.shape[1])
     wh = np.clip(both[:, ::-1], 0.0, None)
     tot = wh.sum(axis=1)
     tot = np.where(tot > 0, tot, 1.0)
@@ -385,9 +394,16 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
             # zero, and a zero-weight test on them would drop directions an SVD keeps.
             ranks[leg] = lim
         disc[leg] = float(drop[ranks[leg]])
-    nk = min(max(int(ranks.max()), int(width)), r)     # stacked width: the ragged mode's
+    nk = min(max(int(ranks.max()), int(width)), min(m, n))   # stacked width: the ragged mode's
     mk = torch.arange(nk, device=mat.device)[None, :] < torch.as_tensor(ranks, device=mat.device)[:, None]
-    Zk = Z[:, :, :nk] * mk[:, None, :]
+    nz = min(nk, r)
+    if side is None and nz == nk:
+        Zk = Z[:, :, :nk] * mk[:, None, :]
+    else:                       # back to the full side, zero rows (and masked columns) elsewhere
+        Zk = torch.zeros((k, n if tall else m, nk), dtype=Z.dtype, device=Z.device)
+        rows = slice(None) if side is None else side
+        Zk[:, rows, :nz] = Z[:, :, :nz]
+        Zk = Zk * mk[:, None, :]
     if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
         # a leg keeping fewer than nk columns has exact zero columns at the end of A Z, which the
         # batched Householder QR turns into NaN; they are filled with fixed Gaussian columns. The
@@ -466,8 +482,17 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
         width = int(min(Bt.caps.max(), a * d, d * b))
     else:
         width = 0
+    # the rows (a, s1) and columns (s2, b) that some leg fills: a below the widest left rank,
+    # b below the widest right rank
+    ra, rb = int(Bt.rank[i].max()), int(Bt.rank[i + 2].max())
+    live = None
+    if ra < a or rb < b:
+        dev = mat.device
+        rows = torch.arange(ra * d, device=dev)
+        cols = (torch.arange(d, device=dev)[:, None] * b + torch.arange(rb, device=dev)[None, :])
+        live = (rows, cols.reshape(-1))
     U, S, Vh, ranks, disc = _gram_cut(mat, Bt.caps, ceiling, svd_tol,
-                                      "left" if moving == "right" else "right", width)
+                                      "left" if moving == "right" else "right", width, live)
     nk = U.shape[2]
     Bt.rank[i + 1] = ranks
     Bt.dims[i + 1] = nk
