@@ -453,6 +453,25 @@ def small_linalg(reps=20):
                                                   reps, stream),
                            "sweeps": int(sw.max()),
                            "rel_err_singular_values": float((S - ref).abs().max() / ref[0])})
+    # the library's eigensolver for the Gram truncation at orders 65..2048 (all n eigenpairs,
+    # float64 Gram matrices with decaying columns) against torch.linalg.eigh (rocSOLVER syevd)
+    out["syev"] = []
+    for n in (512, 1024, 2048):
+        X = torch.randn((n, n), dtype=torch.float64, device=dev, generator=g)
+        X = X * torch.exp(-torch.arange(n, dtype=torch.float64, device=dev) / (n / 20.0))
+        A = (X @ X.T).contiguous()
+        A = 0.5 * (A + A.T)
+        w, Z = kernels.syev_top(A, n)
+        ref = torch.linalg.eigvalsh(A)
+        lam = torch.flip(w, dims=[0])
+        out["syev"].append({
+            "n": n, "k": n, "dtype": "f64",
+            "own_ms": event_time(lambda: kernels.syev_top(A, n, check_status=False), reps, stream),
+            "torch_ms": event_time(lambda: torch.linalg.eigh(A), reps, stream),
+            "rel_err_eigenvalues": float((w - ref).abs().max() / ref.abs().max()),
+            "rel_residual": float((A @ Z - Z * lam[None, :]).abs().max() / ref.abs().max()),
+            "orthogonality": float((Z.T @ Z - torch.eye(n, dtype=torch.float64, device=dev))
+                                   .abs().max())})
     return out
 
 
@@ -639,7 +658,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the chi=1024 sweep-time measurement (Heisenberg N=64, 3 sweeps)")
     ap.add_argument("--small-linalg", action="store_true",
-                    help="only time the Jacobi eigh / thin SVD kernels against torch (rocSOLVER)")
+                    help="only time the Jacobi eigh / thin SVD kernels and the order-512..2048 eigensolver "
+                         "against torch (rocSOLVER)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
