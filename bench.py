@@ -93,6 +93,41 @@ def event_time(fn, reps, stream):
     return s.elapsed_time(e) / reps
 
 
+def cgs2_bench(n, nlive=41, reps=20):
+    """One Lanczos orthogonalisation (`kernels.krylov_orth`: the three-term subtraction, alpha,
+    two classical Gram-Schmidt passes and beta) on a float64 basis of `nlive` rows of length n,
+    HIP events on the launch stream. Algorithmic bytes per call: the pre-pass reads w, V[j] and
+    V[j-1] and writes w (4 n 8 B); each of the three passes reads the nlive rows and w and writes
+    w ((nlive + 2) n 8 B)."""
+    import torch
+    from tnkit_amd import kernels
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    V = torch.randn((nlive, n), dtype=torch.float64, device=dev, generator=g)
+    V, _ = torch.linalg.qr(V.T)
+    V = V.T.contiguous()
+    w0 = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    w = w0.clone()
+    kw = kernels.KrylovWorkspace(n, dev)
+    j = nlive - 1
+
+    def call():
+        w.copy_(w0)
+        kernels.krylov_orth(V, nlive, w, j, [0.25], j - 1, kw)
+    call()
+    resid = float((V @ w).abs().max() / w.norm())
+    stream = torch.cuda.current_stream()
+    ms = event_time(call, reps, stream)
+    ms_copy = event_time(lambda: w.copy_(w0), reps, stream)
+    byts = (4 + 3 * (nlive + 2)) * n * 8
+    t = ms - ms_copy
+    return {"n": n, "nlive": nlive, "ms": t, "bytes": byts, "TBps": byts / (t * 1e-3) / 1e12,
+            "orthogonality_after": resid,
+            "note": "per call, less the timed refresh copy of w; HBM peak 8 TB/s, a flat "
+                    "float64 read measured 6.4 TB/s on this part"}
+
+
 def build_local_problem(N, chi, seed=0):
     """Bulk two-site problem of the XXZ chain from a random mixed-canonical MPS."""
     import numpy as np
@@ -811,6 +846,7 @@ def main():
     if not args.no_local:
         local = local_solves(th, op64, op32, ws, args.chi, op16=op16)
     del op64, op32, op16, ws, out, y32, y64
+    cgs2 = cgs2_bench(th.numel())
 
     setup_s = time.perf_counter() - t_setup
     block = {} if args.no_block else block_bench(max(5, args.steps // 2), args.warmup,
@@ -857,6 +893,7 @@ def main():
         "local_dE": local,
         "reference_h100_mig_fp64_tflops": 5.67,
         "roofline": roof,
+        "cgs2": cgs2,
         "block_u1": block,
         "config2_ramp": ramp,
         "setup_s": setup_s,
