@@ -7,9 +7,13 @@ the kept side only (tnkit/_dmrg.py:700-705) less its part in the span of x. The 
 keep their weights; P competes only for the rest, so the basis gains the directions the
 Hamiltonian couples to, and on block tensors the charge sectors that x lacks.
 
-Dense tensors are torch tensors (the factorisations are rocSOLVER through torch); block tensors
-are `block_ops.BlockTensor`, whose contractions run as grouped GEMM launches of libtnkit_hip and
-whose factorisations are `block_linalg.truncated_svd` with tnkit's zero_tol rule.
+Dense tensors are torch tensors: their contractions and products run on the library's K-major
+GEMM (`contract.einsum`, `linalg._mm`), the thin SVD of the centre on the Jacobi kernel where it
+fits one workgroup (rocSOLVER above), and the density matrix's leading eigenvectors on the
+library's eigensolvers (`linalg._eig_desc`: Jacobi to order 64, divide and conquer for real
+orders to 2048). Block tensors are `block_ops.BlockTensor`, whose contractions run as grouped
+GEMM launches of libtnkit_hip and whose factorisations are `block_linalg.truncated_svd` with
+tnkit's zero_tol rule.
 """
 from __future__ import annotations
 
@@ -18,8 +22,9 @@ from typing import Tuple
 import torch
 
 from . import block_linalg as BL
-from . import kernels
+from . import contract, kernels
 from .block_ops import BlockTensor, einsum as beinsum
+from .linalg import _eig_desc, _kmajor, _mm
 
 
 def _top(mat: torch.Tensor, k: int, rel_tol: float):
@@ -29,6 +34,24 @@ def _top(mat: torch.Tensor, k: int, rel_tol: float):
     s = S.cpu()
     n = min(int(k), max(1, int((s > rel_tol * s[0]).sum())))
     return U[:, :n], S[:n], Vh[:n]
+
+
+def _top_psd(rho: torch.Tensor, k: int, rel_tol: float):
+    """The k leading eigenvectors of a Hermitian positive semidefinite rho whose eigenvalues
+    exceed rel_tol times the largest, at least one: the leading left singular vectors of rho,
+    as `_top` takes them, from its eigendecomposition (eigenvalues by magnitude, as the
+    singular values of a Hermitian matrix are)."""
+    w, V = _eig_desc(0.5 * (rho + rho.mH))
+    a = w.abs()
+    order = torch.argsort(a, descending=True)
+    s = a[order].cpu()
+    n = min(int(k), max(1, int((s > rel_tol * s[0]).sum())))
+    return V[:, order[:n]]
+
+
+def _hmm(X: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    """X^H @ Y on the library's K-major GEMM."""
+    return kernels.gemm_tn(_kmajor(X), Y.contiguous())
 
 
 def _axpy(x: BlockTensor, y: BlockTensor, a: float) -> BlockTensor:
@@ -45,19 +68,20 @@ def _split_dense(x, extra, alpha, chi, right, rel_tol):
     if right:
         U, _, _ = _top(X, min(X.shape), rel_tol)
         E = extra.reshape(a * s1, -1)
-        E = E - U @ (U.t() @ E)
-        rho = X @ X.t() + alpha ** 2 * (E @ E.t())
-        Ur, _, _ = _top(rho, chi, rel_tol ** 2)
+        E = E - _mm(U, _hmm(U, E))
+        Xt, Et = X.t().contiguous(), E.t().contiguous()
+        rho = _hmm(Xt, Xt) + alpha ** 2 * _hmm(Et, Et)             # X X^T + a^2 E E^T
+        Ur = _top_psd(rho, chi, rel_tol ** 2)
         A = Ur.reshape(a, s1, -1)
-        C = (Ur.t() @ X).reshape(-1, s2, b)
+        C = _hmm(Ur, X).reshape(-1, s2, b)
     else:
         _, _, Vh = _top(X, min(X.shape), rel_tol)
         E = extra.reshape(-1, s2 * b)
-        E = E - (E @ Vh.t()) @ Vh
-        rho = X.t() @ X + alpha ** 2 * (E.t() @ E)
-        _, _, Vr = _top(rho, chi, rel_tol ** 2)
+        E = E - _mm(_mm(E, Vh.t()), Vh)
+        rho = _hmm(X, X) + alpha ** 2 * _hmm(E, E)                 # X^T X + a^2 E^T E
+        Vr = _top_psd(rho, chi, rel_tol ** 2).t()
         A = Vr.reshape(-1, s2, b)
-        C = (X @ Vr.t()).reshape(a, s1, -1)
+        C = _mm(X, Vr.t()).reshape(a, s1, -1)
     nx, nc = float(torch.linalg.norm(x)), float(torch.linalg.norm(C))
     return A.contiguous(), C.contiguous(), max(nx * nx - nc * nc, 0.0)
 
@@ -111,7 +135,7 @@ def expansion_term(x, L, W1, W2, R, right: bool):
             return beinsum("wmsi,pwijb->psmjb", W1, beinsum("pwq,qijb->pwijb", L, x))
         return beinsum("wrjt,astdr->aswjd", W2, beinsum("drb,astb->astdr", R, x))
     if right:
-        t = torch.einsum("pwq,qijb->pwijb", L, x)
-        return torch.einsum("wmsi,pwijb->psmjb", W1, t)
-    t = torch.einsum("drb,astb->astdr", R, x)
-    return torch.einsum("wrjt,astdr->aswjd", W2, t)
+        t = contract.einsum("pwq,qijb->pwijb", L, x)
+        return contract.einsum("wmsi,pwijb->psmjb", W1, t)
+    t = contract.einsum("drb,astb->astdr", R, x)
+    return contract.einsum("wrjt,astdr->aswjd", W2, t)
