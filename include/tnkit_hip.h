@@ -301,6 +301,13 @@ int tnh_grouped_gemm(int dtype, const long* desc, long ntiles, const int* tiles,
 int tnh_grouped_small(int dtype, const long* desc, long nchunks, const int* chunks, const void* A,
                       const void* B, void* C, tnh_stream_t stream);
 
+/* The same grouped product when every group has R <= 32 and K <= 32: the two MPO tensors of
+ * the two-site matvec (tnkit/_dmrg.py:166-167) contracted into one operator applied in one
+ * pass, whose charge groups have up to D d^2 rows and columns. Same descriptors and chunk table
+ * as tnh_grouped_small; the group's A is staged in LDS. */
+int tnh_grouped_mid(int dtype, const long* desc, long nchunks, const int* chunks, const void* A,
+                    const void* B, void* C, tnh_stream_t stream);
+
 /* Batched eigendecomposition of small Hermitian (TNH_C128) or real symmetric (TNH_F64) matrices,
  * 1 <= n <= 64: the eigh of tnkit's Gram truncation (tnkit/linalg.py:149, batched over the legs
  * in tnkit/batch_tdvp.py:572-599). Matrix b is A + b*sA with row i at i*lda (both triangles read),

@@ -60,14 +60,32 @@ def test_gather_runs_match_host_expansion(hip):
     assert np.array_equal(got, ref.astype(np.float32).astype(np.float64))
 
 
-@pytest.mark.parametrize("case", CASES)
+def long_operands(dev):
+    """L and theta with sectors of 40-90 states (K > 32 in the L-theta contraction, the plans
+    whose stacks carry K padding for the direct-to-LDS tile configs), random entries."""
+    a = B.Leg((-2, -1, 0, 1, 2), (7, 40, 90, 40, 7))
+    p = B.Leg((-1, 1), (1, 1))
+    wl = B.Leg((-1, 0, 1), (1, 3, 1))
+    g = torch.Generator(device=dev)
+    g.manual_seed(11)
+    out = {}
+    for n, lg in (("theta", (a, p, p, a.flip())), ("L", (a, wl.flip(), a.flip()))):
+        s = B.canonical(lg, 0)
+        out[n] = BO.BlockTensor(s, torch.randn(s.size, dtype=torch.float64, device=dev,
+                                               generator=g))
+    return out
+
+
+@pytest.mark.parametrize("case", CASES + ("long",))
 @pytest.mark.parametrize("cfg", BO.TILE_CONFIGS)
 def test_grouped_gemm_every_tile_config(hip, case, cfg):
     """One contraction (L with theta) in each tile config against per-group torch matmuls."""
     print(f"grouped gemm {case} cfg {cfg}", flush=True)
-    ops = operands(case, hip)
+    ops = long_operands(hip) if case == "long" else operands(case, hip)
     L, th = ops["L"], ops["theta"]
     p = B.plan(L.struct, th.struct, (2,), (0,))
+    if cfg in BO.DMA_CONFIGS and p.kpad != B.K_PAD:
+        pytest.skip("the direct-to-LDS configs serve plans with K > 32 (padded stacks) only")
     for dt, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
         A = BO.gather(p.dev_a, L.data, dt)
         Bm = BO.gather(p.dev_b, th.data, dt)

@@ -76,6 +76,7 @@ SIGNATURES = {
     "tnh_grouped_tile": (c_int, [c_int, c_int, _P, _P, _P]),
     "tnh_grouped_gemm": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, c_int, c_int, _P]),
     "tnh_grouped_small": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, _P]),
+    "tnh_grouped_mid": (c_int, [c_int, _P, c_long, _P, _P, _P, _P, _P]),
     "tnh_svd_jacobi_lds_bytes": (c_size_t, [c_int, c_int, c_int]),
     "tnh_svd_jacobi": (c_int, [c_int, c_long, c_int, c_int, _P, c_long, c_long, _P, c_long, _P,
                                c_long, c_long, _P, c_long, c_long, c_int, c_double, _P, _P]),

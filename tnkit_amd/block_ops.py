@@ -10,9 +10,10 @@ result buffer in tnkit's layout.
 `BlockTwoSite` is the effective Hamiltonian of tnkit's block DMRG (`apply_two_site` through the
 BlockBackend einsum, tnkit/_dmrg.py:162-169, tnkit/block_backend.py:121-131, and the flattening
 of tnkit/_dmrg.py:638-644): the operator tensors L, W1, W2, R are stacked once per local
-problem, every buffer is allocated once, and each apply is five gathers and four grouped GEMMs on
-the current stream, so it can be captured in a hipGraph (`graph()`), which replaces
-tnkit's block_graph.MatvecGraph (tnkit/block_graph.py:89-180).
+problem (W1 and W2 contracted into one operator), every buffer is allocated once, and each apply
+is four gathers and three grouped products on the current stream, so it can be captured in a
+hipGraph (`graph()`), which replaces tnkit's block_graph.MatvecGraph
+(tnkit/block_graph.py:89-180).
 """
 from __future__ import annotations
 
@@ -29,8 +30,13 @@ from ._lib import check, dcode, ptr
 
 _REAL = (torch.float64, torch.float32)
 TWO_SITE_SUBS = ("pwq,qijb->pwijb", "wmsi,pwijb->pmsjb", "mrtj,pmsjb->pstrb", "drb,pstrb->pstd")
-TILE_CONFIGS = (1, 2, 3, 4, 5)
+# the applied form: the two MPO tensors contracted into one (w, s1, s2) -> (s1', s2', w') operator
+MPO_PAIR_SUB = "wmsi,mrtj->wijstr"
+APPLY_SUBS = ("pwq,qijb->pwijb", "wijstr,pwijb->pstrb", "drb,pstrb->pstd")
+TILE_CONFIGS = (1, 2, 3, 4, 5, 11, 12, 13, 15)   # 11..15: direct-to-LDS forms of 1, 2, 3, 5
+DMA_CONFIGS = (11, 12, 13, 15)
 SMALL = 9              # the memory-bound kernel for groups with R, K <= 8 (not a tile shape)
+MID = 10               # the same with R, K <= 32 (the operand staged in LDS)
 _SMALL_COLS = 1024
 
 
@@ -90,23 +96,27 @@ def _device_runs(g: B.Gather, device, src_size: int) -> _DevRuns:
 
 
 def gather(g: B.Gather, src: torch.Tensor, dtype: Optional[torch.dtype] = None,
-           out: Optional[torch.Tensor] = None) -> torch.Tensor:
-    """out[j] = src[position j of g] (zero where g covers nothing), converted to `dtype`."""
+           out: Optional[torch.Tensor] = None, prezeroed: bool = False) -> torch.Tensor:
+    """out[j] = src[position j of g] (zero where g covers nothing), converted to `dtype`.
+    `prezeroed`: out's uncovered positions already hold zeros (a buffer allocated with zeros
+    and only ever written by this gather), so they are not written again."""
     if src.device.type != "cuda" or src.dtype not in _REAL or src.dim() != 1 \
             or not src.is_contiguous():
         raise ValueError(f"gather: need a contiguous 1-D float64/float32 device buffer, got "
                          f"{src.dtype} {tuple(src.shape)} on {src.device}")
     dtype = dtype or src.dtype
-    if out is None:
-        out = torch.empty(g.n, dtype=dtype, device=src.device)
+    d = _device_runs(g, src.device, src.numel())
+    if out is None:   # with tiles the run pass leaves padding alone: start from zeros
+        out = (torch.zeros if d.ntiles else torch.empty)(g.n, dtype=dtype, device=src.device)
     if out.numel() != g.n or out.dtype != dtype or not out.is_contiguous():
         raise ValueError("gather: output buffer does not match")
-    d = _device_runs(g, src.device, src.numel())
     lib, st = _lib.load(), _stream(src.device)
-    # with tiles the run pass leaves uncovered positions alone (tiles and padding); padding is
-    # never read into a stored result by the grouped GEMM
+    # with tiles the run pass leaves uncovered positions alone (tiles and padding): the padding
+    # of a reused buffer keeps what it was allocated with (zeros, see BlockTwoSite), and the
+    # zero K-padding rows are the only padding the grouped GEMM sums into a stored result
     check(lib.tnh_gather_runs(dcode(src.dtype), dcode(dtype), g.n, ptr(src), ptr(out), d.nruns,
-                              ptr(d.runs), ptr(d.chunks), int(d.ntiles == 0), st), "gather_runs")
+                              ptr(d.runs), ptr(d.chunks), int(d.ntiles == 0 and not prezeroed),
+                              st), "gather_runs")
     if d.ntiles:
         check(lib.tnh_gather_tiles(dcode(src.dtype), dcode(dtype), d.ntiles, ptr(src), ptr(out),
                                    ptr(d.tiles), st), "gather_tiles")
@@ -179,14 +189,16 @@ def tile_table(desc: np.ndarray, bm: int, bn: int, panel: int = 8) -> np.ndarray
 class _Grouped:
     """A plan's grouped-GEMM state on one device: descriptors, and per tile config its tile table
     and grid."""
-    __slots__ = ("desc", "desc_np", "tables", "cfg", "size_a", "size_b", "size_out", "macs")
+    __slots__ = ("desc", "desc_np", "tables", "cfg", "size_a", "size_b", "size_out", "macs",
+                 "dma")
 
     def __init__(self, p: B.Plan, device):
         d = p.desc
         for qa, lda, qb, ldb, po, R, K, C in d.tolist():
+            kp = -(-K // p.kpad) * p.kpad
             if not (qa % 4 == 0 and qb % 4 == 0 and lda % 4 == 0 and ldb % 4 == 0
-                    and lda >= R and ldb >= C and qa + K * lda <= p.size_a
-                    and qb + K * ldb <= p.size_b and po + R * C <= p.out.size):
+                    and lda >= R and ldb >= C and qa + kp * lda <= p.size_a
+                    and qb + kp * ldb <= p.size_b and po + R * C <= p.out.size):
                 raise ValueError("grouped GEMM descriptor outside its buffers")
         self.desc_np = d
         self.desc = torch.from_numpy(np.ascontiguousarray(d, np.int64)).to(device)
@@ -194,15 +206,19 @@ class _Grouped:
         self.cfg = {}
         self.size_a, self.size_b, self.size_out = p.size_a, p.size_b, p.out.size
         self.macs = p.macs
+        self.dma = p.kpad == B.K_PAD
 
-    def small_ok(self) -> bool:
+    def small_ok(self, limit: int = 8) -> bool:
         d = self.desc_np
-        return len(d) > 0 and int(d[:, 5].max()) <= 8 and int(d[:, 6].max()) <= 8
+        return len(d) > 0 and int(d[:, 5].max()) <= limit and int(d[:, 6].max()) <= limit
+
+    def mid_ok(self) -> bool:
+        return self.small_ok(32)
 
     def table(self, dtype, cfg, device):
         key = (dtype, cfg)
         t = self.tables.get(key)
-        if t is None and cfg == SMALL:
+        if t is None and cfg in (SMALL, MID):
             C = self.desc_np[:, 7]
             nch = -(-C // _SMALL_COLS)
             g = np.repeat(np.arange(len(C)), nch)
@@ -242,11 +258,21 @@ def grouped_gemm(p: B.Plan, A: torch.Tensor, Bm: torch.Tensor, out: torch.Tensor
         cfg = autotune(p, A, Bm, out)
     if cfg == SMALL and not gs.small_ok():
         raise ValueError("grouped_gemm: the small kernel needs R, K <= 8 in every group")
+    if cfg == MID and not gs.mid_ok():
+        raise ValueError("grouped_gemm: the mid kernel needs R, K <= 32 in every group")
+    if cfg in DMA_CONFIGS and not gs.dma:
+        raise ValueError("grouped_gemm: the direct-to-LDS configs need a plan with K padding "
+                         "(some group with K > 32)")
     tiles, ntiles, grid = gs.table(A.dtype, cfg, dev)
     if cfg == SMALL:
         check(_lib.load().tnh_grouped_small(dcode(A.dtype), ptr(gs.desc), ntiles, ptr(tiles),
                                             ptr(A), ptr(Bm), ptr(out), _stream(dev)),
               "grouped_small")
+        return out
+    if cfg == MID:
+        check(_lib.load().tnh_grouped_mid(dcode(A.dtype), ptr(gs.desc), ntiles, ptr(tiles),
+                                          ptr(A), ptr(Bm), ptr(out), _stream(dev)),
+              "grouped_mid")
         return out
     check(_lib.load().tnh_grouped_gemm(dcode(A.dtype), ptr(gs.desc), ntiles, ptr(tiles), ptr(A),
                                        ptr(Bm), ptr(out), cfg, grid, _stream(dev)),
@@ -266,13 +292,16 @@ def _signature(p: B.Plan, dtype) -> tuple:
 
 
 def heuristic_cfg(p: B.Plan, dtype) -> int:
-    """The tile config without timing: the skinny 16-row tile when every group has at most 16
-    rows, else 64x64 for float64 and 128x128 for float32 when there are enough tiles to fill the
-    chip, else 32x32."""
+    """The tile config without timing: the memory-bound kernels when every group has R, K <= 8
+    (SMALL) or <= 32 (MID), the skinny 16-row tile when every group has at most 16 rows, else
+    64x64 for float64 and 128x128 for float32 when there are enough tiles to fill the chip, else
+    32x32."""
     d = p.desc
     R, C = d[:, 5], d[:, 7]
     if R.max(initial=0) <= 8 and d[:, 6].max(initial=0) <= 8:
         return SMALL
+    if R.max(initial=0) <= 32 and d[:, 6].max(initial=0) <= 32:
+        return MID
     if R.max(initial=0) <= 16:
         return 4
     t64 = int(((-(-R // 64)) * (-(-C // 64))).sum())
@@ -296,9 +325,12 @@ def autotune(p: B.Plan, A, Bm, out, reps: int = 3) -> int:
         _TUNED[sig] = cfg
         return cfg
     R = p.desc[:, 5]
-    cands = [c for c in TILE_CONFIGS if not (c == 4 and R.max(initial=0) > 64)]
+    cands = [c for c in TILE_CONFIGS if not (c == 4 and R.max(initial=0) > 64)
+             and (c not in DMA_CONFIGS or _grouped(p, A.device).dma)]
     if _grouped(p, A.device).small_ok():
         cands.append(SMALL)
+    if _grouped(p, A.device).mid_ok():
+        cands.append(MID)
     if torch.cuda.is_current_stream_capturing():
         cfg = heuristic_cfg(p, A.dtype)
         return cfg
@@ -421,7 +453,12 @@ class BlockTwoSite:
 
     `dtype` is the arithmetic type (float64, or float32 for the FP32 matrix-core operator); the
     input and output buffers keep their own dtype (float64 Krylov vectors), converted inside the
-    first and last gathers. `flops` = 2 * sum of the four plans' macs."""
+    first and last gathers. The operator is tnkit's four contractions (`TWO_SITE_SUBS`); it is
+    applied as three (`APPLY_SUBS`): W1 and W2 are contracted once, at construction, into one
+    six-leg operator (w, s1, s2) -> (s1', s2', w'), so the memory-bound middle of the matvec is
+    one pass over the w-wide intermediate instead of two passes and a gather between them.
+    `flops` = 2 * sum of the macs of tnkit's four contractions (the count the TF/s figures use);
+    `stages` are the three applied ones."""
 
     def __init__(self, L: BlockTensor, W1: BlockTensor, W2: BlockTensor, R: BlockTensor,
                  theta_legs=None, charge: int = 0, dtype: Optional[torch.dtype] = None,
@@ -437,16 +474,24 @@ class BlockTwoSite:
                           R.legs[2].flip())
         self.in_struct = B.canonical(tuple(theta_legs), int(charge))
         x = self.in_struct
-        self.stages = []
         self.macs = []
         for sub, T in zip(TWO_SITE_SUBS, (L, W1, W2, R)):
             ax_a, ax_b, perm = B.parse_einsum(sub)
             p = B.plan(T.struct, x, ax_a, ax_b)
-            A = gather(p.dev_a, T.data, self.dtype)
-            Bbuf = torch.empty(p.size_b, dtype=self.dtype, device=dev)
+            self.macs.append(p.macs)
+            x = B.permuted(p.out, perm)
+        W12 = einsum(MPO_PAIR_SUB, W1, W2)
+        x = self.in_struct
+        self.stages = []
+        for sub, T in zip(APPLY_SUBS, (L, W12, R)):
+            ax_a, ax_b, perm = B.parse_einsum(sub)
+            p = B.plan(T.struct, x, ax_a, ax_b)
+            # zeros: the K-padding rows of the stacks are never written by a gather
+            A = gather(p.dev_a, T.data, self.dtype,
+                       torch.zeros(p.size_a, dtype=self.dtype, device=dev))
+            Bbuf = torch.zeros(p.size_b, dtype=self.dtype, device=dev)
             out = torch.empty(p.out.size, dtype=self.dtype, device=dev)
             self.stages.append([p, A, Bbuf, out, None])
-            self.macs.append(p.macs)
             x = B.permuted(p.out, perm)
         self.out_struct = B.canonical(x.legs, x.charge)
         if self.out_struct is not self.in_struct:
@@ -470,7 +515,7 @@ class BlockTwoSite:
         src = theta.reshape(-1)
         for st in self.stages:
             p, A, Bbuf, o, cfg = st
-            gather(p.dev_b, src, self.dtype, Bbuf)
+            gather(p.dev_b, src, self.dtype, Bbuf, prezeroed=True)
             if cfg is None:
                 cfg = st[4] = autotune(p, A, Bbuf, o)
             grouped_gemm(p, A, Bbuf, o, cfg)

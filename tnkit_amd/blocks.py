@@ -465,6 +465,11 @@ def _pad(n, q: int = 4):
     return (np.asarray(n, np.int64) + q - 1) // q * q
 
 
+# K rows of the device stacks of long-K plans are padded with zero rows to a multiple of this
+# (the grouped GEMM's direct-to-LDS form reads whole slabs of up to 32 rows)
+K_PAD = 32
+
+
 class Plan:
     """A contraction of two layouts (tnkit/blocks.py:1073-1100).
 
@@ -475,13 +480,15 @@ class Plan:
              GEMM results, stacked at po with row length C.
     ref_a, ref_b : tnkit's gathers into (R, K) and (K, C) matrices (parity checks).
     dev_a  : operand a stacked K-major, (K, R) per group with leading dimension lda (a multiple
-             of 4) at offset qa; padding zero.
+             of 4) at offset qa, followed by zero rows up to a multiple of kpad rows.
     dev_b  : operand b stacked (K, C) per group with leading dimension ldb (a multiple of 4) at
-             offset qb; padding zero.
+             offset qb, followed by zero rows up to a multiple of kpad rows.
+    kpad   : K_PAD when some group has more than K_PAD rows of K (the plans the tiled kernels
+             serve), else 1 (short-K plans, served by the memory-bound kernels).
     desc   : (G, 8) int64 rows (qa, lda, qb, ldb, po, R, K, C) for the grouped GEMM.
     """
     __slots__ = ("gemms", "macs", "nmm", "out", "dev_a", "dev_b", "desc", "size_a", "size_b",
-                 "_cache", "_ref")
+                 "kpad", "_cache", "_ref")
 
     @property
     def ref_a(self) -> "Gather":
@@ -503,7 +510,13 @@ class Plan:
         return g
 
 
-def _stack(sp: SymPlan, s: Struct, axes, side: int, ld_pad: bool):
+def _k_totals(sp: SymPlan, s: Struct, axes) -> np.ndarray:
+    """Per group, the number of contracted (K) rows."""
+    return _group_offsets(_item_shapes(sp.k_sec, axes, s.legs).prod(1), sp.k_grp, sp.k_beg,
+                          sp.k_end)[1]
+
+
+def _stack(sp: SymPlan, s: Struct, axes, side: int, ld_pad: bool, kpad: int = 1):
     """Runs stacking operand `side` (0: a, 1: b) of `sp`. For side 0 with ld_pad=False: tnkit's
     (R, K) matrices; for side 0 with ld_pad: K-major (K, R) matrices with padded leading
     dimension; for side 1: (K, C) matrices, padded when ld_pad. Returns (runs, total size,
@@ -531,7 +544,9 @@ def _stack(sp: SymPlan, s: Struct, axes, side: int, ld_pad: bool):
         dst0 = base[g] + o[oi] * ld[g] + ko[ki]
     else:                                     # (K, outer): contracted legs first
         ld = _pad(n) if ld_pad else n
-        gsz = K * ld
+        # device stacks of plans with long K carry zero rows up to a multiple of kpad, so a tile
+        # loop reads whole K slabs
+        gsz = (_pad(K, kpad) if ld_pad else K) * ld
         base = _excl(_pad(gsz)) if ld_pad else _excl(gsz)
         perm = list(tuple(axes) + free)
         nrow = len(axes)
@@ -563,8 +578,10 @@ def plan(sa: Struct, sb: Struct, ax_a: Tuple[int, ...], ax_b: Tuple[int, ...]) -
                              "match and the signs be opposite")
     sp = _sym_plan(sa.labels, sb.labels, tuple(ax_a), tuple(ax_b))
     p = Plan()
-    da, p.size_a, qa, lda, (r_shp, ro, R), K = _stack(sp, sa, ax_a, 0, True)
-    db, p.size_b, qb, ldb, (c_shp, co, C), _ = _stack(sp, sb, ax_b, 1, True)
+    Kg = _k_totals(sp, sa, ax_a)
+    p.kpad = K_PAD if len(Kg) and int(Kg.max()) > K_PAD else 1
+    da, p.size_a, qa, lda, (r_shp, ro, R), K = _stack(sp, sa, ax_a, 0, True, p.kpad)
+    db, p.size_b, qb, ldb, (c_shp, co, C), _ = _stack(sp, sb, ax_b, 1, True, p.kpad)
     p.dev_a = Gather(da[0], p.size_a, sa.size, da[1])
     p.dev_b = Gather(db[0], p.size_b, sb.size, db[1])
     pa, pb, po = _excl(R * K), _excl(K * C), _excl(R * C)

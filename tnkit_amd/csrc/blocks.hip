@@ -217,6 +217,47 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// The same product for groups with R <= 32 and K <= 32 (the two MPO tensors of the two-site
+// matvec contracted into one six-leg operator, whose (w, s1, s2) x (s1', s2', w') charge groups
+// have up to D d^2 rows and columns): the group's A (K x R) is staged in LDS and read as a
+// broadcast, each thread keeps one column of B (K values) in registers and writes its R outputs;
+// B rows are read and C rows written contiguously across the wavefront.
+constexpr int kMid = 32;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    grouped_mid_kernel(const long* __restrict__ desc, const int* __restrict__ chunks,
+                       const T* __restrict__ A, const T* __restrict__ B, T* __restrict__ C) {
+  __shared__ T a[kMid][kMid];
+  const int g = chunks[2 * blockIdx.x];
+  const int c0 = chunks[2 * blockIdx.x + 1];
+  const long* d = desc + 8 * static_cast<long>(g);
+  const T* Ag = A + d[0];
+  const long lda = d[1];
+  const T* Bg = B + d[2];
+  const long ldb = d[3];
+  T* Cg = C + d[4];
+  const int R = static_cast<int>(d[5]), K = static_cast<int>(d[6]), N = static_cast<int>(d[7]);
+  for (int e = threadIdx.x; e < kMid * kMid; e += 256) {
+    const int k = e / kMid, r = e % kMid;
+    a[k][r] = (k < K && r < R) ? Ag[k * lda + r] : T(0);
+  }
+  __syncthreads();
+  for (int q = 0; q < kSmallCols / 256; ++q) {
+    const int c = c0 + q * 256 + threadIdx.x;
+    if (c >= N || c >= c0 + kSmallCols) break;
+    T b[kMid];
+#pragma unroll
+    for (int k = 0; k < kMid; ++k) b[k] = k < K ? Bg[k * ldb + c] : T(0);
+    for (int r = 0; r < R; ++r) {
+      T acc = T(0);
+#pragma unroll
+      for (int k = 0; k < kMid; ++k) acc += a[k][r] * b[k];
+      Cg[static_cast<long>(r) * N + c] = acc;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------------------------
 // grouped GEMM
 // ---------------------------------------------------------------------------------------------
@@ -333,6 +374,136 @@ __global__ void __launch_bounds__(WM* WN * 64) grouped_gemm_kernel(GroupArgs ga)
   }
 }
 
+// The grouped GEMM with direct-to-LDS loads: the ring of gemm_dma_kernel (tn's three LDS stages,
+// tiles moved by global_load_lds_dwordx4 with no VGPR staging, rows rotated by 16 elements per
+// (k row mod 4) for distinct banks) on the persistent tile loop above. Requirements the host
+// plans meet: every group's K-major stack has its K rows padded with zero rows to a multiple of
+// 32 (so whole BK slabs are read), lda / ldb are multiples of 4 elements and the stacks 16-byte
+// aligned. A chunk whose columns start past the padded row (a tile hanging over R or C) is
+// loaded from the row's last chunk instead: it only feeds output rows or columns past R / C,
+// which are not stored, and every load stays inside the group's rows.
+template <typename T, int BM, int BN, int BK>
+struct GroupDmaStages {
+  T a0[BK * BM], a1[BK * BM], a2[BK * BM];
+  T b0[BK * BN], b1[BK * BN], b2[BK * BN];
+};
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(WM* WN * 64) grouped_gemm_dma_kernel(GroupArgs ga) {
+  using MF = Mfma<T>;
+  using acc_t = typename MF::acc_t;
+  constexpr int NT = WM * WN * 64;
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int EPC = 16 / sizeof(T);             // elements per 16-byte chunk
+  constexpr int CPA = BM / EPC, CPB = BN / EPC;   // chunks per k row of each tile
+  constexpr int NJA = BK * CPA / NT, NJB = BK * CPB / NT;
+  constexpr int ROT = 16 / EPC;                   // the 16-element rotation, in chunks
+  static_assert(NJA >= 1 && NJB >= 1 && NJA * NT == BK * CPA && NJB * NT == BK * CPB &&
+                    BK % 4 == 0 && FM >= 1 && FN >= 1,
+                "grouped dma tile shape");
+  __shared__ __align__(16) GroupDmaStages<T, BM, BN, BK> sm;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave / WN, wn = wave % WN, row16 = lane & 15, kq = lane >> 4;
+  const int aoff = wm * (BM / WM) + row16 + 16 * kq, boff = wn * (BN / WN) + row16 + 16 * kq;
+
+  for (long pos = blockIdx.x; pos < ga.ntiles; pos += gridDim.x) {
+    const int g = ga.tiles[3 * pos];
+    if (g < 0) continue;
+    const int m0 = ga.tiles[3 * pos + 1] * BM;
+    const int n0 = ga.tiles[3 * pos + 2] * BN;
+    const long* d = ga.desc + 8 * static_cast<long>(g);
+    const T* __restrict__ A = static_cast<const T*>(ga.A) + d[0];
+    const long lda = d[1];
+    const T* __restrict__ B = static_cast<const T*>(ga.B) + d[2];
+    const long ldb = d[3];
+    T* __restrict__ C = static_cast<T*>(ga.C) + d[4];
+    const int M = static_cast<int>(d[5]);
+    const int K = static_cast<int>(d[6]);
+    const int N = static_cast<int>(d[7]);
+    long goffA[NJA], goffB[NJB];
+#pragma unroll
+    for (int j = 0; j < NJA; ++j) {
+      const int c = j * NT + t, r = c / CPA, p = c % CPA;
+      const int q = ((p - ROT * (r & 3)) % CPA + CPA) % CPA;
+      const long col = min(static_cast<long>(m0 + EPC * q), lda - EPC);
+      goffA[j] = static_cast<long>(r) * lda + col;
+    }
+#pragma unroll
+    for (int j = 0; j < NJB; ++j) {
+      const int c = j * NT + t, r = c / CPB, p = c % CPB;
+      const int q = ((p - ROT * (r & 3)) % CPB + CPB) % CPB;
+      const long col = min(static_cast<long>(n0 + EPC * q), ldb - EPC);
+      goffB[j] = static_cast<long>(r) * ldb + col;
+    }
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+    const int nk = (K + BK - 1) / BK;
+    auto issue = [&](int kt, T* as, T* bs) __attribute__((always_inline)) {
+      const T* a = A + static_cast<long>(kt) * BK * lda;
+      const T* b = B + static_cast<long>(kt) * BK * ldb;
+#pragma unroll
+      for (int j = 0; j < NJA; ++j)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
+                                         (lds_void_t)(as + (j * NT + wave * 64) * EPC), 16, 0, 0);
+#pragma unroll
+      for (int j = 0; j < NJB; ++j)
+        __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
+                                         (lds_void_t)(bs + (j * NT + wave * 64) * EPC), 16, 0, 0);
+    };
+    auto compute = [&](const T* as, const T* bs) __attribute__((always_inline)) {
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        T af[FM], bf[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * BM + ((aoff + i * 16) & (BM - 1))];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + kq) * BN + ((boff + j * 16) & (BN - 1))];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) acc[i][j] = MF::run(af[i], bf[j], acc[i][j]);
+      }
+    };
+    auto step = [&](auto stage, int kt) __attribute__((always_inline)) {
+      constexpr int S = decltype(stage)::value;
+      if (kt + 1 < nk)
+        wait_vmcnt<NJA + NJB>();
+      else
+        wait_vmcnt<0>();
+      __builtin_amdgcn_s_barrier();
+      if (kt + 2 < nk) issue(kt + 2, dma_stage_a<(S + 2) % 3>(sm), dma_stage_b<(S + 2) % 3>(sm));
+      compute(dma_stage_a<S>(sm), dma_stage_b<S>(sm));
+    };
+    // the previous tile's stores and every wave's reads of the stages are done before the
+    // stages are refilled
+    wait_vmcnt<0>();
+    __syncthreads();
+    if (nk > 0) issue(0, sm.a0, sm.b0);
+    if (nk > 1) issue(1, sm.a1, sm.b1);
+    for (int kt = 0; kt < nk; kt += 3) {
+      step(std::integral_constant<int, 0>{}, kt);
+      if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
+      if (kt + 2 < nk) step(std::integral_constant<int, 2>{}, kt + 2);
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int n = n0 + wn * (BN / WN) + j * 16 + row16;
+        if (n >= N) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + wm * (BM / WM) + i * 16 + MF::row(kq, r);
+          if (m < M) C[static_cast<long>(m) * N + n] = acc[i][j][r];
+        }
+      }
+    }
+  }
+}
+
 struct TileShape {
   int bm, bn, bk, wm, wn;
 };
@@ -346,6 +517,10 @@ TileShape tile_shape(int cfg) {
     case 3: return {32, 32, 16, 2, 2};
     case 4: return {16, 128, 16, 1, 4};
     case 5: return {64, 128, 16, 2, 2};
+    case 11: return {128, 128, dbl ? 16 : 32, 2, 2};
+    case 12: return {64, 64, dbl ? 16 : 32, 2, 2};
+    case 13: return {32, 32, dbl ? 16 : 32, 2, 2};
+    case 15: return {64, 128, dbl ? 16 : 32, 2, 2};
     default: return {0, 0, 0, 0, 0};
   }
 }
@@ -371,11 +546,36 @@ int launch_grouped_cfg(const GroupArgs& ga, int grid, hipStream_t stream, int* m
   return TNH_OK;
 }
 
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+int launch_grouped_dma_cfg(const GroupArgs& ga, int grid, hipStream_t stream, int* max_grid) {
+  auto kern = grouped_gemm_dma_kernel<T, BM, BN, BK, WM, WN>;
+  if (max_grid) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, WM * WN * 64, 0) !=
+            hipSuccess) {
+      set_error("grouped_gemm: device query failed");
+      return TNH_ERR_HIP;
+    }
+    *max_grid = (cus * (per_cu > 0 ? per_cu : 1) + 7) / 8 * 8;
+    return TNH_OK;
+  }
+  if (ga.ntiles == 0) return TNH_OK;
+  hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(grid)), dim3(WM * WN * 64), 0, stream, ga);
+  TNH_CHECK_LAUNCH("grouped_gemm_dma");
+  return TNH_OK;
+}
+
 template <typename T>
 int launch_grouped(int cfg, const GroupArgs& ga, int grid, hipStream_t stream, int* max_grid) {
   constexpr bool dbl = sizeof(T) == 8;
   constexpr int BKb = dbl ? 16 : 32;
   switch (cfg) {
+    case 11: return launch_grouped_dma_cfg<T, 128, 128, BKb, 2, 2>(ga, grid, stream, max_grid);
+    case 12: return launch_grouped_dma_cfg<T, 64, 64, BKb, 2, 2>(ga, grid, stream, max_grid);
+    case 13: return launch_grouped_dma_cfg<T, 32, 32, BKb, 2, 2>(ga, grid, stream, max_grid);
+    case 15: return launch_grouped_dma_cfg<T, 64, 128, BKb, 2, 2>(ga, grid, stream, max_grid);
     case 1: return launch_grouped_cfg<T, 128, 128, BKb, 2, 2>(ga, grid, stream, max_grid);
     case 2: return launch_grouped_cfg<T, 64, 64, 16, 2, 2>(ga, grid, stream, max_grid);
     case 3: return launch_grouped_cfg<T, 32, 32, 16, 2, 2>(ga, grid, stream, max_grid);
@@ -450,6 +650,30 @@ extern "C" int tnh_grouped_small(int dtype, const long* desc, long nchunks, cons
     return TNH_ERR_UNSUPPORTED;
   }
   TNH_CHECK_LAUNCH("grouped_small");
+  return TNH_OK;
+}
+
+extern "C" int tnh_grouped_mid(int dtype, const long* desc, long nchunks, const int* chunks,
+                               const void* A, const void* B, void* C, tnh_stream_t stream) {
+  TNH_REQUIRE(nchunks >= 0 && nchunks < (1L << 31), TNH_ERR_ARG,
+              "tnh_grouped_mid: bad chunk count");
+  if (nchunks == 0) return TNH_OK;
+  TNH_REQUIRE(desc && chunks && A && B && C, TNH_ERR_ARG, "tnh_grouped_mid: null pointer");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(static_cast<unsigned>(nchunks));
+  if (dtype == TNH_F64) {
+    hipLaunchKernelGGL(grouped_mid_kernel<double>, grid, dim3(256), 0, s, desc, chunks,
+                       static_cast<const double*>(A), static_cast<const double*>(B),
+                       static_cast<double*>(C));
+  } else if (dtype == TNH_F32) {
+    hipLaunchKernelGGL(grouped_mid_kernel<float>, grid, dim3(256), 0, s, desc, chunks,
+                       static_cast<const float*>(A), static_cast<const float*>(B),
+                       static_cast<float*>(C));
+  } else {
+    set_error("tnh_grouped_mid: unsupported dtype %d", dtype);
+    return TNH_ERR_UNSUPPORTED;
+  }
+  TNH_CHECK_LAUNCH("grouped_mid");
   return TNH_OK;
 }
 
