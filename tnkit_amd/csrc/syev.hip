@@ -130,6 +130,40 @@ __device__ __forceinline__ double wsum(double v) {
   return (lane_d(v, 0) + lane_d(v, 16)) + (lane_d(v, 32) + lane_d(v, 48));
 }
 
+// Wave sums of eight per-lane values s[0..7] at once (the eight own rows of a workgroup's pass):
+// two halving exchanges (xor 1, then xor 2) leave each lane two of the eight rows, summed over
+// its lane pair and quad, then row rotations by 4 and 8 and exchanges across the four 16-lane
+// rows finish the sums. Lanes 0..3 hold rows 4 b0 + 2 b1 + {0, 1} (b0, b1 the lane's low bits)
+// and write them to dst[0..7]. About a quarter of the instructions of eight wsum calls.
+__device__ __forceinline__ void wsum8_store(const double (&s)[8], double* dst, int lane) {
+  const bool b0 = lane & 1, b1 = (lane >> 1) & 1;
+  double t[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = b0 ? s[i + 4] : s[i];
+    const double send = b0 ? s[i] : s[i + 4];
+    t[i] = keep + dpp_d<0xB1>(send);      // quad_perm [1, 0, 3, 2]: lane ^ 1
+  }
+  double u[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = b1 ? t[i + 2] : t[i];
+    const double send = b1 ? t[i] : t[i + 2];
+    u[i] = keep + dpp_d<0x4E>(send);      // quad_perm [2, 3, 0, 1]: lane ^ 2
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    u[i] += dpp_d<0x124>(u[i]);           // row_ror:4
+    u[i] += dpp_d<0x128>(u[i]);           // row_ror:8
+    u[i] += __shfl_xor(u[i], 16, 64);
+    u[i] += __shfl_xor(u[i], 32, 64);
+  }
+  if (lane < 4) {
+    dst[4 * b0 + 2 * b1] = u[0];
+    dst[4 * b0 + 2 * b1 + 1] = u[1];
+  }
+}
+
 struct TrdArgs {
   int n;
   int ns;         // slot length of P and Rp (n rounded up, plus a gap)
@@ -218,7 +252,14 @@ __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
     if (j > 0) {
       const double* Pj = a.P + (long)(j - 1) * a.ns;
       const double* pj = a.part + (long)(j - 1) * a.gs;
-      const double ps = t < G ? pj[t] : 0.0;
+      // every wave sums the G partials itself, in one fixed order (gamma is then the same in
+      // every wave and workgroup without an LDS round trip)
+      double ps = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBarMaxG / 64; ++q) {
+        const int k = lane + 64 * q;
+        ps += k < G ? pj[k] : 0.0;
+      }
       double Pl[kTrdCols];
 #pragma unroll
       for (int c = 0; c < kTrdCols; ++c) {
@@ -227,7 +268,7 @@ __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
         R[c] = l >= j && l < n ? Rj[l] : 0.0;
       }
       const double pjj = Pj[j], pj1 = Pj[j + 1], rjj = Rj[j], rj1 = Rj[j + 1];
-      gamma = 0.5 * tau_prev * trd_sum1(ps, red, par);
+      gamma = 0.5 * tau_prev * wsum(ps);
       const double wj = pjj - gamma, vx = vnext;
 #pragma unroll
       for (int c = 0; c < kTrdCols; ++c) {
@@ -289,28 +330,30 @@ __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
       }
       break;
     }
-    __syncthreads();   // own_vn and vnext written
+    // (own_vn and vnext, written just above, are read only after the next barrier of this
+    // workgroup and in the next column; the pass reads own_v and own_p, written in the last one)
 
     // one pass over the own rows: update by (v_{j-1}, w_{j-1}) on columns >= j, p_j on rows > j,
     // row j+1 published by its owner
     const int rpub = (j + 1 < n - 1 && g == (j + 1) % G) ? (j + 1 - g) / G : -1;
     double (*rb)[kTrdRows] = red8[par8];
     par8 ^= 1;
+    double sacc[kTrdRows];
 #pragma unroll
     for (int r = 0; r < kTrdRows; ++r) {
       const double vi = own_v[r], wi = own_p[r] - gamma * own_v[r];
       const bool live = r < nr && g + r * G >= j;
-      double sacc = 0.0;
+      double acc = 0.0;
 #pragma unroll
       for (int c = 0; c < kTrdCols; ++c) {
         const int l = t + kTrdThreads * c;
         if (live && l >= j && l < n) x[r][c] -= vi * wr[c] + wi * vr[c];
-        sacc += x[r][c] * vn[c];
+        acc += x[r][c] * vn[c];
         if (r == rpub && l > j && l < n) st_shared(a.Rp + (long)(j + 1) * a.ns + l, x[r][c]);
       }
-      sacc = wsum((r < nr && g + r * G > j) ? sacc : 0.0);
-      if (lane == 0) rb[wv][r] = sacc;
+      sacc[r] = (r < nr && g + r * G > j) ? acc : 0.0;
     }
+    wsum8_store(sacc, rb[wv], lane);
     __syncthreads();
     double partial = 0.0;
     if (t < nr) {
