@@ -104,6 +104,38 @@ def test_grouped_gemm_every_tile_config(hip, case, cfg):
 
 
 @pytest.mark.parametrize("case", CASES)
+def test_short_k_kernels_on_the_mpo_operator(hip, case):
+    """The memory-bound grouped kernels on the two-site operator's middle contraction (W1 and W2
+    contracted into one operator, applied to L x theta): the R, K <= 8 kernel where it applies
+    and the R, K <= 32 kernel always, against per-group torch matmuls of tnkit's stacking."""
+    print(f"short-K kernels {case}", flush=True)
+    ops = operands(case, hip)
+    X1 = BO.einsum(BO.APPLY_SUBS[0], ops["L"], ops["theta"])
+    W12 = BO.einsum(BO.MPO_PAIR_SUB, ops["W"], ops["W"])
+    ax_a, ax_b, _ = B.parse_einsum(BO.APPLY_SUBS[1])
+    p = B.plan(W12.struct, X1.struct, ax_a, ax_b)
+    gs = BO._grouped(p, hip)
+    assert gs.mid_ok() and p.kpad == 1
+    cfgs = [BO.MID] + ([BO.SMALL] if gs.small_ok() else [])
+    ia = torch.from_numpy(p.ref_a.expand()).to(hip)
+    ib = torch.from_numpy(p.ref_b.expand()).to(hip)
+    for dt, tol in ((torch.float64, 1e-13), (torch.float32, 2e-6)):
+        A = BO.gather(p.dev_a, W12.data, dt)
+        Bm = BO.gather(p.dev_b, X1.data, dt)
+        Ar, Br = W12.data.double()[ia], X1.data.double()[ib]
+        for cfg in cfgs:
+            out = torch.full((p.out.size,), float("nan"), dtype=dt, device=hip)
+            BO.grouped_gemm(p, A, Bm, out, cfg=cfg)
+            for pa, pb, po, R, K, C in p.gemms.tolist():
+                want = (Ar[pa:pa + R * K].reshape(R, K) @ Br[pb:pb + K * C].reshape(K, C)
+                        if K else torch.zeros(R, C, dtype=torch.float64, device=hip))
+                got = out[po:po + R * C].reshape(R, C).double()
+                assert torch.isfinite(got).all()
+                scale = max(float(want.abs().max()) if want.numel() else 0.0, 1.0)
+                assert float((got - want).abs().max()) <= tol * scale * max(K, 1), (cfg, dt)
+
+
+@pytest.mark.parametrize("case", CASES)
 def test_block_two_site_matches_tnkit(hip, case):
     """H_eff theta on blocks equals tnkit's block matvec (recorded) to 1e-12, eagerly in
     float64; the float32 operator with float64 vectors to 5e-6."""
