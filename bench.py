@@ -578,6 +578,58 @@ def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, lo
     return out
 
 
+def config4_modes(legs=32, N=64, n_steps=4, dt=0.05, log=None):
+    """The batched TDVP's ragged modes on binding caps: `legs` TFIM quenches as in `config4`
+    with svd_tol = 0 (every leg keeps its cap once its bond reaches it), run for n_steps in
+    'pad' (fixed widest shapes), 'tight' (widest kept rank) and 'group' (one batch per entry
+    bond profile). Leg-steps per second per mode, and the largest difference of the final
+    <Z_j> from the 'pad' run."""
+    import numpy as np
+    import torch
+    from tnkit_amd import _dmrg as H
+    from tnkit_amd import tdvp as T
+    from tnkit_amd.models import tfim_mpo
+    from tnkit_amd.observables import local_expectations
+    Z = np.diag([1.0, -1.0])
+    g0 = np.random.default_rng(1).uniform(1.2, 2.0, legs)
+    caps = [int(c) for c in np.linspace(64, 256, legs).round()]
+    states = []
+    for j, g in enumerate(g0):
+        _, mps, _ = H.dmrg(tfim_mpo(N, float(g)), 16, n_sweeps=4, tol=1e-12, rtol=0.0, seed=j,
+                           backend="hip-real")
+        states.append([t.to(torch.complex128) for t in mps])
+        if log:
+            print(f"ground state {j} g0={g:.3f}", file=log, flush=True)
+    mpo1 = tfim_mpo(N, 1.0)
+    meas = lambda ss: [local_expectations(s, Z) for s in ss]  # noqa: E731
+    out = {"workload": f"{legs} TFIM quenches N={N} -> g=1, TDVP2 dt={dt} x {n_steps} steps, "
+                       "per-leg caps 64..256, svd_tol=0", "caps": caps}
+    zref = None
+    for mode in ("pad", "tight", "group"):
+        stats = {}
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prog = (lambda k, t, mode=mode, t1=t1: print(
+            f"{mode} step {k} t={t:.2f} {time.perf_counter() - t1:.1f}s", file=log,
+            flush=True)) if log else None
+        _, rec = T.evolve_batch(states, mpo1, dt, n_steps, chi_max=caps, krylov_m=20,
+                                keep_device=True, measure=meas, meas_every=n_steps,
+                                stats=stats, svd_tol=0.0, ragged=mode, progress=prog)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t1
+        z = np.asarray(rec[-1][1])
+        if zref is None:
+            zref = z
+        out[mode] = {"wall_s": wall, "leg_steps_per_s": legs * n_steps / wall,
+                     "final_bonds_max": max(max(b) for b in stats["bonds"]),
+                     "z_max_diff_vs_pad": float(np.abs(z - zref).max())}
+        if log:
+            print(json.dumps({mode: out[mode]}), file=log, flush=True)
+    out["tight_vs_pad_speedup"] = out["tight"]["leg_steps_per_s"] / out["pad"]["leg_steps_per_s"]
+    out["group_vs_pad_speedup"] = out["group"]["leg_steps_per_s"] / out["pad"]["leg_steps_per_s"]
+    return out
+
+
 def config5(N=64, chi=256, fd_sites=(0, 21, 42, 63), h=1e-5, log=None):
     """Differentiation: the TFIM with per-site fields g_i = 1 + 0.1 N(0,1) (default_rng(2)),
     two-site DMRG at chi (float64), then dE/dg_i by the hand-written backward kernels
@@ -684,6 +736,8 @@ def main():
     ap.add_argument("--config4", action="store_true",
                     help="batched TDVP study: 32 TFIM quenches N=64, per-leg caps 64..256")
     ap.add_argument("--legs", type=int, default=32)
+    ap.add_argument("--config4-modes", action="store_true",
+                    help="batched TDVP ragged modes pad/tight/group on binding caps (svd_tol=0)")
     ap.add_argument("--config5", action="store_true",
                     help="differentiation study: TFIM N=64 per-site fields, chi=256")
     ap.add_argument("--tsteps", type=int, default=40)
@@ -730,6 +784,10 @@ def main():
         return
     if args.config5:
         print(json.dumps(config5(64 if args.N == 100 else args.N, log=sys.stderr)))
+        return
+    if args.config4_modes:
+        print(json.dumps(config4_modes(args.legs, 64 if args.N == 100 else args.N,
+                                       4 if args.tsteps == 40 else args.tsteps, log=sys.stderr)))
         return
     if args.config4:
         print(json.dumps(config4(args.legs, 64 if args.N == 100 else args.N, args.tsteps,

@@ -259,10 +259,11 @@ def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None, deferred: 
     """Batched eigh (ascending) of the Hermitian positive semidefinite G + diag(add) on the
     device. Orders up to 64 go to the one-workgroup-per-matrix Jacobi kernel
     (`kernels.eigh_jacobi`); larger ones, and any matrix the kernel leaves unconverged, to
-    rocSOLVER's batched eigh. That one returns non-finite eigenvectors for some complex Gram
-    matrices of low rank in wide padded bonds; those legs are decomposed instead by an SVD of the
-    unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it returns ascending like
-    eigh. Raises if a leg stays non-finite. With `deferred`, a Jacobi result is returned with
+    rocSOLVER's batched eigh. That one returns non-finite eigenvectors for complex matrices with
+    exactly decoupled rows (a leg's padded rows), so the matrices it gets carry a 1e-30 coupling
+    on the first off-diagonals (`_couple`); a leg that still comes back non-finite is decomposed
+    by an SVD of the unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it
+    returns ascending like eigh. Raises if a leg stays non-finite. With `deferred`, a Jacobi result is returned with
     the device mask of legs to redo, (w, Z, redo), and the caller, which reads w back anyway,
     finishes with `_eigh_redo`; otherwise (w, Z) after the check here."""
     if (G.shape[-1] <= kernels.EIGH_JACOBI_MAX_N and G.dtype in (torch.complex128, torch.float64)
@@ -292,9 +293,23 @@ def _eigh_redo(G: torch.Tensor, add: Optional[torch.Tensor], w: torch.Tensor, Z:
 _JACOBI = os.environ.get("TNKIT_AMD_JACOBI", "1") != "0"
 
 
+def _couple(Gs: torch.Tensor) -> torch.Tensor:
+    """Gs plus 1e-30 of its largest entry on the first off-diagonals. The batched complex eigh
+    (rocSOLVER) returns non-finite vectors for a matrix with exactly decoupled rows (the padded
+    rows of a leg narrower than the batch); this coupling is below the rounding of every nonzero
+    entry, so it only replaces exact zeros and moves no eigenvalue by more than 1e-30 of the
+    spectrum's scale."""
+    r = Gs.shape[-1]
+    if r < 2:
+        return Gs
+    eps = 1e-30 * Gs.abs().amax(dim=(-2, -1))
+    band = torch.diag_embed(eps[:, None].expand(-1, r - 1).to(Gs.dtype), offset=1)
+    return Gs + band + band.mH
+
+
 def _eigh_robust_vendor(G: torch.Tensor, add: Optional[torch.Tensor] = None):
     Gs = G if add is None else G + torch.diag_embed(add.to(G.dtype))
-    w, Z = torch.linalg.eigh(Gs)
+    w, Z = torch.linalg.eigh(_couple(Gs))
     bad = (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
     if not bool(bad.any()):
         return w, Z
