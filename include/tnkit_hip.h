@@ -359,6 +359,23 @@ size_t tnh_syev_top_ws_bytes(int n, int k);
 int tnh_syev_top(int n, int k, const double* A, long lda, double* w, double* Z, long ldz,
                  int* info, void* ws, size_t ws_bytes, tnh_stream_t stream);
 
+/* All eigenvalues and the k leading eigenvectors of a batch of complex Hermitian (complex128)
+ * matrices, 2 <= n <= 1024: the complex eigh of tnkit's Gram truncation at orders above 64
+ * (tnkit/linalg.py:138-159, `be.eigh(A @ A^H)` at :149), batched over the legs of the batched
+ * TDVP (tnkit/batch_tdvp.py:572-599, one torch.linalg.eigh over the stacked Gram matrices).
+ * Matrix m is A + m*sA (complex elements, row i at i*lda, both triangles read as given: pass a
+ * Hermitian matrix). Householder reduction to a real tridiagonal T (LAPACK zhetd2's
+ * reflectors, one cooperative launch per group of resident matrices), the divide and conquer of
+ * tnh_syev_top on T, and the complex WY back-transformation. Writes w + m*sw (n eigenvalues,
+ * ascending) and Z + m*sZ (n x k complex, row stride ldz): column c = eigenvector of the c-th
+ * largest eigenvalue. info (device int, may be NULL): 0; bit 0: a wait of a reduction ran out;
+ * bit 1: a leaf eigensolve did not converge (over the whole batch). The workspace size depends on
+ * the device (how many matrices one launch holds): query it on the device of the call. */
+size_t tnh_heev_top_ws_bytes(int batch, int n, int k);
+int tnh_heev_top(int batch, int n, int k, const void* A, long lda, long sA, double* w, long sw,
+                 void* Z, long ldz, long sZ, int* info, void* ws, size_t ws_bytes,
+                 tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

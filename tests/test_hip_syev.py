@@ -119,3 +119,93 @@ def test_gram_truncation_on_the_solver(hip, m, n, chi, iso):
     sv = np.linalg.svd(A, compute_uv=False)
     err = np.linalg.norm(A - (U * S) @ Vh) ** 2
     assert abs(err - np.sum(sv[chi:] ** 2)) <= 1e-9 * sv[0] ** 2
+
+
+def _hermitian(kind, n, rng):
+    """Complex Hermitian test matrices: the complex counterparts of _matrix, plus a tridiagonal
+    with complex couplings (every reflector has nothing below its subdiagonal entry but a phase)
+    and a padded Gram matrix (rows and columns beyond the live extent exactly zero)."""
+    def cn(*s):
+        return rng.standard_normal(s) + 1j * rng.standard_normal(s)
+    if kind == "gram":
+        X = cn(n, n) * np.exp(-np.arange(n) / max(1.0, n / 20.0))[None, :]
+        A = X @ X.conj().T
+    elif kind == "lowrank":
+        X = cn(n, max(1, n // 7))
+        A = X @ X.conj().T
+    elif kind == "degenerate":
+        Q, _ = np.linalg.qr(cn(n, n))
+        A = (Q * np.repeat(np.arange(1.0, 6.0), (n + 4) // 5)[:n]) @ Q.conj().T
+    elif kind == "graded":
+        Q, _ = np.linalg.qr(cn(n, n))
+        A = (Q * np.logspace(0, -30, n)) @ Q.conj().T
+    elif kind == "random":
+        X = cn(n, n)
+        A = X + X.conj().T
+    elif kind == "tridiagonal":
+        A = np.diag(rng.standard_normal(n)).astype(complex)
+        c = cn(n - 1)
+        A += np.diag(c, -1) + np.diag(c.conj(), 1)
+    elif kind == "padded":
+        m = max(1, (2 * n) // 3)
+        X = np.zeros((n, n), complex)
+        X[:m, :m] = cn(m, m)
+        A = X @ X.conj().T
+    elif kind == "real":
+        X = rng.standard_normal((n, n))
+        A = (X @ X.T).astype(complex)
+    else:
+        A = np.zeros((n, n), complex)
+    return 0.5 * (A + A.conj().T)
+
+
+def _check_c(A, w, Z, k):
+    wr = np.linalg.eigvalsh(A)
+    sc = max(np.abs(wr).max(), 1e-300)
+    assert np.abs(w - wr).max() <= 1e-13 * sc
+    lam = w[::-1][:k]
+    assert np.abs(A @ Z - Z * lam[None, :]).max() <= 1e-13 * sc
+    assert np.abs(Z.conj().T @ Z - np.eye(k)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("n,kind", [(65, "random"), (100, "gram"), (128, "lowrank"),
+                                    (200, "degenerate"), (257, "graded"), (300, "tridiagonal"),
+                                    (96, "zero"), (384, "padded"), (500, "gram"), (512, "real"),
+                                    (777, "lowrank"), (1024, "gram"), (2, "random"),
+                                    (3, "tridiagonal")])
+def test_heev_top_against_numpy(hip, n, kind):
+    """The batched complex solver on one matrix: eigenvalues to 1e-13 of the spectral radius,
+    A Z = Z diag(w) to 1e-13, Z^H Z = 1 to 1e-12 for k = 1, n/2, n."""
+    rng = np.random.default_rng(n + 11)
+    A = _hermitian(kind, n, rng)
+    At = torch.as_tensor(A, device=hip).unsqueeze(0).contiguous()
+    for k in sorted({1, max(1, n // 2), n}):
+        w, Z = kernels.heev_top(At, k)
+        _check_c(A, w[0].cpu().numpy(), Z[0].cpu().numpy(), k)
+
+
+@pytest.mark.parametrize("n,batch", [(128, 5), (512, 9), (1024, 3)])
+def test_heev_top_batch_matches_single_solves(hip, n, batch):
+    """A batch that needs several cooperative launches (more matrices than one launch holds at
+    n = 512 and 1024): every matrix to the same bar, and bit-identical to its solve alone (each
+    matrix is reduced by the same workgroup layout whatever the batch)."""
+    rng = np.random.default_rng(n * batch)
+    kinds = ["gram", "padded", "random", "lowrank", "tridiagonal"]
+    As = np.stack([_hermitian(kinds[i % len(kinds)], n, rng) for i in range(batch)])
+    At = torch.as_tensor(As, device=hip).contiguous()
+    k = n // 4
+    w, Z = kernels.heev_top(At, k)
+    wn, Zn = w.cpu().numpy(), Z.cpu().numpy()
+    for i in range(batch):
+        _check_c(As[i], wn[i], Zn[i], k)
+    for i in (0, batch - 1):
+        w1, Z1 = kernels.heev_top(At[i:i + 1].contiguous(), k)
+        assert torch.equal(w1[0], w[i]) and torch.equal(Z1[0], Z[i])
+
+
+def test_heev_top_rejects_bad_shapes(hip):
+    A = torch.zeros((1, 1025, 1025), dtype=torch.complex128, device=hip)
+    with pytest.raises(ValueError):
+        kernels.heev_top(A, 4)
+    with pytest.raises(ValueError):
+        kernels.heev_top(torch.zeros((2, 8, 8), dtype=torch.complex128, device=hip), 9)

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "cplx.h"
 #include "tnh_common.h"
 
 namespace tnh {
@@ -51,10 +52,11 @@ __device__ __forceinline__ void st_shared(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ bool grid_sync(const GridBar& b, unsigned epoch, int* flag) {
+// The barrier over the G workgroups g = 0..G-1 of one group (a grid can hold several groups,
+// each with its own barrier words).
+__device__ bool grid_sync_of(const GridBar& b, unsigned epoch, int* flag, unsigned G, unsigned g) {
   __builtin_amdgcn_s_waitcnt(0);   // this wave's stores are complete
   __syncthreads();
-  const unsigned G = gridDim.x, g = blockIdx.x;
   if (threadIdx.x == 0)
     __hip_atomic_store(b.mem + g * kBarStride, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (g == 0) {
@@ -92,6 +94,10 @@ __device__ bool grid_sync(const GridBar& b, unsigned epoch, int* flag) {
   }
   __syncthreads();
   return *flag == 0;
+}
+
+__device__ bool grid_sync(const GridBar& b, unsigned epoch, int* flag) {
+  return grid_sync_of(b, epoch, flag, gridDim.x, blockIdx.x);
 }
 
 __global__ void bar_reset_kernel(unsigned* mem, int* err) {
@@ -1020,67 +1026,82 @@ int gemm(int M, int N, int K, const double* A, long lda, const double* B, long l
   return tnh_gemm_tn(TNH_F64, M, N, K, A, lda, B, ldb, C, ldc, batch, sA, sB, sC, acc, 0, st);
 }
 
-int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z, long ldz,
-             void* ws, size_t ws_bytes, int* info, hipStream_t st) {
+// The workspace of one solve, carved by syev_pieces.
+struct SyevBufs {
+  char* trd;
+  double *Vr, *Vc, *d, *e, *tau, *dp, *ep, *Qt[2], *Mb, *leafA, *leafZ, *wv[2];
+  int* sweeps;
+  DcLevel Lv;
+  double *Zw, *S, *negTt, *Y, *Y2;
+};
+
+SyevBufs syev_carve(const SyevPlan& p, void* ws) {
   std::vector<char*> ptrs;
-  {
-    char* base = reinterpret_cast<char*>(align_up(reinterpret_cast<size_t>(ws), 256));
-    size_t off = 0;
-    syev_pieces(p, [&](const char*, size_t b) {
-      ptrs.push_back(base + off);
-      off += align_up(b, 256);
-    });
-  }
+  char* base = reinterpret_cast<char*>(align_up(reinterpret_cast<size_t>(ws), 256));
+  size_t off = 0;
+  syev_pieces(p, [&](const char*, size_t b) {
+    ptrs.push_back(base + off);
+    off += align_up(b, 256);
+  });
   int q = 0;
   auto next = [&]() { return ptrs[q++]; };
-  char* trd = next();
-  double* Vr = reinterpret_cast<double*>(next());
-  double* Vc = reinterpret_cast<double*>(next());
-  double* d = reinterpret_cast<double*>(next());
-  double* e = reinterpret_cast<double*>(next());
-  double* tau = reinterpret_cast<double*>(next());
-  double* dp = reinterpret_cast<double*>(next());
-  double* ep = reinterpret_cast<double*>(next());
-  double* Qt[2] = {reinterpret_cast<double*>(next()), reinterpret_cast<double*>(next())};
-  double* Mb = reinterpret_cast<double*>(next());
-  double* leafA = reinterpret_cast<double*>(next());
-  double* leafZ = reinterpret_cast<double*>(next());
-  double* wv[2] = {reinterpret_cast<double*>(next()), reinterpret_cast<double*>(next())};
-  int* sweeps = reinterpret_cast<int*>(next());
-  DcLevel Lv;
-  Lv.perm = reinterpret_cast<int*>(next());
-  Lv.inv = reinterpret_cast<int*>(next());
-  Lv.kept = reinterpret_cast<int*>(next());
-  Lv.dl = reinterpret_cast<int*>(next());
-  Lv.rot_j = reinterpret_cast<int*>(next());
-  Lv.rot_jl = reinterpret_cast<int*>(next());
-  Lv.ent = reinterpret_cast<int*>(next());
-  Lv.Ds = reinterpret_cast<double*>(next());
-  Lv.zs = reinterpret_cast<double*>(next());
-  Lv.rot_c = reinterpret_cast<double*>(next());
-  Lv.rot_s = reinterpret_cast<double*>(next());
-  Lv.mu = reinterpret_cast<double*>(next());
-  Lv.dor = reinterpret_cast<double*>(next());
-  Lv.zh = reinterpret_cast<double*>(next());
-  Lv.nrm = reinterpret_cast<double*>(next());
-  Lv.cnt = reinterpret_cast<int*>(next());
-  Lv.rhop = reinterpret_cast<double*>(next());
-  double* Zw = reinterpret_cast<double*>(next());
-  double* S = reinterpret_cast<double*>(next());
-  double* negTt = reinterpret_cast<double*>(next());
-  double* Y = reinterpret_cast<double*>(next());
-  double* Y2 = reinterpret_cast<double*>(next());
-  int* dinfo = reinterpret_cast<int*>(next());
-  (void)dinfo;
+  auto dbl = [&]() { return reinterpret_cast<double*>(next()); };
+  auto itg = [&]() { return reinterpret_cast<int*>(next()); };
+  SyevBufs b;
+  b.trd = next();
+  b.Vr = dbl();
+  b.Vc = dbl();
+  b.d = dbl();
+  b.e = dbl();
+  b.tau = dbl();
+  b.dp = dbl();
+  b.ep = dbl();
+  b.Qt[0] = dbl();
+  b.Qt[1] = dbl();
+  b.Mb = dbl();
+  b.leafA = dbl();
+  b.leafZ = dbl();
+  b.wv[0] = dbl();
+  b.wv[1] = dbl();
+  b.sweeps = itg();
+  b.Lv.perm = itg();
+  b.Lv.inv = itg();
+  b.Lv.kept = itg();
+  b.Lv.dl = itg();
+  b.Lv.rot_j = itg();
+  b.Lv.rot_jl = itg();
+  b.Lv.ent = itg();
+  b.Lv.Ds = dbl();
+  b.Lv.zs = dbl();
+  b.Lv.rot_c = dbl();
+  b.Lv.rot_s = dbl();
+  b.Lv.mu = dbl();
+  b.Lv.dor = dbl();
+  b.Lv.zh = dbl();
+  b.Lv.nrm = dbl();
+  b.Lv.cnt = itg();
+  b.Lv.rhop = dbl();
+  b.Zw = dbl();
+  b.S = dbl();
+  b.negTt = dbl();
+  b.Y = dbl();
+  b.Y2 = dbl();
+  return b;
+}
+
+// Divide and conquer on the tridiagonal (d, e) of order p.n: all eigenvalues ascending into w,
+// the eigenvectors of the k largest into rows 0..n-1 of b.Zw (n x k, column c the c-th largest).
+int run_dc(const SyevPlan& p, SyevBufs& b, const double* d, const double* e, double* w,
+           hipStream_t st) {
   const int n = p.n, N = p.N, k = p.k, s = p.s;
-
-  // 1. tridiagonalisation
-  Arena tar(trd, trd_ws_bytes(n));
-  int* err = nullptr;
-  int rc = launch_sytrd(n, A, lda, Vr, d, e, tau, tar, &err, st);
-  if (rc != TNH_OK) return rc;
-
-  // 2. divide and conquer
+  double* const* Qt = b.Qt;
+  double* const* wv = b.wv;
+  DcLevel& Lv = b.Lv;
+  double* Mb = b.Mb;
+  double* Zw = b.Zw;
+  double *dp = b.dp, *ep = b.ep, *leafA = b.leafA, *leafZ = b.leafZ;
+  int* sweeps = b.sweeps;
+  int rc;
   hipLaunchKernelGGL(dc_pad_kernel, dim3(1), dim3(1024), 0, st, n, N, d, e, dp, ep);
   hipLaunchKernelGGL(dc_leaves_kernel, dim3(N / s), dim3(256), 0, st, s, N, dp, ep, leafA);
   TNH_CHECK_LAUNCH("syev leaves");
@@ -1138,6 +1159,26 @@ int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z,
   hipError_t he = hipMemcpyAsync(w, wv[cur] + (N - n), (size_t)n * sizeof(double),
                                  hipMemcpyDeviceToDevice, st);
   TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: eigenvalue copy: %s", hipGetErrorString(he));
+  return TNH_OK;
+}
+
+int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z, long ldz,
+             void* ws, size_t ws_bytes, int* info, hipStream_t st) {
+  (void)ws_bytes;
+  SyevBufs b = syev_carve(p, ws);
+  const int n = p.n, N = p.N, k = p.k, s = p.s;
+  double *Vr = b.Vr, *Vc = b.Vc, *tau = b.tau, *Zw = b.Zw, *S = b.S, *negTt = b.negTt;
+  double *Y = b.Y, *Y2 = b.Y2;
+
+  // 1. tridiagonalisation
+  Arena tar(b.trd, trd_ws_bytes(n));
+  int* err = nullptr;
+  int rc = launch_sytrd(n, A, lda, Vr, b.d, b.e, tau, tar, &err, st);
+  if (rc != TNH_OK) return rc;
+
+  // 2. divide and conquer
+  rc = run_dc(p, b, b.d, b.e, w, st);
+  if (rc != TNH_OK) return rc;
 
   // 3. back-transformation of the k vectors (rows 0..n-1 of Zw)
   const int NV = p.npan * kBtNb;
@@ -1150,23 +1191,516 @@ int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z,
   if (rc != TNH_OK) return rc;
   hipLaunchKernelGGL(larft_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S, tau, negTt);
   TNH_CHECK_LAUNCH("syev larft");
-  for (int b = p.npan - 1; b >= 0; --b) {
-    const int jb = b * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
+  for (int bb = p.npan - 1; bb >= 0; --bb) {
+    const int jb = bb * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
     rc = gemm(nb, k, rows, Vc + (long)jb * NV + jb, NV, Zw + (long)jb * k, k, Y, k, 1, 0, 0, 0, 0,
               st);
     if (rc != TNH_OK) return rc;
-    rc = gemm(nb, k, nb, negTt + (long)b * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, 1, 0, 0, 0, 0, st);
+    rc = gemm(nb, k, nb, negTt + (long)bb * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, 1, 0, 0, 0, 0,
+              st);
     if (rc != TNH_OK) return rc;
     rc = gemm(rows, k, nb, Vr + (long)jb * n + jb, n, Y2, k, Zw + (long)jb * k, k, 1, 0, 0, 0, 1,
               st);
     if (rc != TNH_OK) return rc;
   }
-  he = hipMemcpy2DAsync(Z, (size_t)ldz * sizeof(double), Zw, (size_t)k * sizeof(double),
-                        (size_t)k * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice, st);
+  hipError_t he = hipMemcpy2DAsync(Z, (size_t)ldz * sizeof(double), Zw, (size_t)k * sizeof(double),
+                                   (size_t)k * sizeof(double), (size_t)n, hipMemcpyDeviceToDevice,
+                                   st);
   TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: vector copy: %s", hipGetErrorString(he));
   if (info) {
-    hipLaunchKernelGGL(syev_status_kernel, dim3(1), dim3(256), 0, st, err, sweeps, N / s, info);
+    hipLaunchKernelGGL(syev_status_kernel, dim3(1), dim3(256), 0, st, err, b.sweeps, N / s, info);
     TNH_CHECK_LAUNCH("syev status");
+  }
+  return TNH_OK;
+}
+
+// ---- complex Hermitian matrices ----------------------------------------------------------------
+// A = Q T Q^H with T real symmetric tridiagonal (LAPACK's zhetd2 in lower form: H_j = I - tau_j
+// v_j v_j^H, H_j^H (alpha; x) = (beta; 0) with beta real, so every subdiagonal entry of T is
+// real), then the real divide and conquer on T, then Z = Q Z_T by complex WY blocks. The
+// reduction is the real kernel's scheme (rows in registers, one grid barrier per column) in
+// complex arithmetic, with 512-thread workgroups holding 8 rows x 2 columns per thread (order
+// <= 1024), and several matrices per launch: workgroup b serves matrix b / G as its workgroup
+// b % G, each matrix with its own barrier words and exchange slots.
+constexpr int kHThreads = 512;
+constexpr int kHCols = 2;
+constexpr int kHRows = 8;
+constexpr int kHWaves = kHThreads / 64;
+constexpr int kHeevMaxN = kHThreads * kHCols;   // 1024
+
+__device__ __forceinline__ c128 cmul(c128 a, c128 b) { return a * b; }
+__device__ __forceinline__ c128 cscale(double s, c128 a) { return {s * a.re, s * a.im}; }
+__device__ __forceinline__ double cabs2(c128 a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ void st_shared_c(c128* p, c128 v) {
+  double* q = reinterpret_cast<double*>(p);
+  st_shared(q, v.re);
+  st_shared(q + 1, v.im);
+}
+
+struct HtrdArgs {
+  int n, ns, gs, G;          // order, slot lengths, workgroups per matrix
+  const c128* A;             // matrix m at A + m sA, leading dimension lda
+  long lda, sA;
+  c128* Vr;                  // n x n per matrix (stride n n): row j = v_j
+  double* d;                 // n per matrix
+  double* e;                 // n per matrix
+  c128* tau;                 // n per matrix
+  c128* P;                   // per matrix n x ns slots: tau_j A v_j
+  c128* Rp;                  // per matrix n x ns slots: row j before its update
+  c128* part;                // per matrix n x gs: per-workgroup partials p^H v
+  unsigned* bar;             // per matrix kBarWords
+  int* err;
+};
+
+__global__ void __launch_bounds__(kHThreads) hetrd_coop_kernel(HtrdArgs a) {
+  __shared__ double red[2 * kHWaves];
+  __shared__ double red8[2][kHWaves][2 * kHRows];
+  __shared__ double ov_re[kHRows], ov_im[kHRows], op_re[kHRows], op_im[kHRows];
+  __shared__ double on_re[kHRows], on_im[kHRows];
+  __shared__ double vnx_re, vnx_im;   // v_{j-1}[j+1]
+  __shared__ int flag;
+  const int n = a.n, G = a.G, m = blockIdx.x / G, g = blockIdx.x % G, t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int nr = (n - g + G - 1) / G;
+  const long nn = (long)n * n, sl = (long)n * a.ns, sp = (long)n * a.gs;
+  const c128* A = a.A + m * a.sA;
+  c128* Vr = a.Vr + m * nn;
+  double* dd = a.d + (long)m * n;
+  double* ee = a.e + (long)m * n;
+  c128* tt = a.tau + (long)m * n;
+  c128* P = a.P + m * sl;
+  c128* Rp = a.Rp + m * sl;
+  c128* part = a.part + m * sp;
+  const GridBar bar{a.bar + (size_t)m * kBarWords, a.err};
+  int par = 0, par8 = 0;
+
+  c128 x[kHRows][kHCols];
+#pragma unroll
+  for (int r = 0; r < kHRows; ++r)
+#pragma unroll
+    for (int c = 0; c < kHCols; ++c) {
+      const int l = t + kHThreads * c;
+      x[r][c] = (r < nr && l < n) ? A[(long)(g + r * G) * a.lda + l] : c128{};
+    }
+  if (n == 1) {
+    if (g == 0 && t == 0) dd[0] = x[0][0].re;
+    return;
+  }
+  if (g == 0)
+#pragma unroll
+    for (int c = 0; c < kHCols; ++c) {
+      const int l = t + kHThreads * c;
+      if (l < n) st_shared_c(Rp + l, x[0][c]);
+    }
+  if (t < kHRows) {
+    ov_re[t] = ov_im[t] = op_re[t] = op_im[t] = 0.0;
+  }
+  unsigned epoch = 1;
+  if (!grid_sync_of(bar, epoch++, &flag, G, g)) return;
+
+  c128 vr[kHCols];   // v_{j-1} at this thread's columns
+  c128 tau_prev{};
+  for (int j = 0; j < n - 1; ++j) {
+    const c128* Rj = Rp + (long)j * a.ns;
+    c128 R[kHCols], wr[kHCols];
+    c128 aw{}, wj{};
+    double dj;
+    c128 alpha;
+    if (j > 0) {
+      const c128* Pj = P + (long)(j - 1) * a.ns;
+      const c128* pj = part + (long)(j - 1) * a.gs;
+      c128 ps{};
+#pragma unroll
+      for (int q = 0; q < kBarMaxG / 64; ++q) {
+        const int kk = lane + 64 * q;
+        if (kk < G) ps = ps + pj[kk];
+      }
+      c128 Pl[kHCols];
+#pragma unroll
+      for (int c = 0; c < kHCols; ++c) {
+        const int l = t + kHThreads * c;
+        Pl[c] = l >= j && l < n ? Pj[l] : c128{};
+        R[c] = l >= j && l < n ? Rj[l] : c128{};
+      }
+      const c128 pjj = Pj[j], pj1 = Pj[j + 1], rjj = Rj[j], rj1 = Rj[j + 1];
+      const c128 S{wsum(ps.re), wsum(ps.im)};                 // p^H v of column j - 1
+      aw = cscale(-0.5, cmul(tau_prev, S));                   // w = p + aw v
+      const c128 vx{vnx_re, vnx_im};
+      wj = pjj + aw;                                          // v_{j-1}[j] = 1
+#pragma unroll
+      for (int c = 0; c < kHCols; ++c) {
+        wr[c] = Pl[c] + cmul(aw, vr[c]);
+        R[c] = R[c] - (conj(wr[c]) + cmul(wj, conj(vr[c])));
+      }
+      dj = rjj.re - 2.0 * wj.re;
+      const c128 w1 = pj1 + cmul(aw, vx);
+      alpha = conj(rj1 - (conj(w1) + cmul(wj, conj(vx))));
+    } else {
+#pragma unroll
+      for (int c = 0; c < kHCols; ++c) {
+        const int l = t + kHThreads * c;
+        R[c] = l < n ? Rj[l] : c128{};
+        wr[c] = c128{};
+      }
+      dj = Rj[0].re;
+      alpha = conj(Rj[1]);
+    }
+    // the reflector of column j: x = conj(row j) right of the subdiagonal
+    double sq = 0.0;
+#pragma unroll
+    for (int c = 0; c < kHCols; ++c) {
+      const int l = t + kHThreads * c;
+      if (l >= j + 2 && l < n) sq += cabs2(R[c]);
+    }
+    double sigma;
+    {
+      double v = wsum(sq);
+      double* rb = red + kHWaves * par;
+      par ^= 1;
+      if (lane == 0) rb[wv] = v;
+      __syncthreads();
+      sigma = 0.0;
+#pragma unroll
+      for (int w = 0; w < kHWaves; ++w) sigma += rb[w];
+    }
+    double beta = alpha.re;
+    c128 tj{}, scale{};
+    if (sigma > 0.0 || alpha.im != 0.0) {
+      beta = -copysign(sqrt(cabs2(alpha) + sigma), alpha.re);
+      tj = c128{(beta - alpha.re) / beta, -alpha.im / beta};
+      const c128 den{alpha.re - beta, alpha.im};              // 1 / (alpha - beta)
+      const double q = 1.0 / cabs2(den);
+      scale = c128{den.re * q, -den.im * q};
+    }
+    c128 vn[kHCols];
+#pragma unroll
+    for (int c = 0; c < kHCols; ++c) {
+      const int l = t + kHThreads * c;
+      vn[c] = l == j + 1 ? c128{1.0, 0.0}
+                         : (l >= j + 2 && l < n ? cmul(conj(R[c]), scale) : c128{});
+      if (l < n && l >= g && (l - g) % G == 0) {
+        const int rl = (l - g) / G;
+        on_re[rl] = vn[c].re;
+        on_im[rl] = vn[c].im;
+        Vr[(long)j * n + l] = vn[c];
+      }
+      if (l == j + 2) {
+        vnx_re = vn[c].re;
+        vnx_im = vn[c].im;
+      }
+    }
+    if (g == j % G && t == 0) {
+      dd[j] = dj;
+      ee[j] = beta;
+      tt[j] = tj;
+    }
+    if (j == n - 2) {   // the last diagonal gets column j-1's update
+      if (g == (n - 1) % G) {
+        const int rl = (n - 1 - g) / G;
+        const c128 vi{ov_re[rl], ov_im[rl]};
+        const c128 wi = c128{op_re[rl], op_im[rl]} + cmul(aw, vi);
+#pragma unroll
+        for (int r = 0; r < kHRows; ++r)
+#pragma unroll
+          for (int c = 0; c < kHCols; ++c)
+            if (r == rl && t + kHThreads * c == n - 1)
+              dd[n - 1] = x[r][c].re - 2.0 * cmul(vi, conj(wi)).re;
+      }
+      break;
+    }
+    // one pass over the own rows: update by (v_{j-1}, w_{j-1}), p_j = tau_j A v_j on rows > j,
+    // row j+1 published by its owner
+    const int rpub = (j + 1 < n - 1 && g == (j + 1) % G) ? (j + 1 - g) / G : -1;
+    double (*rb)[2 * kHRows] = red8[par8];
+    par8 ^= 1;
+    double s_re[kHRows], s_im[kHRows];
+#pragma unroll
+    for (int r = 0; r < kHRows; ++r) {
+      const c128 vi{ov_re[r], ov_im[r]};
+      const c128 wi = c128{op_re[r], op_im[r]} + cmul(aw, vi);
+      const bool live = r < nr && g + r * G >= j;
+      c128 acc{};
+#pragma unroll
+      for (int c = 0; c < kHCols; ++c) {
+        const int l = t + kHThreads * c;
+        if (live && l >= j && l < n)
+          x[r][c] = x[r][c] - (cmul(vi, conj(wr[c])) + cmul(wi, conj(vr[c])));
+        acc = acc + cmul(x[r][c], vn[c]);
+        if (r == rpub && l > j && l < n) st_shared_c(Rp + (long)(j + 1) * a.ns + l, x[r][c]);
+      }
+      const bool keep = r < nr && g + r * G > j;
+      s_re[r] = keep ? acc.re : 0.0;
+      s_im[r] = keep ? acc.im : 0.0;
+    }
+    wsum8_store(s_re, rb[wv], lane);
+    wsum8_store(s_im, rb[wv] + kHRows, lane);
+    __syncthreads();
+    c128 partial{};
+    if (t < nr) {
+      const int i = g + t * G;
+      c128 pv{};
+      if (i >= j + 1) {
+        c128 sum{};
+#pragma unroll
+        for (int w = 0; w < kHWaves; ++w) sum = sum + c128{rb[w][t], rb[w][kHRows + t]};
+        pv = cmul(tj, sum);
+        st_shared_c(P + (long)j * a.ns + i, pv);
+        partial = cmul(conj(pv), c128{on_re[t], on_im[t]});
+      }
+      op_re[t] = pv.re;
+      op_im[t] = pv.im;
+      ov_re[t] = on_re[t];
+      ov_im[t] = on_im[t];
+    }
+    if (wv == 0) {
+      const c128 ps{wsum(partial.re), wsum(partial.im)};
+      if (t == 0) st_shared_c(part + (long)j * a.gs + g, ps);
+    }
+    tau_prev = tj;
+#pragma unroll
+    for (int c = 0; c < kHCols; ++c) vr[c] = vn[c];
+    if (!grid_sync_of(bar, epoch++, &flag, G, g)) return;
+  }
+}
+
+int htrd_grid(int n) { return (n + kHRows - 1) / kHRows; }
+
+// Per matrix: P and Rp slots (n x ns complex), partials (n x gs complex), barrier words.
+size_t htrd_ws_per_matrix(int n) {
+  const int G = htrd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
+  return align_up((size_t)n * ns * 16, 256) * 2 + align_up((size_t)n * gs * 16, 256) +
+         align_up(kBarWords * sizeof(unsigned), 256);
+}
+
+// Complex back-transformation helpers.
+// Vc[i][j] = Vr[j][i] and Vh[i][j] = conj(Vr[j][i]) for reflectors j < n - 1 (zero otherwise).
+__global__ void transpose_c_kernel(int n, int NV, const c128* __restrict__ src,
+                                   c128* __restrict__ Vc, c128* __restrict__ Vh) {
+  __shared__ double tre[32][33], tim[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int j = by + r, i = bx + threadIdx.x;
+    const c128 v = (j < n - 1 && i < n) ? src[(long)j * n + i] : c128{};
+    tre[r][threadIdx.x] = v.re;
+    tim[r][threadIdx.x] = v.im;
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int i = bx + r, j = by + threadIdx.x;
+    if (i < n && j < NV) {
+      const c128 v{tre[threadIdx.x][r], tim[threadIdx.x][r]};
+      Vc[(long)i * NV + j] = v;
+      Vh[(long)i * NV + j] = conj(v);
+    }
+  }
+}
+
+// Panel b: T of Q_b = I - V T V^H (T[0:c, c] = -tau_c T[0:c, 0:c] S[0:c, c], T[c][c] = tau_c with
+// S = V^H V), written as A_b[p][q] = -T[q][p], the K-major left operand of Y' = -T Y.
+__global__ void __launch_bounds__(256) larft_c_kernel(int nrefl, const c128* S, const c128* tau,
+                                                      c128* negTt) {
+  __shared__ double Tre[kBtNb][kBtNb + 1], Tim[kBtNb][kBtNb + 1];
+  __shared__ double mre[kBtNb], mim[kBtNb];
+  const int b = blockIdx.x, jb = b * kBtNb, nb = min(kBtNb, nrefl - jb), t = threadIdx.x;
+  const c128* Sb = S + (long)b * kBtNb * kBtNb;
+  for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) {
+    Tre[idx / kBtNb][idx % kBtNb] = 0.0;
+    Tim[idx / kBtNb][idx % kBtNb] = 0.0;
+  }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    const c128 tc = tau[jb + c];
+    if (t < c) {
+      c128 s{};
+      for (int q = t; q < c; ++q) s = s + cmul(c128{Tre[t][q], Tim[t][q]}, Sb[q * kBtNb + c]);
+      mre[t] = s.re;
+      mim[t] = s.im;
+    }
+    __syncthreads();
+    if (t < c) {
+      const c128 v = cmul(tc, c128{mre[t], mim[t]});
+      Tre[t][c] = -v.re;
+      Tim[t][c] = -v.im;
+    }
+    if (t == 0) {
+      Tre[c][c] = tc.re;
+      Tim[c][c] = tc.im;
+    }
+    __syncthreads();
+  }
+  c128* out = negTt + (long)b * kBtNb * kBtNb;
+  for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) {
+    const int p = idx / kBtNb, q = idx % kBtNb;
+    out[idx] = c128{-Tre[q][p], -Tim[q][p]};
+  }
+}
+
+// Zc (n x k complex) = Zw (n x k real).
+__global__ void real_to_c_kernel(long cnt, const double* __restrict__ src, c128* __restrict__ dst) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < cnt; i += (long)gridDim.x * 256)
+    dst[i] = c128{src[i], 0.0};
+}
+
+__global__ void heev_status_kernel(const int* err, const int* sweeps, int nleaf, int* info) {
+  int bad = 0;
+  for (int i = threadIdx.x; i < nleaf; i += blockDim.x) bad |= sweeps[i] < 0;
+  bad = __syncthreads_or(bad);
+  if (threadIdx.x == 0) *info |= (*err ? 1 : 0) | (bad ? 2 : 0);
+}
+
+int gemm_c(int M, int N, int K, const c128* A, long lda, const c128* B, long ldb, c128* C,
+           long ldc, int batch, long sA, long sB, long sC, int acc, hipStream_t st) {
+  return tnh_gemm_tn(TNH_C128, M, N, K, A, lda, B, ldb, C, ldc, batch, sA, sB, sC, acc, 0, st);
+}
+
+struct HeevPlan {
+  SyevPlan sp;
+  int batch, mpl;   // matrices, matrices per reduction launch
+};
+
+// Workspace pieces of the batched complex solve, in carving order: the real solver's pieces
+// (its divide and conquer is reused as is), then per launch of h.mpl matrices the reduction's
+// exchange slots and barrier words, the reflectors and T, then the back-transformation's blocks.
+template <typename F>
+void heev_pieces(const HeevPlan& h, F&& f) {
+  const SyevPlan& p = h.sp;
+  const size_t n = p.n, k = p.k, NV = (size_t)p.npan * kBtNb, B = h.mpl;
+  const size_t G = htrd_grid(p.n), ns = trd_slot(p.n), gs = (G + 31) / 32 * 32 + 32;
+  f("syev", syev_ws_bytes(p));
+  f("P", B * n * ns * 16);
+  f("Rp", B * n * ns * 16);
+  f("part", B * n * gs * 16);
+  f("bars", B * kBarWords * sizeof(unsigned));
+  f("err", 256);
+  f("Vr", B * n * n * 16);
+  f("d", B * n * 8);
+  f("e", B * n * 8);
+  f("tau", B * n * 16);
+  f("Vc", n * NV * 16);
+  f("Vh", n * NV * 16);
+  f("S", (size_t)p.npan * kBtNb * kBtNb * 16);
+  f("negTt", (size_t)p.npan * kBtNb * kBtNb * 16);
+  f("Zc", n * k * 16);
+  f("Y", (size_t)kBtNb * k * 16);
+  f("Y2", (size_t)kBtNb * k * 16);
+}
+
+size_t heev_ws_bytes(const HeevPlan& h) {
+  size_t tot = 0;
+  heev_pieces(h, [&](const char*, size_t b) { tot += align_up(b, 256); });
+  return tot + 256;
+}
+
+// Matrices per reduction launch: as many whole matrices as the device keeps resident at once.
+int heev_mpl(int n, int batch, int* mpl) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(
+          &per_cu, reinterpret_cast<const void*>(&hetrd_coop_kernel), kHThreads, 0) !=
+          hipSuccess) {
+    set_error("heev: device query failed");
+    return TNH_ERR_HIP;
+  }
+  const int G = htrd_grid(n), cap = (cus * per_cu) / G;
+  TNH_REQUIRE(cap >= 1, TNH_ERR_UNSUPPORTED,
+              "heev: the %d workgroups of one matrix are not resident at once", G);
+  *mpl = std::min(cap, batch);
+  return TNH_OK;
+}
+
+// Per launch of up to h.mpl matrices: the reductions in one cooperative launch, then per matrix
+// the divide and conquer on (d, e) (the real solver's) and Z = Q Z_T.
+int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, long sw, c128* Z,
+             long ldz, long sZ, void* ws, int* info, hipStream_t st) {
+  const SyevPlan& p = h.sp;
+  std::vector<char*> ptrs;
+  {
+    char* base = reinterpret_cast<char*>(align_up(reinterpret_cast<size_t>(ws), 256));
+    size_t off = 0;
+    heev_pieces(h, [&](const char*, size_t b) {
+      ptrs.push_back(base + off);
+      off += align_up(b, 256);
+    });
+  }
+  int q = 0;
+  auto cp = [&]() { return reinterpret_cast<c128*>(ptrs[q++]); };
+  auto dp = [&]() { return reinterpret_cast<double*>(ptrs[q++]); };
+  char* ws_syev = ptrs[q++];
+  c128* P = cp();
+  c128* Rp = cp();
+  c128* part = cp();
+  unsigned* bars = reinterpret_cast<unsigned*>(ptrs[q++]);
+  int* err = reinterpret_cast<int*>(ptrs[q++]);
+  c128* Vr = cp();
+  double* d = dp();
+  double* e = dp();
+  c128* tau = cp();
+  c128* Vc = cp();
+  c128* Vh = cp();
+  c128* S = cp();
+  c128* negTt = cp();
+  c128* Zc = cp();
+  c128* Y = cp();
+  c128* Y2 = cp();
+  const int n = p.n, k = p.k, G = htrd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
+  const long nn = (long)n * n;
+  const int NV = p.npan * kBtNb, nrefl = n - 1;
+  SyevBufs b = syev_carve(p, ws_syev);
+
+  hipError_t he = hipMemsetAsync(err, 0, sizeof(int), st);
+  if (he == hipSuccess && info) he = hipMemsetAsync(info, 0, sizeof(int), st);
+  TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: reset: %s", hipGetErrorString(he));
+  for (int m0 = 0; m0 < h.batch; m0 += h.mpl) {
+    const int cnt = std::min(h.mpl, h.batch - m0);
+    he = hipMemsetAsync(bars, 0, (size_t)cnt * kBarWords * sizeof(unsigned), st);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: barrier reset: %s", hipGetErrorString(he));
+    HtrdArgs args{n, ns, gs, G, A + (long)m0 * sA, lda, sA, Vr, d, e, tau, P, Rp, part, bars, err};
+    void* kargs[] = {&args};
+    he = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&hetrd_coop_kernel),
+                                    dim3(G * cnt), dim3(kHThreads), kargs, 0u, st);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP,
+                "heev: cooperative launch of %d workgroups refused: %s", G * cnt,
+                hipGetErrorString(he));
+    for (int r = 0; r < cnt; ++r) {
+      const long m = m0 + r;
+      int rc = run_dc(p, b, d + (long)r * n, e + (long)r * n, w + m * sw, st);
+      if (rc != TNH_OK) return rc;
+      const long cntz = (long)n * k;
+      hipLaunchKernelGGL(real_to_c_kernel,
+                         dim3((unsigned)std::min<long>((cntz + 255) / 256, 4096)), dim3(256), 0,
+                         st, cntz, b.Zw, Zc);
+      const c128* Vrm = Vr + r * nn;
+      hipLaunchKernelGGL(transpose_c_kernel, dim3((n + 31) / 32, (NV + 31) / 32), dim3(32, 8), 0,
+                         st, n, NV, Vrm, Vc, Vh);
+      TNH_CHECK_LAUNCH("heev transpose");
+      rc = gemm_c(kBtNb, kBtNb, n, Vh, NV, Vc, NV, S, kBtNb, p.npan, kBtNb, kBtNb,
+                  (long)kBtNb * kBtNb, 0, st);
+      if (rc != TNH_OK) return rc;
+      hipLaunchKernelGGL(larft_c_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S,
+                         tau + (long)r * n, negTt);
+      TNH_CHECK_LAUNCH("heev larft");
+      for (int bb = p.npan - 1; bb >= 0; --bb) {
+        const int jb = bb * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
+        rc = gemm_c(nb, k, rows, Vh + (long)jb * NV + jb, NV, Zc + (long)jb * k, k, Y, k, 1, 0,
+                    0, 0, 0, st);
+        if (rc != TNH_OK) return rc;
+        rc = gemm_c(nb, k, nb, negTt + (long)bb * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, 1, 0, 0, 0,
+                    0, st);
+        if (rc != TNH_OK) return rc;
+        rc = gemm_c(rows, k, nb, Vrm + (long)jb * n + jb, n, Y2, k, Zc + (long)jb * k, k, 1, 0,
+                    0, 0, 1, st);
+        if (rc != TNH_OK) return rc;
+      }
+      he = hipMemcpy2DAsync(Z + m * sZ, (size_t)ldz * 16, Zc, (size_t)k * 16, (size_t)k * 16,
+                            (size_t)n, hipMemcpyDeviceToDevice, st);
+      TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: vector copy: %s", hipGetErrorString(he));
+      if (info) {
+        hipLaunchKernelGGL(heev_status_kernel, dim3(1), dim3(256), 0, st, err, b.sweeps,
+                           p.N / p.s, info);
+        TNH_CHECK_LAUNCH("heev status");
+      }
+    }
   }
   return TNH_OK;
 }
@@ -1214,5 +1748,32 @@ extern "C" int tnh_syev_top(int n, int k, const double* A, long lda, double* w, 
   TNH_REQUIRE(ws && ws_bytes >= tnh::syev_ws_bytes(p), TNH_ERR_WS,
               "tnh_syev_top: workspace %zu < %zu bytes", ws_bytes, tnh::syev_ws_bytes(p));
   return tnh::syev_top(p, A, lda, w, Z, ldz, ws, ws_bytes, info,
+                       static_cast<hipStream_t>(stream));
+}
+
+extern "C" size_t tnh_heev_top_ws_bytes(int batch, int n, int k) {
+  if (batch < 1 || n < 2 || n > tnh::kHeevMaxN || k < 1 || k > n) return 0;
+  int mpl = 1;
+  if (tnh::heev_mpl(n, batch, &mpl) != TNH_OK) return 0;
+  return tnh::heev_ws_bytes(tnh::HeevPlan{tnh::syev_plan(n, k), batch, mpl});
+}
+
+extern "C" int tnh_heev_top(int batch, int n, int k, const void* A, long lda, long sA, double* w,
+                            long sw, void* Z, long ldz, long sZ, int* info, void* ws,
+                            size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(n >= 2 && n <= tnh::kHeevMaxN, TNH_ERR_UNSUPPORTED,
+              "tnh_heev_top: n = %d outside 2..%d", n, tnh::kHeevMaxN);
+  TNH_REQUIRE(batch >= 1 && k >= 1 && k <= n && lda >= n && ldz >= k && A && w && Z &&
+                  (batch == 1 || (sA >= lda * (n - 1) + n && sw >= n && sZ >= ldz * (n - 1) + k)),
+              TNH_ERR_ARG, "tnh_heev_top: bad arguments (batch=%d n=%d k=%d lda=%ld ldz=%ld)",
+              batch, n, k, lda, ldz);
+  int mpl = 1;
+  int rc = tnh::heev_mpl(n, batch, &mpl);
+  if (rc != TNH_OK) return rc;
+  const tnh::HeevPlan h{tnh::syev_plan(n, k), batch, mpl};
+  TNH_REQUIRE(ws && ws_bytes >= tnh::heev_ws_bytes(h), TNH_ERR_WS,
+              "tnh_heev_top: workspace %zu < %zu bytes", ws_bytes, tnh::heev_ws_bytes(h));
+  return tnh::heev_top(h, static_cast<const tnh::c128*>(A), lda, sA, w, sw,
+                       static_cast<tnh::c128*>(Z), ldz, sZ, ws, info,
                        static_cast<hipStream_t>(stream));
 }

@@ -312,6 +312,41 @@ def syev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.
     return w, Z
 
 
+HEEV_MAX_N = 1024
+
+
+def heev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Eigendecomposition of a batch of complex Hermitian (B, n, n) complex128 matrices,
+    2 <= n <= 1024, by the library's own solver (complex Householder reduction to a real
+    tridiagonal, divide and conquer, complex WY back-transformation): (w (B, n) ascending,
+    Z (B, n, k) whose column c is the eigenvector of the c-th largest eigenvalue). Both triangles
+    are read as given, so pass Hermitian matrices. With `check_status` the solver's status word
+    is read back (a host sync) and a failed solve raises."""
+    _need(G, "G", 3, (torch.complex128,))
+    B, n = int(G.shape[0]), int(G.shape[1])
+    if G.shape[2] != n or not 2 <= n <= HEEV_MAX_N or B < 1:
+        raise ValueError(f"heev_top: a batch of square matrices of order 2..{HEEV_MAX_N}, "
+                         f"got {tuple(G.shape)}")
+    if not 1 <= int(k) <= n:
+        raise ValueError(f"heev_top: k = {k} outside 1..{n}")
+    lib = _lib.load()
+    w = torch.empty((B, n), dtype=torch.float64, device=G.device)
+    Z = torch.empty((B, n, int(k)), dtype=torch.complex128, device=G.device)
+    info = torch.zeros(1, dtype=torch.int32, device=G.device)
+    with torch.cuda.device(G.device):
+        nb = lib.tnh_heev_top_ws_bytes(B, n, int(k))
+    if nb == 0:
+        raise _lib.HipError(f"heev_top: no workspace size for batch {B}, order {n}, k {k}")
+    ws = _ws(nb, G.device)
+    check(lib.tnh_heev_top(B, n, int(k), ptr(G), n, n * n, ptr(w), n, ptr(Z), int(k),
+                           n * int(k), ptr(info), ptr(ws), ws.numel(), stream(G.device)),
+          "heev_top")
+    if check_status and int(info.item()) != 0:
+        raise _lib.HipError(f"heev_top: the solver reported status {int(info.item())} "
+                            "(1: a workgroup wait ran out, 2: a leaf did not converge)")
+    return w, Z
+
+
 def eigh_top(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor]:
     """(w ascending (n,), Z (n, k) for the k largest eigenvalues, descending) of a symmetric
     float64 (n, n) matrix on the device: the batched Jacobi kernel up to order 64, the library's
