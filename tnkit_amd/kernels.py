@@ -315,13 +315,14 @@ def syev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.
 HEEV_MAX_N = 1024
 
 
-def heev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+def heev_top_async(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
     """Eigendecomposition of a batch of complex Hermitian (B, n, n) complex128 matrices,
     2 <= n <= 1024, by the library's own solver (complex Householder reduction to a real
-    tridiagonal, divide and conquer, complex WY back-transformation): (w (B, n) ascending,
-    Z (B, n, k) whose column c is the eigenvector of the c-th largest eigenvalue). Both triangles
-    are read as given, so pass Hermitian matrices. With `check_status` the solver's status word
-    is read back (a host sync) and a failed solve raises."""
+    tridiagonal, divide and conquer, complex WY back-transformation), without a host sync:
+    (w (B, n) ascending, Z (B, n, k) whose column c is the eigenvector of the c-th largest
+    eigenvalue, info (1,) int32 on the device: 0, or bit 0 when a reduction's bounded wait ran
+    out, bit 1 when a leaf eigensolve did not converge, over the whole batch). Both triangles are
+    read as given, so pass Hermitian matrices."""
     _need(G, "G", 3, (torch.complex128,))
     B, n = int(G.shape[0]), int(G.shape[1])
     if G.shape[2] != n or not 2 <= n <= HEEV_MAX_N or B < 1:
@@ -341,7 +342,13 @@ def heev_top(G: torch.Tensor, k: int, check_status: bool = True) -> Tuple[torch.
     check(lib.tnh_heev_top(B, n, int(k), ptr(G), n, n * n, ptr(w), n, ptr(Z), int(k),
                            n * int(k), ptr(info), ptr(ws), ws.numel(), stream(G.device)),
           "heev_top")
-    if check_status and int(info.item()) != 0:
+    return w, Z, info
+
+
+def heev_top(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor]:
+    """`heev_top_async` followed by a read of its status word: raises if the solve failed."""
+    w, Z, info = heev_top_async(G, k)
+    if int(info.item()) != 0:
         raise _lib.HipError(f"heev_top: the solver reported status {int(info.item())} "
                             "(1: a workgroup wait ran out, 2: a leaf did not converge)")
     return w, Z

@@ -255,42 +255,58 @@ def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
     return M
 
 
-def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None, deferred: bool = False):
+def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None, deferred: bool = False,
+                 k: Optional[int] = None):
     """Batched eigh (ascending) of the Hermitian positive semidefinite G + diag(add) on the
     device. Orders up to 64 go to the one-workgroup-per-matrix Jacobi kernel
-    (`kernels.eigh_jacobi`); larger ones, and any matrix the kernel leaves unconverged, to
+    (`kernels.eigh_jacobi`); complex orders 65..1024 to the library's batched divide-and-conquer
+    solver (`kernels.heev_top_async`), which returns only the k leading eigenvectors when k is
+    given (Z then has k columns, those of the k largest eigenvalues, ascending like the rest);
+    other orders, and any matrix a library solver leaves unconverged or non-finite, go to
     rocSOLVER's batched eigh. That one returns non-finite eigenvectors for complex matrices with
     exactly decoupled rows (a leg's padded rows), so the matrices it gets carry a 1e-30 coupling
     on the first off-diagonals (`_couple`); a leg that still comes back non-finite is decomposed
     by an SVD of the unshifted G (G = U diag(S) U^H for a PSD matrix), whose eigenvalues it
-    returns ascending like eigh. Raises if a leg stays non-finite. With `deferred`, a Jacobi result is returned with
-    the device mask of legs to redo, (w, Z, redo), and the caller, which reads w back anyway,
-    finishes with `_eigh_redo`; otherwise (w, Z) after the check here."""
-    if (G.shape[-1] <= kernels.EIGH_JACOBI_MAX_N and G.dtype in (torch.complex128, torch.float64)
+    returns ascending like eigh. Raises if a leg stays non-finite. With `deferred`, a library
+    result is returned with the device mask of legs to redo, (w, Z, redo), and the caller, which
+    reads w back anyway, finishes with `_eigh_redo`; otherwise (w, Z) after the check here."""
+    r = int(G.shape[-1])
+    if (r <= kernels.EIGH_JACOBI_MAX_N and G.dtype in (torch.complex128, torch.float64)
             and _JACOBI):
         w, Z, sw = kernels.eigh_jacobi(G.contiguous(), None if add is None
                                        else add.to(torch.float64).contiguous())
         redo = (sw < 0) | (~torch.isfinite(Z).flatten(1).all(1)) | (~torch.isfinite(w).all(1))
-        if deferred:
-            return w, Z, redo
-        if not bool(redo.any()):
-            return w, Z
-        idx = torch.nonzero(redo).flatten()
-        w[idx], Z[idx] = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
+    elif G.dtype == torch.complex128 and r <= kernels.HEEV_MAX_N and _HEEV:
+        kk = r if k is None else max(1, min(int(k), r))
+        Gs = G if add is None else G + torch.diag_embed(add.to(G.dtype))
+        w, Zt, info = kernels.heev_top_async(Gs.contiguous(), kk)
+        Z = torch.flip(Zt, dims=[2])
+        redo = ((info != 0).expand(G.shape[0]) | (~torch.isfinite(Z).flatten(1).all(1))
+                | (~torch.isfinite(w).all(1)))
+    else:
+        w, Z = _eigh_robust_vendor(G, add)
+        return (w, Z, None) if deferred else (w, Z)
+    if deferred:
+        return w, Z, redo
+    if not bool(redo.any()):
         return w, Z
-    w, Z = _eigh_robust_vendor(G, add)
-    return (w, Z, None) if deferred else (w, Z)
+    idx = torch.nonzero(redo).flatten()
+    wv, Zv = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
+    w[idx], Z[idx] = wv, Zv[:, :, r - Z.shape[2]:]
+    return w, Z
 
 
 def _eigh_redo(G: torch.Tensor, add: Optional[torch.Tensor], w: torch.Tensor, Z: torch.Tensor,
                redo: np.ndarray) -> None:
-    """Replace in place the legs of a deferred `_eigh_robust` that the Jacobi kernel left
+    """Replace in place the legs of a deferred `_eigh_robust` that a library solver left
     unconverged or non-finite (host mask) by the vendor eigensolver's."""
     idx = torch.as_tensor(np.nonzero(redo)[0], device=G.device)
-    w[idx], Z[idx] = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
+    wv, Zv = _eigh_robust_vendor(G[idx], None if add is None else add[idx])
+    w[idx], Z[idx] = wv, Zv[:, :, Zv.shape[2] - Z.shape[2]:]
 
 
 _JACOBI = os.environ.get("TNKIT_AMD_JACOBI", "1") != "0"
+_HEEV = os.environ.get("TNKIT_AMD_HEEV", "1") != "0"
 
 
 def _couple(Gs: torch.Tensor) -> torch.Tensor:
@@ -380,7 +396,10 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
     dg = torch.diagonal(G, dim1=1, dim2=2).real
     shift = -(1.0 + torch.arange(r_, device=G.device, dtype=torch.float64) / r_)
     add = torch.where(dg == 0, shift[None, :], torch.zeros_like(dg))
-    w, Z, redo = _eigh_robust(G, add, deferred=True)
+    # only the leading vectors any leg can keep are formed (the stacked width below is at most
+    # this many)
+    kz = min(r_, max(int(np.minimum(ranks_cap, ceiling).max()), int(width), 1))
+    w, Z, redo = _eigh_robust(G, add, deferred=True, k=kz)
     # one host read: the spectra, and which legs the Jacobi kernel asks to redo
     both = torch.cat([w, redo.to(w.dtype)[:, None]], 1) if redo is not None else w
     both = both.cpu().numpy()
