@@ -376,6 +376,20 @@ int tnh_heev_top(int batch, int n, int k, const void* A, long lda, long sA, doub
                  void* Z, long ldz, long sZ, int* info, void* ws, size_t ws_bytes,
                  tnh_stream_t stream);
 
+/* Thin QR A = Q R of a batch of tall float64 (TNH_F64, m <= 2048) or complex128 (TNH_C128,
+ * m <= 1024) matrices, 1 <= n <= m: the QR that re-orthonormalises the projected factor A Z of
+ * tnkit's Gram truncation when the Gram matrix is taken on the short side (tnkit/linalg.py:155-159,
+ * `Q, R = be.qr(A @ Z)`; tnkit/blocks.py:1345-1377 per sector). Matrix q is A + q*sA (m x n,
+ * row stride lda). LAPACK geqrf's reflectors (R with a real diagonal, the R geqrf returns) in one
+ * cooperative launch per group of resident matrices, Q = H_0 ... H_{n-1} [I; 0] (ungqr's Q) by
+ * WY blocks on tnh_gemm_tn. Writes Q + q*sQ (m x n, row stride ldq) and R + q*sR (n x n upper,
+ * zeros below, row stride ldr). info (device int, may be NULL): 0, or 1 when a bounded wait ran
+ * out. The workspace size depends on the device: query it on the device of the call. */
+size_t tnh_qr_ws_bytes(int dtype, int batch, int m, int n);
+int tnh_qr(int dtype, int batch, int m, int n, const void* A, long lda, long sA, void* Q, long ldq,
+           long sQ, void* R, long ldr, long sR, int* info, void* ws, size_t ws_bytes,
+           tnh_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -87,6 +87,9 @@ SIGNATURES = {
     "tnh_syev_top_ws_bytes": (c_size_t, [c_int, c_int]),
     "tnh_syev_top": (c_int, [c_int, c_int, _P, c_long, _P, _P, c_long, _P, _P, c_size_t, _P]),
     "tnh_heev_top_ws_bytes": (c_size_t, [c_int, c_int, c_int]),
+    "tnh_qr_ws_bytes": (c_size_t, [c_int, c_int, c_int, c_int]),
+    "tnh_qr": (c_int, [c_int, c_int, c_int, c_int, _P, c_long, c_long, _P, c_long, c_long, _P,
+                       c_long, c_long, _P, _P, c_size_t, _P]),
     "tnh_heev_top": (c_int, [c_int, c_int, c_int, _P, c_long, c_long, _P, c_long, _P, c_long,
                              c_long, _P, _P, c_size_t, _P]),
 }

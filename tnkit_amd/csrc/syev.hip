@@ -1705,6 +1705,364 @@ int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, lon
   return TNH_OK;
 }
 
+// ---- thin QR of tall matrices (float64 and complex128) -------------------------------------------
+// A = Q R (m x n, n <= m <= 2048, n <= 2048) with LAPACK's geqrf reflectors (H_j = I - tau_j v_j
+// v_j^H, v_j[j] = 1, H_j^H (alpha; x) = (beta; 0), beta real, so R has a real diagonal and is
+// the R that geqrf returns), and Q = H_0 ... H_{n-1} [I; 0] (ungqr's Q) by WY blocks. The
+// reduction is one cooperative launch per group of resident matrices: workgroup g of a matrix
+// holds columns g, g + G, ... of A (as rows of A^H, conjugated) in registers, 8 of them, each
+// thread two of the m entries. Column j: its owner forms the reflector from its register copy
+// and publishes v_j (one grid barrier), then every workgroup applies H_j^H to its own columns
+// right of j (a dot product with v_j and an update per column, reduced within the workgroup),
+// and the owner of column j+1 forms the next reflector from its updated copy at once.
+__device__ __forceinline__ double sre(double x) { return x; }
+__device__ __forceinline__ double sre(c128 x) { return x.re; }
+__device__ __forceinline__ double sim(double) { return 0.0; }
+__device__ __forceinline__ double sim(c128 x) { return x.im; }
+__device__ __forceinline__ double cj(double x) { return x; }
+__device__ __forceinline__ c128 cj(c128 x) { return conj(x); }
+__device__ __forceinline__ double nrm2(double x) { return x * x; }
+__device__ __forceinline__ double nrm2(c128 x) { return cabs2(x); }
+template <typename T>
+__device__ __forceinline__ T mk(double re, double im);
+template <>
+__device__ __forceinline__ double mk<double>(double re, double) { return re; }
+template <>
+__device__ __forceinline__ c128 mk<c128>(double re, double im) { return c128{re, im}; }
+__device__ __forceinline__ void st_shared_s(double* p, double v) { st_shared(p, v); }
+__device__ __forceinline__ void st_shared_s(c128* p, c128 v) { st_shared_c(p, v); }
+
+constexpr int kQrRows = 8;   // columns of A per workgroup
+
+template <typename T>
+struct QrArgs {
+  int m, n, G;           // A is m x n; G workgroups per matrix
+  const T* A;            // matrix q of the launch at A + q sA, row stride lda
+  long lda, sA;
+  T* V;                  // per matrix n slots of ldv: v_j (zero above j, 1 at j), tau_j at m
+  int ldv;               // slot length: m + 1 rounded up to 32 elements (whole cache lines)
+  T* tau;                // per matrix n: tau_j again, for the host side
+  T* R;                  // per matrix n x n at R + q sR, row stride ldr (upper; zeros below)
+  long ldr, sR;
+  unsigned* bar;         // per matrix kBarWords
+  int* err;
+};
+
+template <typename T, int NT, int C>
+__global__ void __launch_bounds__(NT) geqrf_coop_kernel(QrArgs<T> a) {
+  constexpr int NW = NT / 64, RW = kQrRows;
+  constexpr bool kCplx = sizeof(T) == 16;
+  __shared__ double red[2 * NW];
+  __shared__ double red8[NW][2 * RW];
+  __shared__ double dots[2 * RW];
+  __shared__ double al_re, al_im;
+  __shared__ int flag;
+  const int m = a.m, n = a.n, G = a.G, mb = blockIdx.x / G, g = blockIdx.x % G, t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  const int nr = (n - g + G - 1) / G;
+  const T* A = a.A + mb * a.sA;
+  const int ldv = a.ldv;
+  T* V = a.V + (long)mb * n * ldv;
+  T* tau = a.tau + (long)mb * n;
+  T* R = a.R + mb * a.sR;
+  const GridBar bar{a.bar + (size_t)mb * kBarWords, a.err};
+  int par = 0;
+
+  T x[RW][C];   // x[q][c] = conj(A[l][g + q G]) at l = t + NT c
+#pragma unroll
+  for (int q = 0; q < RW; ++q)
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int l = t + NT * c;
+      x[q][c] = (q < nr && l < m) ? cj(A[(long)l * a.lda + g + q * G]) : T{};
+    }
+
+  // the owner of column p forms reflector p from its register copy, publishes v_p and tau_p,
+  // writes column p of R and leaves (beta, 0, ...) in its registers
+  auto pivot = [&](int p) {
+    const int qp = (p - g) / G;
+    double sq = 0.0;
+#pragma unroll
+    for (int q = 0; q < RW; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int l = t + NT * c;
+        if (q == qp && l > p && l < m) sq += nrm2(x[q][c]);
+        if (q == qp && l == p) {
+          al_re = sre(x[q][c]);
+          al_im = -sim(x[q][c]);
+        }
+      }
+    double v = wsum(sq);
+    double* rb = red + NW * par;
+    par ^= 1;
+    if (lane == 0) rb[wv] = v;
+    __syncthreads();
+    double sigma = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) sigma += rb[w];
+    const double ar = al_re, ai = al_im;
+    double beta = ar;
+    T tj{}, scale{};
+    if (sigma > 0.0 || ai != 0.0) {
+      beta = -copysign(sqrt(ar * ar + ai * ai + sigma), ar);
+      tj = mk<T>((beta - ar) / beta, -ai / beta);
+      const double dr = ar - beta, di = ai, qq = 1.0 / (dr * dr + di * di);
+      scale = mk<T>(dr * qq, -di * qq);
+    }
+#pragma unroll
+    for (int q = 0; q < RW; ++q)
+#pragma unroll
+      for (int c = 0; c < C; ++c) {
+        const int l = t + NT * c;
+        if (q == qp) {
+          const T vv = l == p ? mk<T>(1.0, 0.0) : (l > p && l < m ? cj(x[q][c]) * scale : T{});
+          if (l < m) st_shared_s(V + (long)p * ldv + l, vv);
+          if (l < n) R[(long)l * a.ldr + p] = l < p ? cj(x[q][c]) : (l == p ? mk<T>(beta, 0.0) : T{});
+          x[q][c] = l == p ? mk<T>(beta, 0.0) : (l > p ? T{} : x[q][c]);
+        }
+      }
+    if (t == 0) {
+      st_shared_s(V + (long)p * ldv + m, tj);
+      tau[p] = tj;
+    }
+  };
+
+  if (g == 0) pivot(0);
+  if (n == 1) return;
+  unsigned epoch = 1;
+  if (!grid_sync_of(bar, epoch++, &flag, G, g)) return;
+  for (int j = 0; j < n - 1; ++j) {
+    // apply H_j^H to the own columns right of j: x -= tau_j (x . v_j) conj(v_j)
+    T vj[C];
+#pragma unroll
+    for (int c = 0; c < C; ++c) {
+      const int l = t + NT * c;
+      vj[c] = (l >= j && l < m) ? V[(long)j * ldv + l] : T{};
+    }
+    const T tj = V[(long)j * ldv + m];
+    double s_re[RW], s_im[RW];
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+      T acc{};
+#pragma unroll
+      for (int c = 0; c < C; ++c) acc = acc + x[q][c] * vj[c];
+      const bool live = q < nr && g + q * G > j;
+      s_re[q] = live ? sre(acc) : 0.0;
+      s_im[q] = live ? sim(acc) : 0.0;
+    }
+    wsum8_store(s_re, red8[wv], lane);
+    if constexpr (kCplx) wsum8_store(s_im, red8[wv] + RW, lane);
+    __syncthreads();
+    if (t < (kCplx ? 2 * RW : RW)) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sum += red8[w][t];
+      dots[t] = sum;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < RW; ++q) {
+      const bool live = q < nr && g + q * G > j;
+      const T f = tj * mk<T>(dots[q], kCplx ? dots[RW + q] : 0.0);
+#pragma unroll
+      for (int c = 0; c < C; ++c)
+        if (live) x[q][c] = x[q][c] - f * cj(vj[c]);
+    }
+    if (g == (j + 1) % G) pivot(j + 1);
+    if (j + 1 < n - 1 && !grid_sync_of(bar, epoch++, &flag, G, g)) return;
+  }
+}
+
+// Vc[i][j] = V[j][i], Vh[i][j] = conj(V[j][i]) for reflectors j < nrefl (rows of length m),
+// zero for nrefl <= j < NV (Vh may be null for real matrices); reflector j at src + j lds.
+template <typename T>
+__global__ void qr_transpose_kernel(int m, int nrefl, int NV, const T* __restrict__ src, int lds,
+                                    T* __restrict__ Vc, T* __restrict__ Vh) {
+  __shared__ double tre[32][33], tim[32][33];
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int j = by + r, i = bx + threadIdx.x;
+    const T v = (j < nrefl && i < m) ? src[(long)j * lds + i] : T{};
+    tre[r][threadIdx.x] = sre(v);
+    tim[r][threadIdx.x] = sim(v);
+  }
+  __syncthreads();
+  for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+    const int i = bx + r, j = by + threadIdx.x;
+    if (i < m && j < NV) {
+      const double vr = tre[threadIdx.x][r], vi = tim[threadIdx.x][r];
+      Vc[(long)i * NV + j] = mk<T>(vr, vi);
+      if (Vh) Vh[(long)i * NV + j] = mk<T>(vr, -vi);
+    }
+  }
+}
+
+// Q (m x n, row stride ldq, cnt matrices at stride sQ) = [I; 0].
+template <typename T>
+__global__ void eye_kernel(int m, int n, T* Q, long ldq, long sQ) {
+  T* Qm = Q + blockIdx.y * sQ;
+  const long cnt = (long)m * n;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < cnt; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / n), c = (int)(i % n);
+    Qm[(long)r * ldq + c] = mk<T>(r == c ? 1.0 : 0.0, 0.0);
+  }
+}
+
+__global__ void qr_status_kernel(const int* err, int* info) {
+  if (threadIdx.x == 0) *info |= *err ? 1 : 0;
+}
+
+// Threads per workgroup for rows of length m (two entries per thread): complex rows of more than
+// 1024 entries would not fit the registers and are refused by the entry.
+template <typename T>
+int qr_threads(int m) {
+  return m <= 512 ? 256 : (m <= 1024 ? 512 : 1024);
+}
+
+template <typename T>
+int qr_max_m() {
+  return sizeof(T) == 16 ? 1024 : 2048;
+}
+
+template <typename T>
+const void* qr_kernel(int m) {
+  if (m <= 512) return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 256, 2>);
+  if constexpr (sizeof(T) == 8) {
+    if (m > 1024) return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 1024, 2>);
+  }
+  return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 512, 2>);
+}
+
+int qr_grid(int n) { return (n + kQrRows - 1) / kQrRows; }
+
+// Matrices per reduction launch (all of them resident at once).
+template <typename T>
+int qr_mpl(int m, int n, int batch, int* mpl) {
+  int dev = 0, cus = 0, per_cu = 0;
+  const int nt = qr_threads<T>(m);
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, qr_kernel<T>(m), nt, 0) !=
+          hipSuccess) {
+    set_error("qr: device query failed");
+    return TNH_ERR_HIP;
+  }
+  const int cap = (cus * per_cu) / qr_grid(n);
+  TNH_REQUIRE(cap >= 1, TNH_ERR_UNSUPPORTED,
+              "qr: the %d workgroups of one matrix are not resident at once", qr_grid(n));
+  *mpl = std::min(cap, batch);
+  return TNH_OK;
+}
+
+int qr_ldv(int m) { return (m + 1 + 31) / 32 * 32; }
+
+template <typename T, typename F>
+void qr_pieces(int m, int n, int mpl, F&& f) {
+  const size_t npan = (n + kBtNb - 1) / kBtNb, NV = npan * kBtNb, B = mpl, E = sizeof(T);
+  f(B * n * qr_ldv(m) * E);                  // V slots
+  f(B * n * E);                              // tau
+  f(B * kBarWords * sizeof(unsigned));       // barrier words
+  f(256);                                    // err
+  f(B * m * NV * E);                         // Vc
+  f(E == 16 ? B * m * NV * E : 0);           // Vh (complex only)
+  f(B * npan * kBtNb * kBtNb * E);           // S
+  f(B * npan * kBtNb * kBtNb * E);           // negTt
+  f(B * kBtNb * n * E);                      // Y
+  f(B * kBtNb * n * E);                      // Y2
+}
+
+template <typename T>
+size_t qr_ws_bytes(int m, int n, int mpl) {
+  size_t tot = 0;
+  qr_pieces<T>(m, n, mpl, [&](size_t b) { tot += align_up(b, 256); });
+  return tot + 256;
+}
+
+inline void larft_any(int np, int nrefl, const double* S, const double* tau, double* negTt,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(larft_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt);
+}
+inline void larft_any(int np, int nrefl, const c128* S, const c128* tau, c128* negTt,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(larft_c_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt);
+}
+
+template <typename T>
+int qr_run(int dtype, int batch, int m, int n, int mpl, const T* A, long lda, long sA, T* Q,
+           long ldq, long sQ, T* R, long ldr, long sR, int* info, void* ws, hipStream_t st) {
+  std::vector<char*> ptrs;
+  {
+    char* base = reinterpret_cast<char*>(align_up(reinterpret_cast<size_t>(ws), 256));
+    size_t off = 0;
+    qr_pieces<T>(m, n, mpl, [&](size_t b) {
+      ptrs.push_back(base + off);
+      off += align_up(b, 256);
+    });
+  }
+  T* V = reinterpret_cast<T*>(ptrs[0]);
+  T* tau = reinterpret_cast<T*>(ptrs[1]);
+  unsigned* bars = reinterpret_cast<unsigned*>(ptrs[2]);
+  int* err = reinterpret_cast<int*>(ptrs[3]);
+  T* Vc = reinterpret_cast<T*>(ptrs[4]);
+  T* Vh = sizeof(T) == 16 ? reinterpret_cast<T*>(ptrs[5]) : Vc;
+  T* S = reinterpret_cast<T*>(ptrs[6]);
+  T* negTt = reinterpret_cast<T*>(ptrs[7]);
+  T* Y = reinterpret_cast<T*>(ptrs[8]);
+  T* Y2 = reinterpret_cast<T*>(ptrs[9]);
+  const int G = qr_grid(n), nt = qr_threads<T>(m), npan = (n + kBtNb - 1) / kBtNb;
+  const int NV = npan * kBtNb, ldv = qr_ldv(m);
+  const long sV = (long)n * ldv, sVc = (long)m * NV, sT = (long)npan * kBtNb * kBtNb,
+             sY = (long)kBtNb * n;
+  hipError_t he = hipMemsetAsync(err, 0, sizeof(int), st);
+  if (he == hipSuccess && info) he = hipMemsetAsync(info, 0, sizeof(int), st);
+  TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "qr: reset: %s", hipGetErrorString(he));
+  for (int m0 = 0; m0 < batch; m0 += mpl) {
+    const int cnt = std::min(mpl, batch - m0);
+    he = hipMemsetAsync(bars, 0, (size_t)cnt * kBarWords * sizeof(unsigned), st);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "qr: barrier reset: %s", hipGetErrorString(he));
+    QrArgs<T> args{m, n, G, A + (long)m0 * sA, lda, sA, V, ldv, tau, R + (long)m0 * sR, ldr, sR,
+                   bars, err};
+    void* kargs[] = {&args};
+    he = hipLaunchCooperativeKernel(qr_kernel<T>(m), dim3(G * cnt), dim3(nt), kargs, 0u, st);
+    TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "qr: cooperative launch of %d workgroups: %s",
+                G * cnt, hipGetErrorString(he));
+    T* Qm = Q + (long)m0 * sQ;
+    hipLaunchKernelGGL(eye_kernel<T>, dim3(std::min(((long)m * n + 255) / 256, 1024L), cnt),
+                       dim3(256), 0, st, m, n, Qm, ldq, sQ);
+    TNH_CHECK_LAUNCH("qr eye");
+    for (int q = 0; q < cnt; ++q) {
+      hipLaunchKernelGGL(qr_transpose_kernel<T>, dim3((m + 31) / 32, (NV + 31) / 32), dim3(32, 8),
+                         0, st, m, n, NV, V + q * sV, ldv, Vc + q * sVc,
+                         sizeof(T) == 16 ? Vh + q * sVc : nullptr);
+      TNH_CHECK_LAUNCH("qr transpose");
+      int rc = tnh_gemm_tn(dtype, kBtNb, kBtNb, m, Vh + q * sVc, NV, Vc + q * sVc, NV, S + q * sT,
+                           kBtNb, npan, kBtNb, kBtNb, (long)kBtNb * kBtNb, 0, 0, st);
+      if (rc != TNH_OK) return rc;
+      larft_any(npan, n, S + q * sT, tau + (long)q * n, negTt + q * sT, st);
+      TNH_CHECK_LAUNCH("qr larft");
+    }
+    // Q <- H_0 ... H_{n-1} [I; 0], last block first; block b changes only columns >= jb
+    for (int bb = npan - 1; bb >= 0; --bb) {
+      const int jb = bb * kBtNb, nb = std::min(kBtNb, n - jb), rows = m - jb, nc = n - jb;
+      int rc = tnh_gemm_tn(dtype, nb, nc, rows, Vh + (long)jb * NV + jb, NV,
+                           Qm + (long)jb * ldq + jb, ldq, Y, nc, cnt, sVc, sQ, sY, 0, 0, st);
+      if (rc != TNH_OK) return rc;
+      rc = tnh_gemm_tn(dtype, nb, nc, nb, negTt + (long)bb * kBtNb * kBtNb, kBtNb, Y, nc, Y2, nc,
+                       cnt, sT, sY, sY, 0, 0, st);
+      if (rc != TNH_OK) return rc;
+      rc = tnh_gemm_tn(dtype, rows, nc, nb, V + (long)jb * ldv + jb, ldv, Y2, nc,
+                       Qm + (long)jb * ldq + jb, ldq, cnt, sV, sY, sQ, 1, 0, st);
+      if (rc != TNH_OK) return rc;
+    }
+  }
+  if (info) {
+    hipLaunchKernelGGL(qr_status_kernel, dim3(1), dim3(64), 0, st, err, info);
+    TNH_CHECK_LAUNCH("qr status");
+  }
+  return TNH_OK;
+}
+
 }  // namespace tnh
 
 extern "C" size_t tnh_sytrd_ws_bytes(int n) {
@@ -1776,4 +2134,52 @@ extern "C" int tnh_heev_top(int batch, int n, int k, const void* A, long lda, lo
   return tnh::heev_top(h, static_cast<const tnh::c128*>(A), lda, sA, w, sw,
                        static_cast<tnh::c128*>(Z), ldz, sZ, ws, info,
                        static_cast<hipStream_t>(stream));
+}
+
+extern "C" size_t tnh_qr_ws_bytes(int dtype, int batch, int m, int n) {
+  if (batch < 1 || n < 1 || n > m) return 0;
+  int mpl = 1;
+  if (dtype == TNH_F64) {
+    if (m > tnh::qr_max_m<double>() || tnh::qr_mpl<double>(m, n, batch, &mpl) != TNH_OK) return 0;
+    return tnh::qr_ws_bytes<double>(m, n, mpl);
+  }
+  if (dtype == TNH_C128) {
+    if (m > tnh::qr_max_m<tnh::c128>() || tnh::qr_mpl<tnh::c128>(m, n, batch, &mpl) != TNH_OK)
+      return 0;
+    return tnh::qr_ws_bytes<tnh::c128>(m, n, mpl);
+  }
+  return 0;
+}
+
+extern "C" int tnh_qr(int dtype, int batch, int m, int n, const void* A, long lda, long sA,
+                      void* Q, long ldq, long sQ, void* R, long ldr, long sR, int* info, void* ws,
+                      size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(dtype == TNH_F64 || dtype == TNH_C128, TNH_ERR_UNSUPPORTED,
+              "tnh_qr: dtype %d (float64 or complex128 only)", dtype);
+  const int mmax = dtype == TNH_F64 ? tnh::qr_max_m<double>() : tnh::qr_max_m<tnh::c128>();
+  TNH_REQUIRE(n >= 1 && n <= m && m <= mmax, TNH_ERR_UNSUPPORTED,
+              "tnh_qr: %d x %d outside 1 <= n <= m <= %d", m, n, mmax);
+  TNH_REQUIRE(batch >= 1 && A && Q && R && lda >= n && ldq >= n && ldr >= n &&
+                  (batch == 1 || (sA >= lda * (m - 1) + n && sQ >= ldq * (m - 1) + n &&
+                                  sR >= ldr * (n - 1) + n)),
+              TNH_ERR_ARG, "tnh_qr: bad arguments (batch=%d m=%d n=%d lda=%ld ldq=%ld ldr=%ld)",
+              batch, m, n, lda, ldq, ldr);
+  const size_t need = tnh_qr_ws_bytes(dtype, batch, m, n);
+  TNH_REQUIRE(need > 0, TNH_ERR_HIP, "tnh_qr: no workspace size on this device");
+  TNH_REQUIRE(ws && ws_bytes >= need, TNH_ERR_WS, "tnh_qr: workspace %zu < %zu bytes", ws_bytes,
+              need);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int mpl = 1;
+  if (dtype == TNH_F64) {
+    int rc = tnh::qr_mpl<double>(m, n, batch, &mpl);
+    if (rc != TNH_OK) return rc;
+    return tnh::qr_run<double>(dtype, batch, m, n, mpl, static_cast<const double*>(A), lda, sA,
+                               static_cast<double*>(Q), ldq, sQ, static_cast<double*>(R), ldr, sR,
+                               info, ws, st);
+  }
+  int rc = tnh::qr_mpl<tnh::c128>(m, n, batch, &mpl);
+  if (rc != TNH_OK) return rc;
+  return tnh::qr_run<tnh::c128>(dtype, batch, m, n, mpl, static_cast<const tnh::c128*>(A), lda,
+                                sA, static_cast<tnh::c128*>(Q), ldq, sQ,
+                                static_cast<tnh::c128*>(R), ldr, sR, info, ws, st);
 }

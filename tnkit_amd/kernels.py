@@ -354,6 +354,46 @@ def heev_top(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor]:
     return w, Z
 
 
+QR_MAX_M = {torch.float64: 2048, torch.complex128: 1024}
+
+
+def qr_supported(A: torch.Tensor) -> bool:
+    """Whether `qr` serves A: float64 rows up to 2048 or complex128 rows up to 1024, 1 <= n <= m."""
+    if A.dtype not in QR_MAX_M or A.dim() not in (2, 3):
+        return False
+    m, n = int(A.shape[-2]), int(A.shape[-1])
+    return 1 <= n <= m <= QR_MAX_M[A.dtype] and (A.dim() == 2 or int(A.shape[0]) >= 1)
+
+
+def qr(A: torch.Tensor, check_status: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Thin QR of a tall (m, n) or batched (B, m, n) float64 / complex128 matrix on the library's
+    kernels: LAPACK geqrf's reflectors in one cooperative launch per group of resident matrices,
+    Q = H_0 ... H_{n-1} [I; 0] by WY blocks. Returns (Q (…, m, n), R (…, n, n) upper with a real
+    diagonal), the factors numpy's and torch's QR return up to rounding. With `check_status` the
+    status word is read back (a host sync) and a failed reduction raises."""
+    if not qr_supported(A):
+        raise ValueError(f"qr: float64 (m <= 2048) or complex128 (m <= 1024) with n <= m, got "
+                         f"{A.dtype} {tuple(A.shape)}")
+    one = A.dim() == 2
+    Ab = (A.unsqueeze(0) if one else A).resolve_conj().resolve_neg().contiguous()
+    _need(Ab, "A", 3, tuple(QR_MAX_M))
+    B, m, n = (int(x) for x in Ab.shape)
+    lib = _lib.load()
+    Q = torch.empty((B, m, n), dtype=A.dtype, device=A.device)
+    R = torch.empty((B, n, n), dtype=A.dtype, device=A.device)
+    info = torch.zeros(1, dtype=torch.int32, device=A.device)
+    with torch.cuda.device(A.device):
+        nb = lib.tnh_qr_ws_bytes(dcode(A.dtype), B, m, n)
+    if nb == 0:
+        raise _lib.HipError(f"qr: no workspace size for batch {B}, {m} x {n}")
+    ws = _ws(nb, A.device)
+    check(lib.tnh_qr(dcode(A.dtype), B, m, n, ptr(Ab), n, m * n, ptr(Q), n, m * n, ptr(R), n,
+                     n * n, ptr(info), ptr(ws), ws.numel(), stream(A.device)), "qr")
+    if check_status and int(info.item()) != 0:
+        raise _lib.HipError("qr: a workgroup wait of the reduction ran out")
+    return (Q[0], R[0]) if one else (Q, R)
+
+
 def eigh_top(G: torch.Tensor, k: int) -> Tuple[torch.Tensor, torch.Tensor]:
     """(w ascending (n,), Z (n, k) for the k largest eigenvalues, descending) of a symmetric
     float64 (n, n) matrix on the device: the batched Jacobi kernel up to order 64, the library's
