@@ -358,8 +358,9 @@ QR_MAX_M = {torch.float64: 2048, torch.complex128: 1024}
 
 
 def qr_supported(A: torch.Tensor) -> bool:
-    """Whether `qr` serves A: float64 rows up to 2048 or complex128 rows up to 1024, 1 <= n <= m."""
-    if A.dtype not in QR_MAX_M or A.dim() not in (2, 3):
+    """Whether `qr` serves A: a device tensor, float64 rows up to 2048 or complex128 rows up to
+    1024, 1 <= n <= m."""
+    if A.device.type != "cuda" or A.dtype not in QR_MAX_M or A.dim() not in (2, 3):
         return False
     m, n = int(A.shape[-2]), int(A.shape[-1])
     return 1 <= n <= m <= QR_MAX_M[A.dtype] and (A.dim() == 2 or int(A.shape[0]) >= 1)

@@ -3,7 +3,9 @@ randomized range-finder SVD (tnkit/linalg.py:81-135).
 
 The Gram matrix is one libtnkit_hip K-major GEMM; its eigendecomposition (order at most 2 chi)
 is the library's own: the batched Jacobi kernel up to order 64, the divide-and-conquer solver
-(`kernels.syev_top`) up to 2048, which also returns only the kept vectors. The range finder's
+(`kernels.syev_top`) up to 2048, which also returns only the kept vectors; the QR that
+re-orthonormalises the projected factor on the long side is the library's (`kernels.qr`). The
+range finder's
 sketch and power products are the same K-major GEMM, its orthonormalisations Gram passes with
 the small eigendecompositions on the same solvers, and the core SVD the one-sided Jacobi kernel
 or the Gram truncation (rocSOLVER only for complex cores above order 64 and orders above 2048).
@@ -36,7 +38,8 @@ def _gram_left(A: torch.Tensor, chi: int):
     wh = torch.flip(w, dims=[0]).cpu().numpy()
     if m <= n:
         return Z, Z.t() @ A, wh, k
-    Q, Rr = torch.linalg.qr(A @ Z)
+    AZ = A @ Z
+    Q, Rr = kernels.qr(AZ) if kernels.qr_supported(AZ) else torch.linalg.qr(AZ)
     return Q, Rr @ Z.t(), wh, k
 
 
@@ -99,7 +102,8 @@ def _orthonormal(Y: torch.Tensor) -> torch.Tensor:
     at rounding level come out as orthonormal rounding noise, as the QR factor tnkit takes
     completes a rank-deficient sketch. Passes stop once max |Q^H Q - 1| <= 180 eps (at most
     eight). A sketch this cannot orthonormalise (exactly zero, or exactly dependent columns
-    that stay dependent) gets the rocSOLVER QR."""
+    that stay dependent) gets a Householder QR (`kernels.qr`; rocSOLVER's in single precision or
+    beyond its row limits)."""
     m, l = int(Y.shape[0]), int(Y.shape[1])
     eps = float(torch.finfo(Y.dtype).eps)
     wide = torch.complex128 if Y.is_complex() else torch.float64
@@ -120,7 +124,7 @@ def _orthonormal(Y: torch.Tensor) -> torch.Tensor:
         shift = min(c * float(wh[0]), 0.1 * float(wh[0]))
         scale = torch.rsqrt(torch.clamp(w, min=shift))
         Q = _mm(Q, (V * scale[None, :].to(V.dtype)).to(Y.dtype))
-    return torch.linalg.qr(Y)[0]
+    return kernels.qr(Y)[0] if kernels.qr_supported(Y) else torch.linalg.qr(Y)[0]
 
 
 def _core_svd(B: torch.Tensor, keep: int):

@@ -439,17 +439,18 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
         Zk[:, rows, :nz] = Z[:, :, :nz]
         Zk = Zk * mk[:, None, :]
     if tall:                                            # A Z = Q R, C = R Z^H (tnkit/linalg.py:155-159)
-        # a leg keeping fewer than nk columns has exact zero columns at the end of A Z, which the
-        # batched Householder QR turns into NaN; they are filled with fixed Gaussian columns. The
-        # kept columns come first, so their Q and R do not see the fill, and the fill's rows of
-        # R meet the zero rows of Z^H below.
+        # a leg keeping fewer than nk columns has exact zero columns at the end of A Z, which
+        # rocSOLVER's batched QR (the fallback beyond the library QR's limits) turns into NaN;
+        # they are filled with fixed Gaussian columns, on either QR, so both give the same
+        # factors. The kept columns come first, so their Q and R do not see the fill, and the
+        # fill's rows of R meet the zero rows of Z^H below.
         AZ = torch.matmul(mat, Zk)
         if bool((ranks < nk).any()):                   # host ranks: no device read
             gen = torch.Generator(device=mat.device)
             gen.manual_seed(12345)
             fill = torch.randn(AZ.shape, generator=gen, device=mat.device, dtype=torch.float64)
             AZ = torch.where(mk[:, None, :], AZ, fill.to(AZ.dtype))
-        Q, Rr = torch.linalg.qr(AZ)
+        Q, Rr = kernels.qr(AZ) if kernels.qr_supported(AZ) else torch.linalg.qr(AZ)
         U = Q * mk[:, None, :]
         C = torch.matmul(Rr, Zk.mH.resolve_conj().contiguous())        # (k, nk, n) carried factor
     else:
