@@ -445,16 +445,19 @@ int launch_sytrd(int n, const double* A, long lda, double* Vr, double* d, double
 //   dc_rows       (wave per row) M = P R (eigenvectors of the rank-one problem), row by row
 //   dc_rotate     (workgroup per node) the deflation rotations on rows of M, in reverse order
 //   two batched GEMMs: Qt_new = M^T-products with the halves' eigenvector blocks.
-// Eigenvectors are stored as rows: Qt[c][i] = component i of eigenvector c.
+// Eigenvectors are stored as rows, node by node: the K x K block of node b (at b K K) holds
+// component i of its eigenvector c at [c][i], so a level's GEMMs have uniform strides. B matrices
+// are solved at once as consecutive N-blocks of one padded problem of order B N (no coupling
+// between the blocks, no node spanning two of them): every launch covers all B matrices.
 
 constexpr int kDcLeaf = 32;
 constexpr int kDcTerms = kSyevMaxN / kWave;     // secular terms per lane
 constexpr double kEps = 2.220446049250313e-16;
 
 struct DcLevel {
-  int K, n1, N, top;        // node order, half, padded order, 1 on the last level
-  const double* w_in;       // N: the halves' eigenvalues, ascending per half
-  const double* Qt_in;      // N x N
+  int K, n1, N, top;        // node order, half, padded order of one matrix, 1 on the last level
+  const double* w_in;       // the halves' eigenvalues, ascending per half
+  const double* Qt_in;      // the halves' eigenvector blocks (n1 x n1 each, node b's at 2b, 2b+1)
   const double* e;          // couplings (length N - 1, padded with zeros)
   double* w_out;            // N: ascending per node
   double* Mb;               // nodes x K x K
@@ -499,8 +502,8 @@ __global__ void __launch_bounds__(1024) dc_prep_kernel(DcLevel L) {
   const double rho = fabs(beta), sg = beta < 0.0 ? -1.0 : 1.0;
   for (int i = t; i < K; i += blockDim.x) {
     D[i] = L.w_in[lo + i];
-    z[i] = i < n1 ? L.Qt_in[(long)(lo + i) * L.N + lo + n1 - 1]
-                  : sg * L.Qt_in[(long)(lo + i) * L.N + lo + n1];
+    z[i] = i < n1 ? L.Qt_in[(long)lo * n1 + (long)i * n1 + n1 - 1]
+                  : sg * L.Qt_in[(long)lo * n1 + (long)n1 * n1 + (long)(i - n1) * n1];
     perm[i] = i;
   }
   __syncthreads();
@@ -856,11 +859,17 @@ __global__ void __launch_bounds__(256) dc_rotate_kernel(DcLevel L, int ncol) {
   Mc[(long)held * ncol] = hv;
 }
 
-// Padding and leaves. One workgroup: the bound |T| <= max|d| + 2 max|e|, the padded d and e
-// (decoupled entries at -3 bound), then leaf matrices with the tearing corrections.
-__global__ void __launch_bounds__(1024) dc_pad_kernel(int n, int N, const double* d,
-                                                      const double* e, double* dp, double* ep) {
+// Padding and leaves. One workgroup per matrix (blockIdx.x; d, e at strides sd, se): the bound
+// |T| <= max|d| + 2 max|e|, the padded d and e (decoupled entries at -3 bound, no coupling to the
+// next matrix) at dp, ep + b N, then leaf matrices with the tearing corrections.
+__global__ void __launch_bounds__(1024) dc_pad_kernel(int n, int N, const double* d, long sd,
+                                                      const double* e, long se, double* dp,
+                                                      double* ep) {
   __shared__ double red[16];
+  d += blockIdx.x * sd;
+  e += blockIdx.x * se;
+  dp += (long)blockIdx.x * N;
+  ep += (long)blockIdx.x * N;
   double m = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     m = fmax(m, fabs(d[i]));
@@ -874,7 +883,7 @@ __global__ void __launch_bounds__(1024) dc_pad_kernel(int n, int N, const double
   const double dummy = bound > 0.0 ? -3.0 * bound : -1.0;
   for (int i = threadIdx.x; i < N; i += blockDim.x) {
     dp[i] = i < n ? d[i] : dummy;
-    if (i < N - 1) ep[i] = i < n - 1 ? e[i] : 0.0;
+    ep[i] = i < n - 1 ? e[i] : 0.0;
   }
 }
 
@@ -897,13 +906,14 @@ __global__ void dc_leaves_kernel(int s, int N, const double* dp, const double* e
   }
 }
 
-// Qt[lo + c][lo + i] = Z_b[i][c] (leaf eigenvectors as rows).
-__global__ void dc_scatter_leaves_kernel(int s, int N, const double* Z, double* Qt) {
-  const int b = blockIdx.x, lo = b * s;
+// Leaf block b of Qt: row c = eigenvector c of leaf b (Qt[b s s + c s + i] = Z_b[i][c]).
+__global__ void dc_scatter_leaves_kernel(int s, const double* Z, double* Qt) {
+  const int b = blockIdx.x;
   const double* Zb = Z + (long)b * s * s;
+  double* Qb = Qt + (long)b * s * s;
   for (int idx = threadIdx.x; idx < s * s; idx += blockDim.x) {
     const int c = idx / s, i = idx - c * s;
-    Qt[(long)(lo + c) * N + lo + i] = Zb[i * s + c];
+    Qb[idx] = Zb[i * s + c];
   }
 }
 
@@ -968,13 +978,14 @@ __global__ void syev_status_kernel(const int* err, const int* sweeps, int nleaf,
 }
 
 struct SyevPlan {
-  int n, k, L, s, N, npan;
+  int n, k, L, s, N, npan, B;   // B: matrices solved together by the divide and conquer
 };
 
-SyevPlan syev_plan(int n, int k) {
+SyevPlan syev_plan(int n, int k, int B = 1) {
   SyevPlan p;
   p.n = n;
   p.k = k;
+  p.B = B;
   p.L = 1;
   while ((n + (1 << p.L) - 1) >> p.L > kDcLeaf) ++p.L;
   p.s = (n + (1 << p.L) - 1) >> p.L;
@@ -986,28 +997,28 @@ SyevPlan syev_plan(int n, int k) {
 // Workspace pieces, in carving order (one function, used for both the size and the carve).
 template <typename F>
 void syev_pieces(const SyevPlan& p, F&& f) {
-  const size_t n = p.n, N = p.N, k = p.k, npan = p.npan;
+  const size_t n = p.n, N = p.N, k = p.k, npan = p.npan, NB = (size_t)p.B * p.N;
   f("trd", tnh::trd_ws_bytes(p.n));
   f("Vr", n * n * 8);
   f("Vc", n * npan * kBtNb * 8);
   f("d", n * 8);
   f("e", n * 8);
   f("tau", n * 8);
-  f("dp", N * 8);
-  f("ep", N * 8);
-  f("QtA", N * N * 8);
-  f("QtB", N * N * 8);
-  f("Mb", N * N * 8);
-  f("leafA", N * p.s * 8);
-  f("leafZ", N * p.s * 8);
-  f("wA", N * 8);
-  f("wB", N * 8);
-  f("sweeps", (N / p.s) * 4);
-  for (const char* nm : {"perm", "inv", "kept", "dl", "rot_j", "rot_jl", "ent"}) f(nm, N * 4);
-  for (const char* nm : {"Ds", "zs", "rot_c", "rot_s", "mu", "dor", "zh", "nrm"}) f(nm, N * 8);
-  f("cnt", N * 8);
-  f("rhop", N * 8);
-  f("Zw", N * k * 8);
+  f("dp", NB * 8);
+  f("ep", NB * 8);
+  f("QtA", NB * N * 8);
+  f("QtB", NB * N * 8);
+  f("Mb", NB * N * 8);
+  f("leafA", NB * p.s * 8);
+  f("leafZ", NB * p.s * 8);
+  f("wA", NB * 8);
+  f("wB", NB * 8);
+  f("sweeps", (NB / p.s) * 4);
+  for (const char* nm : {"perm", "inv", "kept", "dl", "rot_j", "rot_jl", "ent"}) f(nm, NB * 4);
+  for (const char* nm : {"Ds", "zs", "rot_c", "rot_s", "mu", "dor", "zh", "nrm"}) f(nm, NB * 8);
+  f("cnt", NB * 8);
+  f("rhop", NB * 8);
+  f("Zw", NB * k * 8);
   f("S", npan * kBtNb * kBtNb * 8);
   f("negTt", npan * kBtNb * kBtNb * 8);
   f("Y", kBtNb * k * 8);
@@ -1089,11 +1100,12 @@ SyevBufs syev_carve(const SyevPlan& p, void* ws) {
   return b;
 }
 
-// Divide and conquer on the tridiagonal (d, e) of order p.n: all eigenvalues ascending into w,
-// the eigenvectors of the k largest into rows 0..n-1 of b.Zw (n x k, column c the c-th largest).
-int run_dc(const SyevPlan& p, SyevBufs& b, const double* d, const double* e, double* w,
-           hipStream_t st) {
-  const int n = p.n, N = p.N, k = p.k, s = p.s;
+// Divide and conquer on the tridiagonals (d + m sd, e + m se) of order p.n of p.B matrices: all
+// eigenvalues ascending into w + m sw, the eigenvectors of the k largest into rows 0..n-1 of
+// b.Zw + m N k (n x k, column c the c-th largest).
+int run_dc(const SyevPlan& p, SyevBufs& b, const double* d, long sd, const double* e, long se,
+           double* w, long sw, hipStream_t st) {
+  const int n = p.n, N = p.N, k = p.k, s = p.s, B = p.B, NB = B * N;
   double* const* Qt = b.Qt;
   double* const* wv = b.wv;
   DcLevel& Lv = b.Lv;
@@ -1102,20 +1114,20 @@ int run_dc(const SyevPlan& p, SyevBufs& b, const double* d, const double* e, dou
   double *dp = b.dp, *ep = b.ep, *leafA = b.leafA, *leafZ = b.leafZ;
   int* sweeps = b.sweeps;
   int rc;
-  hipLaunchKernelGGL(dc_pad_kernel, dim3(1), dim3(1024), 0, st, n, N, d, e, dp, ep);
-  hipLaunchKernelGGL(dc_leaves_kernel, dim3(N / s), dim3(256), 0, st, s, N, dp, ep, leafA);
+  hipLaunchKernelGGL(dc_pad_kernel, dim3(B), dim3(1024), 0, st, n, N, d, sd, e, se, dp, ep);
+  hipLaunchKernelGGL(dc_leaves_kernel, dim3(NB / s), dim3(256), 0, st, s, NB, dp, ep, leafA);
   TNH_CHECK_LAUNCH("syev leaves");
-  rc = tnh_eigh_jacobi(TNH_F64, N / s, s, leafA, s, (long)s * s, nullptr, 0, wv[0], s, leafZ, s,
+  rc = tnh_eigh_jacobi(TNH_F64, NB / s, s, leafA, s, (long)s * s, nullptr, 0, wv[0], s, leafZ, s,
                        (long)s * s, 60, 2e-16, sweeps, st);
   if (rc != TNH_OK) return rc;
-  hipLaunchKernelGGL(dc_scatter_leaves_kernel, dim3(N / s), dim3(256), 0, st, s, N, leafZ, Qt[0]);
+  hipLaunchKernelGGL(dc_scatter_leaves_kernel, dim3(NB / s), dim3(256), 0, st, s, leafZ, Qt[0]);
   TNH_CHECK_LAUNCH("syev scatter");
   int cur = 0;
   Lv.N = N;
   Lv.e = ep;
   Lv.Mb = Mb;
   for (int l = 1; l <= p.L; ++l) {
-    const int K = s << l, nodes = N / K, top = l == p.L;
+    const int K = s << l, nodes = NB / K, top = l == p.L;
     Lv.K = K;
     Lv.n1 = K / 2;
     Lv.top = top;
@@ -1138,26 +1150,28 @@ int run_dc(const SyevPlan& p, SyevBufs& b, const double* d, const double* e, dou
                        ncol);
     TNH_CHECK_LAUNCH("syev merge");
     const int n1 = K / 2;
+    const long sK = (long)K * K, sH = sK / 2;   // a node's block, its two halves' blocks
     if (!top) {
       double* Qo = Qt[cur ^ 1];
-      const long sq = (long)K * (N + 1);
-      rc = gemm(K, n1, n1, Mb, K, Qt[cur], N, Qo, N, nodes, (long)K * K, sq, sq, 0, st);
+      rc = gemm(K, n1, n1, Mb, K, Qt[cur], n1, Qo, K, nodes, sK, sH, sK, 0, st);
       if (rc != TNH_OK) return rc;
-      rc = gemm(K, n1, n1, Mb + (long)n1 * K, K, Qt[cur] + (long)n1 * N + n1, N, Qo + n1, N,
-                nodes, (long)K * K, sq, sq, 0, st);
+      rc = gemm(K, n1, n1, Mb + (long)n1 * K, K, Qt[cur] + (long)n1 * n1, n1, Qo + n1, K, nodes,
+                sK, sH, sK, 0, st);
       if (rc != TNH_OK) return rc;
     } else {
-      rc = gemm(n1, k, n1, Qt[cur], N, Mb, k, Zw, k, 1, 0, 0, 0, 0, st);
+      const long sM = (long)K * k, sZ = (long)N * k;
+      rc = gemm(n1, k, n1, Qt[cur], n1, Mb, k, Zw, k, nodes, sH, sM, sZ, 0, st);
       if (rc != TNH_OK) return rc;
-      rc = gemm(n1, k, n1, Qt[cur] + (long)n1 * N + n1, N, Mb + (long)n1 * k, k,
-                Zw + (long)n1 * k, k, 1, 0, 0, 0, 0, st);
+      rc = gemm(n1, k, n1, Qt[cur] + (long)n1 * n1, n1, Mb + (long)n1 * k, k, Zw + (long)n1 * k,
+                k, nodes, sH, sM, sZ, 0, st);
       if (rc != TNH_OK) return rc;
     }
     cur ^= 1;
   }
-  // eigenvalues: the top level's ascending values above the N - n padding entries
-  hipError_t he = hipMemcpyAsync(w, wv[cur] + (N - n), (size_t)n * sizeof(double),
-                                 hipMemcpyDeviceToDevice, st);
+  // eigenvalues: each matrix's top-level ascending values above its N - n padding entries
+  hipError_t he = hipMemcpy2DAsync(w, (size_t)sw * sizeof(double), wv[cur] + (N - n),
+                                   (size_t)N * sizeof(double), (size_t)n * sizeof(double),
+                                   (size_t)B, hipMemcpyDeviceToDevice, st);
   TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "syev: eigenvalue copy: %s", hipGetErrorString(he));
   return TNH_OK;
 }
@@ -1177,7 +1191,7 @@ int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z,
   if (rc != TNH_OK) return rc;
 
   // 2. divide and conquer
-  rc = run_dc(p, b, b.d, b.e, w, st);
+  rc = run_dc(p, b, b.d, n, b.e, n, w, n, st);
   if (rc != TNH_OK) return rc;
 
   // 3. back-transformation of the k vectors (rows 0..n-1 of Zw)
@@ -1473,10 +1487,14 @@ size_t htrd_ws_per_matrix(int n) {
 }
 
 // Complex back-transformation helpers.
-// Vc[i][j] = Vr[j][i] and Vh[i][j] = conj(Vr[j][i]) for reflectors j < n - 1 (zero otherwise).
+// Vc[i][j] = Vr[j][i] and Vh[i][j] = conj(Vr[j][i]) for reflectors j < n - 1 (zero otherwise),
+// for matrix blockIdx.z (Vr at z n n, Vc and Vh at z n NV).
 __global__ void transpose_c_kernel(int n, int NV, const c128* __restrict__ src,
                                    c128* __restrict__ Vc, c128* __restrict__ Vh) {
   __shared__ double tre[32][33], tim[32][33];
+  src += blockIdx.z * (long)n * n;          // matrix blockIdx.z
+  Vc += blockIdx.z * (long)n * NV;
+  Vh += blockIdx.z * (long)n * NV;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   for (int r = threadIdx.y; r < 32; r += blockDim.y) {
     const int j = by + r, i = bx + threadIdx.x;
@@ -1495,13 +1513,17 @@ __global__ void transpose_c_kernel(int n, int NV, const c128* __restrict__ src,
   }
 }
 
-// Panel b: T of Q_b = I - V T V^H (T[0:c, c] = -tau_c T[0:c, 0:c] S[0:c, c], T[c][c] = tau_c with
-// S = V^H V), written as A_b[p][q] = -T[q][p], the K-major left operand of Y' = -T Y.
+// Panel blockIdx.x of matrix blockIdx.y: T of Q_b = I - V T V^H (T[0:c, c] = -tau_c T[0:c, 0:c]
+// S[0:c, c], T[c][c] = tau_c with S = V^H V), written as A_b[p][q] = -T[q][p], the K-major left
+// operand of Y' = -T Y. S, tau and negTt of matrix y at y sS, y stau, y sS.
 __global__ void __launch_bounds__(256) larft_c_kernel(int nrefl, const c128* S, const c128* tau,
-                                                      c128* negTt) {
+                                                      c128* negTt, long sS, long stau) {
   __shared__ double Tre[kBtNb][kBtNb + 1], Tim[kBtNb][kBtNb + 1];
   __shared__ double mre[kBtNb], mim[kBtNb];
   const int b = blockIdx.x, jb = b * kBtNb, nb = min(kBtNb, nrefl - jb), t = threadIdx.x;
+  S += blockIdx.y * sS;
+  tau += blockIdx.y * stau;
+  negTt += blockIdx.y * sS;
   const c128* Sb = S + (long)b * kBtNb * kBtNb;
   for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) {
     Tre[idx / kBtNb][idx % kBtNb] = 0.0;
@@ -1535,8 +1557,11 @@ __global__ void __launch_bounds__(256) larft_c_kernel(int nrefl, const c128* S, 
   }
 }
 
-// Zc (n x k complex) = Zw (n x k real).
-__global__ void real_to_c_kernel(long cnt, const double* __restrict__ src, c128* __restrict__ dst) {
+// Zc (n x k complex, matrix blockIdx.y at y cnt) = Zw (n x k real, matrix y at y sw).
+__global__ void real_to_c_kernel(long cnt, const double* __restrict__ src, long sw,
+                                 c128* __restrict__ dst) {
+  src += blockIdx.y * sw;
+  dst += blockIdx.y * cnt;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < cnt; i += (long)gridDim.x * 256)
     dst[i] = c128{src[i], 0.0};
 }
@@ -1558,31 +1583,32 @@ struct HeevPlan {
   int batch, mpl;   // matrices, matrices per reduction launch
 };
 
-// Workspace pieces of the batched complex solve, in carving order: the real solver's pieces
-// (its divide and conquer is reused as is), then per launch of h.mpl matrices the reduction's
-// exchange slots and barrier words, the reflectors and T, then the back-transformation's blocks.
+// Workspace pieces of the batched complex solve, in carving order: the real solver's pieces for
+// the divide and conquer of all h.batch tridiagonals at once, the reduction's exchange slots and
+// barrier words for one launch of h.mpl matrices, then per matrix the reflectors, T and the
+// back-transformation's blocks.
 template <typename F>
 void heev_pieces(const HeevPlan& h, F&& f) {
   const SyevPlan& p = h.sp;
-  const size_t n = p.n, k = p.k, NV = (size_t)p.npan * kBtNb, B = h.mpl;
+  const size_t n = p.n, k = p.k, NV = (size_t)p.npan * kBtNb, B = h.batch, C = h.mpl;
   const size_t G = htrd_grid(p.n), ns = trd_slot(p.n), gs = (G + 31) / 32 * 32 + 32;
   f("syev", syev_ws_bytes(p));
-  f("P", B * n * ns * 16);
-  f("Rp", B * n * ns * 16);
-  f("part", B * n * gs * 16);
-  f("bars", B * kBarWords * sizeof(unsigned));
+  f("P", C * n * ns * 16);
+  f("Rp", C * n * ns * 16);
+  f("part", C * n * gs * 16);
+  f("bars", C * kBarWords * sizeof(unsigned));
   f("err", 256);
   f("Vr", B * n * n * 16);
   f("d", B * n * 8);
   f("e", B * n * 8);
   f("tau", B * n * 16);
-  f("Vc", n * NV * 16);
-  f("Vh", n * NV * 16);
-  f("S", (size_t)p.npan * kBtNb * kBtNb * 16);
-  f("negTt", (size_t)p.npan * kBtNb * kBtNb * 16);
-  f("Zc", n * k * 16);
-  f("Y", (size_t)kBtNb * k * 16);
-  f("Y2", (size_t)kBtNb * k * 16);
+  f("Vc", B * n * NV * 16);
+  f("Vh", B * n * NV * 16);
+  f("S", B * p.npan * kBtNb * kBtNb * 16);
+  f("negTt", B * p.npan * kBtNb * kBtNb * 16);
+  f("Zc", B * n * k * 16);
+  f("Y", B * kBtNb * k * 16);
+  f("Y2", B * kBtNb * k * 16);
 }
 
 size_t heev_ws_bytes(const HeevPlan& h) {
@@ -1609,8 +1635,8 @@ int heev_mpl(int n, int batch, int* mpl) {
   return TNH_OK;
 }
 
-// Per launch of up to h.mpl matrices: the reductions in one cooperative launch, then per matrix
-// the divide and conquer on (d, e) (the real solver's) and Z = Q Z_T.
+// The reductions, h.mpl matrices per cooperative launch; one divide and conquer over all the
+// tridiagonals; Z = Q Z_T for all matrices at once (batched GEMMs per block of reflectors).
 int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, long sw, c128* Z,
              long ldz, long sZ, void* ws, int* info, hipStream_t st) {
   const SyevPlan& p = h.sp;
@@ -1643,7 +1669,8 @@ int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, lon
   c128* Zc = cp();
   c128* Y = cp();
   c128* Y2 = cp();
-  const int n = p.n, k = p.k, G = htrd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
+  const int n = p.n, k = p.k, B = h.batch, G = htrd_grid(n), ns = trd_slot(n);
+  const int gs = (G + 31) / 32 * 32 + 32;
   const long nn = (long)n * n;
   const int NV = p.npan * kBtNb, nrefl = n - 1;
   SyevBufs b = syev_carve(p, ws_syev);
@@ -1651,56 +1678,62 @@ int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, lon
   hipError_t he = hipMemsetAsync(err, 0, sizeof(int), st);
   if (he == hipSuccess && info) he = hipMemsetAsync(info, 0, sizeof(int), st);
   TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: reset: %s", hipGetErrorString(he));
-  for (int m0 = 0; m0 < h.batch; m0 += h.mpl) {
-    const int cnt = std::min(h.mpl, h.batch - m0);
+  for (int m0 = 0; m0 < B; m0 += h.mpl) {
+    const int cnt = std::min(h.mpl, B - m0);
     he = hipMemsetAsync(bars, 0, (size_t)cnt * kBarWords * sizeof(unsigned), st);
     TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: barrier reset: %s", hipGetErrorString(he));
-    HtrdArgs args{n, ns, gs, G, A + (long)m0 * sA, lda, sA, Vr, d, e, tau, P, Rp, part, bars, err};
+    HtrdArgs args{n,  ns, gs, G, A + (long)m0 * sA, lda, sA, Vr + m0 * nn, d + (long)m0 * n,
+                  e + (long)m0 * n, tau + (long)m0 * n, P, Rp, part, bars, err};
     void* kargs[] = {&args};
     he = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&hetrd_coop_kernel),
                                     dim3(G * cnt), dim3(kHThreads), kargs, 0u, st);
     TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP,
                 "heev: cooperative launch of %d workgroups refused: %s", G * cnt,
                 hipGetErrorString(he));
-    for (int r = 0; r < cnt; ++r) {
-      const long m = m0 + r;
-      int rc = run_dc(p, b, d + (long)r * n, e + (long)r * n, w + m * sw, st);
-      if (rc != TNH_OK) return rc;
-      const long cntz = (long)n * k;
-      hipLaunchKernelGGL(real_to_c_kernel,
-                         dim3((unsigned)std::min<long>((cntz + 255) / 256, 4096)), dim3(256), 0,
-                         st, cntz, b.Zw, Zc);
-      const c128* Vrm = Vr + r * nn;
-      hipLaunchKernelGGL(transpose_c_kernel, dim3((n + 31) / 32, (NV + 31) / 32), dim3(32, 8), 0,
-                         st, n, NV, Vrm, Vc, Vh);
-      TNH_CHECK_LAUNCH("heev transpose");
-      rc = gemm_c(kBtNb, kBtNb, n, Vh, NV, Vc, NV, S, kBtNb, p.npan, kBtNb, kBtNb,
-                  (long)kBtNb * kBtNb, 0, st);
-      if (rc != TNH_OK) return rc;
-      hipLaunchKernelGGL(larft_c_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S,
-                         tau + (long)r * n, negTt);
-      TNH_CHECK_LAUNCH("heev larft");
-      for (int bb = p.npan - 1; bb >= 0; --bb) {
-        const int jb = bb * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
-        rc = gemm_c(nb, k, rows, Vh + (long)jb * NV + jb, NV, Zc + (long)jb * k, k, Y, k, 1, 0,
-                    0, 0, 0, st);
-        if (rc != TNH_OK) return rc;
-        rc = gemm_c(nb, k, nb, negTt + (long)bb * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, 1, 0, 0, 0,
-                    0, st);
-        if (rc != TNH_OK) return rc;
-        rc = gemm_c(rows, k, nb, Vrm + (long)jb * n + jb, n, Y2, k, Zc + (long)jb * k, k, 1, 0,
-                    0, 0, 1, st);
-        if (rc != TNH_OK) return rc;
-      }
-      he = hipMemcpy2DAsync(Z + m * sZ, (size_t)ldz * 16, Zc, (size_t)k * 16, (size_t)k * 16,
-                            (size_t)n, hipMemcpyDeviceToDevice, st);
+  }
+  int rc = run_dc(p, b, d, n, e, n, w, sw, st);
+  if (rc != TNH_OK) return rc;
+
+  // Z = Q Z_T for all matrices: Zc = Z_T, then the blocks of reflectors last first
+  const bool direct = ldz == k && sZ == (long)n * k;   // Z itself holds the running product
+  c128* Zr = direct ? Z : Zc;
+  const long cntz = (long)n * k, sVc = (long)n * NV, sT = (long)p.npan * kBtNb * kBtNb;
+  const long sY = (long)kBtNb * k;
+  hipLaunchKernelGGL(real_to_c_kernel, dim3((unsigned)std::min<long>((cntz + 255) / 256, 1024), B),
+                     dim3(256), 0, st, cntz, b.Zw, (long)p.N * k, Zr);
+  hipLaunchKernelGGL(transpose_c_kernel, dim3((n + 31) / 32, (NV + 31) / 32, B), dim3(32, 8), 0,
+                     st, n, NV, Vr, Vc, Vh);
+  TNH_CHECK_LAUNCH("heev transpose");
+  for (int m = 0; m < B; ++m) {
+    rc = gemm_c(kBtNb, kBtNb, n, Vh + m * sVc, NV, Vc + m * sVc, NV, S + m * sT, kBtNb, p.npan,
+                kBtNb, kBtNb, (long)kBtNb * kBtNb, 0, st);
+    if (rc != TNH_OK) return rc;
+  }
+  hipLaunchKernelGGL(larft_c_kernel, dim3(p.npan, B), dim3(256), 0, st, nrefl, S, tau, negTt, sT,
+                     (long)n);
+  TNH_CHECK_LAUNCH("heev larft");
+  for (int bb = p.npan - 1; bb >= 0; --bb) {
+    const int jb = bb * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
+    rc = gemm_c(nb, k, rows, Vh + (long)jb * NV + jb, NV, Zr + (long)jb * k, k, Y, k, B, sVc,
+                cntz, sY, 0, st);
+    if (rc != TNH_OK) return rc;
+    rc = gemm_c(nb, k, nb, negTt + (long)bb * kBtNb * kBtNb, kBtNb, Y, k, Y2, k, B, sT, sY, sY, 0,
+                st);
+    if (rc != TNH_OK) return rc;
+    rc = gemm_c(rows, k, nb, Vr + (long)jb * n + jb, n, Y2, k, Zr + (long)jb * k, k, B, nn, sY,
+                cntz, 1, st);
+    if (rc != TNH_OK) return rc;
+  }
+  if (!direct)
+    for (int m = 0; m < B; ++m) {
+      he = hipMemcpy2DAsync(Z + m * sZ, (size_t)ldz * 16, Zc + m * cntz, (size_t)k * 16,
+                            (size_t)k * 16, (size_t)n, hipMemcpyDeviceToDevice, st);
       TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP, "heev: vector copy: %s", hipGetErrorString(he));
-      if (info) {
-        hipLaunchKernelGGL(heev_status_kernel, dim3(1), dim3(256), 0, st, err, b.sweeps,
-                           p.N / p.s, info);
-        TNH_CHECK_LAUNCH("heev status");
-      }
     }
+  if (info) {
+    hipLaunchKernelGGL(heev_status_kernel, dim3(1), dim3(256), 0, st, err, b.sweeps,
+                       B * p.N / p.s, info);
+    TNH_CHECK_LAUNCH("heev status");
   }
   return TNH_OK;
 }
@@ -1985,7 +2018,7 @@ inline void larft_any(int np, int nrefl, const double* S, const double* tau, dou
 }
 inline void larft_any(int np, int nrefl, const c128* S, const c128* tau, c128* negTt,
                       hipStream_t st) {
-  hipLaunchKernelGGL(larft_c_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt);
+  hipLaunchKernelGGL(larft_c_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt, 0L, 0L);
 }
 
 template <typename T>
@@ -2113,7 +2146,7 @@ extern "C" size_t tnh_heev_top_ws_bytes(int batch, int n, int k) {
   if (batch < 1 || n < 2 || n > tnh::kHeevMaxN || k < 1 || k > n) return 0;
   int mpl = 1;
   if (tnh::heev_mpl(n, batch, &mpl) != TNH_OK) return 0;
-  return tnh::heev_ws_bytes(tnh::HeevPlan{tnh::syev_plan(n, k), batch, mpl});
+  return tnh::heev_ws_bytes(tnh::HeevPlan{tnh::syev_plan(n, k, batch), batch, mpl});
 }
 
 extern "C" int tnh_heev_top(int batch, int n, int k, const void* A, long lda, long sA, double* w,
@@ -2128,7 +2161,7 @@ extern "C" int tnh_heev_top(int batch, int n, int k, const void* A, long lda, lo
   int mpl = 1;
   int rc = tnh::heev_mpl(n, batch, &mpl);
   if (rc != TNH_OK) return rc;
-  const tnh::HeevPlan h{tnh::syev_plan(n, k), batch, mpl};
+  const tnh::HeevPlan h{tnh::syev_plan(n, k, batch), batch, mpl};
   TNH_REQUIRE(ws && ws_bytes >= tnh::heev_ws_bytes(h), TNH_ERR_WS,
               "tnh_heev_top: workspace %zu < %zu bytes", ws_bytes, tnh::heev_ws_bytes(h));
   return tnh::heev_top(h, static_cast<const tnh::c128*>(A), lda, sA, w, sw,
