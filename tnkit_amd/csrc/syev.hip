@@ -941,12 +941,16 @@ __global__ void transpose_kernel(int n, int NV, const double* __restrict__ src,
 }
 
 // Panel b (one workgroup): from S = V_b^T V_b (nb x nb) and tau, writes A_b = -T^T (row p, col q
-// = -T[q][p]), the left operand of Y' = -T Y in the K-major GEMM.
+// = -T[q][p]), the left operand of Y' = -T Y in the K-major GEMM; S, tau and negTt of matrix
+// blockIdx.y at y sS, y stau, y sS.
 __global__ void __launch_bounds__(256) larft_kernel(int nrefl, const double* S, const double* tau,
-                                                    double* negTt) {
+                                                    double* negTt, long sS, long stau) {
   __shared__ double T[kBtNb][kBtNb + 1];
   __shared__ double tmp[kBtNb];
   const int b = blockIdx.x, jb = b * kBtNb, nb = min(kBtNb, nrefl - jb), t = threadIdx.x;
+  S += blockIdx.y * sS;       // matrix blockIdx.y
+  tau += blockIdx.y * stau;
+  negTt += blockIdx.y * sS;
   const double* Sb = S + (long)b * kBtNb * kBtNb;
   for (int idx = t; idx < kBtNb * kBtNb; idx += blockDim.x) T[idx / kBtNb][idx % kBtNb] = 0.0;
   __syncthreads();
@@ -1203,7 +1207,7 @@ int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z,
   rc = gemm(kBtNb, kBtNb, n, Vc, NV, Vc, NV, S, kBtNb, p.npan, kBtNb, kBtNb,
             (long)kBtNb * kBtNb, 0, st);
   if (rc != TNH_OK) return rc;
-  hipLaunchKernelGGL(larft_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S, tau, negTt);
+  hipLaunchKernelGGL(larft_kernel, dim3(p.npan), dim3(256), 0, st, nrefl, S, tau, negTt, 0L, 0L);
   TNH_CHECK_LAUNCH("syev larft");
   for (int bb = p.npan - 1; bb >= 0; --bb) {
     const int jb = bb * kBtNb, nb = std::min(kBtNb, nrefl - jb), rows = n - jb;
@@ -1233,14 +1237,12 @@ int syev_top(const SyevPlan& p, const double* A, long lda, double* w, double* Z,
 // v_j v_j^H, H_j^H (alpha; x) = (beta; 0) with beta real, so every subdiagonal entry of T is
 // real), then the real divide and conquer on T, then Z = Q Z_T by complex WY blocks. The
 // reduction is the real kernel's scheme (rows in registers, one grid barrier per column) in
-// complex arithmetic, with 512-thread workgroups holding 8 rows x 2 columns per thread (order
-// <= 1024), and several matrices per launch: workgroup b serves matrix b / G as its workgroup
+// complex arithmetic, with workgroups of 256 or 512 threads holding 8 rows x 2 columns per thread
+// (order <= 1024), and several matrices per launch: workgroup b serves matrix b / G as its workgroup
 // b % G, each matrix with its own barrier words and exchange slots.
-constexpr int kHThreads = 512;
 constexpr int kHCols = 2;
 constexpr int kHRows = 8;
-constexpr int kHWaves = kHThreads / 64;
-constexpr int kHeevMaxN = kHThreads * kHCols;   // 1024
+constexpr int kHeevMaxN = 512 * kHCols;   // 1024 (512 threads; 256 up to order 512)
 
 __device__ __forceinline__ c128 cmul(c128 a, c128 b) { return a * b; }
 __device__ __forceinline__ c128 cscale(double s, c128 a) { return {s * a.re, s * a.im}; }
@@ -1266,7 +1268,9 @@ struct HtrdArgs {
   int* err;
 };
 
+template <int kHThreads>
 __global__ void __launch_bounds__(kHThreads) hetrd_coop_kernel(HtrdArgs a) {
+  constexpr int kHWaves = kHThreads / 64;
   __shared__ double red[2 * kHWaves];
   __shared__ double red8[2][kHWaves][2 * kHRows];
   __shared__ double ov_re[kHRows], ov_im[kHRows], op_re[kHRows], op_im[kHRows];
@@ -1479,6 +1483,15 @@ __global__ void __launch_bounds__(kHThreads) hetrd_coop_kernel(HtrdArgs a) {
 
 int htrd_grid(int n) { return (n + kHRows - 1) / kHRows; }
 
+// Threads per workgroup of the reduction: one thread per two columns, so orders up to 512 take
+// 256 threads, and twice as many matrices stay resident per launch.
+int htrd_threads(int n) { return n <= 512 ? 256 : 512; }
+
+const void* htrd_kernel(int n) {
+  return n <= 512 ? reinterpret_cast<const void*>(&hetrd_coop_kernel<256>)
+                  : reinterpret_cast<const void*>(&hetrd_coop_kernel<512>);
+}
+
 // Per matrix: P and Rp slots (n x ns complex), partials (n x gs complex), barrier words.
 size_t htrd_ws_per_matrix(int n) {
   const int G = htrd_grid(n), ns = trd_slot(n), gs = (G + 31) / 32 * 32 + 32;
@@ -1622,9 +1635,8 @@ int heev_mpl(int n, int batch, int* mpl) {
   int dev = 0, cus = 0, per_cu = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(
-          &per_cu, reinterpret_cast<const void*>(&hetrd_coop_kernel), kHThreads, 0) !=
-          hipSuccess) {
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, htrd_kernel(n), htrd_threads(n),
+                                                   0) != hipSuccess) {
     set_error("heev: device query failed");
     return TNH_ERR_HIP;
   }
@@ -1685,8 +1697,8 @@ int heev_top(const HeevPlan& h, const c128* A, long lda, long sA, double* w, lon
     HtrdArgs args{n,  ns, gs, G, A + (long)m0 * sA, lda, sA, Vr + m0 * nn, d + (long)m0 * n,
                   e + (long)m0 * n, tau + (long)m0 * n, P, Rp, part, bars, err};
     void* kargs[] = {&args};
-    he = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&hetrd_coop_kernel),
-                                    dim3(G * cnt), dim3(kHThreads), kargs, 0u, st);
+    he = hipLaunchCooperativeKernel(htrd_kernel(n), dim3(G * cnt), dim3(htrd_threads(n)), kargs,
+                                    0u, st);
     TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP,
                 "heev: cooperative launch of %d workgroups refused: %s", G * cnt,
                 hipGetErrorString(he));
@@ -1911,8 +1923,11 @@ __global__ void __launch_bounds__(NT) geqrf_coop_kernel(QrArgs<T> a) {
 // zero for nrefl <= j < NV (Vh may be null for real matrices); reflector j at src + j lds.
 template <typename T>
 __global__ void qr_transpose_kernel(int m, int nrefl, int NV, const T* __restrict__ src, int lds,
-                                    T* __restrict__ Vc, T* __restrict__ Vh) {
+                                    long ssrc, T* __restrict__ Vc, T* __restrict__ Vh, long sdst) {
   __shared__ double tre[32][33], tim[32][33];
+  src += blockIdx.z * ssrc;   // matrix blockIdx.z
+  Vc += blockIdx.z * sdst;
+  if (Vh) Vh += blockIdx.z * sdst;
   const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
   for (int r = threadIdx.y; r < 32; r += blockDim.y) {
     const int j = by + r, i = bx + threadIdx.x;
@@ -2012,13 +2027,15 @@ size_t qr_ws_bytes(int m, int n, int mpl) {
   return tot + 256;
 }
 
-inline void larft_any(int np, int nrefl, const double* S, const double* tau, double* negTt,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(larft_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt);
+inline void larft_any(int np, int cnt, int nrefl, const double* S, const double* tau,
+                      double* negTt, long sS, long stau, hipStream_t st) {
+  hipLaunchKernelGGL(larft_kernel, dim3(np, cnt), dim3(256), 0, st, nrefl, S, tau, negTt, sS,
+                     stau);
 }
-inline void larft_any(int np, int nrefl, const c128* S, const c128* tau, c128* negTt,
-                      hipStream_t st) {
-  hipLaunchKernelGGL(larft_c_kernel, dim3(np), dim3(256), 0, st, nrefl, S, tau, negTt, 0L, 0L);
+inline void larft_any(int np, int cnt, int nrefl, const c128* S, const c128* tau, c128* negTt,
+                      long sS, long stau, hipStream_t st) {
+  hipLaunchKernelGGL(larft_c_kernel, dim3(np, cnt), dim3(256), 0, st, nrefl, S, tau, negTt, sS,
+                     stau);
 }
 
 template <typename T>
@@ -2064,17 +2081,17 @@ int qr_run(int dtype, int batch, int m, int n, int mpl, const T* A, long lda, lo
     hipLaunchKernelGGL(eye_kernel<T>, dim3(std::min(((long)m * n + 255) / 256, 1024L), cnt),
                        dim3(256), 0, st, m, n, Qm, ldq, sQ);
     TNH_CHECK_LAUNCH("qr eye");
+    hipLaunchKernelGGL(qr_transpose_kernel<T>, dim3((m + 31) / 32, (NV + 31) / 32, cnt),
+                       dim3(32, 8), 0, st, m, n, NV, V, ldv, sV, Vc,
+                       sizeof(T) == 16 ? Vh : nullptr, sVc);
+    TNH_CHECK_LAUNCH("qr transpose");
     for (int q = 0; q < cnt; ++q) {
-      hipLaunchKernelGGL(qr_transpose_kernel<T>, dim3((m + 31) / 32, (NV + 31) / 32), dim3(32, 8),
-                         0, st, m, n, NV, V + q * sV, ldv, Vc + q * sVc,
-                         sizeof(T) == 16 ? Vh + q * sVc : nullptr);
-      TNH_CHECK_LAUNCH("qr transpose");
       int rc = tnh_gemm_tn(dtype, kBtNb, kBtNb, m, Vh + q * sVc, NV, Vc + q * sVc, NV, S + q * sT,
                            kBtNb, npan, kBtNb, kBtNb, (long)kBtNb * kBtNb, 0, 0, st);
       if (rc != TNH_OK) return rc;
-      larft_any(npan, n, S + q * sT, tau + (long)q * n, negTt + q * sT, st);
-      TNH_CHECK_LAUNCH("qr larft");
     }
+    larft_any(npan, cnt, n, S, tau, negTt, sT, (long)n, st);
+    TNH_CHECK_LAUNCH("qr larft");
     // Q <- H_0 ... H_{n-1} [I; 0], last block first; block b changes only columns >= jb
     for (int bb = npan - 1; bb >= 0; --bb) {
       const int jb = bb * kBtNb, nb = std::min(kBtNb, n - jb), rows = m - jb, nc = n - jb;
