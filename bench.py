@@ -461,8 +461,10 @@ def small_linalg(reps=20):
     """The small factorisations of the truncations on the device: the Jacobi kernels
     (kernels.eigh_jacobi, kernels.svd_jacobi) against torch.linalg (rocSOLVER) on the shapes the
     sweeps meet: a batch of 32 complex Gram matrices of order 50 (the batched TDVP at bonds 25)
-    and float64 SVDs of 16 x 64, 32 x 32 and 64 x 64 (two-site matrices at bonds 8-32).
-    Milliseconds per call with HIP events; every result is checked against torch's."""
+    and float64 SVDs of 16 x 64, 32 x 32 and 64 x 64 (two-site matrices at bonds 8-32); the
+    library's float64 eigensolver at orders 512..2048; its batched complex eigensolver and QR on
+    the batched TDVP's truncation shapes at binding caps. Milliseconds per call with HIP events;
+    every result is checked against torch's."""
     import torch
     from tnkit_amd import kernels
     dev = torch.device("cuda", 0)
@@ -506,6 +508,39 @@ def small_linalg(reps=20):
             "rel_err_eigenvalues": float((w - ref).abs().max() / ref.abs().max()),
             "rel_residual": float((A @ Z - Z * lam[None, :]).abs().max() / ref.abs().max()),
             "orthogonality": float((Z.T @ Z - torch.eye(n, dtype=torch.float64, device=dev))
+                                   .abs().max())})
+    # the batched TDVP's truncation at binding caps: 32 complex Gram matrices of order 256 and
+    # 512 (all eigenpairs) and 32 complex QRs of the projected factor (512 x 128, 512 x 256)
+    out["heev"], out["qr"] = [], []
+    for n in (256, 512):
+        X = torch.randn((32, n, n), dtype=torch.complex128, device=dev, generator=g)
+        X = X * torch.exp(-torch.arange(n, dtype=torch.float64, device=dev) / (n / 20.0))
+        A = (X @ X.mH.resolve_conj()).contiguous()
+        A = 0.5 * (A + A.mH)
+        w, Z = kernels.heev_top(A, n)
+        ref = torch.linalg.eigvalsh(A)
+        lam = torch.flip(w, dims=[1])
+        sc = ref.abs().amax(dim=1, keepdim=True)
+        out["heev"].append({
+            "batch": 32, "n": n, "k": n, "dtype": "c128",
+            "own_ms": event_time(lambda: kernels.heev_top_async(A, n), reps, stream),
+            "torch_ms": event_time(lambda: torch.linalg.eigh(A), reps, stream),
+            "rel_err_eigenvalues": float(((w - ref).abs() / sc).max()),
+            "rel_residual": float(((A @ Z - Z * lam[:, None, :]).abs().amax(dim=(1, 2))
+                                   / sc[:, 0]).max()),
+            "orthogonality": float((Z.mH @ Z - torch.eye(n, dtype=Z.dtype, device=dev))
+                                   .abs().max())})
+    for m, n in ((512, 128), (512, 256)):
+        A = torch.randn((32, m, n), dtype=torch.complex128, device=dev, generator=g)
+        Q, R = kernels.qr(A)
+        Qr, Rr = torch.linalg.qr(A)
+        out["qr"].append({
+            "batch": 32, "m": m, "n": n, "dtype": "c128",
+            "own_ms": event_time(lambda: kernels.qr(A, check_status=False), reps, stream),
+            "torch_ms": event_time(lambda: torch.linalg.qr(A), reps, stream),
+            "max_diff_R_vs_torch": float((R - Rr).abs().max()),
+            "max_diff_Q_vs_torch": float((Q - Qr).abs().max()),
+            "orthogonality": float((Q.mH @ Q - torch.eye(n, dtype=Q.dtype, device=dev))
                                    .abs().max())})
     return out
 
