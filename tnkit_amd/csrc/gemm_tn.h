@@ -362,6 +362,123 @@ __global__ void __launch_bounds__(WNC * 128) gemm_dma_kernel(GemmArgs g) {
     }
 }
 
+// Complex128 form of gemm_dma_kernel: the same three-stage direct-to-LDS ring and 16-element row
+// rotation, one complex element per 16-byte chunk (interleaved re, im as stored), and four real
+// float64 MFMAs per fragment product (re += ar br - ai bi, im += ar bi + ai br). A complex tile of
+// BK rows carries twice the flops per LDS byte of the float64 one. 128 x 128 tiles, 8 waves of
+// 64 x 32 (two per SIMD). Requires M, N % 128 == 0, K % BK == 0 (the launcher checks).
+struct alignas(16) c128_pod {
+  double re, im;
+};
+
+template <int BK>
+__global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj_b) {
+  using MF = Mfma<double>;
+  using acc_t = typename MF::acc_t;
+  using T = c128_pod;
+  constexpr int WNC = 4, NT = 512;
+  constexpr int BM = 128, FM = 4, FN = 2, WCOL = 32;
+  constexpr int CPR = BM;              // chunks (elements) per tile row
+  constexpr int NJ = BK * CPR / NT;    // chunks per thread per operand tile
+  constexpr int ROT = 16;
+  static_assert(NJ * NT == BK * CPR && BK % 4 == 0, "dma tile shape");
+  __shared__ __align__(16) DmaStages<T, BK> sm;
+  const int M = g.M, N = g.N, K = g.K;
+  int tm, tn;
+  tile_of(blockIdx.x, M / BM, N / BM, tm, tn);
+  const int m0 = tm * BM, n0 = tn * BM;
+  const long bz = blockIdx.y;
+  const T* __restrict__ A = static_cast<const T*>(g.A) + bz * g.strideA;
+  const T* __restrict__ B = static_cast<const T*>(g.B) + bz * g.strideB;
+  T* __restrict__ C = static_cast<T*>(g.C) + bz * g.strideC;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = wave / WNC, wn = wave % WNC, row16 = lane & 15, kq = lane >> 4;
+  const double bsign = conj_b ? -1.0 : 1.0;
+  long goffA[NJ], goffB[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int c = j * NT + t;
+    const int r = c / CPR, p = c % CPR;
+    const int q = (p - ROT * (r & 3)) & (CPR - 1);
+    goffA[j] = (long)r * g.lda + m0 + q;
+    goffB[j] = (long)r * g.ldb + n0 + q;
+  }
+  acc_t cr[FM][FN], ci[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) cr[i][j] = ci[i][j] = acc_t{};
+  const int nk = K / BK;
+  const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * WCOL + row16 + 16 * kq;
+  auto issue = [&](int kt, T* as, T* bs) {
+    const T* a = A + (long)kt * BK * g.lda;
+    const T* b = B + (long)kt * BK * g.ldb;
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
+                                       (lds_void_t)(as + j * NT + wave * 64), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
+                                       (lds_void_t)(bs + j * NT + wave * 64), 16, 0, 0);
+    }
+  };
+  auto compute = [&](const T* as, const T* bs) {
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T af[FM], bf[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) af[i] = as[(kk + kq) * BM + ((aoff + i * 16) & (BM - 1))];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + kq) * BM + ((boff + j * 16) & (BM - 1))];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const double ar = af[i].re, ai = af[i].im, nai = -ai;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          const double br = bf[j].re, bi = bsign * bf[j].im;
+          cr[i][j] = MF::run(ar, br, cr[i][j]);
+          ci[i][j] = MF::run(ar, bi, ci[i][j]);
+          cr[i][j] = MF::run(nai, bi, cr[i][j]);
+          ci[i][j] = MF::run(ai, br, ci[i][j]);
+        }
+      }
+    }
+  };
+  auto step = [&](auto stage, int kt) {
+    constexpr int S = decltype(stage)::value;
+    if (kt + 1 < nk)
+      wait_vmcnt<2 * NJ>();
+    else
+      wait_vmcnt<0>();
+    __builtin_amdgcn_s_barrier();
+    if (kt + 2 < nk) issue(kt + 2, dma_stage_a<(S + 2) % 3>(sm), dma_stage_b<(S + 2) % 3>(sm));
+    compute(dma_stage_a<S>(sm), dma_stage_b<S>(sm));
+  };
+  if (nk > 0) issue(0, sm.a0, sm.b0);
+  if (nk > 1) issue(1, sm.a1, sm.b1);
+  for (int kt = 0; kt < nk; kt += 3) {
+    step(std::integral_constant<int, 0>{}, kt);
+    if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
+    if (kt + 2 < nk) step(std::integral_constant<int, 2>{}, kt + 2);
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      const int n = n0 + wn * WCOL + j * 16 + row16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm * 64 + i * 16 + MF::row(kq, r);
+        T* dst = C + (long)m * g.ldc + n;
+        T v{cr[i][j][r], ci[i][j][r]};
+        if (g.accumulate) {
+          v.re += dst->re;
+          v.im += dst->im;
+        }
+        *dst = v;
+      }
+    }
+}
+
 // Complex K-major GEMM: C[m][n] = sum_k A[k][m] * B'[k][n] with B' = B or conj(B), complex
 // elements stored interleaved. Real and imaginary parts sit in separate LDS planes and each
 // complex fragment product is four real MFMAs (re += ar br - ai bi, im += ar bi + ai br), so the
