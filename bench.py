@@ -779,6 +779,8 @@ def main():
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
     ap.add_argument("--no-block", action="store_true")
+    ap.add_argument("--no-linalg", action="store_true",
+                    help="skip the truncation factorisations against rocSOLVER (small_linalg)")
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the chi=1024 sweep-time measurement (Heisenberg N=64, 3 sweeps)")
     ap.add_argument("--small-linalg", action="store_true",
@@ -941,6 +943,9 @@ def main():
     del op64, op32, op16, ws, out, y32, y64
     cgs2 = cgs2_bench(th.numel())
 
+    # the truncation's factorisations on the library's solvers against rocSOLVER (Jacobi eigh
+    # and SVD, the order 512..2048 eigensolver, the batched complex eigensolver and QR)
+    linalg_res = {} if args.no_linalg else small_linalg(5)
     setup_s = time.perf_counter() - t_setup
     block = {} if args.no_block else block_bench(max(5, args.steps // 2), args.warmup,
                                                  local=not args.no_local)
@@ -989,6 +994,7 @@ def main():
         "cgs2": cgs2,
         "block_u1": block,
         "config2_ramp": ramp,
+        "truncation_linalg": linalg_res,
         "setup_s": setup_s,
     }
     if not args.no_cpu:
