@@ -142,6 +142,40 @@ def test_duck_type_helpers(hip):
     assert np.abs(G @ Zh - Zh * w.cpu().numpy()).max() < 1e-12 * np.abs(G).max()
 
 
+@pytest.mark.parametrize("shape,cplx", [((150, 150), True), ((3, 200, 200), True),
+                                        ((300, 300), False)])
+def test_backend_eigh_above_the_jacobi_orders(hip, shape, cplx):
+    """be.eigh past order 64 on the library's solvers (complex batched, real single): ascending
+    eigenvalues to 1e-12 of the radius, A Z = Z diag(w), Z^H Z = 1."""
+    rng = np.random.default_rng(sum(shape))
+    X = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+    G = X + np.swapaxes(X, -1, -2).conj()
+    be = get_backend("hip")
+    w, Z = be.eigh(torch.as_tensor(G).to(hip))
+    w, Z = w.cpu().numpy(), Z.cpu().numpy()
+    wr = np.linalg.eigvalsh(G)
+    sc = np.abs(wr).max()
+    assert np.abs(w - wr).max() < 1e-12 * sc
+    assert np.abs(G @ Z - Z * w[..., None, :]).max() < 1e-12 * sc
+    n = shape[-1]
+    assert np.abs(np.swapaxes(Z, -1, -2).conj() @ Z - np.eye(n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("shape,cplx", [((300, 120), False), ((4, 256, 64), True),
+                                        ((20, 40), True)])
+def test_backend_qr(hip, shape, cplx):
+    """be.qr: the library's QR for tall matrices (numpy's Q and R to 1e-12), the vendor's for
+    wide ones; Q R = A either way."""
+    rng = np.random.default_rng(sum(shape) + 1)
+    A = rng.standard_normal(shape) + (1j * rng.standard_normal(shape) if cplx else 0)
+    Q, R = get_backend("hip").qr(torch.as_tensor(A).to(hip))
+    Q, R = Q.cpu().numpy(), R.cpu().numpy()
+    Qn, Rn = np.linalg.qr(A)
+    assert np.abs(Q @ R - A).max() < 1e-12 * np.abs(A).max()
+    assert np.abs(R - Rn).max() < 1e-12 * np.abs(A).max() * 10
+    assert np.abs(Q - Qn).max() < 1e-11
+
+
 @pytest.mark.parametrize("dt", [torch.float64, torch.complex128])
 def test_matvec2_bwd_is_the_adjoint(hip, dt):
     """tnh_matvec2_bwd(ybar) = H_eff^H ybar of the dense H_eff (the oracle's explicit matrix)."""

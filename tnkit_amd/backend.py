@@ -254,13 +254,18 @@ class HipBackend:
         return kernels.thin_svd(A)
 
     def qr(self, A):
+        """Reduced QR: the library's batched Householder QR for tall float64 (rows <= 2048) and
+        complex128 (rows <= 1024) device matrices, the vendor QR otherwise."""
+        from . import kernels
+        if kernels.qr_supported(A):
+            return kernels.qr(A)
         return self.torch.linalg.qr(A, mode="reduced")
 
     def eigh(self, A):
         """Hermitian eigendecomposition (ascending): the batched Jacobi kernel for orders up to
-        64 in float64 / complex128, the library's divide-and-conquer solver for real float64
-        matrices of order 65..2048, the vendor solver otherwise or where Jacobi did not
-        converge."""
+        64 in float64 / complex128, the library's divide-and-conquer solvers for real float64
+        matrices of order 65..2048 and (batched) complex128 ones of order 65..1024, the vendor
+        solver otherwise or where a library solver reports a failure."""
         from . import kernels
         t = self.torch
         if (A.is_cuda and A.dim() in (2, 3) and A.dtype in (t.float64, t.complex128)
@@ -268,6 +273,15 @@ class HipBackend:
             G = (A.unsqueeze(0) if A.dim() == 2 else A).resolve_conj().contiguous()
             w, Z, sw = kernels.eigh_jacobi(G)
             if bool((sw > 0).all()):
+                return (w[0], Z[0]) if A.dim() == 2 else (w, Z)
+        if (A.is_cuda and A.dim() in (2, 3) and A.dtype == t.complex128 and A.shape[0] >= 1
+                and kernels.EIGH_JACOBI_MAX_N < A.shape[-1] <= kernels.HEEV_MAX_N):
+            G = A.unsqueeze(0) if A.dim() == 2 else A
+            G = (0.5 * (G + G.mH)).resolve_conj().contiguous()
+            n = int(A.shape[-1])
+            w, Z, info = kernels.heev_top_async(G, n)
+            if int(info.item()) == 0 and bool(t.isfinite(w).all()):
+                Z = t.flip(Z, dims=[2])
                 return (w[0], Z[0]) if A.dim() == 2 else (w, Z)
         if (A.is_cuda and A.dim() == 2 and A.dtype == t.float64
                 and kernels.EIGH_JACOBI_MAX_N < A.shape[-1] <= kernels.SYEV_MAX_N):
