@@ -68,32 +68,36 @@ def test_gemm_tn_direct_to_lds(hip, M, N, K, dtype):
             assert _rel(C, want) < tol
 
 
-@pytest.mark.parametrize("M,N,K,ldx", [(128, 128, 8, 0), (128, 256, 24, 0), (384, 128, 40, 0),
+@pytest.mark.parametrize("dt", [torch.complex128, torch.complex64])
+@pytest.mark.parametrize("M,N,K,ldx", [(128, 128, 16, 0), (128, 256, 48, 0), (384, 128, 80, 0),
                                        (256, 384, 96, 0), (256, 256, 64, 16), (128, 128, 12, 0)])
-def test_gemm_tn_complex_direct_to_lds(hip, M, N, K, ldx):
-    """Complex128 shapes that take the direct-to-LDS complex kernel (M, N multiples of 128, K of
-    8: one stage, a full ring, a ring that wraps; padded leading dimensions), batched with strides
-    and accumulating, through the C ABI, against numpy; K = 12 falls back to the staged kernel.
-    Tile 4 asks for it explicitly."""
-    print(f"gemm_tn complex dma M={M} N={N} K={K} ldx={ldx}", flush=True)
+def test_gemm_tn_complex_direct_to_lds(hip, M, N, K, ldx, dt):
+    """Complex shapes that take the direct-to-LDS complex kernel (M, N multiples of 128, K of its
+    step, 8 complex128 / 16 complex64 rows: one stage, a full ring, a ring that wraps; padded
+    leading dimensions), batched with strides and accumulating, through the C ABI, against
+    numpy; K = 12 falls back to the staged kernel. Tile 4 asks for it explicitly."""
+    print(f"gemm_tn complex dma M={M} N={N} K={K} ldx={ldx} {dt}", flush=True)
     rng = np.random.default_rng(M + 5 * N + 11 * K)
     nb = 3
     lda, ldb, ldc = M + ldx, N + ldx, N + ldx
     A = rng.standard_normal((nb, K, lda)) + 1j * rng.standard_normal((nb, K, lda))
     B = rng.standard_normal((nb, K, ldb)) + 1j * rng.standard_normal((nb, K, ldb))
+    if dt == torch.complex64:
+        A, B = A.astype(np.complex64).astype(complex), B.astype(np.complex64).astype(complex)
     ref = np.einsum("bkm,bkn->bmn", A[:, :, :M], B[:, :, :N])
-    Ad, Bd = torch.as_tensor(A, device=hip), torch.as_tensor(B, device=hip)
+    Ad, Bd = torch.as_tensor(A).to(dt).to(hip), torch.as_tensor(B).to(dt).to(hip)
+    tol = 1e-13 if dt == torch.complex128 else 4e-6 * np.sqrt(K)
     lib = _lib.load()
     for tile in (0, 4):
-        C = torch.full((nb, M, ldc), 0.5 + 0.25j, dtype=torch.complex128, device=hip)
+        C = torch.full((nb, M, ldc), 0.5 + 0.25j, dtype=dt, device=hip)
         for acc in (0, 1):
-            rc = lib.tnh_gemm_tn(_lib.dcode(torch.complex128), M, N, K, Ad.data_ptr(), lda,
-                                 Bd.data_ptr(), ldb, C.data_ptr(), ldc, nb, K * lda, K * ldb,
-                                 M * ldc, acc, tile, kernels.stream(hip))
+            rc = lib.tnh_gemm_tn(_lib.dcode(dt), M, N, K, Ad.data_ptr(), lda, Bd.data_ptr(), ldb,
+                                 C.data_ptr(), ldc, nb, K * lda, K * ldb, M * ldc, acc, tile,
+                                 kernels.stream(hip))
             assert rc == 0, lib.tnh_last_error()
             want = ref if acc == 0 else 2 * ref
             got = C[:, :, :N].cpu().numpy()
-            assert np.abs(got - want).max() <= 1e-13 * np.abs(want).max()
+            assert np.abs(got - want).max() <= tol * np.abs(want).max()
         if ldx:
             assert bool((C[:, :, N:] == 0.5 + 0.25j).all())
 

@@ -150,22 +150,27 @@ int launch_c(const GemmArgs& g, int batch, int conj_b, hipStream_t stream) {
 }  // namespace
 
 namespace {
-// The direct-to-LDS complex128 kernel when the shape fits it (128 x 128 tiles, K a multiple of
-// its 8-row step; complex128 operands are 16-byte chunks, so any leading dimension serves).
+// The direct-to-LDS complex kernel when the operands are stored in the compute type and the
+// shape fits it (128 x 128 tiles, K a multiple of its step, 16-byte chunks: any leading
+// dimension for complex128, even ones for complex64).
 template <typename TR, typename TA, typename TB, typename TO>
-bool try_dma_c128(const GemmArgs& g, int batch, int conj_b, hipStream_t stream, int& rc) {
-  if constexpr (std::is_same<TR, double>::value && std::is_same<TA, c128>::value &&
-                std::is_same<TB, c128>::value && std::is_same<TO, c128>::value) {
-    constexpr int BK = 8;
+bool try_dma_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t stream, int& rc) {
+  if constexpr (std::is_same<TA, cpx<TR>>::value && std::is_same<TB, cpx<TR>>::value &&
+                (std::is_same<TO, c128>::value || std::is_same<TO, c64>::value)) {
+    constexpr int EPC = 16 / sizeof(TA);
+    constexpr int BK = sizeof(TR) == 8 ? 8 : 16;
     if (batch < 1 || g.M <= 0 || g.N <= 0 || g.M % 128 || g.N % 128 || g.K < BK || g.K % BK)
       return false;
     if (reinterpret_cast<uintptr_t>(g.A) % 16 || reinterpret_cast<uintptr_t>(g.B) % 16 ||
-        reinterpret_cast<uintptr_t>(g.C) % 16)
+        reinterpret_cast<uintptr_t>(g.C) % sizeof(TO))
       return false;
+    if (g.lda % EPC || g.ldb % EPC) return false;
+    if (batch > 1 && (g.strideA % EPC || g.strideB % EPC)) return false;
     const long tiles = (long)(g.M / 128) * (g.N / 128);
     if (tiles >= (1L << 31) || batch > 65535) return false;
-    hipLaunchKernelGGL((gemm_dma_c128_kernel<BK>), dim3(static_cast<unsigned>(tiles), batch),
-                       dim3(512), 0, stream, g, conj_b);
+    using TOR = typename std::conditional<std::is_same<TO, c128>::value, double, float>::type;
+    hipLaunchKernelGGL((gemm_dma_cplx_kernel<TR, TOR, BK>),
+                       dim3(static_cast<unsigned>(tiles), batch), dim3(512), 0, stream, g, conj_b);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
       set_error("gemm_tn (complex dma): launch failed: %s", hipGetErrorString(e));
@@ -182,7 +187,7 @@ bool try_dma_c128(const GemmArgs& g, int batch, int conj_b, hipStream_t stream, 
 }  // namespace
 
 // Complex tiles: 1 = 64x64 (4 waves of 32x32), 2 = 32x32, 3 = 16x64 (skinny M), 4 = the
-// direct-to-LDS 128x128 complex128 kernel (when the shape fits it). 0 = automatic: the
+// direct-to-LDS 128x128 kernel (when the shape and storage fit it). 0 = automatic: the
 // direct-to-LDS kernel when it fits, else by tile count.
 template <typename TR, typename TA, typename TB, typename TO>
 int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t stream,
@@ -191,7 +196,7 @@ int launch_gemm_tn_cplx(const GemmArgs& g, int batch, int conj_b, hipStream_t st
   int cfg = tile_hint;
   if (cfg == 0 || cfg == 4) {
     int rc = TNH_OK;
-    if (try_dma_c128<TR, TA, TB, TO>(g, batch, conj_b, stream, rc)) return rc;
+    if (try_dma_cplx<TR, TA, TB, TO>(g, batch, conj_b, stream, rc)) return rc;
     if (cfg == 4) cfg = 1;
   }
   if (cfg == 0) {

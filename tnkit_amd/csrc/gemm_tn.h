@@ -362,25 +362,30 @@ __global__ void __launch_bounds__(WNC * 128) gemm_dma_kernel(GemmArgs g) {
     }
 }
 
-// Complex128 form of gemm_dma_kernel: the same three-stage direct-to-LDS ring and 16-element row
-// rotation, one complex element per 16-byte chunk (interleaved re, im as stored), and four real
-// float64 MFMAs per fragment product (re += ar br - ai bi, im += ar bi + ai br). A complex tile of
-// BK rows carries twice the flops per LDS byte of the float64 one. 128 x 128 tiles, 8 waves of
-// 64 x 32 (two per SIMD). Requires M, N % 128 == 0, K % BK == 0 (the launcher checks).
-struct alignas(16) c128_pod {
-  double re, im;
+// Complex form of gemm_dma_kernel (complex128 on float64 MFMAs, complex64 on float32 ones): the
+// same three-stage direct-to-LDS ring and 16-element row rotation over the interleaved (re, im)
+// elements as stored, and four real MFMAs per fragment product (re += ar br - ai bi,
+// im += ar bi + ai br). A complex tile of BK rows carries twice the flops per LDS byte of the
+// real one. 128 x 128 tiles, 8 waves of 64 x 32 (two per SIMD); C in TOR's complex type (the
+// complex64 operator's last GEMM writes complex128). Requires M, N % 128 == 0, K % BK == 0 and
+// 16-byte chunks (the launcher checks).
+template <typename TR>
+struct alignas(2 * sizeof(TR)) cpod {
+  TR re, im;
 };
 
-template <int BK>
-__global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj_b) {
-  using MF = Mfma<double>;
+template <typename TR, typename TOR, int BK>
+__global__ void __launch_bounds__(512) gemm_dma_cplx_kernel(GemmArgs g, int conj_b) {
+  using MF = Mfma<TR>;
   using acc_t = typename MF::acc_t;
-  using T = c128_pod;
+  using T = cpod<TR>;
+  using TO = cpod<TOR>;
   constexpr int WNC = 4, NT = 512;
   constexpr int BM = 128, FM = 4, FN = 2, WCOL = 32;
-  constexpr int CPR = BM;              // chunks (elements) per tile row
+  constexpr int EPC = 16 / sizeof(T);  // complex elements per 16-byte chunk
+  constexpr int CPR = BM / EPC;        // chunks per tile row
   constexpr int NJ = BK * CPR / NT;    // chunks per thread per operand tile
-  constexpr int ROT = 16;
+  constexpr int ROT = 16 / EPC;        // the 16-element row rotation, in chunks
   static_assert(NJ * NT == BK * CPR && BK % 4 == 0, "dma tile shape");
   __shared__ __align__(16) DmaStages<T, BK> sm;
   const int M = g.M, N = g.N, K = g.K;
@@ -390,18 +395,18 @@ __global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj
   const long bz = blockIdx.y;
   const T* __restrict__ A = static_cast<const T*>(g.A) + bz * g.strideA;
   const T* __restrict__ B = static_cast<const T*>(g.B) + bz * g.strideB;
-  T* __restrict__ C = static_cast<T*>(g.C) + bz * g.strideC;
+  TO* __restrict__ C = static_cast<TO*>(g.C) + bz * g.strideC;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave / WNC, wn = wave % WNC, row16 = lane & 15, kq = lane >> 4;
-  const double bsign = conj_b ? -1.0 : 1.0;
+  const TR bsign = conj_b ? TR(-1) : TR(1);
   long goffA[NJ], goffB[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
     const int c = j * NT + t;
     const int r = c / CPR, p = c % CPR;
     const int q = (p - ROT * (r & 3)) & (CPR - 1);
-    goffA[j] = (long)r * g.lda + m0 + q;
-    goffB[j] = (long)r * g.ldb + n0 + q;
+    goffA[j] = (long)r * g.lda + m0 + EPC * q;
+    goffB[j] = (long)r * g.ldb + n0 + EPC * q;
   }
   acc_t cr[FM][FN], ci[FM][FN];
 #pragma unroll
@@ -416,9 +421,9 @@ __global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(a + goffA[j]),
-                                       (lds_void_t)(as + j * NT + wave * 64), 16, 0, 0);
+                                       (lds_void_t)(as + (j * NT + wave * 64) * EPC), 16, 0, 0);
       __builtin_amdgcn_global_load_lds(static_cast<const void*>(b + goffB[j]),
-                                       (lds_void_t)(bs + j * NT + wave * 64), 16, 0, 0);
+                                       (lds_void_t)(bs + (j * NT + wave * 64) * EPC), 16, 0, 0);
     }
   };
   auto compute = [&](const T* as, const T* bs) {
@@ -431,10 +436,10 @@ __global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj
       for (int j = 0; j < FN; ++j) bf[j] = bs[(kk + kq) * BM + ((boff + j * 16) & (BM - 1))];
 #pragma unroll
       for (int i = 0; i < FM; ++i) {
-        const double ar = af[i].re, ai = af[i].im, nai = -ai;
+        const TR ar = af[i].re, ai = af[i].im, nai = -ai;
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-          const double br = bf[j].re, bi = bsign * bf[j].im;
+          const TR br = bf[j].re, bi = bsign * bf[j].im;
           cr[i][j] = MF::run(ar, br, cr[i][j]);
           ci[i][j] = MF::run(ar, bi, ci[i][j]);
           cr[i][j] = MF::run(nai, bi, cr[i][j]);
@@ -468,8 +473,8 @@ __global__ void __launch_bounds__(512) gemm_dma_c128_kernel(GemmArgs g, int conj
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm * 64 + i * 16 + MF::row(kq, r);
-        T* dst = C + (long)m * g.ldc + n;
-        T v{cr[i][j][r], ci[i][j][r]};
+        TO* dst = C + (long)m * g.ldc + n;
+        TO v{static_cast<TOR>(cr[i][j][r]), static_cast<TOR>(ci[i][j][r])};
         if (g.accumulate) {
           v.re += dst->re;
           v.im += dst->im;

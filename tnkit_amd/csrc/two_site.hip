@@ -989,10 +989,13 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   // the mixed form narrows theta first when its batches are packed (the workspace query
   // reserves room for it)
   const long nth = Q * dd * B;
-  constexpr bool kNarrow = std::is_same<TC, float>::value && std::is_same<TIO, double>::value;
-  float* Th = nullptr;
+  constexpr bool kNarrowR = std::is_same<TC, float>::value && std::is_same<TIO, double>::value;
+  constexpr bool kNarrowC = std::is_same<TC, c64>::value && std::is_same<TIO, c128>::value;
+  constexpr bool kNarrow = kNarrowR || kNarrowC;
+  constexpr long kParts = kNarrowC ? 2 : 1;   // doubles narrowed per element
+  TC* Th = nullptr;
   if constexpr (kNarrow) {
-    if (batch == 1 || sth == nth) Th = ar.take<float>(nth * batch);
+    if (batch == 1 || sth == nth) Th = ar.take<TC>(nth * batch);
   }
   TNH_REQUIRE(ar.ok(), TNH_ERR_WS, "apply_two_site_fused: workspace %zu < %zu bytes", ws_bytes,
               ar.used);
@@ -1004,7 +1007,7 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
       fused = try_gemm1_mix<TC>(batch, a, d1, d2, b, wl, wr, theta, sth, LT, sLT, S, sS, X2,
                                 nx2, st, rc);
       if (fused && rc != TNH_OK) return rc;
-    } else if constexpr (kNarrow) {
+    } else if constexpr (kNarrowR) {
       if (Th != nullptr) {
         const long n = nth * batch;
         long gsz = (n / 2 + 255) / 256;
@@ -1026,12 +1029,12 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   if ((stages & 1) && !fused) {  // GEMM1: M = d1 d2 b, N = p wl, K = q
     if constexpr (kNarrow) {
       if (Th != nullptr) {
-        const long n = nth * batch;
+        const long n = nth * batch * kParts;
         long g = (n / 2 + 255) / 256;
         if (g > 4096) g = 4096;
         if (g < 1) g = 1;
         hipLaunchKernelGGL(narrow_kernel, dim3(static_cast<unsigned>(g)), dim3(256), 0, st, n,
-                           static_cast<const double*>(theta), Th);
+                           static_cast<const double*>(theta), reinterpret_cast<float*>(Th));
         TNH_CHECK_LAUNCH("narrow");
         GemmArgs g1{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), Th,
                     dd * B, nth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
@@ -1295,8 +1298,9 @@ size_t tnh_two_site_ws_bytes(int dtype, int a, int d1, int d2, int b, int wl, in
   if (dtype == TNH_BF16)  // X1 in float, X2 in bf16
     return align_up(dd * b * a * wl * 4, 256) + align_up((size_t)wr * b * a * dd * 2, 256) + 256;
   const size_t e = dsize(dtype);
-  // float32: room for theta narrowed to float32 (used when the vectors are float64)
-  const size_t th = dtype == TNH_F32 ? align_up((size_t)a * dd * b * 4, 256) : 0;
+  // float32 / complex64: room for theta narrowed from float64 / complex128 vectors
+  const size_t th = (dtype == TNH_F32 || dtype == TNH_C64) ? align_up((size_t)a * dd * b * e, 256)
+                                                           : 0;
   return align_up(dd * b * a * wl * e, 256) + align_up((size_t)wr * b * a * dd * e, 256) + th +
          256;
 }
