@@ -545,13 +545,15 @@ def small_linalg(reps=20):
     return out
 
 
-def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, log=None):
+def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, log=None,
+            forms=("c128", "c64"), ragged="pad"):
     """Batched time evolution: `legs` TFIM quenches (N sites) from DMRG ground states at chi = 16
     for fields g0 ~ U[1.2, 2.0] (default_rng(1)) to g = 1, two-site TDVP with per-leg bond caps
     spread over 64..256 and a relative discarded-weight budget svd_tol per cut (so each leg's
-    bond follows its entanglement up to its cap), complex128 and complex64-operator (complex128
-    Krylov) forms, <Z_j>(t) against the exact free-fermion curves and against single-leg
-    tdvp2_evolve for two legs."""
+    bond follows its entanglement up to its cap; svd_tol = 0 keeps every leg at its cap once the
+    bond reaches it), complex128 and complex64-operator (complex128 Krylov) forms (`forms`), the
+    batch held as `ragged` ('pad', 'tight' or 'group'), <Z_j>(t) against the exact free-fermion
+    curves and, in the complex128 form, against single-leg tdvp2_evolve for two legs."""
     import numpy as np
     import torch
     from tnkit_amd import _dmrg as H
@@ -576,10 +578,13 @@ def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, lo
     mpo1 = tfim_mpo(N, 1.0)
     meas = lambda ss: [local_expectations(s, Z) for s in ss]  # noqa: E731
     out = {"workload": f"{legs} TFIM quenches N={N}, g0~U[1.2,2.0] (seed 1) -> g=1, TDVP2 "
-                       f"dt={dt} x {n_steps} steps, per-leg caps 64..256, svd_tol={svd_tol}",
+                       f"dt={dt} x {n_steps} steps, per-leg caps 64..256, svd_tol={svd_tol}, "
+                       f"ragged={ragged}",
            "legs": legs, "N": N, "caps": caps, "ground_state_s": t_gs}
     seq = {}
-    for name, low in (("c128", False), ("c64_krylov_c128", True)):
+    runs = [(nm, low) for nm, low in (("c128", False), ("c64_krylov_c128", True))
+            if nm.split("_")[0] in forms]
+    for name, low in runs:
         stats = {}
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -589,7 +594,7 @@ def config4(legs=32, N=64, n_steps=40, dt=0.05, meas_every=10, svd_tol=1e-12, lo
         res, rec = T.evolve_batch(states, mpo1, dt, n_steps, chi_max=caps, krylov_m=20,
                                   keep_device=True, measure=meas, meas_every=meas_every,
                                   stats=stats, low_precision=low, progress=prog,
-                                  svd_tol=svd_tol)
+                                  svd_tol=svd_tol, ragged=ragged)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t1
         err = max(float(np.abs(np.asarray(zs[j]) - ff[j][k]).max())
@@ -776,6 +781,13 @@ def main():
     ap.add_argument("--config5", action="store_true",
                     help="differentiation study: TFIM N=64 per-site fields, chi=256")
     ap.add_argument("--tsteps", type=int, default=40)
+    ap.add_argument("--svd-tol", type=float, default=1e-12,
+                    help="--config4: relative discarded-weight budget per cut (0: caps bind)")
+    ap.add_argument("--c4-forms", type=str, default="c128,c64",
+                    help="--config4: forms to run, from c128 (complex128) and c64 (complex64 "
+                         "operator, complex128 Krylov)")
+    ap.add_argument("--ragged", type=str, default="pad",
+                    help="--config4: how the batch holds uneven bonds: pad, tight or group")
     ap.add_argument("--block", action="store_true",
                     help="only the U(1) block-sparse matvec at chi=2048 (grouped launches)")
     ap.add_argument("--no-block", action="store_true")
@@ -828,7 +840,8 @@ def main():
         return
     if args.config4:
         print(json.dumps(config4(args.legs, 64 if args.N == 100 else args.N, args.tsteps,
-                                 log=sys.stderr)))
+                                 svd_tol=args.svd_tol, log=sys.stderr,
+                                 forms=tuple(args.c4_forms.split(",")), ragged=args.ragged)))
         return
     if args.sweep_cost:
         chis = [int(c) for c in args.sweep_cost.split(",")]
