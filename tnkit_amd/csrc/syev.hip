@@ -1268,8 +1268,11 @@ struct HtrdArgs {
   int* err;
 };
 
+// 256-thread workgroups are held to three per CU (168 VGPRs, a few spilled): twelve matrices of
+// order 512 per launch instead of eight, 10 % less time for 32 of them.
 template <int kHThreads>
-__global__ void __launch_bounds__(kHThreads) hetrd_coop_kernel(HtrdArgs a) {
+__global__ void __launch_bounds__(kHThreads, kHThreads == 256 ? 3 : 1)
+    hetrd_coop_kernel(HtrdArgs a) {
   constexpr int kHWaves = kHThreads / 64;
   __shared__ double red[2 * kHWaves];
   __shared__ double red8[2][kHWaves][2 * kHRows];
