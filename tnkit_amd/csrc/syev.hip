@@ -106,11 +106,8 @@ __global__ void bar_reset_kernel(unsigned* mem, int* err) {
 }
 
 // ---- phase 1: tridiagonalisation -------------------------------------------------------------
-constexpr int kTrdThreads = 1024;
-constexpr int kTrdCols = 2;                     // columns per thread (l = t + 1024 c)
 constexpr int kTrdRows = 8;                     // rows per workgroup
-constexpr int kSyevMaxN = kTrdThreads * kTrdCols;   // 2048
-constexpr int kTrdWaves = kTrdThreads / 64;
+constexpr int kSyevMaxN = 2048;                 // threads x columns per thread at the largest
 
 // Wave sum on VALU only: four DPP lane permutations (xor 1 and xor 2 inside quads, half-row and
 // row mirrors) leave every lane with its 16-lane row's sum, and the four row sums are read out of
@@ -188,6 +185,7 @@ struct TrdArgs {
 
 // Sum over the workgroup of one value per thread with a single barrier: red is used by
 // alternate calls in two halves, so a call's writes cannot meet the previous call's reads.
+template <int kTrdWaves>
 __device__ __forceinline__ double trd_sum1(double v, double* red, int& par) {
   v = wsum(v);
   double* rb = red + kTrdWaves * par;
@@ -200,17 +198,20 @@ __device__ __forceinline__ double trd_sum1(double v, double* red, int& par) {
   return s;
 }
 
-// Grid of G workgroups (G <= n, ceil(n / G) <= kTrdRows) of 1024 threads, launched
+// Grid of G workgroups (G <= n, ceil(n / G) <= kTrdRows) of kTrdThreads threads, launched
 // cooperatively. Row i of A lives in the REGISTERS of workgroup i mod G for the whole reduction:
-// thread t holds columns t and t + 1024 of its workgroup's (at most 8) rows, so the rank-2 update
-// and the product A v are register arithmetic and the only on-chip traffic is the reductions.
+// thread t holds columns t + kTrdThreads c (c < kTrdCols) of its workgroup's (at most 8) rows,
+// so the rank-2 update and the product A v are register arithmetic and the only on-chip traffic
+// is the reductions.
 // A is read once (row i by workgroup i mod G, as given: the caller passes a symmetric matrix).
 //
 // Column j after the barrier: gamma from the partials, w_{j-1} = p - gamma v (v_{j-1}[j] = 1),
 // row j = its published copy - v w^T - w v^T at row j, the reflector v_j, then one pass over
 // the own rows that applies the rank-2 update of column j-1 and accumulates p_j = tau_j A v_j;
 // the owner of row j+1 publishes it from the same registers.
+template <int kTrdThreads, int kTrdCols>
 __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
+  constexpr int kTrdWaves = kTrdThreads / 64;
   __shared__ double red[2 * kTrdWaves];
   __shared__ double red8[2][kTrdWaves][kTrdRows];
   __shared__ double own_v[kTrdRows], own_p[kTrdRows], own_vn[kTrdRows];
@@ -301,7 +302,7 @@ __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
       const int l = t + kTrdThreads * c;
       if (l >= j + 2 && l < n) sq += R[c] * R[c];
     }
-    const double sigma = trd_sum1(sq, red, par);
+    const double sigma = trd_sum1<kTrdWaves>(sq, red, par);
     double beta = alpha, tj = 0.0, scale = 0.0;
     if (sigma > 0.0) {
       beta = -copysign(sqrt(alpha * alpha + sigma), alpha);
@@ -389,6 +390,19 @@ __global__ void __launch_bounds__(kTrdThreads) sytrd_coop_kernel(TrdArgs a) {
 
 int trd_grid(int n) { return (n + kTrdRows - 1) / kTrdRows; }
 
+// Workgroup shape of the reduction by order (8 rows each): 256 threads of 2 columns up to order
+// 512, 512 threads of 2 columns up to 1024, 512 threads of 4 columns up to 2048. Measured at
+// 512 / 1024 / 2048: 3.8 / 7.7 / 18.4 ms for the whole solve, against 4.8 / 9.2 / 19.5 with
+// 1024 threads of 2 columns at every order (idle columns, and wave sums over 16 waves), and
+// 21.6 ms at 2048 with 256 threads of 8.
+int trd_threads(int n) { return n <= 512 ? 256 : 512; }
+
+const void* trd_kernel(int n) {
+  if (n <= 512) return reinterpret_cast<const void*>(&sytrd_coop_kernel<256, 2>);
+  if (n <= 1024) return reinterpret_cast<const void*>(&sytrd_coop_kernel<512, 2>);
+  return reinterpret_cast<const void*>(&sytrd_coop_kernel<512, 4>);
+}
+
 int trd_slot(int n) { return (n + 31) / 32 * 32 + 256; }
 
 size_t trd_ws_bytes(int n) {
@@ -420,8 +434,8 @@ int launch_sytrd(int n, const double* A, long lda, double* Vr, double* d, double
   hipError_t he;
   TrdArgs args{n, ns, gs, A, lda, Vr, d, e, tau, P, Rp, part, GridBar{bar, err}};
   void* kargs[] = {&args};
-  he = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&sytrd_coop_kernel), dim3(G),
-                                  dim3(kTrdThreads), kargs, (unsigned)smem, st);
+  const int nt = trd_threads(n);
+  he = hipLaunchCooperativeKernel(trd_kernel(n), dim3(G), dim3(nt), kargs, (unsigned)smem, st);
   TNH_REQUIRE(he == hipSuccess, TNH_ERR_HIP,
               "sytrd: cooperative launch of %d workgroups (%zu B LDS each) refused: %s", G, smem,
               hipGetErrorString(he));
