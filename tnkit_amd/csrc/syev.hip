@@ -586,8 +586,15 @@ __global__ void __launch_bounds__(1024) dc_prep_kernel(DcLevel L) {
     const double tol = 8.0 * kEps * fmax(dmax, rhop);
     int nk = 0, nr = 0, jl = -1;
     double zl = 0.0, dl = 0.0;
+    // entry j + 1 is read while entry j is processed (an iteration writes entries j and jl < j
+    // only), so the LDS latency of the next read overlaps this iteration's work
+    double zn = zs[0], dn = Ds[0];
     for (int j = 0; j < K; ++j) {
-      double zj = zs[j], dj = Ds[j];
+      double zj = zn, dj = dn;
+      if (j + 1 < K) {
+        zn = zs[j + 1];
+        dn = Ds[j + 1];
+      }
       if (rhop * fabs(zj) <= tol) {
         zs[j] = 0.0;
         sinv[j] = -1;
