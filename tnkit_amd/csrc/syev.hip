@@ -1985,11 +1985,12 @@ __global__ void qr_status_kernel(const int* err, int* info) {
   if (threadIdx.x == 0) *info |= *err ? 1 : 0;
 }
 
-// Threads per workgroup for rows of length m (two entries per thread): complex rows of more than
-// 1024 entries would not fit the registers and are refused by the entry.
+// Threads per workgroup for rows of length m: two entries per thread up to 1024, float64 rows up
+// to 2048 as 512 threads of four (the tridiagonalisation's measured best shape); complex rows of
+// more than 1024 entries would not fit the registers and are refused by the entry.
 template <typename T>
 int qr_threads(int m) {
-  return m <= 512 ? 256 : (m <= 1024 ? 512 : 1024);
+  return m <= 512 ? 256 : 512;
 }
 
 template <typename T>
@@ -2001,7 +2002,7 @@ template <typename T>
 const void* qr_kernel(int m) {
   if (m <= 512) return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 256, 2>);
   if constexpr (sizeof(T) == 8) {
-    if (m > 1024) return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 1024, 2>);
+    if (m > 1024) return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 512, 4>);
   }
   return reinterpret_cast<const void*>(&geqrf_coop_kernel<T, 512, 2>);
 }
