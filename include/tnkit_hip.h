@@ -169,6 +169,22 @@ int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1
                                long sout, void* ws, size_t ws_bytes, int tile1, int tile3,
                                int stages, tnh_stream_t stream);
 
+/* The batched matvec over legs of different bonds held in one padded batch (tnkit's ragged
+ * batches, tnkit/batch_tdvp.py:42-55 and :572-681, where every leg computes at the stacked
+ * width): live[2z], live[2z+1] (device ints) = leg z's live left and right bonds (a_z <= a,
+ * b_z <= b; theta, L and R are zero outside them, as the masked batch holds them). Same operands
+ * and workspace as tnh_apply_two_site_batched with all three stages; complex128 vectors with a
+ * complex128 or complex64 operator. On shapes that take the direct-to-LDS GEMMs (stacked
+ * d1 d2 b, a wl, a d1 d2 and b multiples of 128) the tiles of a leg that lie outside its live
+ * block skip their products and store zeros, and the K loops stop at its live extent, so a leg
+ * costs about its own bonds rather than the stacked ones; other shapes compute in full. The
+ * output is zero outside every leg's live block either way. */
+int tnh_apply_two_site_ragged(int dtype, int io_dtype, int batch, int a, int d1, int d2, int b,
+                              int wl, int wr, const void* LT, long sLT, const void* S, long sS,
+                              const void* RT, long sRT, const void* theta, long sth, void* out,
+                              long sout, const int* live, void* ws, size_t ws_bytes,
+                              tnh_stream_t stream);
+
 /* update_left_env / update_right_env over a batch (tnkit/batch_tdvp.py:231-243); complex inputs
  * take conj(A) / conj(B) as tnkit does. */
 int tnh_update_left_env_batched(int dtype, int batch, int a, int d, int c, int wl, int wr,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "tnh_fused_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 7 + [_P]),
     "tnh_apply_two_site_batched": (c_int, [c_int] * 9 + [_P, c_long] * 5 + [_P, c_size_t, c_int,
                                                                           c_int, c_int, _P]),
+    "tnh_apply_two_site_ragged": (c_int, [c_int] * 9 + [_P, c_long] * 5 + [_P, _P, c_size_t, _P]),
     "tnh_update_left_env_batched": (c_int, [c_int] * 7 + [_P, c_long] * 4 + [_P, c_size_t, _P]),
     "tnh_update_right_env_batched": (c_int, [c_int] * 7 + [_P, c_long] * 4 + [_P, c_size_t, _P]),
     "tnh_expm_ws_bytes": (c_size_t, [c_int]),

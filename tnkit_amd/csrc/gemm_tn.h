@@ -56,6 +56,15 @@ struct GemmArgs {
   long ldc;
   long strideC;
   int accumulate;  // C += A^T B instead of C = A^T B
+  // Live extents of batch entries (the complex direct-to-LDS kernel only; others ignore them):
+  // live[2z], live[2z+1] = (x, y) of entry z, and the product outside its live block is zero by
+  // construction of the operands. live_mode 1 (the two-site GEMM1): a tile of rows m0 with
+  // m0 mod lp1 >= y or of columns n0 >= x lp2 is dead, K stops at x. live_mode 2 (GEMM3): rows
+  // m0 >= x lp2 or columns n0 >= y are dead, and of K only rows k with k mod lp1 < y are read.
+  // Dead tiles store zeros (or leave C as it is when accumulating).
+  const int* live = nullptr;
+  int live_mode = 0;
+  int lp1 = 0, lp2 = 0;
 };
 
 // Tile -> block mapping. Workgroups are dispatched round-robin over the 8 XCDs, each with its
@@ -413,9 +422,37 @@ __global__ void __launch_bounds__(512) gemm_dma_cplx_kernel(GemmArgs g, int conj
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j) cr[i][j] = ci[i][j] = acc_t{};
-  const int nk = K / BK;
+  // K steps: all of them, or the live ones of a ragged batch entry (runs of spr steps every kmod
+  // rows; a step is read whole, its rows past the live extent are zeros)
+  int nk = K / BK, spr = nk, kstride = nk;
+  if (g.live != nullptr && g.live_mode != 0) {
+    const int x = g.live[2 * bz], y = g.live[2 * bz + 1];
+    bool dead;
+    int klim, kmod;
+    if (g.live_mode == 1) {
+      dead = (m0 % g.lp1) >= y || n0 >= x * g.lp2;
+      klim = x;
+      kmod = K;
+    } else {
+      dead = m0 >= x * g.lp2 || n0 >= y;
+      klim = y;
+      kmod = g.lp1;
+    }
+    if (dead) {
+      if (!g.accumulate) {
+        const TO z{TOR(0), TOR(0)};
+        for (int e = t; e < BM * BM; e += NT) C[(long)(m0 + e / BM) * g.ldc + n0 + e % BM] = z;
+      }
+      return;
+    }
+    kstride = kmod / BK;
+    spr = (min(klim, kmod) + BK - 1) / BK;
+    nk = (K / kmod) * spr;
+  }
+  auto kphys = [&](int i) { return (i / spr) * kstride + i % spr; };
   const int aoff = wm * 64 + row16 + 16 * kq, boff = wn * WCOL + row16 + 16 * kq;
-  auto issue = [&](int kt, T* as, T* bs) {
+  auto issue = [&](int i, T* as, T* bs) {
+    const int kt = kphys(i);
     const T* a = A + (long)kt * BK * g.lda;
     const T* b = B + (long)kt * BK * g.ldb;
 #pragma unroll

@@ -980,7 +980,7 @@ template <typename TC, typename TIO>
 int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, const void* LT,
                    long sLT, const void* S, long sS, const void* RT, long sRT, const void* theta,
                    long sth, void* out, long sout, void* ws, size_t ws_bytes, hipStream_t st,
-                   int tile1, int tile3, int stages) {
+                   int tile1, int tile3, int stages, const int* live = nullptr) {
   const long P = a, Q = a, B = b, dd = (long)d1 * d2;
   const long nx1 = dd * B * P * wl, nx2 = (long)wr * B * P * dd;
   Arena ar(ws, ws_bytes);
@@ -1037,13 +1037,15 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
                            static_cast<const double*>(theta), reinterpret_cast<float*>(Th));
         TNH_CHECK_LAUNCH("narrow");
         GemmArgs g1{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), Th,
-                    dd * B, nth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+                    dd * B, nth, LT, P * wl, sLT, X1, P * wl, nx1, 0,
+                    live, live ? 1 : 0, static_cast<int>(B), wl};
         TNH_TRY((gemm_any<TC, TC, TC, TC>(g1, batch, 0, st, tile1)));
       }
     }
     if (Th == nullptr) {
       GemmArgs g{static_cast<int>(dd * B), static_cast<int>(P * wl), static_cast<int>(Q), theta,
-                 dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0};
+                 dd * B, sth, LT, P * wl, sLT, X1, P * wl, nx1, 0,
+                 live, live ? 1 : 0, static_cast<int>(B), wl};
       TNH_TRY((gemm_any<TC, TIO, TC, TC>(g, batch, 0, st, tile1)));
     }
   }
@@ -1054,7 +1056,8 @@ int apply_two_site(int batch, int a, int d1, int d2, int b, int wl, int wr, cons
   }
   if (stages & 4) {  // GEMM3: M = p d1 d2, N = b', K = wr b
     GemmArgs g{static_cast<int>(P * dd), static_cast<int>(B), static_cast<int>(wr * B), X2, P * dd,
-               nx2, RT, B, sRT, out, B, sout, 0};
+               nx2, RT, B, sRT, out, B, sout, 0,
+               live, live ? 2 : 0, static_cast<int>(B), static_cast<int>(dd)};
     TNH_TRY((gemm_any<TC, TC, TC, TIO>(g, batch, 0, st, tile3)));
   }
   return TNH_OK;
@@ -1330,6 +1333,27 @@ int tnh_apply_two_site_batched(int dtype, int io_dtype, int batch, int a, int d1
                                        theta, sth, out, sout, ws, ws_bytes, st, tile1, tile3,
                                        stages);
   set_error("tnh_apply_two_site_fused: unsupported dtype pair (%d, %d)", dtype, io_dtype);
+  return TNH_ERR_UNSUPPORTED;
+}
+
+int tnh_apply_two_site_ragged(int dtype, int io_dtype, int batch, int a, int d1, int d2, int b,
+                              int wl, int wr, const void* LT, long sLT, const void* S, long sS,
+                              const void* RT, long sRT, const void* theta, long sth, void* out,
+                              long sout, const int* live, void* ws, size_t ws_bytes,
+                              tnh_stream_t stream) {
+  TNH_REQUIRE(a > 0 && b > 0 && wl > 0 && wr > 0 && d1 > 0 && d2 > 0 && live, TNH_ERR_ARG,
+              "tnh_apply_two_site_ragged: sizes must be positive and live given");
+  TNH_REQUIRE(batch >= 1 && batch <= 65535, TNH_ERR_ARG, "tnh_apply_two_site_ragged: batch %d",
+              batch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dtype == TNH_C128 && io_dtype == TNH_C128)
+    return apply_two_site<c128, c128>(batch, a, d1, d2, b, wl, wr, LT, sLT, S, sS, RT, sRT, theta,
+                                      sth, out, sout, ws, ws_bytes, st, 0, 0, 7, live);
+  if (dtype == TNH_C64 && io_dtype == TNH_C128)
+    return apply_two_site<c64, c128>(batch, a, d1, d2, b, wl, wr, LT, sLT, S, sS, RT, sRT, theta,
+                                     sth, out, sout, ws, ws_bytes, st, 0, 0, 7, live);
+  set_error("tnh_apply_two_site_ragged: complex128 vectors with a complex128 or complex64 "
+            "operator only (dtype pair %d, %d)", dtype, io_dtype);
   return TNH_ERR_UNSUPPORTED;
 }
 

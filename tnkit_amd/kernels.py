@@ -558,8 +558,12 @@ def fused_two_site_b(L: torch.Tensor, W1: torch.Tensor, W2: torch.Tensor, R: tor
 
 
 def apply_two_site_b(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor, RT: torch.Tensor,
-                     out: Optional[torch.Tensor] = None, ws: Optional[torch.Tensor] = None):
-    """H_eff theta for k legs: theta (k, a, d1, d2, b) with the operator of `fused_two_site_b`."""
+                     out: Optional[torch.Tensor] = None, ws: Optional[torch.Tensor] = None,
+                     live: Optional[torch.Tensor] = None):
+    """H_eff theta for k legs: theta (k, a, d1, d2, b) with the operator of `fused_two_site_b`.
+    `live` (k, 2) int32 on the device: each leg's live left and right bonds in a padded batch
+    (theta, L and R zero outside them); complex vectors then go through the ragged entry, whose
+    products skip what lies outside each leg's live block."""
     _need_b(theta, "theta", 5)
     k, q, d1, d2, b = theta.shape
     if LT.shape[:2] != (k, q) or RT.shape[0] != k or RT.shape[2] != b:
@@ -574,6 +578,16 @@ def apply_two_site_b(theta: torch.Tensor, LT: torch.Tensor, S: torch.Tensor, RT:
         ws = _ws(nbytes, theta.device)
     if out is None:
         out = torch.empty(theta.shape, dtype=theta.dtype, device=theta.device)
+    if live is not None and theta.is_complex():
+        if (live.dtype != torch.int32 or tuple(live.shape) != (k, 2) or not live.is_contiguous()
+                or live.device != theta.device):
+            raise ValueError(f"apply_two_site_b: live must be a contiguous (k, 2) int32 device "
+                             f"tensor, got {tuple(live.shape)} {live.dtype}")
+        check(lib.tnh_apply_two_site_ragged(
+            dt, io, k, q, d1, d2, b, wl, wr, ptr(LT), _bstride(LT), ptr(S), _bstride(S), ptr(RT),
+            _bstride(RT), ptr(theta), _bstride(theta), ptr(out), _bstride(out), ptr(live),
+            ptr(ws), ws.numel(), stream(theta.device)), "apply_two_site_b")
+        return out
     check(lib.tnh_apply_two_site_batched(
         dt, io, k, q, d1, d2, b, wl, wr, ptr(LT), _bstride(LT), ptr(S), _bstride(S), ptr(RT),
         _bstride(RT), ptr(theta), _bstride(theta), ptr(out), _bstride(out), ptr(ws), ws.numel(), 0, 0,

@@ -466,32 +466,44 @@ def _gram_cut(mat: torch.Tensor, ranks_cap: np.ndarray, ceiling: np.ndarray, svd
 # --------------------------------------------------------------------------------------------
 
 
+def _live(Bt: _Batch, left: int, right: int) -> torch.Tensor:
+    """(k, 2) int32 device tensor of each leg's true bonds at positions left and right: the live
+    block of the padded batch, outside which the legs' tensors are zero."""
+    lv = np.stack([Bt.rank[left], Bt.rank[right]], axis=1).astype(np.int32)
+    return torch.as_tensor(lv, device=Bt.M[0].device)
+
+
 def _matvec_two(Bt: _Batch, i: int, shape, low: bool):
-    """(matvec, graph context) of the batched two-site H_eff at bond i."""
+    """(matvec, graph context) of the batched two-site H_eff at bond i. The legs' true bonds ride
+    along (`_live`), so the matvec's products skip what lies outside each leg's own block."""
     op = kernels.fused_two_site_b(Bt.L[i], Bt.W[i], Bt.W[i + 1], Bt.R[i + 2],
                                   dtype=torch.complex64 if low else None)
+    op = (*op, _live(Bt, i, i + 2))
 
     def make(o):
         def mv(x):
-            return kernels.apply_two_site_b(x.reshape(shape), *o).reshape(shape[0], -1)
+            return kernels.apply_two_site_b(x.reshape(shape), *o[:3], live=o[3]).reshape(
+                shape[0], -1)
         return mv
     key = ("two", tuple(shape), tuple(tuple(t.shape) for t in op), low)
     return make(op), (key, op, make)
 
 
-def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool):
+def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool, live: torch.Tensor):
     """One-site H_eff (tnkit/_dmrg.py:192-199) as the two-site kernel with d2 = 1 and the
     identity of the right MPO bond as the second MPO tensor: S[(s r)][(w s')] = W[w][r][s][s'].
-    Returns (matvec, graph context)."""
+    `live` = the legs' true bonds either side (see `_live`). Returns (matvec, graph context)."""
     k, a, d, b = shape
     wr = int(W.shape[2])
     Wid = torch.eye(wr, dtype=W.dtype, device=W.device).reshape(1, wr, wr, 1, 1)
     Wid = Wid.expand(k, -1, -1, -1, -1).contiguous()
     op = kernels.fused_two_site_b(L, W, Wid, R, dtype=torch.complex64 if low else None)
+    op = (*op, live)
 
     def make(o):
         def mv(x):
-            return kernels.apply_two_site_b(x.reshape(k, a, d, 1, b), *o).reshape(k, -1)
+            return kernels.apply_two_site_b(x.reshape(k, a, d, 1, b), *o[:3],
+                                            live=o[3]).reshape(k, -1)
         return mv
     key = ("one", tuple(shape), tuple(tuple(t.shape) for t in op), low)
     return make(op), (key, op, make)
@@ -539,7 +551,8 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
         Bt.L[i + 1] = kernels.update_left_env_b(Bt.L[i], Bt.M[i], Bt.W[i])
         C = (Sc[:, :, None] * Vh).reshape(k, nk, d, b).contiguous()
         if i < Bt.N - 2:
-            mv, gctx = _matvec_one(Bt, Bt.L[i + 1], Bt.W[i + 1], Bt.R[i + 2], (k, nk, d, b), low)
+            mv, gctx = _matvec_one(Bt, Bt.L[i + 1], Bt.W[i + 1], Bt.R[i + 2], (k, nk, d, b), low,
+                                   _live(Bt, i + 1, i + 2))
             C = lanczos_expm_fixed_batched(mv, C.reshape(k, -1), 0.5j * dt, m, resid,
                                            gctx).reshape(k, nk, d, b)
         Bt.M[i + 1] = C.contiguous()
@@ -548,7 +561,8 @@ def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, m
         Bt.R[i + 1] = kernels.update_right_env_b(Bt.R[i + 2], Bt.M[i + 1], Bt.W[i + 1])
         C = (U * Sc[:, None, :]).reshape(k, a, d, nk).contiguous()
         if i > 0:
-            mv, gctx = _matvec_one(Bt, Bt.L[i], Bt.W[i], Bt.R[i + 1], (k, a, d, nk), low)
+            mv, gctx = _matvec_one(Bt, Bt.L[i], Bt.W[i], Bt.R[i + 1], (k, a, d, nk), low,
+                                   _live(Bt, i, i + 1))
             C = lanczos_expm_fixed_batched(mv, C.reshape(k, -1), 0.5j * dt, m, resid,
                                            gctx).reshape(k, a, d, nk)
         Bt.M[i] = C.contiguous()
