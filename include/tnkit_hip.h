@@ -207,6 +207,16 @@ size_t tnh_expm_ws_bytes(int batch);
 int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w, long sw,
                     double* ab, long sab, void* ws, size_t ws_bytes, tnh_stream_t stream);
 
+/* tnh_expm_step_c over a padded batch of legs with different bonds (tnkit's ragged batches,
+ * tnkit/batch_tdvp.py:42-55): live[2z], live[2z+1] (device ints) = leg z's live extents (x, y) in
+ * vectors laid out as rows of `rowlen` entries whose last index runs over `bpad` (theta (a, d1,
+ * d2, b): rowlen = d1 d2 b, bpad = b; x, y = the leg's left and right bonds). Entry i of a leg is
+ * live when i / rowlen < x and i % bpad < y; the others are zero in every vector of the leg and
+ * the passes skip them, with the same result. */
+int tnh_expm_step_c_ragged(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w,
+                           long sw, double* ab, long sab, const int* live, int rowlen, int bpad,
+                           void* ws, size_t ws_bytes, tnh_stream_t stream);
+
 /* out[leg] = sum_{k<nrows} c[leg][k] V[leg][k] with complex coefficients c (device, interleaved
  * re/im, leg stride sc doubles): the final combination beta0 * expm(tau T)[:, 0] . V. */
 int tnh_lincomb_c(int batch, long n, int nrows, const void* V, long ldv, long sV, const double* c,

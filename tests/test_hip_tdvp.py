@@ -288,3 +288,32 @@ def test_ragged_matvec_equals_the_padded_one(hip, a, b, low):
         assert np.abs(rag[z][blk] - full[z][blk]).max() <= 1e-14 * max(np.abs(full[z]).max(), 1e-300)
         assert np.abs(rag[z][~blk]).max(initial=0.0) == 0.0
         assert np.abs(full[z][~blk]).max(initial=0.0) == 0.0
+
+
+def test_ragged_krylov_step_equals_the_padded_one(hip):
+    """tnh_expm_step_c_ragged on a batch whose legs are zero outside their own bonds gives the
+    basis and coefficients of the padded step, bit for bit (the skipped entries are zeros)."""
+    from tnkit_amd import kernels
+    rng = np.random.default_rng(11)
+    k, m, a, d, b = 4, 6, 48, 2, 40
+    ra, rb = np.array([48, 20, 7, 33]), np.array([40, 9, 40, 1])
+    n = a * d * d * b
+    V = np.zeros((k, m, a, d, d, b), complex)
+    w = np.zeros((k, a, d, d, b), complex)
+    for z in range(k):
+        sh = (ra[z], d, d, rb[z])
+        V[z, :, :ra[z], :, :, :rb[z]] = rng.standard_normal((m,) + sh) + 1j * rng.standard_normal((m,) + sh)
+        w[z, :ra[z], :, :, :rb[z]] = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    live = torch.as_tensor(np.stack([ra, rb], 1).astype(np.int32), device=hip)
+    outs = []
+    for rag in (None, (live, d * d * b, b)):
+        Vt = torch.as_tensor(V.reshape(k, m, n)).to(hip)
+        Vt[:, 0] /= torch.linalg.vector_norm(Vt[:, 0], dim=1, keepdim=True)
+        ab = torch.zeros((k, 2 * m), dtype=torch.float64, device=hip)
+        ws = kernels.expm_ws(k, hip)
+        for j in range(3):
+            wt = torch.as_tensor(w.reshape(k, n) * (j + 1)).to(hip)
+            kernels.expm_step_c(Vt, wt, ab, j, ws, ragged=rag)
+        outs.append((Vt[:, :4].cpu().numpy(), ab.cpu().numpy()))
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0], outs[1][0])

@@ -57,6 +57,8 @@ SIGNATURES = {
     "tnh_expm_ws_bytes": (c_size_t, [c_int]),
     "tnh_expm_step_c": (c_int, [c_int, c_long, c_int, c_int, _P, c_long, c_long, _P, c_long, _P,
                                 c_long, _P, c_size_t, _P]),
+    "tnh_expm_step_c_ragged": (c_int, [c_int, c_long, c_int, c_int, _P, c_long, c_long, _P, c_long,
+                                       _P, c_long, _P, c_int, c_int, _P, c_size_t, _P]),
     "tnh_lincomb_c": (c_int, [c_int, c_long, c_int, _P, c_long, c_long, _P, c_long, _P, c_long,
                               _P]),
     "tnh_krylov_ws_bytes": (c_size_t, [c_long, c_int]),

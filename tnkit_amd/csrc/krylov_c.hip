@@ -25,14 +25,30 @@ __device__ __forceinline__ double wsum(double v) {
 // One pass over the live basis of every leg: optional three-term subtraction, subtraction of
 // the previous pass's projection (complex coefficients `coef`), then partial sums of the next
 // projection <V[k], w> (mode 0) or of |w|^2 (mode 1). part is [leg][block][2*NV] doubles.
+// Ragged batches: `live` (2 ints per leg, may be null) = the leg's live extents (x, y) in a
+// vector laid out as rows of `rowlen` entries whose last index runs over `bpad`: entry i is live
+// when i / rowlen < x and i % bpad < y. The others are zero in every vector of the leg, so a pass
+// skips their loads and stores (the sums are the same).
+struct Ragged {
+  const int* live;
+  int rowlen, bpad;
+};
+
+__device__ __forceinline__ bool live_entry(const Ragged& rg, int lx, int ly, long i) {
+  if (rg.live == nullptr) return true;
+  const int ii = static_cast<int>(i);
+  return ii / rg.rowlen < lx && ii % rg.bpad < ly;
+}
+
 template <int NV>
 __global__ void __launch_bounds__(256)
     cgs_c_kernel(long n, int nv, const c128* __restrict__ V, long ldv, long sV, c128* __restrict__ w,
                  long sw, const double* __restrict__ coef, long scoef, const double* __restrict__ beta,
-                 long sbeta, int beta_row, int mode, double* __restrict__ part, int nbl) {
+                 long sbeta, int beta_row, int mode, double* __restrict__ part, int nbl, Ragged rg) {
   const int leg = blockIdx.y;
   V += leg * sV;
   w += leg * sw;
+  const int lx = rg.live ? rg.live[2 * leg] : 0, ly = rg.live ? rg.live[2 * leg + 1] : 0;
   __shared__ double cs[2 * NV];
   __shared__ double red[4][2 * NV];
   for (int k = threadIdx.x; k < 2 * NV; k += 256)
@@ -44,6 +60,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
   for (int k = 0; k < NV; ++k) ar[k] = ai[k] = 0.0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)nbl * 256) {
+    if (!live_entry(rg, lx, ly, i)) continue;
     c128 wi = w[i];
     if (beta) {
       const c128 vp = V[(long)beta_row * ldv + i];
@@ -145,18 +162,19 @@ __global__ void lincomb_c_kernel(long n, int nrows, const c128* __restrict__ V, 
 template <int NV>
 int cgs_pass(int batch, long n, int nv, const c128* V, long ldv, long sV, c128* w, long sw,
              const double* coef, long scoef, const double* beta, long sbeta, int beta_row, int mode,
-             double* part, int nbl, hipStream_t st) {
+             double* part, int nbl, const Ragged& rg, hipStream_t st) {
   hipLaunchKernelGGL(cgs_c_kernel<NV>, dim3(nbl, batch), dim3(256), 0, st, n, nv, V, ldv, sV, w, sw,
-                     coef, scoef, beta, sbeta, beta_row, mode, part, nbl);
+                     coef, scoef, beta, sbeta, beta_row, mode, part, nbl, rg);
   TNH_CHECK_LAUNCH("cgs_c");
   return TNH_OK;
 }
 
 int cgs_any(int batch, long n, int nv, const c128* V, long ldv, long sV, c128* w, long sw,
             const double* coef, long scoef, const double* beta, long sbeta, int beta_row, int mode,
-            double* part, int nbl, hipStream_t st) {
+            double* part, int nbl, const Ragged& rg, hipStream_t st) {
 #define TNH_CGS(NV) \
-  cgs_pass<NV>(batch, n, nv, V, ldv, sV, w, sw, coef, scoef, beta, sbeta, beta_row, mode, part, nbl, st)
+  cgs_pass<NV>(batch, n, nv, V, ldv, sV, w, sw, coef, scoef, beta, sbeta, beta_row, mode, part, nbl, \
+               rg, st)
   if (nv <= 4) return TNH_CGS(4);
   if (nv <= 8) return TNH_CGS(8);
   if (nv <= 16) return TNH_CGS(16);
@@ -176,8 +194,13 @@ size_t tnh_expm_ws_bytes(int batch) {
          2 * align_up((size_t)(batch > 0 ? batch : 1) * 2 * kKMax * sizeof(double), 256) + 256;
 }
 
-int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w, long sw,
-                    double* ab, long sab, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+}  // extern "C"
+
+namespace tnh {
+namespace {
+int expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w, long sw,
+                double* ab, long sab, void* ws, size_t ws_bytes, const Ragged& rg,
+                tnh_stream_t stream) {
   TNH_REQUIRE(batch >= 1 && batch <= 65535 && n > 0 && ldv >= n, TNH_ERR_ARG,
               "tnh_expm_step_c: bad sizes (batch %d, n %ld, ldv %ld)", batch, n, ldv);
   TNH_REQUIRE(m >= 1 && m <= kKMax && j >= 0 && j < m, TNH_ERR_ARG,
@@ -205,15 +228,15 @@ int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV,
   } while (0)
   // three-term subtraction fused into pass 1's projection
   TNH_T(cgs_any(batch, n, nv, Vc, ldv, sV, wc, sw, nullptr, 0, j > 0 ? ab + m + (j - 1) : nullptr,
-                sab, j - 1, 0, part, (int)nbl, st));
+                sab, j - 1, 0, part, (int)nbl, rg, st));
   hipLaunchKernelGGL(reduce_c_kernel, dim3(2 * nv, batch), dim3(64), 0, st, (int)nbl, 2 * nv, pst,
                      part, c1, sc, 2, j, m, ab, sab);
   TNH_CHECK_LAUNCH("reduce_c");
-  TNH_T(cgs_any(batch, n, nv, Vc, ldv, sV, wc, sw, c1, sc, nullptr, 0, 0, 0, part, (int)nbl, st));
+  TNH_T(cgs_any(batch, n, nv, Vc, ldv, sV, wc, sw, c1, sc, nullptr, 0, 0, 0, part, (int)nbl, rg, st));
   hipLaunchKernelGGL(reduce_c_kernel, dim3(2 * nv, batch), dim3(64), 0, st, (int)nbl, 2 * nv, pst,
                      part, c2, sc, 0, j, m, ab, sab);
   TNH_CHECK_LAUNCH("reduce_c");
-  TNH_T(cgs_any(batch, n, nv, Vc, ldv, sV, wc, sw, c2, sc, nullptr, 0, 0, 1, part, (int)nbl, st));
+  TNH_T(cgs_any(batch, n, nv, Vc, ldv, sV, wc, sw, c2, sc, nullptr, 0, 0, 1, part, (int)nbl, rg, st));
   hipLaunchKernelGGL(reduce_c_kernel, dim3(1, batch), dim3(64), 0, st, (int)nbl, 1, pst, part,
                      c2, sc, 1, j, m, ab, sab);
   TNH_CHECK_LAUNCH("reduce_c");
@@ -227,6 +250,27 @@ int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV,
   }
 #undef TNH_T
   return TNH_OK;
+}
+}  // namespace
+}  // namespace tnh
+
+extern "C" {
+
+int tnh_expm_step_c(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w, long sw,
+                    double* ab, long sab, void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  return tnh::expm_step_c(batch, n, m, j, V, ldv, sV, w, sw, ab, sab, ws, ws_bytes,
+                          tnh::Ragged{nullptr, 1, 1}, stream);
+}
+
+int tnh_expm_step_c_ragged(int batch, long n, int m, int j, void* V, long ldv, long sV, void* w,
+                           long sw, double* ab, long sab, const int* live, int rowlen, int bpad,
+                           void* ws, size_t ws_bytes, tnh_stream_t stream) {
+  TNH_REQUIRE(live && rowlen >= 1 && bpad >= 1 && rowlen % bpad == 0 && n % rowlen == 0 &&
+                  n < (1L << 31),
+              TNH_ERR_ARG, "tnh_expm_step_c_ragged: bad layout (n %ld, rowlen %d, bpad %d)", n,
+              rowlen, bpad);
+  return tnh::expm_step_c(batch, n, m, j, V, ldv, sV, w, sw, ab, sab, ws, ws_bytes,
+                          tnh::Ragged{live, rowlen, bpad}, stream);
 }
 
 int tnh_lincomb_c(int batch, long n, int nrows, const void* V, long ldv, long sV, const double* c,

@@ -633,10 +633,19 @@ def update_right_env_b(R: torch.Tensor, B: torch.Tensor, W: torch.Tensor) -> tor
     return out
 
 
-def expm_step_c(V: torch.Tensor, w: torch.Tensor, ab: torch.Tensor, j: int, ws: torch.Tensor):
+def expm_step_c(V: torch.Tensor, w: torch.Tensor, ab: torch.Tensor, j: int, ws: torch.Tensor,
+                ragged=None):
     """Step j of the batched complex Krylov exponential (see tnh_expm_step_c): V (k, m, n),
-    w (k, n) complex128, ab (k, 2m) float64 on the device."""
+    w (k, n) complex128, ab (k, 2m) float64 on the device. `ragged` = (live (k, 2) int32 device
+    tensor, rowlen, bpad): the legs' live extents in a padded batch (tnh_expm_step_c_ragged)."""
     k, m, n = V.shape
+    if ragged is not None:
+        live, rowlen, bpad = ragged
+        check(_lib.load().tnh_expm_step_c_ragged(k, n, m, j, ptr(V), V.stride(1), V.stride(0),
+                                                 ptr(w), w.stride(0), ptr(ab), ab.stride(0),
+                                                 ptr(live), int(rowlen), int(bpad), ptr(ws),
+                                                 ws.numel(), stream(V.device)), "expm_step_c")
+        return
     check(_lib.load().tnh_expm_step_c(k, n, m, j, ptr(V), V.stride(1), V.stride(0), ptr(w),
                                       w.stride(0), ptr(ab), ab.stride(0), ptr(ws), ws.numel(),
                                       stream(V.device)), "expm_step_c")

@@ -86,7 +86,7 @@ def clear_graphs() -> None:
     _graph_bytes[0] = 0
 
 
-def _krylov_graph(key, op, make_mv, k: int, m: int, n: int, dev):
+def _krylov_graph(key, op, make_mv, k: int, m: int, n: int, dev, rag=None):
     """The captured Krylov loop for `key`, or None to run eagerly. As tnkit's site-step graphs
     (tnkit/cudagraph.py:87-342, block_graph.py:137-151) a shape signature runs eagerly the first
     time and is captured the second: the m matvecs and orthogonalisation steps become one
@@ -113,10 +113,11 @@ def _krylov_graph(key, op, make_mv, k: int, m: int, n: int, dev):
     mv = make_mv(sop)
     torch.cuda.synchronize(dev)
     graph = torch.cuda.CUDAGraph()
+    srag = None if rag is None else (sop[rag[0]], rag[1], rag[2])
     with torch.cuda.graph(graph):
         for j in range(m):
             w = mv(V[:, j]).reshape(k, n)
-            kernels.expm_step_c(V, w, ab, j, ws)
+            kernels.expm_step_c(V, w, ab, j, ws, ragged=srag)
     entry = (graph, sop, V, ab, nbytes, ws)
     _GRAPHS[key] = entry
     _graph_bytes[0] += nbytes
@@ -160,16 +161,21 @@ def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
     their eigendecompositions on the device, so a solve reads nothing back. `v` is (k, n)
     complex128. `resid` (length k: a device float64 tensor, or a numpy array, which costs a host
     read) is raised to each leg's Saad a-posteriori error estimate
-    |beta_m tau [e^{tau T}]_{m-1,0}| beta0. `graph` = (key, operator tensors, make_mv) lets the
-    Krylov loop run as a captured hipGraph."""
+    |beta_m tau [e^{tau T}]_{m-1,0}| beta0. `graph` = (key, operator tensors, make_mv[, rag])
+    lets the Krylov loop run as a captured hipGraph; rag = (index of the legs' live extents among
+    the operator tensors, rowlen, bpad) lets its orthogonalisation skip each leg's padding."""
     k, n = v.shape
     m = max(1, min(int(m), n))
     dev = v.device
     beta0 = torch.linalg.vector_norm(v, dim=1)
     entry = None
+    rag = None
+    if graph is not None and len(graph) > 3 and graph[3] is not None:
+        rag = (graph[1][graph[3][0]], graph[3][1], graph[3][2])
     if graph is not None and graphs_enabled():
-        key, op, make_mv = graph
-        entry = _krylov_graph((key, m), op, make_mv, k, m, n, dev)
+        key, op, make_mv = graph[:3]
+        entry = _krylov_graph((key, m), op, make_mv, k, m, n, dev,
+                              None if len(graph) < 4 else graph[3])
     if entry is not None:
         g, sop, V, ab, _, _ = entry
         for st, t in zip(sop, op):
@@ -183,7 +189,7 @@ def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
         ws = kernels.expm_ws(k, dev)
         for j in range(m):
             w = matvec_b(V[:, j]).reshape(k, n).to(torch.complex128).contiguous()
-            kernels.expm_step_c(V, w, ab, j, ws)
+            kernels.expm_step_c(V, w, ab, j, ws, ragged=rag)
     c, est = _expm_tridiag(ab, beta0, tau, m)
     if isinstance(resid, torch.Tensor):
         torch.maximum(resid, est, out=resid)
@@ -486,7 +492,8 @@ def _matvec_two(Bt: _Batch, i: int, shape, low: bool):
                 shape[0], -1)
         return mv
     key = ("two", tuple(shape), tuple(tuple(t.shape) for t in op), low)
-    return make(op), (key, op, make)
+    k, a, d1, d2, b = shape
+    return make(op), (key, op, make, (3, d1 * d2 * b, b))
 
 
 def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool, live: torch.Tensor):
@@ -506,7 +513,7 @@ def _matvec_one(Bt: _Batch, L, W, R, shape, low: bool, live: torch.Tensor):
                                             live=o[3]).reshape(k, -1)
         return mv
     key = ("one", tuple(shape), tuple(tuple(t.shape) for t in op), low)
-    return make(op), (key, op, make)
+    return make(op), (key, op, make, (3, d * b, b))
 
 
 def _step(Bt: _Batch, i: int, dt: float, m: int, svd_tol: float, renorm: bool, moving: str,
