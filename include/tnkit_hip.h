@@ -221,6 +221,12 @@ int tnh_expm_step_c_ragged(int batch, long n, int m, int j, void* V, long ldv, l
  * re/im, leg stride sc doubles): the final combination beta0 * expm(tau T)[:, 0] . V. */
 int tnh_lincomb_c(int batch, long n, int nrows, const void* V, long ldv, long sV, const double* c,
                   long sc, void* out, long sout, tnh_stream_t stream);
+/* tnh_lincomb_c over a padded batch of legs with different bonds: live, rowlen and bpad as for
+ * tnh_expm_step_c_ragged; a leg's entries outside its live extents are written as zeros without
+ * reading V (they are zero in V). */
+int tnh_lincomb_c_ragged(int batch, long n, int nrows, const void* V, long ldv, long sV,
+                         const double* c, long sc, void* out, long sout, const int* live,
+                         int rowlen, int bpad, tnh_stream_t stream);
 
 /* Workspace bytes of the Krylov-vector entries below. */
 size_t tnh_krylov_ws_bytes(long n, int nmax);

@@ -291,8 +291,9 @@ def test_ragged_matvec_equals_the_padded_one(hip, a, b, low):
 
 
 def test_ragged_krylov_step_equals_the_padded_one(hip):
-    """tnh_expm_step_c_ragged on a batch whose legs are zero outside their own bonds gives the
-    basis and coefficients of the padded step, bit for bit (the skipped entries are zeros)."""
+    """tnh_expm_step_c_ragged and tnh_lincomb_c_ragged on a batch whose legs are zero outside
+    their own bonds give the basis, coefficients and combination of the padded forms, bit for bit
+    (the skipped entries are zeros)."""
     from tnkit_amd import kernels
     rng = np.random.default_rng(11)
     k, m, a, d, b = 4, 6, 48, 2, 40
@@ -305,6 +306,7 @@ def test_ragged_krylov_step_equals_the_padded_one(hip):
         V[z, :, :ra[z], :, :, :rb[z]] = rng.standard_normal((m,) + sh) + 1j * rng.standard_normal((m,) + sh)
         w[z, :ra[z], :, :, :rb[z]] = rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
     live = torch.as_tensor(np.stack([ra, rb], 1).astype(np.int32), device=hip)
+    cf = torch.as_tensor(rng.standard_normal((k, m)) + 1j * rng.standard_normal((k, m))).to(hip)
     outs = []
     for rag in (None, (live, d * d * b, b)):
         Vt = torch.as_tensor(V.reshape(k, m, n)).to(hip)
@@ -314,6 +316,9 @@ def test_ragged_krylov_step_equals_the_padded_one(hip):
         for j in range(3):
             wt = torch.as_tensor(w.reshape(k, n) * (j + 1)).to(hip)
             kernels.expm_step_c(Vt, wt, ab, j, ws, ragged=rag)
-        outs.append((Vt[:, :4].cpu().numpy(), ab.cpu().numpy()))
+        out = torch.full((k, n), 7.0 + 0j, dtype=torch.complex128, device=hip)
+        kernels.lincomb_c(Vt, cf, out, 4, ragged=rag)
+        outs.append((Vt[:, :4].cpu().numpy(), ab.cpu().numpy(), out.cpu().numpy()))
     assert np.array_equal(outs[0][1], outs[1][1])
     assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][2], outs[1][2])

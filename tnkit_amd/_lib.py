@@ -61,6 +61,8 @@ SIGNATURES = {
                                        _P, c_long, _P, c_int, c_int, _P, c_size_t, _P]),
     "tnh_lincomb_c": (c_int, [c_int, c_long, c_int, _P, c_long, c_long, _P, c_long, _P, c_long,
                               _P]),
+    "tnh_lincomb_c_ragged": (c_int, [c_int, c_long, c_int, _P, c_long, c_long, _P, c_long, _P,
+                                     c_long, _P, c_int, c_int, _P]),
     "tnh_krylov_ws_bytes": (c_size_t, [c_long, c_int]),
     "tnh_krylov_orth": (c_int, [c_long, c_int, _P, c_long, _P, c_int, c_int, c_int, _PD, _P, _P,
                                 c_size_t, _P]),

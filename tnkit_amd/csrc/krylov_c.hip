@@ -22,9 +22,6 @@ __device__ __forceinline__ double wsum(double v) {
   return v;
 }
 
-// One pass over the live basis of every leg: optional three-term subtraction, subtraction of
-// the previous pass's projection (complex coefficients `coef`), then partial sums of the next
-// projection <V[k], w> (mode 0) or of |w|^2 (mode 1). part is [leg][block][2*NV] doubles.
 // Ragged batches: `live` (2 ints per leg, may be null) = the leg's live extents (x, y) in a
 // vector laid out as rows of `rowlen` entries whose last index runs over `bpad`: entry i is live
 // when i / rowlen < x and i % bpad < y. The others are zero in every vector of the leg, so a pass
@@ -40,6 +37,9 @@ __device__ __forceinline__ bool live_entry(const Ragged& rg, int lx, int ly, lon
   return ii / rg.rowlen < lx && ii % rg.bpad < ly;
 }
 
+// One pass over the live basis of every leg: optional three-term subtraction, subtraction of
+// the previous pass's projection (complex coefficients `coef`), then partial sums of the next
+// projection <V[k], w> (mode 0) or of |w|^2 (mode 1). part is [leg][block][2*NV] doubles.
 template <int NV>
 __global__ void __launch_bounds__(256)
     cgs_c_kernel(long n, int nv, const c128* __restrict__ V, long ldv, long sV, c128* __restrict__ w,
@@ -139,15 +139,21 @@ __global__ void next_vec_kernel(long n2, const double* __restrict__ w, long sw2,
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n2; i += (long)gridDim.x * 256) Vn[i] = w[i] * s;
 }
 
-// out[leg] = sum_{k<nrows} c[leg][k] V[leg][k], complex coefficients on the device
+// out[leg] = sum_{k<nrows} c[leg][k] V[leg][k], complex coefficients on the device; with live
+// extents (Ragged), a leg's entries outside them are zero in V and written as zeros unread
 __global__ void lincomb_c_kernel(long n, int nrows, const c128* __restrict__ V, long ldv, long sV,
                                  const double* __restrict__ c, long sc, c128* __restrict__ out,
-                                 long sout) {
+                                 long sout, Ragged rg) {
   const int leg = blockIdx.y;
   V += leg * sV;
   out += leg * sout;
   c += leg * sc;
+  const int lx = rg.live ? rg.live[2 * leg] : 0, ly = rg.live ? rg.live[2 * leg + 1] : 0;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) {
+    if (!live_entry(rg, lx, ly, i)) {
+      out[i] = c128(0.0, 0.0);
+      continue;
+    }
     double re = 0.0, im = 0.0;
     for (int k = 0; k < nrows; ++k) {
       const c128 v = V[(long)k * ldv + i];
@@ -275,14 +281,25 @@ int tnh_expm_step_c_ragged(int batch, long n, int m, int j, void* V, long ldv, l
 
 int tnh_lincomb_c(int batch, long n, int nrows, const void* V, long ldv, long sV, const double* c,
                   long sc, void* out, long sout, tnh_stream_t stream) {
+  return tnh_lincomb_c_ragged(batch, n, nrows, V, ldv, sV, c, sc, out, sout, nullptr, 1, 1,
+                              stream);
+}
+
+int tnh_lincomb_c_ragged(int batch, long n, int nrows, const void* V, long ldv, long sV,
+                         const double* c, long sc, void* out, long sout, const int* live,
+                         int rowlen, int bpad, tnh_stream_t stream) {
   TNH_REQUIRE(batch >= 1 && batch <= 65535 && n >= 0 && nrows >= 0 && (nrows == 0 || ldv >= n),
               TNH_ERR_ARG, "tnh_lincomb_c: bad sizes");
+  TNH_REQUIRE(!live || (rowlen >= 1 && bpad >= 1 && rowlen % bpad == 0 && n % rowlen == 0 &&
+                        n < (1L << 31)),
+              TNH_ERR_ARG, "tnh_lincomb_c_ragged: bad layout (n %ld, rowlen %d, bpad %d)", n,
+              rowlen, bpad);
   if (n == 0) return TNH_OK;
   long g = (n + 255) / 256;
   if (g > 4096) g = 4096;
   hipLaunchKernelGGL(lincomb_c_kernel, dim3((unsigned)g, batch), dim3(256), 0,
                      static_cast<hipStream_t>(stream), n, nrows, static_cast<const c128*>(V), ldv, sV,
-                     c, sc, static_cast<c128*>(out), sout);
+                     c, sc, static_cast<c128*>(out), sout, Ragged{live, rowlen, bpad});
   TNH_CHECK_LAUNCH("lincomb_c");
   return TNH_OK;
 }

@@ -655,10 +655,20 @@ def expm_ws(k: int, device) -> torch.Tensor:
     return _ws(_lib.load().tnh_expm_ws_bytes(k), device)
 
 
-def lincomb_c(V: torch.Tensor, c: torch.Tensor, out: torch.Tensor, nrows: int) -> torch.Tensor:
-    """out[leg] = sum_{r<nrows} c[leg, r] V[leg, r] for complex128 V (k, m, n), c (k, m)."""
+def lincomb_c(V: torch.Tensor, c: torch.Tensor, out: torch.Tensor, nrows: int,
+              ragged=None) -> torch.Tensor:
+    """out[leg] = sum_{r<nrows} c[leg, r] V[leg, r] for complex128 V (k, m, n), c (k, m).
+    `ragged` = (live, rowlen, bpad) as for `expm_step_c`: each leg's padding is written as zeros
+    without being read."""
     k, m, n = V.shape
     cr = torch.view_as_real(c.contiguous())
+    if ragged is not None:
+        live, rowlen, bpad = ragged
+        check(_lib.load().tnh_lincomb_c_ragged(k, n, nrows, ptr(V), V.stride(1), V.stride(0),
+                                               ptr(cr), cr.stride(0), ptr(out), out.stride(0),
+                                               ptr(live), int(rowlen), int(bpad),
+                                               stream(V.device)), "lincomb_c")
+        return out
     check(_lib.load().tnh_lincomb_c(k, n, nrows, ptr(V), V.stride(1), V.stride(0), ptr(cr),
                                     cr.stride(0), ptr(out), out.stride(0), stream(V.device)),
           "lincomb_c")

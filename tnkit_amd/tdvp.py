@@ -196,7 +196,7 @@ def lanczos_expm_fixed_batched(matvec_b, v: torch.Tensor, tau: complex, m: int,
     elif resid is not None:
         resid[:] = np.maximum(resid, est.cpu().numpy())
     out = torch.empty((k, n), dtype=torch.complex128, device=dev)
-    kernels.lincomb_c(V, c, out, m)
+    kernels.lincomb_c(V, c, out, m, ragged=rag)
     return out
 
 
