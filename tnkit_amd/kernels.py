@@ -366,6 +366,14 @@ def qr_supported(A: torch.Tensor) -> bool:
     return 1 <= n <= m <= QR_MAX_M[A.dtype] and (A.dim() == 2 or int(A.shape[0]) >= 1)
 
 
+def qr_any(A: torch.Tensor) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Reduced QR on the library's kernels where `qr` serves A, rocSOLVER's otherwise (wide
+    matrices, single precision, rows beyond the limits)."""
+    if qr_supported(A):
+        return qr(A)
+    return torch.linalg.qr(A)
+
+
 def qr(A: torch.Tensor, check_status: bool = True) -> Tuple[torch.Tensor, torch.Tensor]:
     """Thin QR of a tall (m, n) or batched (B, m, n) float64 / complex128 matrix on the library's
     kernels: LAPACK geqrf's reflectors in one cooperative launch per group of resident matrices,

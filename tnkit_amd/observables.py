@@ -34,11 +34,12 @@ def _device_mps(mps) -> List[torch.Tensor]:
 
 
 def right_canonical(mps) -> List[torch.Tensor]:
-    """A right-canonical, normalised copy of a dense MPS on the device (LQ per site)."""
+    """A right-canonical, normalised copy of a dense MPS on the device (LQ per site, on the
+    library's QR)."""
     M = [t.clone() for t in _device_mps(mps)]
     for i in range(len(M) - 1, 0, -1):
         Dl, d, Dr = M[i].shape
-        Q, Rr = torch.linalg.qr(M[i].reshape(Dl, d * Dr).mT)
+        Q, Rr = kernels.qr_any(M[i].reshape(Dl, d * Dr).mT)
         M[i] = Q.mT.reshape(-1, d, Dr).contiguous()
         M[i - 1] = torch.tensordot(M[i - 1], Rr.mT, dims=([2], [0])).contiguous()
     M[0] = (M[0] / torch.linalg.norm(M[0])).contiguous()

@@ -249,16 +249,33 @@ class _Batch:
 
 
 def _right_canonical(M: List[torch.Tensor]) -> List[torch.Tensor]:
-    """Right-canonical form of one complex MPS by LQ per site, centre at 0, normalised."""
-    M = [t.clone() for t in M]
-    for i in range(len(M) - 1, 0, -1):
-        Dl, d, Dr = M[i].shape
-        Q, Rr = torch.linalg.qr(M[i].reshape(Dl, d * Dr).mT)
-        M[i] = Q.mT.reshape(-1, d, Dr).contiguous()
-        M[i - 1] = torch.tensordot(M[i - 1], Rr.mT, dims=([2], [0])).contiguous()
-        M[i - 1] = M[i - 1] / torch.linalg.norm(M[i - 1])
-    M[0] = M[0] / torch.linalg.norm(M[0])
-    return M
+    """Right-canonical form of one complex MPS by LQ per site (the library's QR), centre at 0,
+    normalised."""
+    return _right_canonical_all([M])[0]
+
+
+def _right_canonical_all(states: Sequence[Sequence[torch.Tensor]]) -> List[List[torch.Tensor]]:
+    """`_right_canonical` of several MPS: states whose site shapes all agree go through each
+    site's LQ together, one batched QR per site (a batched factorisation equals the single ones
+    bit for bit)."""
+    groups = {}
+    for q, M in enumerate(states):
+        groups.setdefault(tuple(tuple(t.shape) for t in M), []).append(q)
+    out: List[Optional[List[torch.Tensor]]] = [None] * len(states)
+    for shapes, idx in groups.items():
+        Ms = [torch.stack([states[q][i] for q in idx]) for i in range(len(shapes))]
+        for i in range(len(Ms) - 1, 0, -1):
+            k, Dl, d, Dr = Ms[i].shape
+            Q, Rr = kernels.qr_any(Ms[i].reshape(k, Dl, d * Dr).mT)
+            Ms[i] = Q.mT.reshape(k, -1, d, Dr).contiguous()
+            Ms[i - 1] = torch.matmul(Ms[i - 1], Rr.mT.unsqueeze(1)).contiguous()
+            Ms[i - 1] = Ms[i - 1] / torch.linalg.vector_norm(
+                Ms[i - 1].reshape(k, -1), dim=1)[:, None, None, None]
+        Ms[0] = Ms[0] / torch.linalg.vector_norm(Ms[0].reshape(len(idx), -1), dim=1)[:, None, None,
+                                                                                      None]
+        for j, q in enumerate(idx):
+            out[q] = [t[j].contiguous() for t in Ms]
+    return out
 
 
 def _eigh_robust(G: torch.Tensor, add: Optional[torch.Tensor] = None, deferred: bool = False,
@@ -631,7 +648,7 @@ def evolve_batch(states: Sequence[Sequence[Any]], hams: Any, dt: float, n_steps:
         .to(dev, torch.complex128)  # noqa: E731
     st = [[c(t) for t in s] for s in states]
     if o.canonicalize:
-        st = [_right_canonical(s) for s in st]
+        st = _right_canonical_all(st)
     wd = {id(h): [c(w) for w in h] for h in mpos}     # one device copy per distinct MPO
     mp = [wd[id(h)] for h in mpos]
     groups = _partition(st, o.ragged)
