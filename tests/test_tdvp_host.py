@@ -92,3 +92,38 @@ def test_gram_cut_on_the_live_extent(monkeypatch, iso, a, b, width):
         pad_rows[:ra[leg] * d] = False
         if iso == "left" and pad_rows.any():            # Householder QR: zero to rounding
             assert float(U2[leg, torch.as_tensor(pad_rows), :r].abs().max()) <= 1e-13
+
+
+def _mps_vector(M):
+    v = M[0]
+    for t in M[1:]:
+        v = torch.tensordot(v, t, dims=([v.dim() - 1], [0]))
+    return v.reshape(-1)
+
+
+def test_right_canonical_all_batches_legs_of_one_shape():
+    """The batched start canonicalisation (one QR per site over the legs whose site shapes agree)
+    on the host: every leg ends right-canonical (sum over s, r of M M^H = 1 at every site but the
+    first), normalised, and the same state as its input up to normalisation; legs of another
+    shape take their own group and the same result as alone."""
+    rng = np.random.default_rng(3)
+    dims = [1, 2, 4, 6, 4, 2, 1]
+    other = [1, 2, 3, 3, 3, 2, 1]
+    def rand_mps(ds):
+        return [torch.as_tensor(rng.standard_normal((ds[i], 2, ds[i + 1]))
+                                + 1j * rng.standard_normal((ds[i], 2, ds[i + 1])))
+                for i in range(len(ds) - 1)]
+    states = [rand_mps(dims), rand_mps(other), rand_mps(dims), rand_mps(dims)]
+    out = T._right_canonical_all(states)
+    for M, C in zip(states, out):
+        assert [tuple(t.shape) for t in C] == [tuple(t.shape) for t in M]
+        for t in C[1:]:
+            a, d, b = t.shape
+            m = t.reshape(a, d * b)
+            assert float((m @ m.mH - torch.eye(a, dtype=m.dtype)).abs().max()) < 1e-13
+        v, w = _mps_vector(M), _mps_vector(C)
+        assert abs(float(torch.linalg.vector_norm(w)) - 1.0) < 1e-13
+        ov = torch.vdot(v, w) / torch.linalg.vector_norm(v)
+        assert abs(abs(complex(ov)) - 1.0) < 1e-12
+    alone = T._right_canonical(states[1])
+    assert all(torch.equal(x, y) for x, y in zip(alone, out[1]))
