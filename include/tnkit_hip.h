@@ -145,7 +145,10 @@ int tnh_env_right_bwd(int dtype, int c, int d, int b, int wl, int wr, const void
 /* ---- hipGraph capture of library calls (replaces tnkit/block_graph.py:89-180 and the capture
  * half of tnkit/cudagraph.py:87-342 for C callers): tnh_graph_begin starts a thread-local
  * capture on `stream` (not the null stream); every library call issued on it is recorded (the
- * library allocates nothing and never synchronises, so all of them can be); tnh_graph_end
+ * library allocates nothing and never synchronises) except the cooperative solvers tnh_sytrd,
+ * tnh_syev_top, tnh_heev_top and tnh_qr, which are for eager streams only: each resets its grid
+ * barrier words before every cooperative launch (tnh_heev_top and tnh_qr with hipMemsetAsync,
+ * whose captured nodes are not replay-safe), so do not issue them inside a capture; tnh_graph_end
  * instantiates the recording into *graph; tnh_graph_launch replays it on a stream;
  * tnh_graph_destroy frees it. Pointers are baked in: replays read and write the same buffers. */
 int tnh_graph_begin(tnh_stream_t stream);
